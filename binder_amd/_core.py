@@ -1,0 +1,369 @@
+"""Python face of the gfx950 backend: the reference's ``BINDER._core`` API
+(names, argument order and keywords of BINDER/src/gpu/bindings.cpp, plus the
+CPU-only ``compute_jacobian_determinant_*`` of BINDER/src/cpu/bindings.cpp)
+implemented as a ctypes shim over the C ABI in include/binder_c.h.
+
+Arrays are accepted in any dtype/contiguity and converted like pybind11's
+``c_style | forcecast``.  Every call runs on the GPU through
+``libbinder_hip.so``; if the library is missing the import fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+backend = "hip"
+has_topology_correction = True
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("BINDER_HIP_LIB", os.path.join(_HERE, "libbinder_hip.so"))
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"BINDER HIP backend library not found at {LIB_PATH}; build it with "
+        "`python binder_amd/build.py` (hipcc, gfx950)")
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+_c_int = ctypes.c_int
+_c_double = ctypes.c_double
+_vp = ctypes.c_void_p
+_P = ctypes.POINTER
+
+
+def _sig(name, restype, *args):
+    f = getattr(_lib, name)
+    f.restype = restype
+    f.argtypes = list(args)
+    return f
+
+
+_lib.binder_last_error.restype = ctypes.c_char_p
+_lib.binder_backend.restype = ctypes.c_char_p
+
+_reg_create = _sig("binder_registration_create", _c_int, _P(_vp), _vp, _c_int, _vp, _vp,
+                   _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double)
+_reg_destroy = _sig("binder_registration_destroy", None, _vp)
+_reg_set_mi = _sig("binder_registration_set_mi_likelihood", _c_int, _vp, _c_double, _vp,
+                   _c_int, _c_int, _c_int, _vp, _vp)
+_reg_set_nlt = _sig("binder_registration_set_nonlinear_transformation", _c_int, _vp,
+                    *([_c_double] * 9), _vp)
+_reg_em = _sig("binder_registration_em", _c_int, _vp, _c_int, _c_double, _c_int, _c_int)
+_reg_sampler = _sig("binder_registration_sampler", _c_int, _vp, *([_c_int] * 8), _vp, _vp)
+_reg_get_final = _sig("binder_registration_get_final_locations", _c_int, _vp, _vp)
+_reg_get_dream = _sig("binder_registration_get_dream_locations", _c_int, _vp, _vp)
+_reg_set_dream = _sig("binder_registration_set_dream_locations", _c_int, _vp, _vp)
+_reg_get_theta = _sig("binder_registration_get_likelihood_parameters", _c_int, _vp, _vp)
+_reg_get_shape = _sig("binder_registration_get_shape", _c_int, _vp, _vp)
+_reg_get_trace = _sig("binder_registration_get_objective_trace", _c_int, _vp, _vp, _P(_c_int))
+_reg_e_step = _sig("binder_registration_e_step", _c_int, _vp, _P(_c_double), _vp)
+_reg_set_prof = _sig("binder_registration_set_profiling", _c_int, _vp, _c_int)
+_reg_get_times = _sig("binder_registration_get_kernel_times", _c_int, _vp, ctypes.c_char_p,
+                      _c_int, _vp, _vp, _P(_c_int))
+_nlt_create = _sig("binder_nlt_create", _c_int, _P(_vp), *([_c_double] * 9), _vp,
+                   _c_int, _c_int, _c_int, _c_int, _c_double, _c_double)
+_nlt_destroy = _sig("binder_nlt_destroy", None, _vp)
+_nlt_smooth = _sig("binder_nlt_smooth_deformation", _c_int, _vp, _vp, _vp, _P(_c_double))
+_nlt_sample = _sig("binder_nlt_sample_deformation_field", _c_int, _vp, _vp, _c_int,
+                   ctypes.c_ulonglong, _vp)
+_nlt_up = _sig("binder_nlt_up_sample", _c_int, _vp, *([_c_int] * 7), _vp)
+_jac3 = _sig("binder_jacobian_determinant_3d", _c_int, _vp, _c_int, _c_int, _c_int,
+             _c_double, _c_double, _c_double, _vp)
+_jac2 = _sig("binder_jacobian_determinant_2d", _c_int, _vp, _c_int, _c_int, _c_double,
+             _c_double, _vp)
+_fold3 = _sig("binder_fold_mask_3d", _c_int, _vp, _c_int, _c_int, _c_int, _c_double,
+              _c_double, _c_double, _c_double, _vp, _P(ctypes.c_longlong))
+_topo3 = _sig("binder_topology_correction_3d", _c_int, _vp, _c_double, _c_double, _c_int,
+              _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp)
+
+
+_philox = _sig("binder_philox4x32_10", _c_int, _vp, _vp, _vp)
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 block (host-side) used by the sampler's random streams."""
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    _check(_philox(_ptr(c), _ptr(k), _ptr(out)))
+    return out
+
+
+def _check(rc):
+    if rc == 0:
+        return
+    msg = _lib.binder_last_error().decode(errors="replace")
+    if rc == 1:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def _arr(a, dtype):
+    """pybind11-style c_style|forcecast conversion."""
+    return np.ascontiguousarray(np.asarray(a), dtype=dtype)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp)
+
+
+class Registration:
+    """Registration(final_locations, threads, voxel_pos, node_pos, Nx_v, Ny_v,
+    Nz_v, Nx_n, Ny_n, Nz_n, spline_order, non_zero_voxels)
+
+    GPU-resident EM registration and Gibbs sampler
+    (BINDER/src/gpu/bindings.cpp:17-157, BINDER/src/cpu/Registration.cpp)."""
+
+    def __init__(self, final_locations, threads, voxel_pos, node_pos, Nx_v, Ny_v, Nz_v,
+                 Nx_n, Ny_n, Nz_n, spline_order, non_zero_voxels):
+        self._h = None
+        fl = _arr(final_locations, np.float32)
+        vp = _arr(voxel_pos, np.float32)
+        N = 2 if int(Nz_v) == 1 else 3
+        n = int(Nx_v) * int(Ny_v) * int(Nz_v) * N
+        if fl.size != n or vp.size != n:
+            raise ValueError(f"final_locations/voxel_pos must have {n} values")
+        h = _vp()
+        _check(_reg_create(ctypes.byref(h), _ptr(fl), int(threads), _ptr(vp), None,
+                           int(Nx_v), int(Ny_v), int(Nz_v), int(Nx_n), int(Ny_n), int(Nz_n),
+                           int(spline_order), float(non_zero_voxels)))
+        self._h = h
+        self._shape = (int(Nx_v), int(Ny_v), int(Nz_v), N)
+        self._nn = int(Nx_n) * int(Ny_n) * int(Nz_n)
+        self._KL = None
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _reg_destroy(self._h)
+            self._h = None
+
+    def setMILikelihood(self, alpha, theta, K, L, threads, binned_nodes, binned_voxels):
+        th = _arr(theta, np.float64)
+        bn = _arr(binned_nodes, np.int32)
+        bv = _arr(binned_voxels, np.int32)
+        if th.size != int(K) * int(L):
+            raise ValueError("theta must have K*L values")
+        if bn.size != self._nn or bv.size != int(np.prod(self._shape[:3])):
+            raise ValueError("binned_nodes / binned_voxels have the wrong size")
+        _check(_reg_set_mi(self._h, float(alpha), _ptr(th), int(K), int(L), int(threads),
+                           _ptr(bn), _ptr(bv)))
+        self._KL = (int(K), int(L))
+
+    def setNonLinearTransformation(self, gamma_x, gamma_y, gamma_z, voxels_size_x,
+                                   voxels_size_y, voxels_size_z, nodes_size_x, nodes_size_y,
+                                   nodes_size_z, D):
+        Dv = _arr(D, np.float32)
+        if Dv.size != int(np.prod(self._shape[:3])):
+            raise ValueError("D must have Nx_v*Ny_v*Nz_v values")
+        _check(_reg_set_nlt(self._h, float(gamma_x), float(gamma_y), float(gamma_z),
+                            float(voxels_size_x), float(voxels_size_y), float(voxels_size_z),
+                            float(nodes_size_x), float(nodes_size_y), float(nodes_size_z),
+                            _ptr(Dv)))
+
+    def EM(self, max_EM_iterations, convergence_th, update_likelihood_parameters, debug):
+        _check(_reg_em(self._h, int(max_EM_iterations), float(convergence_th),
+                       int(bool(update_likelihood_parameters)), int(bool(debug))))
+
+    def sampler(self, seed, N_b, N_s, sample_gamma, sample_gamma_every, sample_posteriors,
+                sample_likelihood_parameters, sample_transformation_parameters):
+        Nx, Ny, Nz, N = self._shape
+        mean = np.empty((Nx, Ny, Nz, N), np.float64)
+        cov = np.empty((Nx, Ny, Nz, N, N), np.float64)
+        _check(_reg_sampler(self._h, int(seed), int(N_b), int(N_s), int(sample_gamma),
+                            int(sample_gamma_every), int(sample_posteriors),
+                            int(sample_likelihood_parameters),
+                            int(sample_transformation_parameters), _ptr(mean), _ptr(cov)))
+        return mean, cov
+
+    def get_final_locations_as_numpy_array(self):
+        out = np.empty(self._shape, np.float32)
+        _check(_reg_get_final(self._h, _ptr(out)))
+        return out
+
+    def get_dream_locations(self):
+        out = np.empty(self._shape, np.float32)
+        _check(_reg_get_dream(self._h, _ptr(out)))
+        return out.ravel().tolist()
+
+    def set_dream_locations(self, dream_locations):
+        d = _arr(dream_locations, np.float32)
+        if d.size != int(np.prod(self._shape)):
+            raise ValueError("dream_locations has the wrong size")
+        _check(_reg_set_dream(self._h, _ptr(d)))
+
+    def get_likelihood_parameters(self):
+        if self._KL is None:
+            raise RuntimeError("setMILikelihood was not called")
+        out = np.empty(self._KL[0] * self._KL[1], np.float64)
+        _check(_reg_get_theta(self._h, _ptr(out)))
+        return out.tolist()
+
+    def get_shape(self):
+        return list(self._shape)
+
+    # -- additions beyond the reference surface ----------------------------
+    def get_objective_trace(self):
+        """Rows of (mll, prior_theta, prior_d, mlp) of the last EM call."""
+        rows = _c_int(0)
+        _check(_reg_get_trace(self._h, None, ctypes.byref(rows)))
+        out = np.empty((max(rows.value, 1), 4), np.float64)
+        _check(_reg_get_trace(self._h, _ptr(out), ctypes.byref(rows)))
+        return out[:rows.value]
+
+    def e_step(self, return_histogram=False):
+        """One E-step; returns mll (and the soft joint histogram)."""
+        mll = _c_double(0.0)
+        hist = None
+        if return_histogram:
+            hist = np.empty(self._KL, np.float64)
+        _check(_reg_e_step(self._h, ctypes.byref(mll), _ptr(hist) if hist is not None else None))
+        return (mll.value, hist) if return_histogram else mll.value
+
+    def set_profiling(self, enable):
+        _check(_reg_set_prof(self._h, int(bool(enable))))
+
+    def get_kernel_times(self):
+        """{kernel name: (total ms, launches)} recorded since set_profiling(True)."""
+        cap = 64
+        names = ctypes.create_string_buffer(4096)
+        ms = np.zeros(cap, np.float64)
+        cnt = np.zeros(cap, np.int64)
+        n = _c_int(cap)
+        _check(_reg_get_times(self._h, names, 4096, _ptr(ms), _ptr(cnt), ctypes.byref(n)))
+        keys = names.value.decode().split("\n")[: n.value]
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(keys)}
+
+
+class NonLinearTransformation:
+    """NonLinearTransformation(gamma_x, gamma_y, gamma_z, voxels_size_x..z,
+    nodes_size_x..z, D, Nx_v, Ny_v, Nz_v, threads, sigma_spline, spline_offset)
+    (BINDER/src/gpu/bindings.cpp:159-255)."""
+
+    def __init__(self, gamma_x, gamma_y, gamma_z, voxels_size_x, voxels_size_y,
+                 voxels_size_z, nodes_size_x, nodes_size_y, nodes_size_z, D, Nx_v, Ny_v,
+                 Nz_v, threads, sigma_spline, spline_offset):
+        self._h = None
+        Dv = _arr(D, np.float32)
+        if Dv.size != int(Nx_v) * int(Ny_v) * int(Nz_v):
+            raise ValueError("D must have Nx_v*Ny_v*Nz_v values")
+        h = _vp()
+        _check(_nlt_create(ctypes.byref(h), float(gamma_x), float(gamma_y), float(gamma_z),
+                           float(voxels_size_x), float(voxels_size_y), float(voxels_size_z),
+                           float(nodes_size_x), float(nodes_size_y), float(nodes_size_z),
+                           _ptr(Dv), int(Nx_v), int(Ny_v), int(Nz_v), int(threads),
+                           float(sigma_spline), float(spline_offset)))
+        self._h = h
+        self._shape = (int(Nx_v), int(Ny_v), int(Nz_v), 2 if int(Nz_v) == 1 else 3)
+        self.seed = 1234
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _nlt_destroy(self._h)
+            self._h = None
+
+    def get_shape(self):
+        return list(self._shape)
+
+    def smooth_deformation(self, deformation):
+        d = _arr(deformation, np.float32)
+        if d.size != int(np.prod(self._shape)):
+            raise ValueError("deformation has the wrong size")
+        out = np.empty(self._shape, np.float32)
+        cost = _c_double(0.0)
+        _check(_nlt_smooth(self._h, _ptr(d), _ptr(out), ctypes.byref(cost)))
+        return cost.value, out
+
+    def sampleDeformationField(self, deformationField, number_of_samples):
+        d = _arr(deformationField, np.float32)
+        if d.size != int(np.prod(self._shape)):
+            raise ValueError("deformationField has the wrong size")
+        S = int(number_of_samples)
+        out = np.empty((S,) + self._shape, np.float32)
+        _check(_nlt_sample(self._h, _ptr(d), S, int(self.seed), _ptr(out)))
+        self.seed += 1
+        return out
+
+    @staticmethod
+    def up_sample(deformation_field, N_x, N_y, N_z, N_x_o, N_y_o, N_z_o, threads):
+        f = _arr(deformation_field, np.float64)
+        N = 2 if int(N_z_o) == 1 else 3
+        out = np.empty((int(N_x_o), int(N_y_o), int(N_z_o), N), np.float64)
+        _check(_nlt_up(_ptr(f), int(N_x), int(N_y), int(N_z), int(N_x_o), int(N_y_o),
+                       int(N_z_o), int(threads), _ptr(out)))
+        return out
+
+
+class topology_correction_3D:
+    """topology_correction_3D().correct_topology(h_init, e1, e2, max_outer_it,
+    max_it, threads, voxel_sizes, Nx_v, Ny_v, Nz_v)
+    (BINDER/src/gpu/bindings.cpp:257-287)."""
+
+    def __init__(self):
+        self.last_stats = None
+
+    def correct_topology(self, h_init, e1, e2, max_outer_it, max_it, threads, voxel_sizes,
+                         Nx_v, Ny_v, Nz_v):
+        h = _arr(h_init, np.float64)
+        vs = _arr(voxel_sizes, np.float64)
+        if h.size != int(Nx_v) * int(Ny_v) * int(Nz_v) * 3 or vs.size < 3:
+            raise ValueError("h_init / voxel_sizes have the wrong size")
+        out = np.empty((int(Nx_v), int(Ny_v), int(Nz_v), 3), np.float64)
+        stats = np.zeros(6, np.float64)
+        _check(_topo3(_ptr(h), float(e1), float(e2), int(max_outer_it), int(max_it),
+                      int(threads), _ptr(vs), int(Nx_v), int(Ny_v), int(Nz_v), _ptr(out),
+                      _ptr(stats)))
+        self.last_stats = stats
+        return out
+
+
+def compute_jacobian_determinant_3D(field, Nx, Ny, Nz, voxel_size_x, voxel_size_y,
+                                    voxel_size_z):
+    """BINDER/src/cpu/bindings.cpp:564-593; FP64, bit-exact with the CPU."""
+    f = _arr(field, np.float64)
+    if f.size != int(Nx) * int(Ny) * int(Nz) * 3:
+        raise ValueError("field must have Nx*Ny*Nz*3 values")
+    out = np.empty((int(Nx), int(Ny), int(Nz)), np.float64)
+    _check(_jac3(_ptr(f), int(Nx), int(Ny), int(Nz), float(voxel_size_x),
+                 float(voxel_size_y), float(voxel_size_z), _ptr(out)))
+    return out
+
+
+def compute_jacobian_determinant_2D(field, Nx, Ny, voxel_size_x, voxel_size_y):
+    """BINDER/src/cpu/bindings.cpp:530-562; FP64, bit-exact with the CPU."""
+    f = _arr(field, np.float64)
+    if f.size != int(Nx) * int(Ny) * 2:
+        raise ValueError("field must have Nx*Ny*2 values")
+    out = np.empty((int(Nx), int(Ny)), np.float64)
+    _check(_jac2(_ptr(f), int(Nx), int(Ny), float(voxel_size_x), float(voxel_size_y),
+                 _ptr(out)))
+    return out
+
+
+def fold_mask_3D(field, Nx, Ny, Nz, voxel_size_x, voxel_size_y, voxel_size_z, e1=0.0):
+    """Voxel-wise fold mask (J < e1) and its count, from the same FP64 Jacobian."""
+    f = _arr(field, np.float64)
+    mask = np.empty((int(Nx), int(Ny), int(Nz)), np.uint8)
+    cnt = ctypes.c_longlong(0)
+    _check(_fold3(_ptr(f), int(Nx), int(Ny), int(Nz), float(voxel_size_x),
+                  float(voxel_size_y), float(voxel_size_z), float(e1), _ptr(mask),
+                  ctypes.byref(cnt)))
+    return mask.astype(bool), int(cnt.value)
+
+
+def exported_symbols():
+    """Names of the C entry points this module binds (for ABI checks)."""
+    return [
+        "binder_last_error", "binder_backend", "binder_philox4x32_10",
+        "binder_registration_create",
+        "binder_registration_destroy", "binder_registration_set_mi_likelihood",
+        "binder_registration_set_nonlinear_transformation", "binder_registration_em",
+        "binder_registration_sampler", "binder_registration_get_final_locations",
+        "binder_registration_get_dream_locations", "binder_registration_set_dream_locations",
+        "binder_registration_get_likelihood_parameters", "binder_registration_get_shape",
+        "binder_registration_get_objective_trace", "binder_registration_e_step",
+        "binder_registration_set_profiling", "binder_registration_get_kernel_times",
+        "binder_nlt_create", "binder_nlt_destroy", "binder_nlt_smooth_deformation",
+        "binder_nlt_sample_deformation_field", "binder_nlt_up_sample",
+        "binder_jacobian_determinant_3d", "binder_jacobian_determinant_2d",
+        "binder_fold_mask_3d", "binder_topology_correction_3d",
+    ]

@@ -1,0 +1,347 @@
+// E-step kernels for the EM and Gibbs paths.
+//
+// One thread per fixed-image voxel i (grid-stride).  From the current
+// position d_i it forms the cubic B-spline prior over the 4x4x4 moving-grid
+// nodes around floor(d_i), weights each node by the likelihood
+// theta[v_j, u_i] (v_j = moving bin of node j, u_i = fixed bin of voxel i;
+// nodes outside the moving grid use moving bin 0), and
+//   EM:      writes the posterior-mean ("dream") location, adds -log Z_i to
+//            the block's mll and the posteriors w_ij / Z_i to the block's
+//            joint histogram;
+//   sampling: draws one node from the posterior with a Philox uniform, writes
+//            its grid coordinates and counts it in the block's histogram.
+// Reference: BINDER/src/cpu/Registration.cpp:219-330 (voxel),
+// :153-165 / :558-571 (loops), BINDER/src/cpu/MILikelihood.cpp:42-55 (theta
+// lookup, out-of-range -> bin 0), :89-124 (histogram), :71-86 (MAP theta),
+// :132-169 (Dirichlet draw), :183-206 (Dirichlet prior);
+// BINDER/src/cpu/Spline.cpp:44-73 (cubic weights).
+//
+// Moving bins are read as packed words: for 3D, word (x,y,z) of a padded grid
+// holds the bins of z..z+3 (bin 0 outside the image), so one 32-bit load
+// serves a whole 4-node z-row and no bounds test is needed after clamping.
+// The likelihood column theta[:, u] and the block histogram live in LDS; the
+// histogram is a 64-bit fixed-point array (exact, order-independent sums).
+// Consecutive neighbours with the same moving bin are merged in registers
+// before the LDS atomic, which removes most atomics in homogeneous regions.
+#include "estep.hpp"
+
+namespace binder {
+namespace {
+
+__device__ __forceinline__ void spline3(float d, float w[4]) {
+  const float d2 = d * d, d3 = d2 * d;
+  const float o = 1.0f - d, o2 = o * o, o3 = o2 * o;
+  w[0] = o3 * (1.0f / 6.0f);
+  w[1] = (2.0f / 3.0f) - d2 + d3 * 0.5f;
+  w[2] = (2.0f / 3.0f) - o2 + o3 * 0.5f;
+  w[3] = d3 * (1.0f / 6.0f);
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// floor() of a position, kept in a range where int arithmetic is safe; NaN
+// maps to the padding and keeps NaN weights.
+__device__ __forceinline__ int safe_floor(float p, float &frac) {
+  float f = floorf(p);
+  frac = p - f;
+  f = fminf(fmaxf(f, -1.0e7f), 1.0e7f);
+  return (int)f;
+}
+
+__device__ __forceinline__ void hist_add(unsigned long long *hist, int idx,
+                                         float v) {
+  const unsigned long long q =
+      (unsigned long long)__double2ll_rn((double)v * kHistScale);
+  if (q) atomicAdd(hist + idx, q);
+}
+
+template <bool IS2D, bool SAMPLE>
+__global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int KL = a.K * a.L;
+  unsigned long long *hist = reinterpret_cast<unsigned long long *>(smem);
+  float *th = reinterpret_cast<float *>(hist + KL);
+  for (int i = threadIdx.x; i < KL; i += blockDim.x) {
+    hist[i] = 0ull;
+    th[i] = a.thT[i];
+  }
+  __syncthreads();
+
+  constexpr int NB = IS2D ? 16 : 64;  // neighbours
+  double mll = 0.0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.nv;
+       i += stride) {
+    const int u = a.bv[i];
+    const float *thu = th + u * a.K;
+    float tx, ty, tz = 0.f;
+    const int ix = safe_floor(a.d[i], tx);
+    const int iy = safe_floor(a.d[a.nv + i], ty);
+    int iz = 0;
+    if (!IS2D) iz = safe_floor(a.d[2 * a.nv + i], tz);
+    float wx[4], wy[4], wz[4];
+    spline3(tx, wx);
+    spline3(ty, wy);
+    if (!IS2D) spline3(tz, wz);
+
+    float w[NB];
+    int vb[NB];
+    float Z = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+#pragma unroll
+    for (int xa = 0; xa < 4; ++xa) {
+      const int px = clampi(ix - 1 + xa, -1, a.nxn) + 1;
+      float zx = 0.f;
+#pragma unroll
+      for (int yb = 0; yb < 4; ++yb) {
+        const int py = clampi(iy - 1 + yb, -1, a.nyn) + 1;
+        const float wxy = wx[xa] * wy[yb];
+        if (IS2D) {
+          const int j = xa * 4 + yb;
+          const int v = a.bins2d[(long long)px * a.Yp + py];
+          const float wj = wxy * thu[v];
+          w[j] = wj;
+          vb[j] = v;
+          zx += wj;
+          sy += wj * (float)(yb - 1);
+        } else {
+          const int pz = clampi(iz - 1, -4, a.nzn) + 4;
+          const uint32_t q = a.quads[((long long)px * a.Yp + py) * a.Zq + pz];
+          float zy = 0.f;
+#pragma unroll
+          for (int zc = 0; zc < 4; ++zc) {
+            const int j = (xa * 4 + yb) * 4 + zc;
+            const int v = (q >> (8 * zc)) & 0xFF;
+            const float wj = wxy * wz[zc] * thu[v];
+            w[j] = wj;
+            vb[j] = v;
+            zy += wj;
+            sz += wj * (float)(zc - 1);
+          }
+          zx += zy;
+          sy += zy * (float)(yb - 1);
+        }
+      }
+      Z += zx;
+      sx += zx * (float)(xa - 1);
+    }
+
+    if (!SAMPLE) {
+      mll -= log((double)Z);
+      const float invZ = 1.0f / Z;
+      a.dream[i] = (float)ix + sx * invZ;
+      a.dream[a.nv + i] = (float)iy + sy * invZ;
+      if (!IS2D) a.dream[2 * a.nv + i] = (float)iz + sz * invZ;
+      if (Z > 0.f) {
+        int cur = vb[0];
+        float acc = w[0];
+#pragma unroll
+        for (int j = 1; j < NB; ++j) {
+          if (vb[j] == cur) {
+            acc += w[j];
+          } else {
+            hist_add(hist, cur * a.L + u, acc * invZ);
+            cur = vb[j];
+            acc = w[j];
+          }
+        }
+        hist_add(hist, cur * a.L + u, acc * invZ);
+      }
+    } else {
+      // categorical draw: first j with r*Z <= sum_{j'<=j} w_j'
+      const u32x4 rr = philox4x32_10(
+          u32x4{(uint32_t)i, (uint32_t)(i >> 32), a.sweep, kStreamEStep}, a.k0,
+          a.k1);
+      const float target = u01f(rr.x) * Z;
+      int sel = -1, last = -1;
+      float cum = 0.f;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        cum += w[j];
+        if (w[j] > 0.f) last = j;
+        if (sel < 0 && target <= cum) sel = j;
+      }
+      // rounding of the running sum against Z can leave target just above
+      // the final cumulative value; the draw then belongs to the last node
+      // with positive weight
+      if (sel < 0) sel = last;
+      if (Z > 0.f && sel >= 0) {
+        const int xa = IS2D ? sel / 4 : sel / 16;
+        const int yb = IS2D ? sel % 4 : (sel / 4) % 4;
+        const int zc = sel % 4;
+        a.dream[i] = (float)(ix - 1 + xa);
+        a.dream[a.nv + i] = (float)(iy - 1 + yb);
+        if (!IS2D) a.dream[2 * a.nv + i] = (float)(iz - 1 + zc);
+        atomicAdd(hist + vb[sel] * a.L + u, (unsigned long long)kHistScale);
+      } else {
+        // no draw (all weights zero): the reference leaves the voxel's
+        // reset dream location (0) and adds nothing to the histogram
+        a.dream[i] = 0.f;
+        a.dream[a.nv + i] = 0.f;
+        if (!IS2D) a.dream[2 * a.nv + i] = 0.f;
+      }
+    }
+  }
+
+  __syncthreads();
+  unsigned long long *out = a.hpart + (long long)blockIdx.x * KL;
+  for (int i = threadIdx.x; i < KL; i += blockDim.x) out[i] = hist[i];
+  // deterministic block reduction of mll
+  __shared__ double red[256];
+  red[threadIdx.x] = mll;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.mllpart[blockIdx.x] = red[0];
+}
+
+constexpr int kRowsPerReduce = 16;
+
+__global__ __launch_bounds__(256) void k_hist_reduce1(
+    const unsigned long long *hpart, int rows, int KL,
+    unsigned long long *hpart2) {
+  const int kl = blockIdx.x * 256 + threadIdx.x;
+  if (kl >= KL) return;
+  const int r0 = blockIdx.y * kRowsPerReduce;
+  const int r1 = min(rows, r0 + kRowsPerReduce);
+  unsigned long long s = 0;
+  for (int r = r0; r < r1; ++r) s += hpart[(long long)r * KL + kl];
+  hpart2[(long long)blockIdx.y * KL + kl] = s;
+}
+
+__global__ __launch_bounds__(256) void k_hist_reduce2(
+    const unsigned long long *hpart2, int rows2, int KL, double *Hd) {
+  const int kl = blockIdx.x * 256 + threadIdx.x;
+  if (kl >= KL) return;
+  unsigned long long s = 0;
+  for (int r = 0; r < rows2; ++r) s += hpart2[(long long)r * KL + kl];
+  Hd[kl] = (double)s * (1.0 / kHistScale);
+}
+
+// Likelihood-parameter step in one block: sum of the mll partials, then the
+// MAP update (H + alpha - 1)/row sum, or a Dirichlet(H + alpha) draw, or no
+// change; then the Dirichlet prior cost and the transposed FP32 table.
+__global__ __launch_bounds__(1024) void k_theta(ThetaArgs a) {
+  const int KL = a.K * a.L;
+  const int tid = threadIdx.x;
+  __shared__ double red[1024];
+  double s = 0.0;
+  for (int i = tid; i < a.nmll; i += 1024) s += a.mllpart[i];
+  red[tid] = s;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  const double mll = red[0];
+  __syncthreads();
+
+  if (a.mode == kThetaDirichlet) {
+    for (int kl = tid; kl < KL; kl += 1024)
+      a.rowbuf[kl] = gamma_mt(a.Hd[kl] + a.alpha,
+                              u32x4{(uint32_t)kl, a.sweep, 0u, kStreamTheta},
+                              a.k0, a.k1);
+    __syncthreads();
+    __threadfence_block();
+    for (int k = tid; k < a.K; k += 1024) {
+      double sum = 0.0;
+      for (int l = 0; l < a.L; ++l) sum += a.rowbuf[k * a.L + l];
+      for (int l = 0; l < a.L; ++l) a.theta[k * a.L + l] = a.rowbuf[k * a.L + l] / sum;
+    }
+  } else if (a.mode == kThetaMap) {
+    for (int k = tid; k < a.K; k += 1024) {
+      double sum = 0.0;
+      for (int l = 0; l < a.L; ++l) sum += (a.Hd[k * a.L + l] + a.alpha - 1.0);
+      for (int l = 0; l < a.L; ++l)
+        a.theta[k * a.L + l] = (a.Hd[k * a.L + l] + a.alpha - 1.0) / sum;
+    }
+  }
+  __syncthreads();
+  __threadfence_block();
+  // -log Dir(theta_k; alpha) per row, summed in row order
+  for (int k = tid; k < a.K; k += 1024) {
+    double lp = a.lg_sum_alpha;
+    for (int l = 0; l < a.L; ++l) lp -= a.lg_alpha;
+    for (int l = 0; l < a.L; ++l) lp += (a.alpha - 1.0) * log(a.theta[k * a.L + l]);
+    a.rowbuf[KL + k] = -lp;
+  }
+  for (int kl = tid; kl < KL; kl += 1024) {
+    const int k = kl / a.L, l = kl % a.L;
+    a.thT[l * a.K + k] = (float)a.theta[kl];
+  }
+  __syncthreads();
+  __threadfence_block();
+  if (tid == 0) {
+    double c = 0.0;
+    for (int k = 0; k < a.K; ++k) c += a.rowbuf[KL + k];
+    a.stats[0] = mll;
+    a.stats[1] = c;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_welford(const float *d, double *mean,
+                                                 double *m2, long long nv,
+                                                 int N, double inv_n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
+       i += stride) {
+    double d1[3], d2[3];
+    for (int c = 0; c < N; ++c) {
+      const double val = (double)d[c * nv + i];
+      const double mo = mean[c * nv + i];
+      d1[c] = val - mo;
+      const double mn = mo + d1[c] * inv_n;
+      mean[c * nv + i] = mn;
+      d2[c] = val - mn;
+    }
+    int p = 0;
+    for (int r = 0; r < N; ++r)
+      for (int c = r; c < N; ++c, ++p) m2[p * nv + i] += d1[r] * d2[c];
+  }
+}
+
+}  // namespace
+
+void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
+                  hipStream_t s) {
+  const size_t lds = (size_t)a.K * a.L * (sizeof(unsigned long long) + sizeof(float));
+  dim3 g(blocks), b(256);
+  if (is2d) {
+    if (sample) hipLaunchKernelGGL((k_estep<true, true>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((k_estep<true, false>), g, b, lds, s, a);
+  } else {
+    if (sample) hipLaunchKernelGGL((k_estep<false, true>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((k_estep<false, false>), g, b, lds, s, a);
+  }
+  HIP_LAUNCH_OK();
+}
+
+void launch_hist_reduce(const unsigned long long *hpart, int rows, int KL,
+                        unsigned long long *hpart2, double *Hd, hipStream_t s) {
+  const int rows2 = (rows + kRowsPerReduce - 1) / kRowsPerReduce;
+  hipLaunchKernelGGL(k_hist_reduce1, dim3((KL + 255) / 256, rows2), dim3(256),
+                     0, s, hpart, rows, KL, hpart2);
+  HIP_LAUNCH_OK();
+  hipLaunchKernelGGL(k_hist_reduce2, dim3((KL + 255) / 256), dim3(256), 0, s,
+                     hpart2, rows2, KL, Hd);
+  HIP_LAUNCH_OK();
+}
+
+void launch_theta(const ThetaArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_theta, dim3(1), dim3(1024), 0, s, a);
+  HIP_LAUNCH_OK();
+}
+
+void launch_welford(const float *d, double *mean, double *m2, long long nv,
+                    int N, double inv_n, hipStream_t s) {
+  long long blocks = (nv + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_welford, dim3((unsigned)blocks), dim3(256), 0, s, d,
+                     mean, m2, nv, N, inv_n);
+  HIP_LAUNCH_OK();
+}
+
+}  // namespace binder

@@ -1,0 +1,57 @@
+// E-step, joint-histogram and likelihood-parameter kernels (declarations).
+#pragma once
+#include "common.hpp"
+
+namespace binder {
+
+// Fixed-point scale of the LDS/partial histograms: posteriors are added as
+// round(w * 2^32) into 64-bit integers, so block and grid reductions are
+// exact and the histogram is independent of scheduling.
+constexpr double kHistScale = 4294967296.0;
+
+struct EStepArgs {
+  const float *d;           // [N][nv] current deformed positions (SoA)
+  const uint8_t *bv;        // [nv] fixed-image bins
+  const uint32_t *quads;    // 3D: padded moving bins, 4 consecutive z per word
+  const uint8_t *bins2d;    // 2D: padded moving bins
+  const float *thT;         // [L][K] likelihood table, transposed
+  int K, L;
+  long long nv;
+  int nxn, nyn, nzn;        // moving grid
+  int Yp, Zq;               // padded strides
+  float *dream;             // [N][nv] out
+  unsigned long long *hpart;  // [blocks][K*L] out
+  double *mllpart;            // [blocks] out
+  uint32_t k0, k1, sweep;     // sampling keys
+};
+
+void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
+                  hipStream_t s);
+
+// Sum `rows` partial histograms into Hd (doubles, divided by the scale).
+void launch_hist_reduce(const unsigned long long *hpart, int rows, int KL,
+                        unsigned long long *hpart2, double *Hd, hipStream_t s);
+
+enum ThetaMode { kThetaKeep = 0, kThetaMap = 1, kThetaDirichlet = 2 };
+
+struct ThetaArgs {
+  const double *Hd;     // [K*L]
+  double *theta;        // [K*L] in/out
+  float *thT;           // [L*K] out
+  double *rowbuf;       // [K*L] scratch
+  const double *mllpart;
+  int nmll;
+  int K, L;
+  double alpha, lg_sum_alpha, lg_alpha;
+  int mode;
+  double *stats;        // stats[0] = mll, stats[1] = prior_theta
+  uint32_t k0, k1, sweep;
+};
+
+void launch_theta(const ThetaArgs &a, hipStream_t s);
+
+// FP64 Welford update of mean [N][nv] and upper-triangular M2 [N(N+1)/2][nv].
+void launch_welford(const float *d, double *mean, double *m2, long long nv,
+                    int N, double inv_n, hipStream_t s);
+
+}  // namespace binder

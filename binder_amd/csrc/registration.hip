@@ -1,0 +1,629 @@
+// Registration and NonLinearTransformation handles behind the C ABI
+// (include/binder_c.h): device-resident state, the EM loop and the Gibbs
+// sampler, driven from the host with one HIP stream per handle.
+//
+// Device layout (all [component][voxel] SoA so every pass is coalesced):
+//   d, x, dream      float [N][nv]   positions (grid units)
+//   bv               uint8 [nv]      fixed-image bins
+//   quads            uint32 [(Nx_n+2)(Ny_n+2)(Nz_n+5)] packed moving bins (3D)
+//   bins2d           uint8 [(Nx_n+2)(Ny_n+2)]          padded moving bins (2D)
+//   theta (f64), thT (f32, transposed), Hd (f64) K*L tables
+//   mean (f64 [N][nv]), m2 (f64 [N(N+1)/2][nv]) sampler accumulators
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "estep.hpp"
+#include "smoothing.hpp"
+
+namespace binder {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+namespace {
+
+// AoS [nv][N] <-> SoA [N][nv]
+template <class T, class U>
+void aos_to_soa(const T *in, U *out, long long nv, int N) {
+  for (long long i = 0; i < nv; ++i)
+    for (int c = 0; c < N; ++c) out[c * nv + i] = (U)in[i * N + c];
+}
+template <class T, class U>
+void soa_to_aos(const T *in, U *out, long long nv, int N) {
+  for (long long i = 0; i < nv; ++i)
+    for (int c = 0; c < N; ++c) out[i * N + c] = (U)in[c * nv + i];
+}
+
+struct Stream {
+  hipStream_t s = nullptr;
+  Stream() { HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+  ~Stream() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+struct PinnedStats {
+  double *p = nullptr;
+  PinnedStats() { HIP_OK(hipHostMalloc((void **)&p, 16 * sizeof(double), 0)); }
+  ~PinnedStats() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+}  // namespace
+}  // namespace binder
+
+using namespace binder;
+
+struct binder_registration {
+  Stream stream;
+  KernelTimer timer;
+  PinnedStats host_stats;
+  int nxv, nyv, nzv, nxn, nyn, nzn, N;
+  long long nv, nn;
+  double nnz;
+  bool is2d;
+  DBuf<float> d, x, dream;
+  // likelihood
+  bool has_mi = false;
+  int K = 0, L = 0;
+  double alpha = 0.0;
+  DBuf<uint8_t> bv, bins2d;
+  DBuf<uint32_t> quads;
+  DBuf<double> theta, Hd, rowbuf, mllpart, stats;
+  DBuf<float> thT;
+  DBuf<unsigned long long> hpart, hpart2;
+  int estep_blocks = 0;
+  // transformation
+  std::unique_ptr<Smoother> sm;
+  // sampler
+  DBuf<double> mean, m2;
+  std::vector<double> trace;
+
+  KernelTimer *tm() { return timer.enabled() ? &timer : nullptr; }
+  hipStream_t s() const { return stream.s; }
+
+  void require_ready() {
+    if (!has_mi)
+      throw Error(BINDER_ESTATE, "setMILikelihood must be called first");
+    if (!sm)
+      throw Error(BINDER_ESTATE, "setNonLinearTransformation must be called first");
+  }
+
+  EStepArgs estep_args(bool sample, uint32_t k0, uint32_t k1, uint32_t sweep) {
+    EStepArgs a{};
+    a.d = d.get();
+    a.bv = bv.get();
+    a.quads = quads.get();
+    a.bins2d = bins2d.get();
+    a.thT = thT.get();
+    a.K = K;
+    a.L = L;
+    a.nv = nv;
+    a.nxn = nxn;
+    a.nyn = nyn;
+    a.nzn = nzn;
+    a.Yp = nyn + 2;
+    a.Zq = nzn + 5;
+    a.dream = dream.get();
+    a.hpart = hpart.get();
+    a.mllpart = mllpart.get();
+    a.k0 = k0;
+    a.k1 = k1;
+    a.sweep = sweep;
+    (void)sample;
+    return a;
+  }
+
+  // E-step + histogram reduction (+ theta step with the given mode).
+  void e_and_theta(bool sample, int theta_mode, uint32_t k0, uint32_t k1,
+                   uint32_t sweep) {
+    EStepArgs a = estep_args(sample, k0, k1, sweep);
+    {
+      TimedLaunch t(tm(), sample ? "estep_sample" : "estep", s());
+      launch_estep(a, is2d, sample, estep_blocks, s());
+    }
+    {
+      TimedLaunch t(tm(), "hist_reduce", s());
+      launch_hist_reduce(hpart.get(), estep_blocks, K * L, hpart2.get(),
+                         Hd.get(), s());
+    }
+    if (theta_mode >= 0) run_theta(theta_mode, k0, k1, sweep);
+  }
+
+  void run_theta(int mode, uint32_t k0, uint32_t k1, uint32_t sweep) {
+    ThetaArgs t{};
+    t.Hd = Hd.get();
+    t.theta = theta.get();
+    t.thT = thT.get();
+    t.rowbuf = rowbuf.get();
+    t.mllpart = mllpart.get();
+    t.nmll = estep_blocks;
+    t.K = K;
+    t.L = L;
+    t.alpha = alpha;
+    // dirichlet_logpdf constants (MILikelihood.cpp:183-206): sum_alpha is
+    // accumulated L times, as the reference does
+    double sa = 0.0;
+    for (int l = 0; l < L; ++l) sa += alpha;
+    t.lg_sum_alpha = std::lgamma(sa);
+    t.lg_alpha = std::lgamma(alpha);
+    t.mode = mode;
+    t.stats = stats.get();
+    t.k0 = k0;
+    t.k1 = k1;
+    t.sweep = sweep;
+    TimedLaunch tl(tm(), mode == kThetaDirichlet ? "dirichlet" : "theta", s());
+    launch_theta(t, s());
+  }
+};
+
+struct binder_nlt {
+  Stream stream;
+  int nx, ny, nz, N;
+  long long nv;
+  std::unique_ptr<Smoother> sm;
+  DBuf<float> in, out;
+  DBuf<double> scal;
+};
+
+extern "C" {
+
+const char *binder_last_error(void) { return g_last_error.c_str(); }
+const char *binder_backend(void) { return "hip"; }
+
+int binder_philox4x32_10(const unsigned int *ctr4, const unsigned int *key2,
+                         unsigned int *out4) {
+  return guarded([&] {
+    BINDER_REQUIRE(ctr4 && key2 && out4, "null argument");
+    const u32x4 r =
+        philox4x32_10(u32x4{ctr4[0], ctr4[1], ctr4[2], ctr4[3]}, key2[0], key2[1]);
+    out4[0] = r.x;
+    out4[1] = r.y;
+    out4[2] = r.z;
+    out4[3] = r.w;
+  });
+}
+
+int binder_registration_create(binder_registration **out,
+                               const float *final_locations, int threads,
+                               const float *voxel_pos, const float *node_pos,
+                               int Nx_v, int Ny_v, int Nz_v, int Nx_n,
+                               int Ny_n, int Nz_n, int spline_order,
+                               double non_zero_voxels) {
+  (void)threads;
+  (void)node_pos;
+  return guarded([&] {
+    BINDER_REQUIRE(out != nullptr, "out handle pointer is null");
+    *out = nullptr;
+    BINDER_REQUIRE(final_locations && voxel_pos, "null input array");
+    BINDER_REQUIRE(Nx_v >= 2 && Ny_v >= 2 && Nz_v >= 1, "bad fixed-grid shape");
+    BINDER_REQUIRE(Nx_n >= 1 && Ny_n >= 1 && Nz_n >= 1, "bad moving-grid shape");
+    BINDER_REQUIRE(spline_order == 3,
+                   "the HIP backend implements the cubic B-spline only "
+                   "(spline_order must be 3)");
+    BINDER_REQUIRE((Nz_v == 1) == (Nz_n == 1),
+                   "fixed and moving images must both be 2D or both 3D");
+    auto r = std::make_unique<binder_registration>();
+    r->nxv = Nx_v;
+    r->nyv = Ny_v;
+    r->nzv = Nz_v;
+    r->nxn = Nx_n;
+    r->nyn = Ny_n;
+    r->nzn = Nz_n;
+    r->N = Nz_v == 1 ? 2 : 3;
+    r->is2d = Nz_v == 1;
+    r->nv = (long long)Nx_v * Ny_v * Nz_v;
+    r->nn = (long long)Nx_n * Ny_n * Nz_n;
+    r->nnz = non_zero_voxels;
+    std::vector<float> tmp((size_t)r->N * r->nv);
+    aos_to_soa(final_locations, tmp.data(), r->nv, r->N);
+    r->d.upload(tmp.data(), tmp.size(), r->s());
+    HIP_OK(hipStreamSynchronize(r->s()));
+    aos_to_soa(voxel_pos, tmp.data(), r->nv, r->N);
+    r->x.upload(tmp.data(), tmp.size(), r->s());
+    r->dream.alloc(tmp.size());
+    r->dream.zero(r->s());
+    r->stats.alloc(16);
+    r->stats.zero(r->s());
+    long long blocks = (r->nv + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    r->estep_blocks = (int)blocks;
+    r->mllpart.alloc(blocks);
+    HIP_OK(hipStreamSynchronize(r->s()));
+    *out = r.release();
+  });
+}
+
+void binder_registration_destroy(binder_registration *reg) {
+  if (reg) {
+    (void)hipStreamSynchronize(reg->s());
+    delete reg;
+  }
+}
+
+int binder_registration_set_mi_likelihood(binder_registration *r, double alpha,
+                                          const double *theta, int K, int L,
+                                          int threads, const int *binned_nodes,
+                                          const int *binned_voxels) {
+  (void)threads;
+  return guarded([&] {
+    BINDER_REQUIRE(r, "null handle");
+    BINDER_REQUIRE(theta && binned_nodes && binned_voxels, "null input array");
+    BINDER_REQUIRE(K >= 1 && K <= 256 && L >= 1 && L <= 256 && K * L <= 4096,
+                   "K and L must be in [1,256] with K*L <= 4096");
+    std::vector<uint8_t> v8(r->nv);
+    for (long long i = 0; i < r->nv; ++i) {
+      const int b = binned_voxels[i];
+      BINDER_REQUIRE(b >= 0 && b < L, "binned_voxels value outside [0, L)");
+      v8[i] = (uint8_t)b;
+    }
+    for (long long i = 0; i < r->nn; ++i)
+      BINDER_REQUIRE(binned_nodes[i] >= 0 && binned_nodes[i] < K,
+                     "binned_nodes value outside [0, K)");
+    r->K = K;
+    r->L = L;
+    r->alpha = alpha;
+    r->bv.upload(v8.data(), v8.size(), r->s());
+    const int Xp = r->nxn + 2, Yp = r->nyn + 2;
+    if (r->is2d) {
+      std::vector<uint8_t> p((size_t)Xp * Yp, 0);
+      for (int x = 0; x < r->nxn; ++x)
+        for (int y = 0; y < r->nyn; ++y)
+          p[(size_t)(x + 1) * Yp + (y + 1)] = (uint8_t)binned_nodes[(long long)x * r->nyn + y];
+      r->bins2d.upload(p.data(), p.size(), r->s());
+      HIP_OK(hipStreamSynchronize(r->s()));
+    } else {
+      const int Zq = r->nzn + 5;
+      std::vector<uint32_t> q((size_t)Xp * Yp * Zq, 0u);
+      for (int x = 0; x < r->nxn; ++x)
+        for (int y = 0; y < r->nyn; ++y) {
+          const int *row = binned_nodes + ((long long)x * r->nyn + y) * r->nzn;
+          uint32_t *qr = &q[((size_t)(x + 1) * Yp + (y + 1)) * Zq];
+          for (int zq = 0; zq < Zq; ++zq) {
+            uint32_t w = 0;
+            for (int c = 0; c < 4; ++c) {
+              const int z = zq - 4 + c;
+              const uint32_t b = (z >= 0 && z < r->nzn) ? (uint32_t)row[z] : 0u;
+              w |= b << (8 * c);
+            }
+            qr[zq] = w;
+          }
+        }
+      r->quads.upload(q.data(), q.size(), r->s());
+      HIP_OK(hipStreamSynchronize(r->s()));
+    }
+    std::vector<float> tt((size_t)K * L);
+    for (int k = 0; k < K; ++k)
+      for (int l = 0; l < L; ++l) tt[(size_t)l * K + k] = (float)theta[(size_t)k * L + l];
+    r->theta.upload(theta, (size_t)K * L, r->s());
+    r->thT.upload(tt.data(), tt.size(), r->s());
+    r->Hd.alloc((size_t)K * L);
+    r->Hd.zero(r->s());
+    r->rowbuf.alloc((size_t)K * L + K + 64);
+    r->hpart.alloc((size_t)r->estep_blocks * K * L);
+    r->hpart2.alloc((size_t)((r->estep_blocks + 15) / 16) * K * L);
+    HIP_OK(hipStreamSynchronize(r->s()));
+    r->has_mi = true;
+  });
+}
+
+int binder_registration_set_nonlinear_transformation(
+    binder_registration *r, double gamma_x, double gamma_y, double gamma_z,
+    double voxels_size_x, double voxels_size_y, double voxels_size_z,
+    double nodes_size_x, double nodes_size_y, double nodes_size_z,
+    const float *D) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && D, "null argument");
+    const double g[3] = {gamma_x, gamma_y, gamma_z};
+    const double rho[3] = {voxels_size_x, voxels_size_y, voxels_size_z};
+    const double eta[3] = {nodes_size_x, nodes_size_y, nodes_size_z};
+    r->sm = std::make_unique<Smoother>(r->nxv, r->nyv, r->nzv, g, rho, eta, D,
+                                       std::sqrt(9.0 / (8.0 * M_PI)), r->s());
+  });
+}
+
+int binder_registration_em(binder_registration *r, int max_it, double th,
+                           int update, int debug) {
+  return guarded([&] {
+    BINDER_REQUIRE(r, "null handle");
+    r->require_ready();
+    r->trace.clear();
+    double mlp_old = 1e100;
+    int it = 0;
+    bool converged = false;
+    SmoothOptions opt;
+    opt.prior_out = r->stats.get() + 2;
+    while (it < max_it && !converged) {
+      r->e_and_theta(false, update ? kThetaMap : kThetaKeep, 0, 0, 0);
+      r->sm->run(r->dream.get(), r->x.get(), r->d.get(), opt, r->tm());
+      HIP_OK(hipMemcpyAsync(r->host_stats.p, r->stats.get(), 3 * sizeof(double),
+                            hipMemcpyDeviceToHost, r->s()));
+      HIP_OK(hipStreamSynchronize(r->s()));
+      const double mll = r->host_stats.p[0], pt = r->host_stats.p[1],
+                   pd = r->host_stats.p[2];
+      const double mlp = mll + pd + pt;
+      r->trace.insert(r->trace.end(), {mll, pt, pd, mlp});
+      if (debug)
+        std::printf("EM %d: mll %.8f prior_theta %.8f prior_d %.8f mlp %.8f "
+                    "per nonzero voxel %.8f\n",
+                    it + 1, mll, pt, pd, mlp, mlp / r->nnz);
+      if ((mlp_old - mlp) / r->nnz < th) converged = true;
+      mlp_old = mlp;
+      ++it;
+    }
+  });
+}
+
+int binder_registration_e_step(binder_registration *r, double *mll,
+                               double *hist_out) {
+  return guarded([&] {
+    BINDER_REQUIRE(r, "null handle");
+    r->require_ready();
+    r->e_and_theta(false, kThetaKeep, 0, 0, 0);
+    HIP_OK(hipMemcpyAsync(r->host_stats.p, r->stats.get(), sizeof(double),
+                          hipMemcpyDeviceToHost, r->s()));
+    if (hist_out)
+      HIP_OK(hipMemcpyAsync(hist_out, r->Hd.get(), sizeof(double) * r->K * r->L,
+                            hipMemcpyDeviceToHost, r->s()));
+    HIP_OK(hipStreamSynchronize(r->s()));
+    if (mll) *mll = r->host_stats.p[0];
+  });
+}
+
+int binder_registration_sampler(binder_registration *r, int seed, int N_b,
+                                int N_s, int sample_gamma, int every, int sp,
+                                int sl, int st, double *mean_out,
+                                double *cov_out) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && mean_out && cov_out, "null argument");
+    BINDER_REQUIRE(N_b >= 0 && N_s >= 0, "sweep counts must be non-negative");
+    r->require_ready();
+    const int N = r->N;
+    const int npairs = N * (N + 1) / 2;
+    r->mean.alloc((size_t)N * r->nv);
+    r->mean.zero(r->s());
+    r->m2.alloc((size_t)npairs * r->nv);
+    r->m2.zero(r->s());
+    const uint32_t k0 = (uint32_t)seed;
+    const uint32_t k1 = 0x85EBCA6Bu ^ (uint32_t)((long long)seed >> 32);
+    for (int sweep = 0; sweep < N_b + N_s; ++sweep) {
+      if (sp) r->e_and_theta(true, -1, k0, k1, (uint32_t)sweep);
+      if (sl) r->run_theta(kThetaDirichlet, k0, k1, (uint32_t)sweep);
+      if (st) {
+        SmoothOptions o;
+        o.sample = true;
+        o.sample_gamma = sample_gamma && every > 0 && (sweep % every == 0);
+        o.key0 = k0;
+        o.key1 = k1;
+        o.sweep = (uint32_t)sweep;
+        o.stream = kStreamDeform;
+        o.prior_out = r->stats.get() + 3;
+        r->sm->run(r->dream.get(), r->x.get(), r->d.get(), o, r->tm());
+      }
+      if (sweep >= N_b) {
+        const double inv_n = 1.0 / ((double)(sweep - N_b) + 1.0);
+        TimedLaunch t(r->tm(), "welford", r->s());
+        launch_welford(r->d.get(), r->mean.get(), r->m2.get(), r->nv, N, inv_n,
+                       r->s());
+      }
+    }
+    std::vector<double> mh((size_t)N * r->nv), m2h((size_t)npairs * r->nv);
+    r->mean.download(mh.data(), mh.size(), r->s());
+    r->m2.download(m2h.data(), m2h.size(), r->s());
+    HIP_OK(hipStreamSynchronize(r->s()));
+    soa_to_aos(mh.data(), mean_out, r->nv, N);
+    const double inv = 1.0 / (N_s - 1);  // M2 / (N_s - 1) as in the reference
+    for (long long i = 0; i < r->nv; ++i) {
+      int p = 0;
+      for (int a = 0; a < N; ++a)
+        for (int b = a; b < N; ++b, ++p) {
+          const double v = m2h[(size_t)p * r->nv + i] * inv;
+          cov_out[(i * N + a) * N + b] = v;
+          cov_out[(i * N + b) * N + a] = v;
+        }
+    }
+  });
+}
+
+int binder_registration_get_final_locations(binder_registration *r,
+                                            float *out) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && out, "null argument");
+    std::vector<float> h((size_t)r->N * r->nv);
+    r->d.download(h.data(), h.size(), r->s());
+    HIP_OK(hipStreamSynchronize(r->s()));
+    soa_to_aos(h.data(), out, r->nv, r->N);
+  });
+}
+
+int binder_registration_get_dream_locations(binder_registration *r,
+                                            float *out) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && out, "null argument");
+    std::vector<float> h((size_t)r->N * r->nv);
+    r->dream.download(h.data(), h.size(), r->s());
+    HIP_OK(hipStreamSynchronize(r->s()));
+    soa_to_aos(h.data(), out, r->nv, r->N);
+  });
+}
+
+int binder_registration_set_dream_locations(binder_registration *r,
+                                            const float *in) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && in, "null argument");
+    std::vector<float> h((size_t)r->N * r->nv);
+    aos_to_soa(in, h.data(), r->nv, r->N);
+    r->dream.upload(h.data(), h.size(), r->s());
+    HIP_OK(hipStreamSynchronize(r->s()));
+  });
+}
+
+int binder_registration_get_likelihood_parameters(binder_registration *r,
+                                                  double *theta_out) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && theta_out, "null argument");
+    if (!r->has_mi) throw Error(BINDER_ESTATE, "setMILikelihood was not called");
+    r->theta.download(theta_out, (size_t)r->K * r->L, r->s());
+    HIP_OK(hipStreamSynchronize(r->s()));
+  });
+}
+
+int binder_registration_get_shape(binder_registration *r, int *shape4) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && shape4, "null argument");
+    shape4[0] = r->nxv;
+    shape4[1] = r->nyv;
+    shape4[2] = r->nzv;
+    shape4[3] = r->N;
+  });
+}
+
+int binder_registration_get_objective_trace(binder_registration *r,
+                                            double *out, int *rows) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && rows, "null argument");
+    const int avail = (int)(r->trace.size() / 4);
+    const int n = *rows < avail ? *rows : avail;
+    if (out && n > 0) std::memcpy(out, r->trace.data(), sizeof(double) * 4 * n);
+    *rows = out ? n : avail;
+  });
+}
+
+int binder_registration_set_profiling(binder_registration *r, int enable) {
+  return guarded([&] {
+    BINDER_REQUIRE(r, "null handle");
+    HIP_OK(hipStreamSynchronize(r->s()));
+    r->timer.enable(enable != 0);
+  });
+}
+
+int binder_registration_get_kernel_times(binder_registration *r, char *names,
+                                         int names_cap, double *total_ms,
+                                         long long *launches, int *count) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && count, "null argument");
+    HIP_OK(hipStreamSynchronize(r->s()));
+    std::vector<std::string> nm;
+    std::vector<double> ms;
+    std::vector<long long> c;
+    r->timer.summary(nm, ms, c);
+    const int n = (int)nm.size() < *count ? (int)nm.size() : *count;
+    std::string joined;
+    for (int i = 0; i < n; ++i) {
+      if (total_ms) total_ms[i] = ms[i];
+      if (launches) launches[i] = c[i];
+      joined += nm[i];
+      joined += '\n';
+    }
+    if (names && names_cap > 0) {
+      std::strncpy(names, joined.c_str(), (size_t)names_cap - 1);
+      names[names_cap - 1] = '\0';
+    }
+    *count = n;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// NonLinearTransformation
+// ---------------------------------------------------------------------------
+int binder_nlt_create(binder_nlt **out, double gamma_x, double gamma_y,
+                      double gamma_z, double vsx, double vsy, double vsz,
+                      double nsx, double nsy, double nsz, const float *D,
+                      int Nx_v, int Ny_v, int Nz_v, int threads,
+                      double sigma_spline, double spline_offset) {
+  (void)threads;
+  (void)spline_offset;
+  return guarded([&] {
+    BINDER_REQUIRE(out && D, "null argument");
+    *out = nullptr;
+    auto n = std::make_unique<binder_nlt>();
+    n->nx = Nx_v;
+    n->ny = Ny_v;
+    n->nz = Nz_v;
+    n->N = Nz_v == 1 ? 2 : 3;
+    n->nv = (long long)Nx_v * Ny_v * Nz_v;
+    const double g[3] = {gamma_x, gamma_y, gamma_z};
+    const double rho[3] = {vsx, vsy, vsz};
+    const double eta[3] = {nsx, nsy, nsz};
+    n->sm = std::make_unique<Smoother>(Nx_v, Ny_v, Nz_v, g, rho, eta, D,
+                                       sigma_spline, n->stream.s);
+    n->in.alloc((size_t)n->N * n->nv);
+    n->out.alloc((size_t)n->N * n->nv);
+    n->scal.alloc(2);
+    *out = n.release();
+  });
+}
+
+void binder_nlt_destroy(binder_nlt *n) {
+  if (n) {
+    (void)hipStreamSynchronize(n->stream.s);
+    delete n;
+  }
+}
+
+int binder_nlt_smooth_deformation(binder_nlt *n, const float *deformation,
+                                  float *out, double *cost) {
+  return guarded([&] {
+    BINDER_REQUIRE(n && deformation && out, "null argument");
+    std::vector<float> h((size_t)n->N * n->nv);
+    aos_to_soa(deformation, h.data(), n->nv, n->N);
+    n->in.upload(h.data(), h.size(), n->stream.s);
+    SmoothOptions o;
+    o.prior_out = n->scal.get();
+    n->sm->run(n->in.get(), nullptr, n->out.get(), o, nullptr);
+    n->out.download(h.data(), h.size(), n->stream.s);
+    double c = 0.0;
+    HIP_OK(hipMemcpyAsync(&c, n->scal.get(), sizeof(double),
+                          hipMemcpyDeviceToHost, n->stream.s));
+    HIP_OK(hipStreamSynchronize(n->stream.s));
+    soa_to_aos(h.data(), out, n->nv, n->N);
+    if (cost) *cost = c;
+  });
+}
+
+int binder_nlt_sample_deformation_field(binder_nlt *n, const float *deformation,
+                                        int S, unsigned long long seed,
+                                        float *out) {
+  return guarded([&] {
+    BINDER_REQUIRE(n && deformation && out, "null argument");
+    BINDER_REQUIRE(S >= 0, "number_of_samples must be non-negative");
+    std::vector<float> h((size_t)n->N * n->nv);
+    aos_to_soa(deformation, h.data(), n->nv, n->N);
+    n->in.upload(h.data(), h.size(), n->stream.s);
+    for (int s = 0; s < S; ++s) {
+      SmoothOptions o;
+      o.sample = true;
+      o.key0 = (uint32_t)seed;
+      o.key1 = (uint32_t)(seed >> 32) ^ 0x27D4EB2Fu;
+      o.sweep = (uint32_t)s;
+      o.stream = kStreamNltSample;
+      o.prior_out = n->scal.get();
+      n->sm->run(n->in.get(), nullptr, n->out.get(), o, nullptr);
+      n->out.download(h.data(), h.size(), n->stream.s);
+      HIP_OK(hipStreamSynchronize(n->stream.s));
+      soa_to_aos(h.data(), out + (size_t)s * n->N * n->nv, n->nv, n->N);
+    }
+  });
+}
+
+int binder_nlt_up_sample(const double *field, int N_x, int N_y, int N_z,
+                         int N_x_o, int N_y_o, int N_z_o, int threads,
+                         double *out) {
+  (void)field;
+  (void)N_x;
+  (void)N_y;
+  (void)N_z;
+  (void)N_x_o;
+  (void)N_y_o;
+  (void)N_z_o;
+  (void)threads;
+  (void)out;
+  return guarded([&] {
+    throw Error(BINDER_EINVAL, "up_sample is not available in this build");
+  });
+}
+
+}  // extern "C"

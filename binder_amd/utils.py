@@ -1,0 +1,82 @@
+"""Host-side producers of the path's inputs (NumPy), restated from
+BINDER/utils.py: intensity binning and the DCT-domain eigenvalue array D."""
+import numpy as np
+
+
+def bin_signal(signal, number_of_bins):
+    """Uniform binning to rounded bin indices (BINDER/utils.py:12-16)."""
+    signal = np.asarray(signal, dtype=np.float64)
+    lo = np.min(signal)
+    width = (np.max(signal) - lo) / (number_of_bins - 1)
+    return np.round((signal - lo) / width).astype(np.int32)
+
+
+def bin_signal_robust(signal, number_of_bins, lower_percentile=0.01, higher_percentile=0.99,
+                      intensity_bins=None):
+    """Robust binning between the 1st/99th percentiles of a 1000-bin histogram
+    (BINDER/utils.py:20-82).  Returns (binned int32, bin edges)."""
+    signal = np.asarray(signal, dtype=np.float64)
+    if intensity_bins is not None:
+        edges = np.asarray(intensity_bins)
+        b = np.digitize(signal, edges).astype(np.int32) - 1
+        b[signal <= edges[0]] = 0
+        b[signal >= edges[-1]] = number_of_bins - 1
+        return b, edges
+    hist, centers = np.histogram(signal, 1000)
+    cdf = np.cumsum(hist / np.sum(hist))
+
+    def _edge(p):
+        idx = np.where(cdf <= p)[0]
+        add = 0.01
+        while len(idx) == 0:
+            idx = np.where(cdf <= p + add)[0]
+            add += 0.01
+        return centers[idx[-1]]
+
+    lo = _edge(lower_percentile)
+    hi = _edge(higher_percentile)
+    width = (hi - lo) / (number_of_bins - 1)
+    edges = np.linspace(lo, hi, number_of_bins + 1)
+    b = np.floor((signal - lo) / width).astype(np.int32)
+    b[signal < lo] = 0
+    b[signal > hi] = number_of_bins - 1
+    return b, edges
+
+
+def initialize_fourier_domain_smoothing(size, DCT=True, finite_difference_order=2,
+                                        number_of_dimensions=3, return_full_matrix=False,
+                                        voxel_size=np.ones(3)):
+    """Eigenvalues D of the finite-difference prior in the DCT (or real-FFT)
+    domain: D = (sum_a (2 - 2 cos(pi k_a / N_a)) / vs_a^2) ** order
+    (BINDER/utils.py:270-365)."""
+    vs = np.asarray(voxel_size, dtype=np.float64)
+    order = finite_difference_order
+    if number_of_dimensions == 1:
+        n = size[0]
+        if DCT:
+            k = np.arange(n * 2)
+            D = ((2.0 - 2 * np.cos(np.pi * k / n)) / vs[0] ** 2) ** order
+            return (D[:n], D) if return_full_matrix else D[:n]
+        k = np.arange(n)
+        D = ((2.0 - 2 * np.cos(2 * np.pi * k / n)) / vs[0] ** 2) ** order
+        half = int(np.floor(len(D) / 2)) + 1
+        return (D[:half], D) if return_full_matrix else D[:half]
+    dims = list(size[:number_of_dimensions])
+    if DCT:
+        full = [np.arange(2 * n) for n in dims]
+        grids = np.meshgrid(*full, indexing="ij") if return_full_matrix else \
+            np.meshgrid(*[np.arange(n) for n in dims], indexing="ij")
+        D = sum((2.0 - 2 * np.cos(np.pi * g / n)) / vs[a] ** 2
+                for a, (g, n) in enumerate(zip(grids, dims)))
+        D = D ** order
+        if return_full_matrix:
+            return D[tuple(slice(0, n) for n in dims)], D
+        return D
+    grids = np.meshgrid(*[np.arange(n) for n in dims], indexing="ij")
+    D = sum((2.0 - 2 * np.cos(2 * np.pi * g / n)) / vs[a] ** 2
+            for a, (g, n) in enumerate(zip(grids, dims)))
+    D = D ** order
+    half = int(np.floor(D.shape[number_of_dimensions - 1] / 2)) + 1
+    if return_full_matrix:
+        return D[..., :half], D
+    return D[..., :half]

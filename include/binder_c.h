@@ -1,0 +1,238 @@
+/*
+ * binder_c.h -- C ABI of the MI355X (gfx950) backend for BINDER's
+ * registration-and-sampling path.
+ *
+ * Each entry point replaces one method of the reference's pybind11 module
+ * `BINDER._core` (GPU bindings: BINDER/src/gpu/bindings.cpp, CPU bindings:
+ * BINDER/src/cpu/bindings.cpp); the citation next to each declaration names
+ * the binding it stands in for.  A ctypes shim (binder_amd/_core.py) turns
+ * these calls back into the reference's Python classes with unchanged names
+ * and keyword arguments.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a non-zero code on failure:
+ *      BINDER_EINVAL  (1)  bad argument (the shim raises ValueError)
+ *      BINDER_EDEVICE (2)  HIP runtime/launch error (RuntimeError)
+ *      BINDER_ESTATE  (3)  call out of order, e.g. EM before setMILikelihood
+ *    binder_last_error() returns a thread-local message for the last failure.
+ *    No C++ exception crosses this boundary.
+ *  - Arrays are host pointers in C order with the reference layout:
+ *    fields are [Nx, Ny, Nz, N] with interleaved components (N = 2 when
+ *    Nz == 1, else 3); images/bins are [Nx, Ny, Nz].  Inputs are copied to
+ *    device memory at set time; outputs are written to caller buffers.
+ *  - A handle owns its device memory and one HIP stream (default priority).
+ *    Handles are not thread-safe; distinct handles may be used concurrently.
+ *  - `threads` arguments are accepted for signature parity and ignored.
+ */
+#ifndef BINDER_C_H
+#define BINDER_C_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BINDER_OK 0
+#define BINDER_EINVAL 1
+#define BINDER_EDEVICE 2
+#define BINDER_ESTATE 3
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *binder_last_error(void);
+
+/* Backend name, "hip" (reference attribute `backend`,
+ * BINDER/src/gpu/bindings.cpp:16). */
+const char *binder_backend(void);
+
+/* Philox4x32-10 block function (host-side, no device work): the generator
+ * behind every random draw of the sampler, exported so its known-answer
+ * vectors can be checked.  No counterpart in the reference (which uses
+ * std::mt19937 / cuRAND). */
+int binder_philox4x32_10(const unsigned int *ctr4, const unsigned int *key2,
+                         unsigned int *out4);
+
+/* ------------------------------------------------------------------ */
+/* Registration (BINDER/src/gpu/bindings.cpp:17-157)                   */
+/* ------------------------------------------------------------------ */
+typedef struct binder_registration binder_registration;
+
+/* Registration(final_locations, threads, voxel_pos, node_pos, Nx_v, Ny_v,
+ * Nz_v, Nx_n, Ny_n, Nz_n, spline_order, non_zero_voxels)
+ * -- BINDER/src/gpu/bindings.cpp:18-47, BINDER/src/cpu/Registration.cpp:25-70.
+ * final_locations and voxel_pos are [Nx_v,Ny_v,Nz_v,N]; node_pos is unused
+ * after construction in the reference and may be NULL here.
+ * spline_order must be 3 (the cubic B-spline; others -> BINDER_EINVAL). */
+int binder_registration_create(binder_registration **out,
+                               const float *final_locations, int threads,
+                               const float *voxel_pos, const float *node_pos,
+                               int Nx_v, int Ny_v, int Nz_v,
+                               int Nx_n, int Ny_n, int Nz_n,
+                               int spline_order, double non_zero_voxels);
+void binder_registration_destroy(binder_registration *reg);
+
+/* setMILikelihood(alpha, theta, K, L, threads, binned_nodes, binned_voxels)
+ * -- BINDER/src/gpu/bindings.cpp:49-70, BINDER/src/cpu/Registration.cpp:455-464.
+ * theta is K x L row-major (k = moving bin, l = fixed bin).  binned_nodes
+ * are [Nx_n,Ny_n,Nz_n] in [0,K), binned_voxels [Nx_v,Ny_v,Nz_v] in [0,L).
+ * K and L must be in [1,256] with K*L <= 4096 (the likelihood table and
+ * the block histogram are held in LDS). */
+int binder_registration_set_mi_likelihood(binder_registration *reg,
+                                          double alpha, const double *theta,
+                                          int K, int L, int threads,
+                                          const int *binned_nodes,
+                                          const int *binned_voxels);
+
+/* setNonLinearTransformation(gamma_x..z, voxels_size_x..z, nodes_size_x..z, D)
+ * -- BINDER/src/gpu/bindings.cpp:72-95, BINDER/src/cpu/Registration.cpp:492-502.
+ * D is the [Nx_v,Ny_v,Nz_v] DCT-domain eigenvalue array. */
+int binder_registration_set_nonlinear_transformation(
+    binder_registration *reg, double gamma_x, double gamma_y, double gamma_z,
+    double voxels_size_x, double voxels_size_y, double voxels_size_z,
+    double nodes_size_x, double nodes_size_y, double nodes_size_z,
+    const float *D);
+
+/* EM(max_EM_iterations, convergence_th, update_likelihood_parameters, debug)
+ * -- BINDER/src/gpu/bindings.cpp:99-103, BINDER/src/cpu/Registration.cpp:122-217.
+ * Iterates E-step -> theta M-step -> deformation M-step until
+ * (mlp_old - mlp)/non_zero_voxels < convergence_th or the cap is reached.
+ * debug != 0 prints the per-iteration objective like the reference. */
+int binder_registration_em(binder_registration *reg, int max_EM_iterations,
+                           double convergence_th,
+                           int update_likelihood_parameters, int debug);
+
+/* sampler(seed, N_b, N_s, sample_gamma, sample_gamma_every,
+ *         sample_posteriors, sample_likelihood_parameters,
+ *         sample_transformation_parameters) -> (mean, cov)
+ * -- BINDER/src/gpu/bindings.cpp:105-131, BINDER/src/cpu/Registration.cpp:515-642.
+ * mean_out: [Nx,Ny,Nz,N] doubles; cov_out: [Nx,Ny,Nz,N,N] doubles
+ * (= M2/(N_s-1)).  Random numbers come from stateless Philox4x32-10 streams
+ * keyed by (seed, sweep, site); accumulators are FP64. */
+int binder_registration_sampler(binder_registration *reg, int seed, int N_b,
+                                int N_s, int sample_gamma,
+                                int sample_gamma_every, int sample_posteriors,
+                                int sample_likelihood_parameters,
+                                int sample_transformation_parameters,
+                                double *mean_out, double *cov_out);
+
+/* get_final_locations_as_numpy_array() -- BINDER/src/gpu/bindings.cpp:142-157. */
+int binder_registration_get_final_locations(binder_registration *reg,
+                                            float *out);
+/* get_dream_locations() -- BINDER/src/gpu/bindings.cpp:133. */
+int binder_registration_get_dream_locations(binder_registration *reg,
+                                            float *out);
+/* set_dream_locations(dream) -- BINDER/src/gpu/bindings.cpp:135-140. */
+int binder_registration_set_dream_locations(binder_registration *reg,
+                                            const float *dream);
+/* get_likelihood_parameters() -> theta[K*L] -- BINDER/src/gpu/bindings.cpp:97. */
+int binder_registration_get_likelihood_parameters(binder_registration *reg,
+                                                  double *theta_out);
+/* get_shape() -> {Nx, Ny, Nz, N} -- BINDER/src/cpu/bindings.cpp:294. */
+int binder_registration_get_shape(binder_registration *reg, int *shape4);
+
+/* Objective trace of the last EM call: rows of (mll, prior_theta, prior_d,
+ * mlp).  The reference only prints these (Registration.cpp:196-202).
+ * Writes min(*rows, available) rows and sets *rows to the number written. */
+int binder_registration_get_objective_trace(binder_registration *reg,
+                                            double *out, int *rows);
+
+/* One EM E-step with the current theta and deformation (the voxel loop of
+ * Registration.cpp:153-165 over estimate_posterior_and_dream_location,
+ * :219-330).  Writes the dream locations on the device, returns
+ * mll = -sum_i log Z_i and, if hist_out != NULL, the soft joint histogram
+ * H[K*L] it accumulated (MILikelihood.cpp:89-124).  Used by tests/bench. */
+int binder_registration_e_step(binder_registration *reg, double *mll,
+                               double *hist_out);
+
+/* Kernel timing on the handle's stream (HIP events around each launch).
+ * enable != 0 starts recording; get_kernel_times returns, for up to
+ * *count kernel names, the total milliseconds and launch counts since the
+ * last enable.  names receives a '\n'-separated list (capacity names_cap). */
+int binder_registration_set_profiling(binder_registration *reg, int enable);
+int binder_registration_get_kernel_times(binder_registration *reg,
+                                         char *names, int names_cap,
+                                         double *total_ms, long long *launches,
+                                         int *count);
+
+/* ------------------------------------------------------------------ */
+/* NonLinearTransformation (BINDER/src/gpu/bindings.cpp:159-255)       */
+/* ------------------------------------------------------------------ */
+typedef struct binder_nlt binder_nlt;
+
+/* NonLinearTransformation(gamma_x..z, voxels_size_x..z, nodes_size_x..z, D,
+ *   Nx_v, Ny_v, Nz_v, threads, sigma_spline, spline_offset)
+ * -- BINDER/src/gpu/bindings.cpp:161-198,
+ *    BINDER/src/cpu/NonLinearTransformation.cpp:15-56. */
+int binder_nlt_create(binder_nlt **out, double gamma_x, double gamma_y,
+                      double gamma_z, double voxels_size_x,
+                      double voxels_size_y, double voxels_size_z,
+                      double nodes_size_x, double nodes_size_y,
+                      double nodes_size_z, const float *D, int Nx_v, int Ny_v,
+                      int Nz_v, int threads, double sigma_spline,
+                      double spline_offset);
+void binder_nlt_destroy(binder_nlt *nlt);
+/* smooth_deformation(field) -> (cost, field)
+ * -- BINDER/src/gpu/bindings.cpp:201-224,
+ *    BINDER/src/cpu/NonLinearTransformation.cpp:226-278. */
+int binder_nlt_smooth_deformation(binder_nlt *nlt, const float *deformation,
+                                  float *out, double *cost);
+/* sampleDeformationField(field, S) -> [S,Nx,Ny,Nz,N]
+ * -- BINDER/src/gpu/bindings.cpp:226-255,
+ *    BINDER/src/cpu/NonLinearTransformation.cpp:280-341.
+ * seed selects the Philox stream (the reference's GPU build fixes 1234). */
+int binder_nlt_sample_deformation_field(binder_nlt *nlt,
+                                        const float *deformation,
+                                        int number_of_samples,
+                                        unsigned long long seed, float *out);
+/* up_sample(field, N_x, N_y, N_z, N_x_o, N_y_o, N_z_o, threads)
+ * -- BINDER/src/cpu/bindings.cpp:408-433,
+ *    BINDER/src/cpu/NonLinearTransformation.cpp:366-430.
+ * Ortho DCT-II at the input size, zero-padded, ortho DCT-III at the output
+ * size; field [N_x,N_y,N_z,N] -> out [N_x_o,N_y_o,N_z_o,N]. */
+int binder_nlt_up_sample(const double *field, int N_x, int N_y, int N_z,
+                         int N_x_o, int N_y_o, int N_z_o, int threads,
+                         double *out);
+
+/* ------------------------------------------------------------------ */
+/* Jacobian determinant and topology correction                       */
+/* ------------------------------------------------------------------ */
+
+/* compute_jacobian_determinant_3D(field, Nx, Ny, Nz, vx, vy, vz)
+ * -- BINDER/src/cpu/bindings.cpp:564-593, BINDER/src/cpu/utils.cpp:100-345.
+ * FP64, same corner order and expression tree as the reference, no FMA
+ * contraction: bit-exact with the reference CPU result.
+ * Also Nz == 1 with a [Nx,Ny,2] field gives the 2D version
+ * (BINDER/src/cpu/utils.cpp:10-98) when called through
+ * binder_jacobian_determinant_2d. */
+int binder_jacobian_determinant_3d(const double *field, int Nx, int Ny,
+                                   int Nz, double voxel_size_x,
+                                   double voxel_size_y, double voxel_size_z,
+                                   double *out);
+/* compute_jacobian_determinant_2D(field, Nx, Ny, vx, vy)
+ * -- BINDER/src/cpu/bindings.cpp:530-562, BINDER/src/cpu/utils.cpp:10-98. */
+int binder_jacobian_determinant_2d(const double *field, int Nx, int Ny,
+                                   double voxel_size_x, double voxel_size_y,
+                                   double *out);
+/* Fold mask and count: mask[i] = (J[i] < e1), *count = number of set voxels.
+ * Same J as binder_jacobian_determinant_3d (reference fold test:
+ * BINDER/src/cpu/topology_correction_3D.cpp:62-64). mask may be NULL. */
+int binder_fold_mask_3d(const double *field, int Nx, int Ny, int Nz,
+                        double voxel_size_x, double voxel_size_y,
+                        double voxel_size_z, double e1, unsigned char *mask,
+                        long long *count);
+
+/* topology_correction_3D().correct_topology(h_init, e1, e2, max_outer_it,
+ *   max_it, threads, voxel_sizes, Nx_v, Ny_v, Nz_v)
+ * -- BINDER/src/gpu/bindings.cpp:257-287,
+ *    BINDER/src/cpu/topology_correction_3D.cpp:17-143.
+ * h_init/out: [Nx,Ny,Nz,3] grid-unit positions; FP64 throughout.
+ * stats (may be NULL) receives {outer iterations, inner iterations total,
+ * final J_min, final J_max, final count below e1, final count above e2}. */
+int binder_topology_correction_3d(const double *h_init, double e1, double e2,
+                                  int max_outer_it, int max_it, int threads,
+                                  const double *voxel_sizes, int Nx_v,
+                                  int Ny_v, int Nz_v, double *out,
+                                  double *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BINDER_C_H */

@@ -1,0 +1,57 @@
+// ORACLE -- test infrastructure, not product code.
+// C entry points of the FP64 CPU oracle (loaded by tests/ and bench.py via
+// ctypes).  Arrays follow the reference layout: fields [Nx,Ny,Nz,N]
+// interleaved, volumes [Nx,Ny,Nz], theta [K,L] row-major.
+#pragma once
+#ifdef __cplusplus
+extern "C" {
+#endif
+void orc_spline(int order, double d, double *out4);
+void orc_dct3(double *data, int nx, int ny, int nz, int inverse, int threads);
+
+void *orc_reg_create(const double *final_locations, int threads,
+                     const double *voxel_pos, int Nx_v, int Ny_v, int Nz_v,
+                     int Nx_n, int Ny_n, int Nz_n, int spline_order,
+                     double non_zero_voxels);
+void orc_reg_destroy(void *h);
+void orc_reg_set_mi(void *h, double alpha, const double *theta, int K, int L,
+                    const int *binned_nodes, const int *binned_voxels);
+void orc_reg_set_nlt(void *h, double gx, double gy, double gz, double rx,
+                     double ry, double rz, double ex, double ey, double ez,
+                     const double *D);
+int orc_reg_em(void *h, int max_it, double th, int update, int debug);
+double orc_reg_estep(void *h, double *H_out);
+void orc_reg_theta_update(void *h);
+double orc_reg_theta_cost(void *h);
+void orc_reg_sampler(void *h, int seed, int Nb, int Ns, int sg, int every,
+                     int sp, int sl, int st, double *mean, double *cov);
+void orc_reg_get_final(void *h, double *out);
+void orc_reg_get_dream(void *h, double *out);
+void orc_reg_set_dream(void *h, const double *in);
+void orc_reg_get_theta(void *h, double *out);
+int orc_reg_get_trace(void *h, double *out, int max_rows);
+
+void *orc_nlt_create(double gx, double gy, double gz, double rx, double ry,
+                     double rz, double ex, double ey, double ez,
+                     const double *D, int nx, int ny, int nz, int threads,
+                     double sigma, double offset);
+void orc_nlt_destroy(void *h);
+double orc_nlt_smooth(void *h, const double *in, double *out);
+void orc_nlt_sample(void *h, const double *in, int S, long long seed,
+                    double *out);
+void orc_up_sample(const double *field, int nx, int ny, int nz, int nxo,
+                   int nyo, int nzo, int threads, double *out);
+
+void orc_jacobian_3d(const double *field, int Nx, int Ny, int Nz, double vx,
+                     double vy, double vz, double *out);
+void orc_jacobian_2d(const double *field, int Nx, int Ny, double vx, double vy,
+                     double *out);
+void orc_topology_3d(const double *h_init, double e1, double e2, int max_outer,
+                     int max_inner, const double *vs, int Nx, int Ny, int Nz,
+                     double *out, double *stats);
+void orc_corner_jacobians_3d(const double *field, int Nx, int Ny, int Nz,
+                             const double *vs, double e1, double e2,
+                             double *Jc_out, long long *counts);
+#ifdef __cplusplus
+}
+#endif

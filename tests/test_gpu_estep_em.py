@@ -1,0 +1,119 @@
+"""HIP E-step, theta update and EM loop against the FP64 oracle
+(reference semantics: BINDER/src/cpu/Registration.cpp:122-330,
+BINDER/src/cpu/MILikelihood.cpp:42-124)."""
+import numpy as np
+import pytest
+
+from binder_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_problem(shape, K, L, seed, jitter=1.5, oob=False):
+    rng = np.random.default_rng(seed)
+    Nx, Ny, Nz = shape
+    is2d = Nz == 1
+    N = 2 if is2d else 3
+    pos = np.stack(np.indices(shape, dtype=np.float64), axis=-1)[..., :N]
+    d = pos + rng.normal(0, jitter, pos.shape)
+    if oob:  # push a slab far outside the moving grid
+        d[: max(1, Nx // 4)] -= 50.0
+    bn = rng.integers(0, K, shape).astype(np.int32)
+    # smooth-ish fixed bins so histogram runs are non-trivial
+    bv = (rng.integers(0, L, shape) * (rng.random(shape) < 0.7)).astype(np.int32)
+    theta = rng.random((K, L)) + 0.05
+    theta /= theta.sum(1, keepdims=True)
+    D = synth.initialize_fourier_domain_smoothing(list(shape[:N]), True, 2, N)
+    D = D.reshape(shape)
+    g = 1e-3 * np.prod(shape)
+    return synth.Problem(shape=shape, final_locations=d, voxel_pos=pos, binned_nodes=bn,
+                         binned_voxels=bv, theta=theta, D=D, gammas=(g, g * 1.3, g * 0.8),
+                         K=K, L=L, alpha=2.0, nnz=int(np.count_nonzero(bv)))
+
+
+@pytest.mark.parametrize("shape,K,L,oob", [
+    ((12, 10, 9), 8, 6, False),
+    ((17, 13, 11), 32, 32, True),
+    ((31, 23, 1), 16, 8, False),
+    ((40, 33, 1), 32, 32, True),
+])
+def test_estep_matches_oracle(core, orc, shape, K, L, oob):
+    print(f"estep parity shape={shape} K={K} L={L} oob={oob}", flush=True)
+    prob = _random_problem(shape, K, L, seed=sum(shape) + K, oob=oob)
+    g = synth.build_registration(core, prob)
+    o = synth.build_registration(orc, prob)
+    mll_g, H_g = g.e_step(return_histogram=True)
+    mll_o, H_o = o.e_step(return_histogram=True)
+    assert abs(mll_g - mll_o) <= 1e-6 * abs(mll_o)
+    # soft histogram: sum is the voxel count; entries agree to FP32 posterior precision
+    assert abs(H_g.sum() - np.prod(shape)) <= 1e-6 * np.prod(shape)
+    assert np.max(np.abs(H_g - H_o)) <= 1e-5 * max(1.0, H_o.max())
+    dg = np.asarray(g.get_dream_locations()).reshape(prob.final_locations.shape)
+    do = np.asarray(o.get_dream_locations()).reshape(prob.final_locations.shape)
+    assert np.max(np.abs(dg - do)) <= 2e-4
+
+
+def test_estep_all_nodes_outside_use_bin0(core, orc):
+    """Every neighbour outside the moving grid -> theta[0, u]: the posterior is
+    the spline prior, so the dream location is the spline-weighted mean."""
+    print("estep all-out-of-bounds", flush=True)
+    prob = _random_problem((9, 8, 7), 6, 5, seed=3)
+    prob.final_locations = prob.voxel_pos + 100.25
+    g = synth.build_registration(core, prob)
+    mll, H = g.e_step(return_histogram=True)
+    # all mass in moving bin 0, distributed over the fixed bins' counts
+    counts = np.bincount(prob.binned_voxels.ravel(), minlength=prob.L)
+    assert np.allclose(H[0], counts, rtol=1e-6, atol=1e-6)
+    assert np.allclose(H[1:], 0.0)
+    expect = -np.sum(np.log(prob.theta[0, prob.binned_voxels.ravel()]))
+    assert abs(mll - expect) <= 1e-6 * abs(expect)
+    dream = np.asarray(g.get_dream_locations()).reshape(prob.final_locations.shape)
+    # cubic B-spline partition of unity with first moment 0 about floor(d)+d
+    assert np.allclose(dream, prob.final_locations, atol=2e-4)
+
+
+@pytest.mark.parametrize("shape,iters", [((20, 18, 16), 4), ((48, 40, 1), 6)])
+def test_em_matches_oracle(core, orc, shape, iters):
+    print(f"EM parity shape={shape} iters={iters}", flush=True)
+    pair = synth.make_pair(shape[:2] if shape[2] == 1 else shape, 4, 1.5, seed=7,
+                           semi=(3.0, 8.0))
+    prob = synth.problem_from_pair(pair, gamma=1e-4)
+    g = synth.build_registration(core, prob)
+    o = synth.build_registration(orc, prob)
+    g.EM(iters, -1e30, 1, 0)  # threshold below any change: run all iterations
+    o.EM(iters, -1e30, 1, 0)
+    tg, to = g.get_objective_trace(), o.get_objective_trace()
+    assert tg.shape == to.shape == (iters, 4)
+    # objective terms (mll, prior_theta, prior_d, mlp) per iteration
+    assert np.all(np.abs(tg - to) <= 1e-5 * np.abs(to) + 1e-6)
+    fg = g.get_final_locations_as_numpy_array().astype(np.float64)
+    fo = o.get_final_locations_as_numpy_array()
+    assert np.max(np.abs(fg - fo)) <= 1e-3
+    thg = np.asarray(g.get_likelihood_parameters())
+    tho = np.asarray(o.get_likelihood_parameters())
+    assert np.max(np.abs(thg - tho)) <= 1e-5
+
+
+def test_em_convergence_rule(core, orc):
+    """Stops on (mlp_old - mlp)/nnz < th exactly like the reference loop."""
+    print("EM convergence rule", flush=True)
+    pair = synth.make_pair((16, 14, 12), 3, 1.0, seed=11, semi=(3.0, 6.0))
+    prob = synth.problem_from_pair(pair, gamma=1e-4)
+    g = synth.build_registration(core, prob)
+    o = synth.build_registration(orc, prob)
+    g.EM(50, 1e-3, 1, 0)
+    n_o = o.EM(50, 1e-3, 1, 0)
+    tg = g.get_objective_trace()
+    assert len(tg) == n_o
+    mlp = np.concatenate([[1e100], tg[:, 3]])
+    steps = (mlp[:-1] - mlp[1:]) / prob.nnz
+    assert np.all(steps[:-1] >= 1e-3) and (steps[-1] < 1e-3 or len(tg) == 50)
+
+
+def test_theta_keep_when_not_updating(core):
+    print("EM without theta update", flush=True)
+    pair = synth.make_pair((12, 12, 10), 3, 1.0, seed=5, semi=(3.0, 6.0))
+    prob = synth.problem_from_pair(pair, gamma=1e-4)
+    g = synth.build_registration(core, prob)
+    g.EM(2, -1e30, 0, 0)
+    assert np.allclose(g.get_likelihood_parameters(), prob.theta.ravel(), rtol=0, atol=0)
