@@ -1,0 +1,203 @@
+"""Benchmark of the registration-and-sampling path on one MI355X.
+
+Workload: the 3D MI registration of 160x192x160 (synthetic 12-ellipsoid
+phantom, 6-voxel-RMS DCT warp, x^0.7 intensity map, 32 bins): EM
+registration from a cold start to convergence, topology correction of the
+result, then Gibbs sampler sweeps.  A step is one Gibbs sweep over all
+Nv = 4 915 200 voxels (one site update per voxel: categorical E-draw,
+Dirichlet theta draw, DCT-domain deformation draw, FP64 mean/covariance
+update).  value = Nv * steps / time, inputs resident in HBM.
+
+Prints one JSON line.  `--gpus N` with N > 1 is refused (single-GPU build).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("3D reg time-to-tolerance (warm-start vs plain) + Gibbs voxel-updates/s, 1 GPU")
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def kernel_model(name, nv, nn, N, dims):
+    """Algorithmic HBM bytes and flops of one launch of `name` at this size."""
+    nx, ny, nz = dims
+    f32 = 4
+    if name in ("estep", "estep_sample"):
+        # read d (N f32) + fixed bin (u8) + moving bins (u8 per node), write dream
+        return nv * (N * f32 + 1 + N * f32) + nn * 1, 0.0
+    if name.startswith("dct_"):
+        axis = {"x": nx, "y": ny, "z": nz}[name[4]]
+        extra = 0
+        if name in ("dct_z_fwd", "dct_z_inv"):
+            extra = nv * N * f32  # x read by the first / last pass
+        return nv * N * 2 * f32 + extra, 2.0 * axis * nv * N
+    if name == "welford":
+        npair = N * (N + 1) // 2
+        return nv * (N * f32 + 2 * 8 * N + 2 * 8 * npair), 0.0
+    return None, None
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run(args):
+    from binder_amd import _core, synth
+
+    t0 = time.perf_counter()
+    pair, prob = synth.config_problem("c2", seed=args.seed)
+    Nx, Ny, Nz = prob.shape
+    nv = Nx * Ny * Nz
+    N = 3
+    gen_s = time.perf_counter() - t0
+    log(f"inputs {Nx}x{Ny}x{Nz} generated in {gen_s:.1f}s")
+
+    # ---- EM registration, cold start, to convergence ----------------------
+    reg = synth.build_registration(_core, prob)
+    t0 = time.perf_counter()
+    reg.EM(args.em_iters, args.convergence_th, 1, 0)
+    em_s = time.perf_counter() - t0
+    trace = reg.get_objective_trace()
+    d_em = reg.get_final_locations_as_numpy_array().astype(np.float64)
+    theta = np.asarray(reg.get_likelihood_parameters())
+    disp_rms = float(np.sqrt(np.mean(np.sum((d_em - prob.voxel_pos) ** 2, -1))))
+    log(f"EM: {len(trace)} iterations in {em_s:.3f}s")
+
+    # ---- fold detection + topology correction ------------------------------
+    t0 = time.perf_counter()
+    _, folds_before = _core.fold_mask_3D(d_em.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0, e1=0.01)
+    fold_s = time.perf_counter() - t0
+    topo = _core.topology_correction_3D()
+    t0 = time.perf_counter()
+    d_tc = topo.correct_topology(d_em.ravel(), 0.01, 100, 10, 10, 1, np.ones(3), Nx, Ny, Nz)
+    topo_s = time.perf_counter() - t0
+    _, folds_after = _core.fold_mask_3D(d_tc.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0, e1=0.01)
+    log(f"topology: {folds_before} -> {folds_after} folded voxels in {topo_s:.3f}s")
+
+    # ---- Gibbs sweeps -------------------------------------------------------
+    prob.final_locations = d_tc
+    prob.theta = theta.reshape(prob.K, prob.L)
+    s = synth.build_registration(_core, prob)
+    s.sampler_run(args.seed, 0, max(args.warmup, 0) or 0)  # untimed warmup sweeps
+    t0 = time.perf_counter()
+    s.sampler_run(args.seed + 1, 0, args.steps)
+    sweep_s = time.perf_counter() - t0
+    ms_per_step = 1e3 * sweep_s / args.steps
+    value = nv * args.steps / sweep_s
+    log(f"sweeps: {args.steps} in {sweep_s:.3f}s")
+
+    # ---- per-kernel times from HIP events on the handle's stream -----------
+    s.set_profiling(True)
+    prof_sweeps = max(1, min(args.steps, 3))
+    s.sampler_run(args.seed + 2, 0, prof_sweeps)
+    ktimes = s.get_kernel_times()
+    s.set_profiling(False)
+    kernels = {}
+    for name, (ms, cnt) in ktimes.items():
+        avg_us = 1e3 * ms / max(cnt, 1)
+        b, fl = kernel_model(name, nv, nv, N, (Nx, Ny, Nz))
+        ent = {"avg_us": round(avg_us, 2), "launches_per_sweep": cnt / prof_sweeps,
+               "ms_per_sweep": round(ms / prof_sweeps, 4)}
+        if b is not None:
+            ent["GBps"] = round(b / (avg_us * 1e-6) / 1e9, 1)
+            ent["hbm_frac"] = round(ent["GBps"] / HBM_PEAK_GBS, 4)
+            if fl:
+                ent["TFLOPs"] = round(fl / (avg_us * 1e-6) / 1e12, 2)
+        kernels[name] = ent
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_sweep"])
+    b, fl = kernel_model(dom, nv, nv, N, (Nx, Ny, Nz))
+    avg_s = kernels[dom]["avg_us"] * 1e-6
+    if fl:
+        roof = {"kernel": dom, "bound": "mfma", "achieved": round(fl / avg_s / 1e12, 3),
+                "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s"}
+    else:
+        roof = {"kernel": dom, "bound": "hbm", "achieved": round(b / avg_s / 1e9, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
+    roof["traffic"] = None
+
+    # EM iteration kernel times
+    reg.set_profiling(True)
+    reg.EM(2, -1e30, 1, 0)
+    em_k = {k: round(1e3 * v[0] / max(v[1], 1), 2) for k, v in reg.get_kernel_times().items()}
+    reg.set_profiling(False)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "voxel-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "Gibbs sweep, 3D MI registration 160x192x160, 32 bins, "
+                               "after cold-start EM + topology correction",
+                   "shape": [Nx, Ny, Nz], "bins": prob.K, "gamma_per_voxel": 1e-6,
+                   "alpha": prob.alpha, "seed": args.seed},
+        "roofline": roof,
+        "em": {"cold_start_s": round(em_s, 4), "iterations": int(len(trace)),
+               "ms_per_iteration": round(1e3 * em_s / max(len(trace), 1), 3),
+               "convergence_th": args.convergence_th, "final_mlp": float(trace[-1, 3]),
+               "displacement_rms_vox": round(disp_rms, 4), "kernel_avg_us": em_k},
+        "topology": {"seconds": round(topo_s, 4), "folds_before": folds_before,
+                     "folds_after": folds_after, "fold_detect_s": round(fold_s, 5),
+                     "outer_inner": [int(topo.last_stats[0]), int(topo.last_stats[1])]},
+        "kernels": kernels,
+        "input_gen_s": round(gen_s, 2),
+    }
+    if not args.no_cpu:
+        log("CPU baseline (oracle, all host cores)")
+        out["cpu_baseline"] = cpu_baseline(prob, nv)
+    return out
+
+
+def cpu_baseline(prob, nv):
+    """FP64 CPU oracle (restatement of the reference CPU backend) on the same
+    workload: one Gibbs sweep over the full volume with all host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pycore
+    from binder_amd import synth
+
+    th = os.cpu_count() or 1
+    r = synth.build_registration(pycore, prob, threads=th)
+    t0 = time.perf_counter()
+    r.sampler(1, 0, 1, 0, 1, 1, 1, 1)
+    t = time.perf_counter() - t0
+    return {"value": round(nv / t, 1), "unit": "voxel-updates/s", "cores": th,
+            "kind": "port", "sample": "1 Gibbs sweep of the FP64 oracle over the full "
+                                      "160x192x160 volume (E-draw, Dirichlet, 3x DCT draw, "
+                                      f"Welford), {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--em-iters", type=int, default=200)
+    ap.add_argument("--convergence-th", type=float, default=2.5e-5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the registration path is single-GPU in this build)", file=sys.stderr)
+        sys.exit(2)
+    print(json.dumps(run(args)), flush=True)
+
+
+if __name__ == "__main__":
+    main()

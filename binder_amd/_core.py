@@ -50,6 +50,8 @@ _reg_set_nlt = _sig("binder_registration_set_nonlinear_transformation", _c_int, 
                     *([_c_double] * 9), _vp)
 _reg_em = _sig("binder_registration_em", _c_int, _vp, _c_int, _c_double, _c_int, _c_int)
 _reg_sampler = _sig("binder_registration_sampler", _c_int, _vp, *([_c_int] * 8), _vp, _vp)
+_reg_sampler_run = _sig("binder_registration_sampler_run", _c_int, _vp, *([_c_int] * 8))
+_reg_sampler_result = _sig("binder_registration_sampler_result", _c_int, _vp, _vp, _vp)
 _reg_get_final = _sig("binder_registration_get_final_locations", _c_int, _vp, _vp)
 _reg_get_dream = _sig("binder_registration_get_dream_locations", _c_int, _vp, _vp)
 _reg_set_dream = _sig("binder_registration_set_dream_locations", _c_int, _vp, _vp)
@@ -173,6 +175,22 @@ class Registration:
                             int(sample_gamma_every), int(sample_posteriors),
                             int(sample_likelihood_parameters),
                             int(sample_transformation_parameters), _ptr(mean), _ptr(cov)))
+        return mean, cov
+
+    def sampler_run(self, seed, N_b, N_s, sample_gamma=0, sample_gamma_every=1,
+                    sample_posteriors=1, sample_likelihood_parameters=1,
+                    sample_transformation_parameters=1):
+        """Run the sweeps, keeping mean/M2 on the device (see sampler_result)."""
+        _check(_reg_sampler_run(self._h, int(seed), int(N_b), int(N_s), int(sample_gamma),
+                                int(sample_gamma_every), int(sample_posteriors),
+                                int(sample_likelihood_parameters),
+                                int(sample_transformation_parameters)))
+
+    def sampler_result(self):
+        Nx, Ny, Nz, N = self._shape
+        mean = np.empty((Nx, Ny, Nz, N), np.float64)
+        cov = np.empty((Nx, Ny, Nz, N, N), np.float64)
+        _check(_reg_sampler_result(self._h, _ptr(mean), _ptr(cov)))
         return mean, cov
 
     def get_final_locations_as_numpy_array(self):
@@ -357,7 +375,8 @@ def exported_symbols():
         "binder_registration_create",
         "binder_registration_destroy", "binder_registration_set_mi_likelihood",
         "binder_registration_set_nonlinear_transformation", "binder_registration_em",
-        "binder_registration_sampler", "binder_registration_get_final_locations",
+        "binder_registration_sampler", "binder_registration_sampler_run",
+        "binder_registration_sampler_result", "binder_registration_get_final_locations",
         "binder_registration_get_dream_locations", "binder_registration_set_dream_locations",
         "binder_registration_get_likelihood_parameters", "binder_registration_get_shape",
         "binder_registration_get_objective_trace", "binder_registration_e_step",

@@ -50,138 +50,286 @@ __device__ __forceinline__ int safe_floor(float p, float &frac) {
   return (int)f;
 }
 
-__device__ __forceinline__ void hist_add(unsigned long long *hist, int idx,
-                                         float v) {
-  const unsigned long long q =
-      (unsigned long long)__double2ll_rn((double)v * kHistScale);
-  if (q) atomicAdd(hist + idx, q);
+// Adds `val` at hist[idx] for every lane with idx >= 0.  Lanes that share an
+// index are first summed across the wavefront (one LDS atomic per distinct
+// index for up to two distinct indices); the rest add directly.  Must be
+// called by all 64 lanes of the wavefront.
+__device__ __forceinline__ void wave_hist_add(unsigned long long *hist, int idx,
+                                              unsigned long long val) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int round = 0; round < 2; ++round) {
+    const unsigned long long pend = __ballot(idx >= 0);
+    if (pend == 0ull) return;
+    const int first = __ffsll((long long)pend) - 1;
+    const int leader = __shfl(idx, first);
+    const bool mine = (idx == leader);
+    unsigned long long part = mine ? val : 0ull;
+    uint32_t lo = (uint32_t)part, hi = (uint32_t)(part >> 32);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint32_t olo = __shfl_xor(lo, off), ohi = __shfl_xor(hi, off);
+      const uint32_t nlo = lo + olo;
+      hi = hi + ohi + (nlo < lo ? 1u : 0u);
+      lo = nlo;
+    }
+    if (lane == first) atomicAdd(hist + leader, ((unsigned long long)hi << 32) | lo);
+    if (mine) idx = -1;
+  }
+  if (idx >= 0) atomicAdd(hist + idx, val);
 }
 
+__device__ __forceinline__ unsigned long long to_fixed(float v) {
+  return (unsigned long long)__float2ll_rn(v * 4294967296.0f);
+}
+
+__device__ __forceinline__ float sel4(const float w[4], int k) {
+  return k == 0 ? w[0] : (k == 1 ? w[1] : (k == 2 ? w[2] : w[3]));
+}
+
+// Packed moving bins of the 4 (x = ix-1+xa, y = iy-1..iy+2) columns: 4 z-bins
+// per word (3D) or the single bin (2D).
+template <bool IS2D>
+__device__ __forceinline__ void load_columns(const EStepArgs &a, int ix, int iy,
+                                             int iz, uint32_t q[16]) {
+#pragma unroll
+  for (int xa = 0; xa < 4; ++xa) {
+    const int px = clampi(ix - 1 + xa, -1, a.nxn) + 1;
+#pragma unroll
+    for (int yb = 0; yb < 4; ++yb) {
+      const int py = clampi(iy - 1 + yb, -1, a.nyn) + 1;
+      if (IS2D) {
+        q[xa * 4 + yb] = a.bins2d[(long long)px * a.Yp + py];
+      } else {
+        const int pz = clampi(iz - 1, -4, a.nzn) + 4;
+        q[xa * 4 + yb] = a.quads[((long long)px * a.Yp + py) * a.Zq + pz];
+      }
+    }
+  }
+}
+
+// q[xa*4 + yb] for a runtime slice index xa, without dynamic register indexing
+__device__ __forceinline__ void slice_columns(const uint32_t q[16], int xa,
+                                              uint32_t s[4]) {
+#pragma unroll
+  for (int yb = 0; yb < 4; ++yb)
+    s[yb] = xa == 0 ? q[yb] : (xa == 1 ? q[4 + yb] : (xa == 2 ? q[8 + yb] : q[12 + yb]));
+}
+
+// The 64 (3D) or 16 (2D) neighbour weights are evaluated twice -- once for
+// the normaliser and the mean, once for the histogram / the draw -- one
+// x-slice (16 or 4 nodes) at a time, so only a slice's packed bins are live.
 template <bool IS2D, bool SAMPLE>
 __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int KL = a.K * a.L;
   unsigned long long *hist = reinterpret_cast<unsigned long long *>(smem);
+  // likelihood columns theta[:, u] with a row stride of K+1 so that lanes with
+  // different fixed bins reading the same moving bin hit different banks
   float *th = reinterpret_cast<float *>(hist + KL);
+  const int Kp = a.K + 1;
   for (int i = threadIdx.x; i < KL; i += blockDim.x) {
     hist[i] = 0ull;
-    th[i] = a.thT[i];
+    th[(i / a.K) * Kp + (i % a.K)] = a.thT[i];
   }
   __syncthreads();
 
-  constexpr int NB = IS2D ? 16 : 64;  // neighbours
+  constexpr int NZ = IS2D ? 1 : 4;
   double mll = 0.0;
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.nv;
-       i += stride) {
-    const int u = a.bv[i];
-    const float *thu = th + u * a.K;
+  // uniform trip count across the block: every lane reaches wave_hist_add
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < a.nv; base += stride) {
+    const long long i = base + threadIdx.x;
+    const bool valid = i < a.nv;
+    const long long ii = valid ? i : a.nv - 1;
+    const int u = a.bv[ii];
+    const float *thu = th + u * Kp;
     float tx, ty, tz = 0.f;
-    const int ix = safe_floor(a.d[i], tx);
-    const int iy = safe_floor(a.d[a.nv + i], ty);
+    const int ix = safe_floor(a.d[ii], tx);
+    const int iy = safe_floor(a.d[a.nv + ii], ty);
     int iz = 0;
-    if (!IS2D) iz = safe_floor(a.d[2 * a.nv + i], tz);
-    float wx[4], wy[4], wz[4];
+    if (!IS2D) iz = safe_floor(a.d[2 * a.nv + ii], tz);
+    float wx[4], wy[4], wz[4] = {1.f, 0.f, 0.f, 0.f};
     spline3(tx, wx);
     spline3(ty, wy);
     if (!IS2D) spline3(tz, wz);
 
-    float w[NB];
-    int vb[NB];
+    uint32_t qall[16];
+    load_columns<IS2D>(a, ix, iy, iz, qall);
+
+    // pass 1: normaliser and first moments; the sampler also keeps the 16
+    // (x,y) column sums for its two-level draw, the EM path walks the
+    // x-slices in a rolled loop so its second pass cannot reuse (and keep
+    // live) all 64 first-pass weights
     float Z = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+    float col[16];
+    if constexpr (SAMPLE) {
 #pragma unroll
-    for (int xa = 0; xa < 4; ++xa) {
-      const int px = clampi(ix - 1 + xa, -1, a.nxn) + 1;
-      float zx = 0.f;
+      for (int xa = 0; xa < 4; ++xa) {
+        float zx = 0.f;
 #pragma unroll
-      for (int yb = 0; yb < 4; ++yb) {
-        const int py = clampi(iy - 1 + yb, -1, a.nyn) + 1;
-        const float wxy = wx[xa] * wy[yb];
-        if (IS2D) {
-          const int j = xa * 4 + yb;
-          const int v = a.bins2d[(long long)px * a.Yp + py];
-          const float wj = wxy * thu[v];
-          w[j] = wj;
-          vb[j] = v;
-          zx += wj;
-          sy += wj * (float)(yb - 1);
-        } else {
-          const int pz = clampi(iz - 1, -4, a.nzn) + 4;
-          const uint32_t q = a.quads[((long long)px * a.Yp + py) * a.Zq + pz];
+        for (int yb = 0; yb < 4; ++yb) {
+          const float wxy = wx[xa] * wy[yb];
+          const uint32_t qq = qall[xa * 4 + yb];
           float zy = 0.f;
 #pragma unroll
-          for (int zc = 0; zc < 4; ++zc) {
-            const int j = (xa * 4 + yb) * 4 + zc;
-            const int v = (q >> (8 * zc)) & 0xFF;
-            const float wj = wxy * wz[zc] * thu[v];
-            w[j] = wj;
-            vb[j] = v;
+          for (int zc = 0; zc < NZ; ++zc) zy += wxy * wz[zc] * thu[(qq >> (8 * zc)) & 0xFF];
+          col[xa * 4 + yb] = zy;
+          zx += zy;
+        }
+        Z += zx;
+      }
+    } else {
+#pragma unroll 1
+      for (int xa = 0; xa < 4; ++xa) {
+        uint32_t q[4];
+        slice_columns(qall, xa, q);
+        const float wxa = sel4(wx, xa);
+        float zx = 0.f;
+#pragma unroll
+        for (int yb = 0; yb < 4; ++yb) {
+          const float wxy = wxa * wy[yb];
+          float zy = 0.f;
+#pragma unroll
+          for (int zc = 0; zc < NZ; ++zc) {
+            const float wj = wxy * wz[zc] * thu[(q[yb] >> (8 * zc)) & 0xFF];
             zy += wj;
-            sz += wj * (float)(zc - 1);
+            if (!IS2D) sz += wj * (float)(zc - 1);
           }
           zx += zy;
           sy += zy * (float)(yb - 1);
         }
+        Z += zx;
+        sx += zx * (float)(xa - 1);
       }
-      Z += zx;
-      sx += zx * (float)(xa - 1);
     }
 
+    int hidx = -1;
+    unsigned long long hval = 0ull;
     if (!SAMPLE) {
-      mll -= log((double)Z);
-      const float invZ = 1.0f / Z;
-      a.dream[i] = (float)ix + sx * invZ;
-      a.dream[a.nv + i] = (float)iy + sy * invZ;
-      if (!IS2D) a.dream[2 * a.nv + i] = (float)iz + sz * invZ;
-      if (Z > 0.f) {
-        int cur = vb[0];
-        float acc = w[0];
+      if (valid) {
+        mll -= log((double)Z);
+        const float invZ = 1.0f / Z;
+        a.dream[i] = (float)ix + sx * invZ;
+        a.dream[a.nv + i] = (float)iy + sy * invZ;
+        if (!IS2D) a.dream[2 * a.nv + i] = (float)iz + sz * invZ;
+      }
+      if (valid && Z > 0.f) {
+        // pass 2: posteriors grouped in runs of equal moving bin
+        const float invZ = 1.0f / Z;
+        int cur = -1;
+        float acc = 0.f;
+#pragma unroll 1
+        for (int xa = 0; xa < 4; ++xa) {
+          uint32_t q[4];
+          slice_columns(qall, xa, q);
+          const float wxa = sel4(wx, xa);
 #pragma unroll
-        for (int j = 1; j < NB; ++j) {
-          if (vb[j] == cur) {
-            acc += w[j];
-          } else {
-            hist_add(hist, cur * a.L + u, acc * invZ);
-            cur = vb[j];
-            acc = w[j];
+          for (int yb = 0; yb < 4; ++yb) {
+            const float wxy = wxa * wy[yb];
+#pragma unroll
+            for (int zc = 0; zc < NZ; ++zc) {
+              const int v = (q[yb] >> (8 * zc)) & 0xFF;
+              const float wj = wxy * wz[zc] * thu[v];
+              if (v == cur) {
+                acc += wj;
+              } else {
+                if (cur >= 0) {
+                  const unsigned long long f = to_fixed(acc * invZ);
+                  if (f) atomicAdd(hist + cur * a.L + u, f);
+                }
+                cur = v;
+                acc = wj;
+              }
+            }
           }
         }
-        hist_add(hist, cur * a.L + u, acc * invZ);
+        hidx = cur * a.L + u;
+        hval = to_fixed(acc * invZ);
       }
     } else {
-      // categorical draw: first j with r*Z <= sum_{j'<=j} w_j'
+      // categorical draw in the reference's node order (x, then y, then z):
+      // the first node whose running sum reaches r*Z.  The (x,y) column is
+      // found from the column sums, then the node within the column.
       const u32x4 rr = philox4x32_10(
-          u32x4{(uint32_t)i, (uint32_t)(i >> 32), a.sweep, kStreamEStep}, a.k0,
+          u32x4{(uint32_t)ii, (uint32_t)(ii >> 32), a.sweep, kStreamEStep}, a.k0,
           a.k1);
       const float target = u01f(rr.x) * Z;
-      int sel = -1, last = -1;
-      float cum = 0.f;
+      int scol = -1, lastcol = -1;
+      float before = 0.f, cum = 0.f, before_last = 0.f;
 #pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        cum += w[j];
-        if (w[j] > 0.f) last = j;
-        if (sel < 0 && target <= cum) sel = j;
+      for (int c = 0; c < 16; ++c) {
+        const float nc = cum + col[c];
+        if (col[c] > 0.f) {
+          lastcol = c;
+          before_last = cum;
+        }
+        if (scol < 0 && target <= nc && col[c] > 0.f) {
+          scol = c;
+          before = cum;
+        }
+        cum = nc;
       }
       // rounding of the running sum against Z can leave target just above
-      // the final cumulative value; the draw then belongs to the last node
+      // the final cumulative value: the draw then falls in the last column
       // with positive weight
-      if (sel < 0) sel = last;
-      if (Z > 0.f && sel >= 0) {
-        const int xa = IS2D ? sel / 4 : sel / 16;
-        const int yb = IS2D ? sel % 4 : (sel / 4) % 4;
-        const int zc = sel % 4;
-        a.dream[i] = (float)(ix - 1 + xa);
-        a.dream[a.nv + i] = (float)(iy - 1 + yb);
-        if (!IS2D) a.dream[2 * a.nv + i] = (float)(iz - 1 + zc);
-        atomicAdd(hist + vb[sel] * a.L + u, (unsigned long long)kHistScale);
-      } else {
-        // no draw (all weights zero): the reference leaves the voxel's
-        // reset dream location (0) and adds nothing to the histogram
-        a.dream[i] = 0.f;
-        a.dream[a.nv + i] = 0.f;
-        if (!IS2D) a.dream[2 * a.nv + i] = 0.f;
+      if (scol < 0) {
+        scol = lastcol;
+        before = before_last;
+      }
+      int sel = -1, selv = 0;
+      if (scol >= 0) {
+        const int xa = scol >> 2, yb = scol & 3;
+        uint32_t qsel = 0;
+        float wxy = 0.f;
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+          if (c == scol) {
+            qsel = qall[c];
+            wxy = wx[c >> 2] * wy[c & 3];
+          }
+        float cz = before;
+        int lastz = -1, lastv = 0;
+#pragma unroll
+        for (int zc = 0; zc < NZ; ++zc) {
+          const int v = (qsel >> (8 * zc)) & 0xFF;
+          const float wj = wxy * wz[zc] * thu[v];
+          cz += wj;
+          if (wj > 0.f) {
+            lastz = zc;
+            lastv = v;
+          }
+          if (sel < 0 && wj > 0.f && target <= cz) {
+            sel = (xa * 4 + yb) * NZ + zc;
+            selv = v;
+          }
+        }
+        if (sel < 0 && lastz >= 0) {
+          sel = (xa * 4 + yb) * NZ + lastz;
+          selv = lastv;
+        }
+      }
+      if (valid) {
+        if (Z > 0.f && sel >= 0) {
+          const int xa = sel / (4 * NZ);
+          const int yb = (sel / NZ) % 4;
+          const int zc = sel % NZ;
+          a.dream[i] = (float)(ix - 1 + xa);
+          a.dream[a.nv + i] = (float)(iy - 1 + yb);
+          if (!IS2D) a.dream[2 * a.nv + i] = (float)(iz - 1 + zc);
+          hidx = selv * a.L + u;
+          hval = (unsigned long long)kHistScale;
+        } else {
+          // no draw (all weights zero): the reference leaves the voxel's
+          // reset dream location (0) and adds nothing to the histogram
+          a.dream[i] = 0.f;
+          a.dream[a.nv + i] = 0.f;
+          if (!IS2D) a.dream[2 * a.nv + i] = 0.f;
+        }
       }
     }
+    if (hval == 0ull) hidx = -1;
+    wave_hist_add(hist, hidx, hval);
   }
 
   __syncthreads();
@@ -307,7 +455,8 @@ __global__ __launch_bounds__(256) void k_welford(const float *d, double *mean,
 
 void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
                   hipStream_t s) {
-  const size_t lds = (size_t)a.K * a.L * (sizeof(unsigned long long) + sizeof(float));
+  const size_t lds = (size_t)a.K * a.L * sizeof(unsigned long long) +
+                     (size_t)(a.K + 1) * a.L * sizeof(float);
   dim3 g(blocks), b(256);
   if (is2d) {
     if (sample) hipLaunchKernelGGL((k_estep<true, true>), g, b, lds, s, a);

@@ -80,6 +80,7 @@ struct binder_registration {
   std::unique_ptr<Smoother> sm;
   // sampler
   DBuf<double> mean, m2;
+  int samples = -1;
   std::vector<double> trace;
 
   KernelTimer *tm() { return timer.enabled() ? &timer : nullptr; }
@@ -228,8 +229,10 @@ int binder_registration_create(binder_registration **out,
     r->dream.zero(r->s());
     r->stats.alloc(16);
     r->stats.zero(r->s());
+    // 8 blocks of 4 wavefronts per CU on 256 CUs; each block keeps one
+    // partial histogram row
     long long blocks = (r->nv + 255) / 256;
-    if (blocks > 1024) blocks = 1024;
+    if (blocks > 2048) blocks = 2048;
     r->estep_blocks = (int)blocks;
     r->mllpart.alloc(blocks);
     HIP_OK(hipStreamSynchronize(r->s()));
@@ -373,12 +376,11 @@ int binder_registration_e_step(binder_registration *r, double *mll,
   });
 }
 
-int binder_registration_sampler(binder_registration *r, int seed, int N_b,
-                                int N_s, int sample_gamma, int every, int sp,
-                                int sl, int st, double *mean_out,
-                                double *cov_out) {
+int binder_registration_sampler_run(binder_registration *r, int seed, int N_b,
+                                    int N_s, int sample_gamma, int every,
+                                    int sp, int sl, int st) {
   return guarded([&] {
-    BINDER_REQUIRE(r && mean_out && cov_out, "null argument");
+    BINDER_REQUIRE(r, "null handle");
     BINDER_REQUIRE(N_b >= 0 && N_s >= 0, "sweep counts must be non-negative");
     r->require_ready();
     const int N = r->N;
@@ -387,6 +389,7 @@ int binder_registration_sampler(binder_registration *r, int seed, int N_b,
     r->mean.zero(r->s());
     r->m2.alloc((size_t)npairs * r->nv);
     r->m2.zero(r->s());
+    r->samples = N_s;
     const uint32_t k0 = (uint32_t)seed;
     const uint32_t k1 = 0x85EBCA6Bu ^ (uint32_t)((long long)seed >> 32);
     for (int sweep = 0; sweep < N_b + N_s; ++sweep) {
@@ -410,12 +413,24 @@ int binder_registration_sampler(binder_registration *r, int seed, int N_b,
                        r->s());
       }
     }
+    HIP_OK(hipStreamSynchronize(r->s()));
+  });
+}
+
+int binder_registration_sampler_result(binder_registration *r,
+                                       double *mean_out, double *cov_out) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && mean_out && cov_out, "null argument");
+    if (r->samples < 0 || !r->mean.get())
+      throw Error(BINDER_ESTATE, "sampler_run has not been called");
+    const int N = r->N;
+    const int npairs = N * (N + 1) / 2;
     std::vector<double> mh((size_t)N * r->nv), m2h((size_t)npairs * r->nv);
     r->mean.download(mh.data(), mh.size(), r->s());
     r->m2.download(m2h.data(), m2h.size(), r->s());
     HIP_OK(hipStreamSynchronize(r->s()));
     soa_to_aos(mh.data(), mean_out, r->nv, N);
-    const double inv = 1.0 / (N_s - 1);  // M2 / (N_s - 1) as in the reference
+    const double inv = 1.0 / (r->samples - 1);  // M2 / (N_s - 1) as the reference
     for (long long i = 0; i < r->nv; ++i) {
       int p = 0;
       for (int a = 0; a < N; ++a)
@@ -426,6 +441,20 @@ int binder_registration_sampler(binder_registration *r, int seed, int N_b,
         }
     }
   });
+}
+
+int binder_registration_sampler(binder_registration *r, int seed, int N_b,
+                                int N_s, int sample_gamma, int every, int sp,
+                                int sl, int st, double *mean_out,
+                                double *cov_out) {
+  if (!mean_out || !cov_out) {
+    set_last_error("null argument");
+    return BINDER_EINVAL;
+  }
+  const int rc = binder_registration_sampler_run(r, seed, N_b, N_s, sample_gamma,
+                                                 every, sp, sl, st);
+  if (rc) return rc;
+  return binder_registration_sampler_result(r, mean_out, cov_out);
 }
 
 int binder_registration_get_final_locations(binder_registration *r,
@@ -612,17 +641,30 @@ int binder_nlt_sample_deformation_field(binder_nlt *n, const float *deformation,
 int binder_nlt_up_sample(const double *field, int N_x, int N_y, int N_z,
                          int N_x_o, int N_y_o, int N_z_o, int threads,
                          double *out) {
-  (void)field;
-  (void)N_x;
-  (void)N_y;
-  (void)N_z;
-  (void)N_x_o;
-  (void)N_y_o;
-  (void)N_z_o;
   (void)threads;
-  (void)out;
   return guarded([&] {
-    throw Error(BINDER_EINVAL, "up_sample is not available in this build");
+    BINDER_REQUIRE(field && out, "null argument");
+    BINDER_REQUIRE(N_x >= 2 && N_y >= 2 && N_z >= 1 && N_x_o >= 2 && N_y_o >= 2 &&
+                       N_z_o >= 1 && (N_z == 1) == (N_z_o == 1),
+                   "bad input/output grid shape");
+    BINDER_REQUIRE(N_x <= 256 && N_y <= 256 && N_z <= 256 && N_x_o <= 256 &&
+                       N_y_o <= 256 && N_z_o <= 256,
+                   "each axis must be at most 256 voxels");
+    const int N = N_z_o == 1 ? 2 : 3;
+    const long long nvi = (long long)N_x * N_y * N_z;
+    const long long nvo = (long long)N_x_o * N_y_o * N_z_o;
+    Stream st;
+    MatrixCache mats;
+    std::vector<float> h((size_t)N * nvi);
+    aos_to_soa(field, h.data(), nvi, N);
+    DBuf<float> in, o;
+    in.upload(h.data(), h.size(), st.s);
+    o.alloc((size_t)N * nvo);
+    up_sample_field(in.get(), N_x, N_y, N_z, N_x_o, N_y_o, N_z_o, o.get(), st.s, mats);
+    std::vector<float> ho((size_t)N * nvo);
+    o.download(ho.data(), ho.size(), st.s);
+    HIP_OK(hipStreamSynchronize(st.s));
+    soa_to_aos(ho.data(), out, nvo, N);
   });
 }
 

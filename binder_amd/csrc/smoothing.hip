@@ -1,17 +1,25 @@
-// Per-axis DCT contractions for the deformation M-step / d-sample.
+// Per-axis DCT contractions for the deformation M-step / d-sample, on MFMA.
 //
-// A component's displacement t = dream - x is transformed with the separable
-// orthonormal DCT-II: one pass per axis, each a batched matrix product whose
-// operands are a dense transform matrix and the data (read with the axis
-// either as the contiguous dimension -- "data-left": rows x n times n x n --
-// or as a strided one -- "matrix-left": n x n times n x columns).  The pass
-// that completes the forward transform applies the prior filter
+// Every pass contracts the contiguous axis of its input and writes its output
+// transposed:  Out[k'][r] = sum_k In[r][k] M[k][k']  for r over all R rows.  A
+// volume laid out [X][Y][Z] therefore rotates to [Z][X][Y], [Y][Z][X] and back
+// to [X][Y][Z] after one pass per axis, so both MFMA operands are always
+// contiguous in the contraction index and no separate transpose is needed.
+//
+// The products use split BF16: a = a_hi + a_lo with a_hi = bf16(a) and
+// a_lo = bf16(a - a_hi) for the data and the (precomputed) matrix, and
+// a*b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi accumulated in FP32 by
+// v_mfma_f32_16x16x32_bf16 (about 2^-16 relative per product).
+//
+// The forward pass that completes the transform applies the prior filter
 // p = c / (1 + gamma sigma^2 D) (+ sqrt(sigma^2/(1+gamma sigma^2 D)) * eps when
-// sampling) and accumulates the prior cost 0.5 gamma D p^2 per block; the
-// inverse passes apply the orthonormal DCT-III and the last one adds x back.
+// sampling) and accumulates the prior cost 0.5 gamma D p^2 (and, for the nu
+// draw, eta^2 rho D p^2) per block; the last inverse pass adds x back.
 // Reference semantics: BINDER/src/cpu/NonLinearTransformation.cpp:58-126,
-// :128-223, :344-364, :433-451; ortho convention BINDER/src/cpu/DCT.cpp:83-175.
+// :128-223, :344-364, :366-430, :433-451; ortho convention
+// BINDER/src/cpu/DCT.cpp:83-175.
 #include <cmath>
+#include <cstring>
 
 #include "smoothing.hpp"
 
@@ -19,146 +27,198 @@ namespace binder {
 
 namespace {
 
-constexpr int BM = 64, BN = 64, BK = 16, kThreads = 256;
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8_t;
+typedef __attribute__((ext_vector_type(4))) float f32x4_t;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4_t;
+
+constexpr int kRowsPerBlock = 64;  // 4 wavefronts x 16 rows
+constexpr int kMaxKSteps = 8;      // contraction length <= 256
 
 enum : int { EPI_STORE = 0, EPI_FILTER = 1, EPI_ADDX = 2 };
 
 struct PassArgs {
-  const float *A, *B;
-  float *C;
-  int M, N, K, lda, ldb, ldc;
-  long long sAc, sBc, sCc;  // per-component strides
-  long long sAb, sBb, sCb;  // per inner-batch strides
-  int nb;                   // inner batches per component
-  const float *X;           // prologue: data operand minus X (same layout)
-  const float *Xo;          // ADDX epilogue: + Xo (same layout as C)
+  const float *in;       // [comp][R][n_in]
+  float *out;            // [comp][n_out][R]
+  const float *xpro;     // prologue: in - xpro (same layout as in), may be null
+  const float *xepi;     // ADDX epilogue: + xepi (same layout as out)
+  const uint32_t *bhi;   // swizzled matrix, bf16 hi / lo
+  const uint32_t *blo;
+  long long R;
+  long long cin, cout;   // component strides of in / out
+  int n_in, n_out, ks;   // ks = ceil(n_in / 32)
   // filter epilogue
   const float *D;
   const double *gamma;
-  double sigma2;
+  float sigma2;
   double etarho[3];
   int noise, raw;
   uint32_t k0, k1, sweep, stream;
   double *cost_part, *raw_part;
 };
 
-template <bool DATA_LEFT, bool PRO, int EPI>
-__global__ __launch_bounds__(kThreads) void k_axis_pass(PassArgs a) {
-  __shared__ float As[BK][BM + 4];
-  __shared__ float Bs[BK][BN + 4];
-  const int tid = threadIdx.x;
-  const int tn = tid & 15, tm = tid >> 4;
-  const int n0 = blockIdx.x * BN, m0 = blockIdx.y * BM;
-  const int comp = blockIdx.z / a.nb, inner = blockIdx.z % a.nb;
-  const float *A = a.A + comp * a.sAc + inner * a.sAb;
-  const float *B = a.B + comp * a.sBc + inner * a.sBb;
-  const float *XA = nullptr, *XB = nullptr;
-  if (PRO) {
-    if (DATA_LEFT)
-      XA = a.X + comp * a.sAc + inner * a.sAb;
-    else
-      XB = a.X + comp * a.sBc + inner * a.sBb;
-  }
-  float acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+__device__ __forceinline__ uint32_t bf16_bits(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;  // round to nearest even
+}
 
-  for (int k0 = 0; k0 < a.K; k0 += BK) {
+// Split 8 floats into bf16 hi and lo vectors (element j in bits 16j..16j+15).
+__device__ __forceinline__ void split8(const float v[8], bf16x8_t &hi,
+                                       bf16x8_t &lo) {
+  u32x4_t h, l;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int e = tid + kThreads * i;
-      const int r = e >> 4, c = e & 15;
-      const int gm = m0 + r, gk = k0 + c;
-      float v = 0.f;
-      if (gm < a.M && gk < a.K) {
-        const long long off = (long long)gm * a.lda + gk;
-        v = A[off];
-        if (PRO && DATA_LEFT) v -= XA[off];
-      }
-      As[c][r] = v;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int e = tid + kThreads * i;
-      const int r = e >> 6, c = e & 63;
-      const int gk = k0 + r, gn = n0 + c;
-      float v = 0.f;
-      if (gk < a.K && gn < a.N) {
-        const long long off = (long long)gk * a.ldb + gn;
-        v = B[off];
-        if (PRO && !DATA_LEFT) v -= XB[off];
-      }
-      Bs[r][c] = v;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; ++kk) {
-      const float4 av = *reinterpret_cast<const float4 *>(&As[kk][tm * 4]);
-      const float4 bv = *reinterpret_cast<const float4 *>(&Bs[kk][tn * 4]);
-      const float ar[4] = {av.x, av.y, av.z, av.w};
-      const float br[4] = {bv.x, bv.y, bv.z, bv.w};
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(ar[i], br[j], acc[i][j]);
-    }
-    __syncthreads();
+  for (int p = 0; p < 4; ++p) {
+    const uint32_t h0 = bf16_bits(v[2 * p]), h1 = bf16_bits(v[2 * p + 1]);
+    const float r0 = v[2 * p] - __uint_as_float(h0 << 16);
+    const float r1 = v[2 * p + 1] - __uint_as_float(h1 << 16);
+    h[p] = h0 | (h1 << 16);
+    l[p] = bf16_bits(r0) | (bf16_bits(r1) << 16);
   }
+  hi = __builtin_bit_cast(bf16x8_t, h);
+  lo = __builtin_bit_cast(bf16x8_t, l);
+}
 
-  float *C = a.C + comp * a.sCc + inner * a.sCb;
-  double cost = 0.0, raw = 0.0;
-  double g = 0.0;
-  if (EPI == EPI_FILTER) g = a.gamma[comp];
+// Four standard normals from one Philox block (Box-Muller, FP32).
+__device__ __forceinline__ void normals4(u32x4 r, float n[4]) {
+  const float u1 = ((float)(r.x >> 8) + 0.5f) * (1.0f / 16777216.0f);
+  const float u2 = (float)(r.y >> 8) * (1.0f / 16777216.0f);
+  const float u3 = ((float)(r.z >> 8) + 0.5f) * (1.0f / 16777216.0f);
+  const float u4 = (float)(r.w >> 8) * (1.0f / 16777216.0f);
+  const float a = sqrtf(-2.0f * __logf(u1)), b = sqrtf(-2.0f * __logf(u3));
+  float s1, c1, s2, c2;
+  __sincosf(6.283185307f * u2, &s1, &c1);
+  __sincosf(6.283185307f * u4, &s2, &c2);
+  n[0] = a * c1;
+  n[1] = a * s1;
+  n[2] = b * c2;
+  n[3] = b * s2;
+}
+
+template <bool VEC, bool PRO, int EPI>
+__global__ __launch_bounds__(256) void k_contract(PassArgs a) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int comp = blockIdx.y;
+  const long long row0 = (long long)blockIdx.x * kRowsPerBlock + wave * 16;
+  const float *in = a.in + comp * a.cin;
+  const float *xp = PRO ? a.xpro + comp * a.cin : nullptr;
+  float *out = a.out + comp * a.cout;
+
+  // A fragments: this lane's row and 8 consecutive k per 32-wide k-step
+  const long long arow = row0 + (lane & 15);
+  const bool rowok = arow < a.R;
+  const int kq = 8 * (lane >> 4);
+  bf16x8_t ahi[kMaxKSteps], alo[kMaxKSteps];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int gm = m0 + tm * 4 + i;
-    if (gm >= a.M) continue;
+  for (int s = 0; s < kMaxKSteps; ++s) {
+    if (s >= a.ks) break;
+    float v[8];
+    const int k0 = s * 32 + kq;
+    if (VEC && rowok && k0 + 8 <= a.n_in) {
+      const float4 *p = reinterpret_cast<const float4 *>(in + arow * a.n_in + k0);
+      const float4 x0 = p[0], x1 = p[1];
+      v[0] = x0.x; v[1] = x0.y; v[2] = x0.z; v[3] = x0.w;
+      v[4] = x1.x; v[5] = x1.y; v[6] = x1.z; v[7] = x1.w;
+      if (PRO) {
+        const float4 *q = reinterpret_cast<const float4 *>(xp + arow * a.n_in + k0);
+        const float4 y0 = q[0], y1 = q[1];
+        v[0] -= y0.x; v[1] -= y0.y; v[2] -= y0.z; v[3] -= y0.w;
+        v[4] -= y1.x; v[5] -= y1.y; v[6] -= y1.z; v[7] -= y1.w;
+      }
+    } else {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int gn = n0 + tn * 4 + j;
-      if (gn >= a.N) continue;
-      const long long off = (long long)gm * a.ldc + gn;
-      float v = acc[i][j];
-      if (EPI == EPI_FILTER) {
-        const long long o = inner * a.sCb + off;  // voxel index in component
-        const float Dv = a.D[o];
-        const float gs2 = (float)(g * a.sigma2);
-        const float den = 1.0f + gs2 * Dv;
-        float p = v * (1.0f / den);
-        if (a.noise) {
-          const u32x4 r = philox4x32_10(
-              u32x4{(uint32_t)o, a.sweep, (uint32_t)comp, a.stream}, a.k0, a.k1);
-          double e0, e1;
-          box_muller(r.x, r.y, e0, e1);
-          p += (float)e0 * sqrtf((float)a.sigma2 / den);
+      for (int j = 0; j < 8; ++j) {
+        const int k = k0 + j;
+        float t = 0.f;
+        if (rowok && k < a.n_in) {
+          t = in[arow * a.n_in + k];
+          if (PRO) t -= xp[arow * a.n_in + k];
         }
+        v[j] = t;
+      }
+    }
+    split8(v, ahi[s], alo[s]);
+  }
+
+  double g = 0.0;
+  float gs2 = 0.f;
+  if (EPI == EPI_FILTER) {
+    g = a.gamma[comp];
+    gs2 = (float)g * a.sigma2;
+  }
+  double cost = 0.0, raw = 0.0;
+  const int nt_count = (a.n_out + 15) >> 4;
+  const long long orow = row0 + 4 * (lane >> 4);  // first of this lane's 4 rows
+  for (int nt = 0; nt < nt_count; ++nt) {
+    f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+    const uint4 *bh = reinterpret_cast<const uint4 *>(a.bhi) + (long long)nt * a.ks * 64 + lane;
+    const uint4 *bl = reinterpret_cast<const uint4 *>(a.blo) + (long long)nt * a.ks * 64 + lane;
+#pragma unroll
+    for (int s = 0; s < kMaxKSteps; ++s) {
+      if (s >= a.ks) break;
+      const bf16x8_t bhi = __builtin_bit_cast(bf16x8_t, bh[s * 64]);
+      const bf16x8_t blo = __builtin_bit_cast(bf16x8_t, bl[s * 64]);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[s], bhi, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[s], blo, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[s], bhi, acc, 0, 0, 0);
+    }
+    // lane holds rows orow..orow+3 of output column kc
+    const int kc = nt * 16 + (lane & 15);
+    if (kc >= a.n_out) continue;
+    float v[4] = {acc[0], acc[1], acc[2], acc[3]};
+    const long long obase = (long long)kc * a.R + orow;
+    if (EPI == EPI_FILTER) {
+      float eps[4] = {0.f, 0.f, 0.f, 0.f};
+      if (a.noise) {
+        // one block of four normals per (output column, 4-row group)
+        const u32x4 r = philox4x32_10(
+            u32x4{(uint32_t)orow, (uint32_t)kc, a.sweep,
+                  ((uint32_t)comp << 16) | a.stream},
+            a.k0, a.k1);
+        normals4(r, eps);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (orow + i >= a.R) break;
+        const float Dv = a.D[obase + i];
+        const float den = 1.0f + gs2 * Dv;
+        float p = v[i] * (1.0f / den);
+        if (a.noise) p += eps[i] * sqrtf(a.sigma2 / den);
         const double pd = (double)p;
         cost += 0.5 * g * (double)Dv * pd * pd;
         if (a.raw) raw += a.etarho[comp] * (double)Dv * pd * pd;
-        v = p;
-      } else if (EPI == EPI_ADDX) {
-        if (a.Xo) v += a.Xo[comp * a.sCc + inner * a.sCb + off];
+        v[i] = p;
       }
-      C[off] = v;
+    } else if (EPI == EPI_ADDX) {
+      if (a.xepi) {
+        const float *xo = a.xepi + comp * a.cout;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (orow + i < a.R) v[i] += xo[obase + i];
+      }
+    }
+    if (VEC && orow + 3 < a.R) {
+      *reinterpret_cast<float4 *>(out + obase) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (orow + i < a.R) out[obase + i] = v[i];
     }
   }
+
   if (EPI == EPI_FILTER) {
-    __shared__ double red[2][kThreads];
-    red[0][tid] = cost;
-    red[1][tid] = raw;
+    __shared__ double red[2][256];
+    const int t = threadIdx.x;
+    red[0][t] = cost;
+    red[1][t] = raw;
     __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-      if (tid < s) {
-        red[0][tid] += red[0][tid + s];
-        red[1][tid] += red[1][tid + s];
+    for (int s = 128; s > 0; s >>= 1) {
+      if (t < s) {
+        red[0][t] += red[0][t + s];
+        red[1][t] += red[1][t + s];
       }
       __syncthreads();
     }
-    if (tid == 0) {
-      const long long b =
-          ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    if (t == 0) {
+      const long long b = (long long)blockIdx.y * gridDim.x + blockIdx.x;
       a.cost_part[b] = red[0][0];
       if (a.raw) a.raw_part[b] = red[1][0];
     }
@@ -190,19 +250,145 @@ __global__ void k_draw_nu(const double *raw, double *gamma, int N, double nv,
   if (threadIdx.x != 0) return;
   const double a_post = 1.0 + 0.5 * (double)N * nv;
   const double b_post = 10000.0 + 0.5 * nv * (*raw);
-  const double g = gamma_mt(a_post, u32x4{0u, sweep, 0u, kStreamNu}, k0, k1);
-  const double nu = g / b_post;
+  const double gm = gamma_mt(a_post, u32x4{0u, sweep, 0u, kStreamNu}, k0, k1);
+  const double nu = gm / b_post;
   gamma[0] = nu * rho_prod * e0 * e0 * nv;
   gamma[1] = nu * rho_prod * e1 * e1 * nv;
   gamma[2] = nu * rho_prod * e2 * e2 * nv;
 }
 
-template <bool DL, bool PRO, int EPI>
+template <bool VEC, bool PRO, int EPI>
 void launch(dim3 grid, hipStream_t s, const PassArgs &a) {
-  hipLaunchKernelGGL((k_axis_pass<DL, PRO, EPI>), grid, dim3(kThreads), 0, s, a);
+  hipLaunchKernelGGL((k_contract<VEC, PRO, EPI>), grid, dim3(256), 0, s, a);
+}
+
+uint16_t host_bf16(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+float host_bf16_to_float(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
 }
 
 }  // namespace
+
+// Swizzle M (n_in x n_out, row-major, double) into the MFMA B-operand order:
+// for n-tile t, k-step s, lane l the 8 bf16 values M[32 s + 8 (l/16) + j][16 t + l%16].
+void build_swizzled(const std::vector<double> &M, int n_in, int n_out,
+                    std::vector<uint32_t> &hi, std::vector<uint32_t> &lo) {
+  const int ks = (n_in + 31) / 32, nt = (n_out + 15) / 16;
+  hi.assign((size_t)nt * ks * 64 * 4, 0u);
+  lo.assign((size_t)nt * ks * 64 * 4, 0u);
+  for (int t = 0; t < nt; ++t)
+    for (int s = 0; s < ks; ++s)
+      for (int l = 0; l < 64; ++l)
+        for (int j = 0; j < 8; ++j) {
+          const int k = 32 * s + 8 * (l / 16) + j, c = 16 * t + (l % 16);
+          const float v = (k < n_in && c < n_out) ? (float)M[(size_t)k * n_out + c] : 0.f;
+          const uint16_t h = host_bf16(v);
+          const uint16_t r = host_bf16(v - host_bf16_to_float(h));
+          const size_t w = (((size_t)t * ks + s) * 64 + l) * 4 + j / 2;
+          hi[w] |= (uint32_t)h << (16 * (j & 1));
+          lo[w] |= (uint32_t)r << (16 * (j & 1));
+        }
+}
+
+// Orthonormal DCT-II matrix entry C[k][j] = s_k cos(pi (2j+1) k / (2n)).
+static double dct_entry(int n, int k, int j) {
+  const double s = k == 0 ? std::sqrt(1.0 / n) : std::sqrt(2.0 / n);
+  return s * std::cos(M_PI * (2.0 * j + 1.0) * k / (2.0 * n));
+}
+
+const PassMatrix &MatrixCache::get(int kind, int n_in, int n_out,
+                                   hipStream_t stream) {
+  const long long key = ((long long)kind << 40) | ((long long)n_in << 20) | n_out;
+  auto it = cache_.find(key);
+  if (it != cache_.end()) return *it->second;
+  std::vector<double> M((size_t)n_in * n_out);
+  for (int k = 0; k < n_in; ++k)
+    for (int c = 0; c < n_out; ++c) {
+      double v;
+      if (kind == kDctForward) {  // Out[c] = sum_k In[k] C_n[c][k]
+        v = dct_entry(n_in, c, k);
+      } else {  // inverse at size n_out from the first n_in coefficients
+        v = dct_entry(n_out, k, c);
+      }
+      M[(size_t)k * n_out + c] = v;
+    }
+  std::vector<uint32_t> hi, lo;
+  build_swizzled(M, n_in, n_out, hi, lo);
+  auto pm = std::make_unique<PassMatrix>();
+  pm->n_in = n_in;
+  pm->n_out = n_out;
+  pm->hi.upload(hi.data(), hi.size(), stream);
+  pm->lo.upload(lo.data(), lo.size(), stream);
+  HIP_OK(hipStreamSynchronize(stream));
+  auto &ref = *pm;
+  cache_[key] = std::move(pm);
+  return ref;
+}
+
+void contract_pass(hipStream_t stream, const PassMatrix &m, const float *in,
+                   float *out, int comps, long long cin, long long cout,
+                   long long R, const float *xpro, const float *xepi, int epi,
+                   const FilterArgs *f, KernelTimer *timer, const char *name) {
+  BINDER_REQUIRE(m.n_in <= 32 * kMaxKSteps, "contraction length above 256");
+  PassArgs a{};
+  a.in = in;
+  a.out = out;
+  a.xpro = xpro;
+  a.xepi = xepi;
+  a.bhi = m.hi.get();
+  a.blo = m.lo.get();
+  a.R = R;
+  a.cin = cin;
+  a.cout = cout;
+  a.n_in = m.n_in;
+  a.n_out = m.n_out;
+  a.ks = (m.n_in + 31) / 32;
+  if (f) {
+    a.D = f->D;
+    a.gamma = f->gamma;
+    a.sigma2 = (float)f->sigma2;
+    for (int i = 0; i < 3; ++i) a.etarho[i] = f->etarho[i];
+    a.noise = f->noise;
+    a.raw = f->raw;
+    a.k0 = f->k0;
+    a.k1 = f->k1;
+    a.sweep = f->sweep;
+    a.stream = f->stream;
+    a.cost_part = f->cost_part;
+    a.raw_part = f->raw_part;
+  }
+  const bool vec = (m.n_in % 8 == 0) && (R % 4 == 0) && (cin % 4 == 0) &&
+                   (cout % 4 == 0);
+  dim3 grid((unsigned)((R + kRowsPerBlock - 1) / kRowsPerBlock), comps);
+  TimedLaunch tl(timer, name, stream);
+  const bool pro = xpro != nullptr;
+#define BINDER_LAUNCH(V)                                               \
+  if (pro) {                                                           \
+    if (epi == EPI_FILTER) launch<V, true, EPI_FILTER>(grid, stream, a); \
+    else launch<V, true, EPI_STORE>(grid, stream, a);                  \
+  } else if (epi == EPI_FILTER) launch<V, false, EPI_FILTER>(grid, stream, a); \
+  else if (epi == EPI_ADDX) launch<V, false, EPI_ADDX>(grid, stream, a); \
+  else launch<V, false, EPI_STORE>(grid, stream, a);
+  if (vec) {
+    BINDER_LAUNCH(true)
+  } else {
+    BINDER_LAUNCH(false)
+  }
+#undef BINDER_LAUNCH
+  HIP_LAUNCH_OK();
+}
+
+long long contract_blocks(long long R, int comps) {
+  return ((R + kRowsPerBlock - 1) / kRowsPerBlock) * comps;
+}
 
 Smoother::Smoother(int nx, int ny, int nz, const double gamma[3],
                    const double rho[3], const double eta[3],
@@ -211,6 +397,8 @@ Smoother::Smoother(int nx, int ny, int nz, const double gamma[3],
       nv_((long long)nx * ny * nz), sigma_(sigma), stream_(stream) {
   BINDER_REQUIRE(nx >= 2 && ny >= 2 && nz >= 1,
                  "deformation grid must be at least 2 x 2 (x 1)");
+  BINDER_REQUIRE(nx <= 256 && ny <= 256 && nz <= 256,
+                 "each grid axis must be at most 256 voxels");
   for (int i = 0; i < 3; ++i) {
     rho_[i] = rho[i];
     eta_[i] = eta[i];
@@ -220,9 +408,8 @@ Smoother::Smoother(int nx, int ny, int nz, const double gamma[3],
   gamma_.upload(gamma, 3, stream_);
   bufA_.alloc((size_t)N_ * nv_);
   bufB_.alloc((size_t)N_ * nv_);
-  // partial sums of the filter pass (axis 0, matrix-left over ny*nz columns)
-  const long long nparts =
-      (long long)((ny * nz + BN - 1) / BN) * ((nx + BM - 1) / BM) * N_;
+  // the filter pass contracts x over R = ny*nz rows
+  const long long nparts = contract_blocks((long long)ny * nz, N_);
   part_cost_.alloc(nparts);
   part_raw_.alloc(nparts);
   scal_.alloc(4);
@@ -249,136 +436,53 @@ double Smoother::flops() const {
   return 2.0 * per_voxel * (double)nv_ * N_;  // forward + inverse
 }
 
-const AxisMatrices &Smoother::mats(int n) {
-  auto it = mats_.find(n);
-  if (it != mats_.end()) return *it->second;
-  std::vector<float> f((size_t)n * n), t((size_t)n * n);
-  for (int k = 0; k < n; ++k) {
-    const double s = k == 0 ? std::sqrt(1.0 / n) : std::sqrt(2.0 / n);
-    for (int j = 0; j < n; ++j) {
-      const double c = s * std::cos(M_PI * (2.0 * j + 1.0) * k / (2.0 * n));
-      f[(size_t)k * n + j] = (float)c;
-      t[(size_t)j * n + k] = (float)c;
-    }
-  }
-  auto m = std::make_unique<AxisMatrices>();
-  m->fwd.upload(f.data(), f.size(), stream_);
-  m->inv.upload(t.data(), t.size(), stream_);
-  HIP_OK(hipStreamSynchronize(stream_));
-  auto &ref = *m;
-  mats_[n] = std::move(m);
-  return ref;
-}
-
-// flags: bit0 = prologue (data - xpro), bits1-2 = epilogue kind.
-void Smoother::launch_pass(int axis, bool forward, const float *in, float *out,
-                           int flags, const float *xpro, const float *xepi,
-                           const SmoothOptions &opt, KernelTimer *timer) {
-  const bool pro = flags & 1;
-  const int epi = (flags >> 1) & 3;
-  const int inner_axis = nz_ > 1 ? 2 : 1;
-  PassArgs a{};
-  a.sigma2 = sigma_ * sigma_;
-  a.D = D_.get();
-  a.gamma = gamma_.get();
-  a.X = xpro;
-  a.Xo = xepi;
-  a.noise = opt.sample ? 1 : 0;
-  a.raw = opt.sample_gamma ? 1 : 0;
-  const double rp = rho_[0] * rho_[1] * rho_[2];
-  for (int i = 0; i < 3; ++i) a.etarho[i] = eta_[i] * eta_[i] * rp;
-  a.k0 = opt.key0;
-  a.k1 = opt.key1;
-  a.sweep = opt.sweep;
-  a.stream = opt.stream;
-  a.cost_part = part_cost_.get();
-  a.raw_part = part_raw_.get();
-  const int dims[3] = {nx_, ny_, nz_};
-  const int n = dims[axis];
-  const AxisMatrices &m = mats(n);
-  const bool data_left = (axis == inner_axis);
-  if (data_left) {
-    a.A = in;
-    a.B = forward ? m.inv.get() : m.fwd.get();
-    a.M = (int)(nv_ / n);
-    a.N = n;
-    a.K = n;
-    a.lda = n;
-    a.ldb = n;
-    a.ldc = n;
-    a.sAc = nv_;
-    a.sBc = 0;
-    a.sCc = nv_;
-    a.nb = 1;
-  } else {
-    a.A = forward ? m.fwd.get() : m.inv.get();
-    a.B = in;
-    a.M = n;
-    a.K = n;
-    a.lda = n;
-    a.sAc = 0;
-    a.sBc = nv_;
-    a.sCc = nv_;
-    if (axis == 1) {  // 3D middle axis, batched over x
-      a.N = nz_;
-      a.ldb = nz_;
-      a.ldc = nz_;
-      a.nb = nx_;
-      a.sBb = (long long)ny_ * nz_;
-      a.sCb = (long long)ny_ * nz_;
-    } else {  // outermost axis
-      a.N = ny_ * nz_;
-      a.ldb = ny_ * nz_;
-      a.ldc = ny_ * nz_;
-      a.nb = 1;
-    }
-  }
-  a.C = out;
-  dim3 grid((a.N + BN - 1) / BN, (a.M + BM - 1) / BM, N_ * a.nb);
-  BINDER_REQUIRE(grid.y <= 65535 && grid.z <= 65535, "grid too large for DCT pass");
-  const char *names[3][2] = {{"dct_x_inv", "dct_x_fwd"},
-                             {"dct_y_inv", "dct_y_fwd"},
-                             {"dct_z_inv", "dct_z_fwd"}};
-  TimedLaunch tl(timer, names[axis][forward ? 1 : 0], stream_);
-  if (data_left) {
-    if (pro) {
-      if (epi == EPI_FILTER) launch<true, true, EPI_FILTER>(grid, stream_, a);
-      else launch<true, true, EPI_STORE>(grid, stream_, a);
-    } else if (epi == EPI_ADDX) launch<true, false, EPI_ADDX>(grid, stream_, a);
-    else if (epi == EPI_FILTER) launch<true, false, EPI_FILTER>(grid, stream_, a);
-    else launch<true, false, EPI_STORE>(grid, stream_, a);
-  } else {
-    if (pro) {
-      if (epi == EPI_FILTER) launch<false, true, EPI_FILTER>(grid, stream_, a);
-      else launch<false, true, EPI_STORE>(grid, stream_, a);
-    } else if (epi == EPI_ADDX) launch<false, false, EPI_ADDX>(grid, stream_, a);
-    else if (epi == EPI_FILTER) launch<false, false, EPI_FILTER>(grid, stream_, a);
-    else launch<false, false, EPI_STORE>(grid, stream_, a);
-  }
-  HIP_LAUNCH_OK();
-}
-
 void Smoother::run(const float *dream, const float *x, float *out,
                    const SmoothOptions &opt, KernelTimer *timer) {
-  // forward axes innermost first, inverse in the opposite order
-  int axes[3], na = 0;
-  if (nz_ > 1) axes[na++] = 2;
-  axes[na++] = 1;
-  axes[na++] = 0;
-  const float *src = dream;
+  // axes contracted innermost first; each pass rotates the layout, so after
+  // one pass per axis the data is back in [X][Y][Z] order
+  int dims[3], na = 0;
+  const char *fwd_names[3], *inv_names[3];
+  if (nz_ > 1) {
+    dims[na] = nz_;
+    fwd_names[na] = "dct_z_fwd";
+    inv_names[na++] = "dct_z_inv";
+  }
+  dims[na] = ny_;
+  fwd_names[na] = "dct_y_fwd";
+  inv_names[na++] = "dct_y_inv";
+  dims[na] = nx_;
+  fwd_names[na] = "dct_x_fwd";
+  inv_names[na++] = "dct_x_inv";
+
+  FilterArgs f{};
+  f.D = D_.get();
+  f.gamma = gamma_.get();
+  f.sigma2 = sigma_ * sigma_;
+  const double rp = rho_[0] * rho_[1] * rho_[2];
+  for (int i = 0; i < 3; ++i) f.etarho[i] = eta_[i] * eta_[i] * rp;
+  f.noise = opt.sample ? 1 : 0;
+  f.raw = opt.sample_gamma ? 1 : 0;
+  f.k0 = opt.key0;
+  f.k1 = opt.key1;
+  f.sweep = opt.sweep;
+  f.stream = opt.stream;
+  f.cost_part = part_cost_.get();
+  f.raw_part = part_raw_.get();
+
   float *bufs[2] = {bufA_.get(), bufB_.get()};
   int bi = 0;
-  // Forward passes; the first subtracts x, the last filters.
+  const float *src = dream;
   for (int i = 0; i < na; ++i) {
-    int flags = 0;
-    if (i == 0 && x) flags |= 1;
-    if (i == na - 1) flags |= (EPI_FILTER << 1);
-    launch_pass(axes[i], true, src, bufs[bi], flags, x, nullptr, opt, timer);
+    const int n = dims[i];
+    const PassMatrix &m = mats_.get(MatrixCache::kDctForward, n, n, stream_);
+    const bool last = (i == na - 1);
+    contract_pass(stream_, m, src, bufs[bi], N_, nv_, nv_, nv_ / n,
+                  i == 0 ? x : nullptr, nullptr, last ? EPI_FILTER : EPI_STORE,
+                  last ? &f : nullptr, timer, fwd_names[i]);
     src = bufs[bi];
     bi ^= 1;
   }
-  const long long nparts =
-      (long long)((ny_ * nz_ + BN - 1) / BN) * ((nx_ + BM - 1) / BM) * N_;
+  const long long nparts = contract_blocks(nv_ / dims[na - 1], N_);
   {
     TimedLaunch tl(timer, "sum_partials", stream_);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, stream_,
@@ -391,13 +495,14 @@ void Smoother::run(const float *dream, const float *x, float *out,
                        part_raw_.get(), nparts, scal_.get() + 1);
     HIP_LAUNCH_OK();
   }
-  // Inverse passes; the last adds x and writes the output.
-  for (int i = na - 1; i >= 0; --i) {
-    const bool last = (i == 0);
+  for (int i = 0; i < na; ++i) {
+    const int n = dims[i];
+    const PassMatrix &m = mats_.get(MatrixCache::kDctInverse, n, n, stream_);
+    const bool last = (i == na - 1);
     float *dst = last ? out : bufs[bi];
-    const int flags = last ? (EPI_ADDX << 1) : 0;
-    launch_pass(axes[i], false, src, dst, flags, nullptr, last ? x : nullptr,
-                opt, timer);
+    contract_pass(stream_, m, src, dst, N_, nv_, nv_, nv_ / n, nullptr,
+                  last ? x : nullptr, last ? EPI_ADDX : EPI_STORE, nullptr, timer,
+                  inv_names[i]);
     src = dst;
     bi ^= 1;
   }
@@ -408,6 +513,57 @@ void Smoother::run(const float *dream, const float *x, float *out,
                        opt.key0, opt.key1, opt.sweep);
     HIP_LAUNCH_OK();
   }
+}
+
+// DCT-domain zero-pad resampling of an N-component field
+// (NonLinearTransformation.cpp:366-430): orthonormal DCT-II at the input size,
+// then DCT-III at the output size from the first n_in coefficients per axis.
+void up_sample_field(const float *in_soa, int nx, int ny, int nz, int nxo,
+                     int nyo, int nzo, float *out_soa, hipStream_t stream,
+                     MatrixCache &mats) {
+  const int N = nzo == 1 ? 2 : 3;
+  const long long nvi = (long long)nx * ny * nz, nvo = (long long)nxo * nyo * nzo;
+  // largest intermediate: the inverse passes change one axis at a time
+  long long big = nvi;
+  for (long long v : {(long long)nx * ny * nzo, (long long)nx * nyo * nzo, nvo})
+    big = v > big ? v : big;
+  DBuf<float> A, B;
+  A.alloc((size_t)N * big);
+  B.alloc((size_t)N * big);
+  // forward: [X][Y][Z] -> [Z][X][Y] -> [Y][Z][X] -> [X][Y][Z] (coefficients)
+  int din[3], dout[3], na = 0;
+  if (nz > 1 || nzo > 1) {
+    din[na] = nz;
+    dout[na++] = nzo;
+  }
+  din[na] = ny;
+  dout[na++] = nyo;
+  din[na] = nx;
+  dout[na++] = nxo;
+  const float *src = in_soa;
+  float *bufs[2] = {A.get(), B.get()};
+  int bi = 0;
+  for (int i = 0; i < na; ++i) {
+    const PassMatrix &m = mats.get(MatrixCache::kDctForward, din[i], din[i], stream);
+    contract_pass(stream, m, src, bufs[bi], N, nvi, nvi, nvi / din[i], nullptr,
+                  nullptr, EPI_STORE, nullptr, nullptr, "upsample_fwd");
+    src = bufs[bi];
+    bi ^= 1;
+  }
+  // inverse at the output size: each pass maps one axis n_in -> n_out
+  long long cur = nvi;  // elements per component of the current layout
+  for (int i = 0; i < na; ++i) {
+    const PassMatrix &m = mats.get(MatrixCache::kDctInverse, din[i], dout[i], stream);
+    const long long R = cur / din[i];
+    const long long next = R * dout[i];
+    float *dst = (i == na - 1) ? out_soa : bufs[bi];
+    contract_pass(stream, m, src, dst, N, cur, (i == na - 1) ? nvo : next, R, nullptr,
+                  nullptr, EPI_STORE, nullptr, nullptr, "upsample_inv");
+    src = dst;
+    bi ^= 1;
+    cur = next;
+  }
+  HIP_OK(hipStreamSynchronize(stream));
 }
 
 }  // namespace binder

@@ -1,23 +1,57 @@
 // DCT-domain smoothing of a deformation on the GPU: the separable orthonormal
 // DCT-II / DCT-III of each displacement component computed as per-axis
-// matrix contractions, with the prior filter, the sampling noise and the
-// prior cost fused into the pass that completes the forward transform.
+// split-BF16 MFMA contractions, with the prior filter, the sampling noise and
+// the prior cost fused into the pass that completes the forward transform.
 // Replaces NonLinearTransformation::constrain_transformation /
-// sample_transformation (BINDER/src/cpu/NonLinearTransformation.cpp:58-126,
-// :190-223) and their cuFFT-based GPU counterparts.
+// sample_transformation / up_sample
+// (BINDER/src/cpu/NonLinearTransformation.cpp:58-126, :190-223, :366-430).
 #pragma once
 #include <map>
 #include <memory>
+#include <vector>
 
 #include "common.hpp"
 
 namespace binder {
 
-// Dense per-axis transform matrices, FP32 on the device.
-struct AxisMatrices {
-  DBuf<float> fwd;  // [n][n]: fwd[k*n + j] = s_k cos(pi (2j+1) k / (2n))
-  DBuf<float> inv;  // [n][n]: inv = fwd^T
+// One pass matrix M (n_in x n_out) in MFMA B-operand order, bf16 hi / lo.
+struct PassMatrix {
+  int n_in = 0, n_out = 0;
+  DBuf<uint32_t> hi, lo;
 };
+
+class MatrixCache {
+ public:
+  enum Kind { kDctForward = 1, kDctInverse = 2 };
+  // kDctForward: M[k][c] = C_{n}[c][k] (n = n_in = n_out);
+  // kDctInverse: M[k][c] = C_{n_out}[k][c] (first n_in coefficients).
+  const PassMatrix &get(int kind, int n_in, int n_out, hipStream_t stream);
+
+ private:
+  std::map<long long, std::unique_ptr<PassMatrix>> cache_;
+};
+
+struct FilterArgs {
+  const float *D;
+  const double *gamma;
+  double sigma2;
+  double etarho[3];
+  int noise, raw;
+  uint32_t k0, k1, sweep, stream;
+  double *cost_part, *raw_part;
+};
+
+// Out[c][k'][r] = sum_k In[c][r][k] M[k][k'] for r < R, c < comps.
+// epi: 0 store, 1 filter (needs f), 2 add xepi.  xpro: In - xpro prologue.
+void contract_pass(hipStream_t stream, const PassMatrix &m, const float *in,
+                   float *out, int comps, long long cin, long long cout,
+                   long long R, const float *xpro, const float *xepi, int epi,
+                   const FilterArgs *f, KernelTimer *timer, const char *name);
+long long contract_blocks(long long R, int comps);
+
+void up_sample_field(const float *in_soa, int nx, int ny, int nz, int nxo,
+                     int nyo, int nzo, float *out_soa, hipStream_t stream,
+                     MatrixCache &mats);
 
 struct SmoothOptions {
   bool sample = false;         // add DCT-domain Gaussian noise
@@ -51,18 +85,13 @@ class Smoother {
   double flops() const;
 
  private:
-  const AxisMatrices &mats(int n);
-  void launch_pass(int axis, bool forward, const float *in, float *out,
-                   int flags, const float *xpro, const float *xepi,
-                   const SmoothOptions &opt, KernelTimer *timer);
-
   int nx_, ny_, nz_, N_;
   long long nv_;
   double sigma_, rho_[3], eta_[3], nu_;
   hipStream_t stream_;
   DBuf<float> D_, bufA_, bufB_;
   DBuf<double> gamma_, part_cost_, part_raw_, scal_;
-  std::map<int, std::unique_ptr<AxisMatrices>> mats_;
+  MatrixCache mats_;
 };
 
 }  // namespace binder

@@ -113,6 +113,19 @@ int binder_registration_sampler(binder_registration *reg, int seed, int N_b,
                                 int sample_transformation_parameters,
                                 double *mean_out, double *cov_out);
 
+/* The two halves of binder_registration_sampler: run_ executes the sweeps
+ * and keeps mean/M2 on the device; result_ copies them out (mean, and
+ * M2/(N_s-1) expanded to the full N x N matrix).  Splitting them lets a
+ * caller time the sweeps with all state resident in HBM. */
+int binder_registration_sampler_run(binder_registration *reg, int seed, int N_b,
+                                    int N_s, int sample_gamma,
+                                    int sample_gamma_every,
+                                    int sample_posteriors,
+                                    int sample_likelihood_parameters,
+                                    int sample_transformation_parameters);
+int binder_registration_sampler_result(binder_registration *reg,
+                                       double *mean_out, double *cov_out);
+
 /* get_final_locations_as_numpy_array() -- BINDER/src/gpu/bindings.cpp:142-157. */
 int binder_registration_get_final_locations(binder_registration *reg,
                                             float *out);

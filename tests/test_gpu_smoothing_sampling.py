@@ -125,3 +125,18 @@ def test_sampler_deformation_mean_pipeline(core, order):
                         sample_posteriors=0, sample_likelihood_parameters=0,
                         sample_transformation_parameters=1)
     assert np.allclose(mean, m2[..., 0], rtol=1e-3, atol=1e-2)
+
+
+@pytest.mark.parametrize("shape_in,shape_out", [((10, 12, 8), (20, 24, 16)),
+                                                ((24, 20, 1), (48, 40, 1)),
+                                                ((13, 7, 9), (26, 14, 18))])
+def test_up_sample_matches_oracle(core, orc, shape_in, shape_out):
+    """DCT zero-pad resampling (NonLinearTransformation.cpp:366-430)."""
+    print(f"up_sample {shape_in} -> {shape_out}", flush=True)
+    rng = np.random.default_rng(sum(shape_in))
+    N = 2 if shape_in[2] == 1 else 3
+    f = rng.standard_normal(shape_in + (N,))
+    g = core.NonLinearTransformation.up_sample(f.ravel(), *shape_in, *shape_out, 1)
+    o = orc.NonLinearTransformation.up_sample(f.ravel(), *shape_in, *shape_out, 1)
+    assert g.shape == o.shape == shape_out + (N,)
+    assert np.max(np.abs(g - o)) <= 2e-5 * max(1.0, np.abs(o).max())
