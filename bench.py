@@ -63,7 +63,7 @@ def run(args):
     # ---- EM registration, cold start, to convergence ----------------------
     reg = synth.build_registration(_core, prob)
     t0 = time.perf_counter()
-    reg.EM(args.em_iters, args.convergence_th, 1, 0)
+    reg.EM(3 if args.quick else args.em_iters, args.convergence_th, 1, 0)
     em_s = time.perf_counter() - t0
     trace = reg.get_objective_trace()
     d_em = reg.get_final_locations_as_numpy_array().astype(np.float64)
@@ -77,7 +77,8 @@ def run(args):
     fold_s = time.perf_counter() - t0
     topo = _core.topology_correction_3D()
     t0 = time.perf_counter()
-    d_tc = topo.correct_topology(d_em.ravel(), 0.01, 100, 10, 10, 1, np.ones(3), Nx, Ny, Nz)
+    d_tc = topo.correct_topology(d_em.ravel(), 0.01, 100, 0 if args.quick else 10, 10, 1,
+                                 np.ones(3), Nx, Ny, Nz)
     topo_s = time.perf_counter() - t0
     _, folds_after = _core.fold_mask_3D(d_tc.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0, e1=0.01)
     log(f"topology: {folds_before} -> {folds_after} folded voxels in {topo_s:.3f}s")
@@ -112,7 +113,8 @@ def run(args):
             if fl:
                 ent["TFLOPs"] = round(fl / (avg_us * 1e-6) / 1e12, 2)
         kernels[name] = ent
-    dom = max(kernels, key=lambda k: kernels[k]["ms_per_sweep"])
+    modelled = [k for k in kernels if kernel_model(k, nv, nv, N, (Nx, Ny, Nz))[0] is not None]
+    dom = max(modelled, key=lambda k: kernels[k]["ms_per_sweep"])
     b, fl = kernel_model(dom, nv, nv, N, (Nx, Ny, Nz))
     avg_s = kernels[dom]["avg_us"] * 1e-6
     if fl:
@@ -182,6 +184,20 @@ def cpu_baseline(prob, nv):
                                       f"Welford), {t:.2f} s"}
 
 
+def copy_calibration(mib=1024):
+    """A plain device-to-device copy of known size (torch elementwise copy),
+    used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy calibration: 3 copies of {mib} MiB (read {mib} MiB + write {mib} MiB each)")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,7 +207,14 @@ def main():
     ap.add_argument("--em-iters", type=int, default=200)
     ap.add_argument("--convergence-th", type=float, default=2.5e-5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--quick", action="store_true",
+                    help="3 EM iterations and no topology iterations (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the copy kernel used to calibrate HBM counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the registration path is single-GPU in this build)", file=sys.stderr)

@@ -173,6 +173,20 @@ struct TimedLaunch {
 };
 
 // ----------------------------------------------------------------------------
+// Work-group placement.  MI355X dispatches consecutive work-groups of a grid
+// round-robin over its 8 XCDs, each with a private L2.  For kernels that
+// re-read a spatial neighbourhood (E-step gathers, Jacobian stencils) a
+// physical block id is mapped to a logical one such that each XCD owns one
+// contiguous range of logical blocks, keeping neighbourhood reuse in one L2.
+// ----------------------------------------------------------------------------
+constexpr int kNumXcd = 8;
+
+__device__ __forceinline__ int xcd_logical_block(int b, int nb) {
+  if (nb % kNumXcd) return b;
+  return (b % kNumXcd) * (nb / kNumXcd) + b / kNumXcd;
+}
+
+// ----------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
 // as 1, 2, 3", SC'11): a stateless, counter-based generator.  Every random
 // number used by the sampler is a pure function of (seed, stream, sweep, site),

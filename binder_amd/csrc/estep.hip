@@ -80,7 +80,7 @@ __device__ __forceinline__ void wave_hist_add(unsigned long long *hist, int idx,
 }
 
 __device__ __forceinline__ unsigned long long to_fixed(float v) {
-  return (unsigned long long)__float2ll_rn(v * 4294967296.0f);
+  return (unsigned long long)__float2uint_rn(v * 2147483648.0f);
 }
 
 __device__ __forceinline__ float sel4(const float w[4], int k) {
@@ -136,11 +136,17 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
 
   constexpr int NZ = IS2D ? 1 : 4;
   double mll = 0.0;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  // uniform trip count across the block: every lane reaches wave_hist_add
-  for (long long base = (long long)blockIdx.x * blockDim.x; base < a.nv; base += stride) {
+  // each logical block owns one contiguous voxel range (XCD-local gathers);
+  // the trip count is uniform across the block, so every lane reaches
+  // wave_hist_add
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const long long per =
+      ((a.nv + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
+  const long long vbeg = (long long)lb * per;
+  const long long vend = vbeg + per < a.nv ? vbeg + per : a.nv;
+  for (long long base = vbeg; base < vend; base += blockDim.x) {
     const long long i = base + threadIdx.x;
-    const bool valid = i < a.nv;
+    const bool valid = i < vend;
     const long long ii = valid ? i : a.nv - 1;
     const int u = a.bv[ii];
     const float *thu = th + u * Kp;
@@ -333,7 +339,7 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
   }
 
   __syncthreads();
-  unsigned long long *out = a.hpart + (long long)blockIdx.x * KL;
+  unsigned long long *out = a.hpart + (long long)lb * KL;
   for (int i = threadIdx.x; i < KL; i += blockDim.x) out[i] = hist[i];
   // deterministic block reduction of mll
   __shared__ double red[256];
@@ -343,89 +349,99 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) a.mllpart[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) a.mllpart[lb] = red[0];
 }
 
 constexpr int kRowsPerReduce = 16;
 
 __global__ __launch_bounds__(256) void k_hist_reduce1(
-    const unsigned long long *hpart, int rows, int KL,
-    unsigned long long *hpart2) {
+    const unsigned long long *__restrict__ hpart, int rows, int KL,
+    unsigned long long *__restrict__ hpart2) {
   const int kl = blockIdx.x * 256 + threadIdx.x;
   if (kl >= KL) return;
   const int r0 = blockIdx.y * kRowsPerReduce;
-  const int r1 = min(rows, r0 + kRowsPerReduce);
   unsigned long long s = 0;
-  for (int r = r0; r < r1; ++r) s += hpart[(long long)r * KL + kl];
+#pragma unroll
+  for (int r = 0; r < kRowsPerReduce; ++r)
+    if (r0 + r < rows) s += hpart[(long long)(r0 + r) * KL + kl];
   hpart2[(long long)blockIdx.y * KL + kl] = s;
 }
 
+// 64 entries per block, four row subsets per entry summed in parallel and
+// combined in a fixed order (integer sums: exact).
 __global__ __launch_bounds__(256) void k_hist_reduce2(
-    const unsigned long long *hpart2, int rows2, int KL, double *Hd) {
-  const int kl = blockIdx.x * 256 + threadIdx.x;
-  if (kl >= KL) return;
+    const unsigned long long *__restrict__ hpart2, int rows2, int KL,
+    double *__restrict__ Hd) {
+  __shared__ unsigned long long part[4][64];
+  const int e = threadIdx.x & 63, sub = threadIdx.x >> 6;
+  const int kl = blockIdx.x * 64 + e;
   unsigned long long s = 0;
-  for (int r = 0; r < rows2; ++r) s += hpart2[(long long)r * KL + kl];
-  Hd[kl] = (double)s * (1.0 / kHistScale);
+  if (kl < KL) {
+#pragma unroll 8
+    for (int r = sub; r < rows2; r += 4) s += hpart2[(long long)r * KL + kl];
+  }
+  part[sub][e] = s;
+  __syncthreads();
+  if (sub == 0 && kl < KL)
+    Hd[kl] = (double)(part[0][e] + part[1][e] + part[2][e] + part[3][e]) *
+             (1.0 / kHistScale);
 }
 
-// Likelihood-parameter step in one block: sum of the mll partials, then the
-// MAP update (H + alpha - 1)/row sum, or a Dirichlet(H + alpha) draw, or no
-// change; then the Dirichlet prior cost and the transposed FP32 table.
+// Likelihood-parameter step in one block, working in LDS: sum of the mll
+// partials, then the MAP update (H + alpha - 1)/row sum, or a
+// Dirichlet(H + alpha) draw, or no change; then the Dirichlet prior cost and
+// the transposed FP32 table.
 __global__ __launch_bounds__(1024) void k_theta(ThetaArgs a) {
   const int KL = a.K * a.L;
   const int tid = threadIdx.x;
   __shared__ double red[1024];
+  __shared__ double sh[4096];   // H, then theta
+  __shared__ double rowc[256];  // per-row prior cost / row sums
   double s = 0.0;
   for (int i = tid; i < a.nmll; i += 1024) s += a.mllpart[i];
   red[tid] = s;
+  for (int kl = tid; kl < KL; kl += 1024) {
+    const double h = a.Hd[kl];
+    if (a.mode == kThetaDirichlet)
+      sh[kl] = gamma_mt(h + a.alpha, u32x4{(uint32_t)kl, a.sweep, 0u, kStreamTheta},
+                        a.k0, a.k1);
+    else if (a.mode == kThetaMap)
+      sh[kl] = h + a.alpha - 1.0;
+    else
+      sh[kl] = a.theta[kl];
+  }
   __syncthreads();
   for (int w = 512; w > 0; w >>= 1) {
     if (tid < w) red[tid] += red[tid + w];
     __syncthreads();
   }
-  const double mll = red[0];
-  __syncthreads();
-
-  if (a.mode == kThetaDirichlet) {
-    for (int kl = tid; kl < KL; kl += 1024)
-      a.rowbuf[kl] = gamma_mt(a.Hd[kl] + a.alpha,
-                              u32x4{(uint32_t)kl, a.sweep, 0u, kStreamTheta},
-                              a.k0, a.k1);
+  if (a.mode != kThetaKeep) {
+    // row sums in index order (MILikelihood.cpp:74-79)
+    for (int k = tid; k < a.K; k += 1024) {
+      double sum = 0.0;
+      for (int l = 0; l < a.L; ++l) sum += sh[k * a.L + l];
+      rowc[k] = sum;
+    }
     __syncthreads();
-    __threadfence_block();
-    for (int k = tid; k < a.K; k += 1024) {
-      double sum = 0.0;
-      for (int l = 0; l < a.L; ++l) sum += a.rowbuf[k * a.L + l];
-      for (int l = 0; l < a.L; ++l) a.theta[k * a.L + l] = a.rowbuf[k * a.L + l] / sum;
-    }
-  } else if (a.mode == kThetaMap) {
-    for (int k = tid; k < a.K; k += 1024) {
-      double sum = 0.0;
-      for (int l = 0; l < a.L; ++l) sum += (a.Hd[k * a.L + l] + a.alpha - 1.0);
-      for (int l = 0; l < a.L; ++l)
-        a.theta[k * a.L + l] = (a.Hd[k * a.L + l] + a.alpha - 1.0) / sum;
-    }
+    for (int kl = tid; kl < KL; kl += 1024) sh[kl] = sh[kl] / rowc[kl / a.L];
+    __syncthreads();
   }
-  __syncthreads();
-  __threadfence_block();
-  // -log Dir(theta_k; alpha) per row, summed in row order
+  // -log Dir(theta_k; alpha) per row, then summed in row order
   for (int k = tid; k < a.K; k += 1024) {
     double lp = a.lg_sum_alpha;
     for (int l = 0; l < a.L; ++l) lp -= a.lg_alpha;
-    for (int l = 0; l < a.L; ++l) lp += (a.alpha - 1.0) * log(a.theta[k * a.L + l]);
-    a.rowbuf[KL + k] = -lp;
+    for (int l = 0; l < a.L; ++l) lp += (a.alpha - 1.0) * log(sh[k * a.L + l]);
+    rowc[k] = -lp;
   }
   for (int kl = tid; kl < KL; kl += 1024) {
-    const int k = kl / a.L, l = kl % a.L;
-    a.thT[l * a.K + k] = (float)a.theta[kl];
+    a.theta[kl] = sh[kl];
+    a.thT[(kl % a.L) * a.K + kl / a.L] = (float)sh[kl];
   }
   __syncthreads();
-  __threadfence_block();
   if (tid == 0) {
     double c = 0.0;
-    for (int k = 0; k < a.K; ++k) c += a.rowbuf[KL + k];
-    a.stats[0] = mll;
+    for (int k = 0; k < a.K; ++k) c += rowc[k];
+    a.stats[0] = red[0];
     a.stats[1] = c;
   }
 }
@@ -474,7 +490,7 @@ void launch_hist_reduce(const unsigned long long *hpart, int rows, int KL,
   hipLaunchKernelGGL(k_hist_reduce1, dim3((KL + 255) / 256, rows2), dim3(256),
                      0, s, hpart, rows, KL, hpart2);
   HIP_LAUNCH_OK();
-  hipLaunchKernelGGL(k_hist_reduce2, dim3((KL + 255) / 256), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_hist_reduce2, dim3((KL + 63) / 64), dim3(256), 0, s,
                      hpart2, rows2, KL, Hd);
   HIP_LAUNCH_OK();
 }

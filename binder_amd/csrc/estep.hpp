@@ -5,9 +5,10 @@
 namespace binder {
 
 // Fixed-point scale of the LDS/partial histograms: posteriors are added as
-// round(w * 2^32) into 64-bit integers, so block and grid reductions are
-// exact and the histogram is independent of scheduling.
-constexpr double kHistScale = 4294967296.0;
+// round(w * 2^31) (one 32-bit conversion, w <= 1) into 64-bit integers, so
+// block and grid reductions are exact and the histogram is independent of
+// scheduling.
+constexpr double kHistScale = 2147483648.0;
 
 struct EStepArgs {
   const float *d;           // [N][nv] current deformed positions (SoA)

@@ -93,7 +93,9 @@ __device__ __forceinline__ void normals4(u32x4 r, float n[4]) {
   n[3] = b * s2;
 }
 
-template <bool VEC, bool PRO, int EPI>
+// KS = number of 32-wide k-steps (compile time, so every fragment loop is
+// straight-line code and the B loads of a whole output tile issue together).
+template <int KS, bool VEC, bool PRO, int EPI>
 __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int comp = blockIdx.y;
@@ -106,10 +108,9 @@ __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
   const long long arow = row0 + (lane & 15);
   const bool rowok = arow < a.R;
   const int kq = 8 * (lane >> 4);
-  bf16x8_t ahi[kMaxKSteps], alo[kMaxKSteps];
+  bf16x8_t ahi[KS], alo[KS];
 #pragma unroll
-  for (int s = 0; s < kMaxKSteps; ++s) {
-    if (s >= a.ks) break;
+  for (int s = 0; s < KS; ++s) {
     float v[8];
     const int k0 = s * 32 + kq;
     if (VEC && rowok && k0 + 8 <= a.n_in) {
@@ -147,24 +148,59 @@ __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
   double cost = 0.0, raw = 0.0;
   const int nt_count = (a.n_out + 15) >> 4;
   const long long orow = row0 + 4 * (lane >> 4);  // first of this lane's 4 rows
-  for (int nt = 0; nt < nt_count; ++nt) {
-    f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-    const uint4 *bh = reinterpret_cast<const uint4 *>(a.bhi) + (long long)nt * a.ks * 64 + lane;
-    const uint4 *bl = reinterpret_cast<const uint4 *>(a.blo) + (long long)nt * a.ks * 64 + lane;
+  const uint4 *bh = reinterpret_cast<const uint4 *>(a.bhi) + lane;
+  const uint4 *bl = reinterpret_cast<const uint4 *>(a.blo) + lane;
+  // B fragments of the current output tile; the next tile's are prefetched
+  // while the current tile's MFMAs run
+  uint4 ch[KS], cl[KS];
 #pragma unroll
-    for (int s = 0; s < kMaxKSteps; ++s) {
-      if (s >= a.ks) break;
-      const bf16x8_t bhi = __builtin_bit_cast(bf16x8_t, bh[s * 64]);
-      const bf16x8_t blo = __builtin_bit_cast(bf16x8_t, bl[s * 64]);
+  for (int s = 0; s < KS; ++s) {
+    ch[s] = bh[s * 64];
+    cl[s] = bl[s * 64];
+  }
+  for (int nt = 0; nt < nt_count; ++nt) {
+    uint4 nh[KS], nl[KS];
+    const int nn = (nt + 1 < nt_count) ? nt + 1 : nt;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      nh[s] = bh[((long long)nn * KS + s) * 64];
+      nl[s] = bl[((long long)nn * KS + s) * 64];
+    }
+    // epilogue operands of this tile (D or x), loaded before the MFMAs
+    const int kc = nt * 16 + (lane & 15);
+    const long long obase = (long long)kc * a.R + orow;
+    const bool colok = kc < a.n_out;
+    float4 epv = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (EPI == EPI_FILTER || (EPI == EPI_ADDX && a.xepi)) {
+      const float *src = EPI == EPI_FILTER ? a.D : a.xepi + comp * a.cout;
+      if (VEC && colok && orow + 3 < a.R) {
+        epv = *reinterpret_cast<const float4 *>(src + obase);
+      } else if (colok) {
+        float t4[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (orow + i < a.R) t4[i] = src[obase + i];
+        epv = make_float4(t4[0], t4[1], t4[2], t4[3]);
+      }
+    }
+    f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const bf16x8_t bhi = __builtin_bit_cast(bf16x8_t, ch[s]);
+      const bf16x8_t blo = __builtin_bit_cast(bf16x8_t, cl[s]);
       acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[s], bhi, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[s], blo, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[s], bhi, acc, 0, 0, 0);
     }
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      ch[s] = nh[s];
+      cl[s] = nl[s];
+    }
     // lane holds rows orow..orow+3 of output column kc
-    const int kc = nt * 16 + (lane & 15);
-    if (kc >= a.n_out) continue;
+    if (!colok) continue;
     float v[4] = {acc[0], acc[1], acc[2], acc[3]};
-    const long long obase = (long long)kc * a.R + orow;
+    const float ep[4] = {epv.x, epv.y, epv.z, epv.w};
     if (EPI == EPI_FILTER) {
       float eps[4] = {0.f, 0.f, 0.f, 0.f};
       if (a.noise) {
@@ -178,7 +214,7 @@ __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         if (orow + i >= a.R) break;
-        const float Dv = a.D[obase + i];
+        const float Dv = ep[i];
         const float den = 1.0f + gs2 * Dv;
         float p = v[i] * (1.0f / den);
         if (a.noise) p += eps[i] * sqrtf(a.sigma2 / den);
@@ -189,10 +225,8 @@ __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
       }
     } else if (EPI == EPI_ADDX) {
       if (a.xepi) {
-        const float *xo = a.xepi + comp * a.cout;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-          if (orow + i < a.R) v[i] += xo[obase + i];
+        for (int i = 0; i < 4; ++i) v[i] += ep[i];
       }
     }
     if (VEC && orow + 3 < a.R) {
@@ -224,6 +258,169 @@ __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
     }
   }
 }
+
+// Persistent variant for contraction lengths up to 192: one 1024-thread block
+// per CU holds the whole pass matrix (bf16 hi/lo, MFMA order) in LDS and its
+// 16 wavefronts stream 16-row tiles; the many wavefronts per CU hide the HBM
+// latency of the tile loads.
+constexpr int kPersistThreads = 1024;
+constexpr size_t kPersistMaxLds = 144 * 1024;
+
+template <int KS, bool VEC, bool PRO, int EPI>
+__global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
+                                                                 int comps) {
+  extern __shared__ __align__(16) uint4 bsh[];  // [nt][KS][64] hi, then lo
+  const int nt_count = (a.n_out + 15) >> 4;
+  const int nfrag = nt_count * KS * 64;
+  {
+    const uint4 *gh = reinterpret_cast<const uint4 *>(a.bhi);
+    const uint4 *gl = reinterpret_cast<const uint4 *>(a.blo);
+    for (int i = threadIdx.x; i < nfrag; i += blockDim.x) {
+      bsh[i] = gh[i];
+      bsh[nfrag + i] = gl[i];
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int waves_per_block = blockDim.x >> 6;
+  const long long tiles_per_comp = (a.R + 15) >> 4;
+  const long long total = tiles_per_comp * comps;
+  const int kq = 8 * (lane >> 4);
+  double cost = 0.0, raw = 0.0;
+  for (long long tile = (long long)blockIdx.x * waves_per_block + wave; tile < total;
+       tile += (long long)gridDim.x * waves_per_block) {
+    const int comp = (int)(tile / tiles_per_comp);
+    const long long row0 = (tile % tiles_per_comp) * 16;
+    const float *in = a.in + comp * a.cin;
+    const float *xp = PRO ? a.xpro + comp * a.cin : nullptr;
+    float *out = a.out + comp * a.cout;
+    const long long arow = row0 + (lane & 15);
+    const bool rowok = arow < a.R;
+    bf16x8_t ahi[KS], alo[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      float v[8];
+      const int k0 = s * 32 + kq;
+      if (VEC && rowok && k0 + 8 <= a.n_in) {
+        const float4 *p = reinterpret_cast<const float4 *>(in + arow * a.n_in + k0);
+        const float4 x0 = p[0], x1 = p[1];
+        v[0] = x0.x; v[1] = x0.y; v[2] = x0.z; v[3] = x0.w;
+        v[4] = x1.x; v[5] = x1.y; v[6] = x1.z; v[7] = x1.w;
+        if (PRO) {
+          const float4 *q = reinterpret_cast<const float4 *>(xp + arow * a.n_in + k0);
+          const float4 y0 = q[0], y1 = q[1];
+          v[0] -= y0.x; v[1] -= y0.y; v[2] -= y0.z; v[3] -= y0.w;
+          v[4] -= y1.x; v[5] -= y1.y; v[6] -= y1.z; v[7] -= y1.w;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int k = k0 + j;
+          float t = 0.f;
+          if (rowok && k < a.n_in) {
+            t = in[arow * a.n_in + k];
+            if (PRO) t -= xp[arow * a.n_in + k];
+          }
+          v[j] = t;
+        }
+      }
+      split8(v, ahi[s], alo[s]);
+    }
+    double g = 0.0;
+    float gs2 = 0.f;
+    if (EPI == EPI_FILTER) {
+      g = a.gamma[comp];
+      gs2 = (float)g * a.sigma2;
+    }
+    const long long orow = row0 + 4 * (lane >> 4);
+    for (int nt = 0; nt < nt_count; ++nt) {
+      const int kc = nt * 16 + (lane & 15);
+      const long long obase = (long long)kc * a.R + orow;
+      const bool colok = kc < a.n_out;
+      float4 epv = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (EPI == EPI_FILTER || (EPI == EPI_ADDX && a.xepi)) {
+        const float *src = EPI == EPI_FILTER ? a.D : a.xepi + comp * a.cout;
+        if (VEC && colok && orow + 3 < a.R) {
+          epv = *reinterpret_cast<const float4 *>(src + obase);
+        } else if (colok) {
+          float t4[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            if (orow + i < a.R) t4[i] = src[obase + i];
+          epv = make_float4(t4[0], t4[1], t4[2], t4[3]);
+        }
+      }
+      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+      const uint4 *bt = bsh + (nt * KS) * 64 + lane;
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const bf16x8_t bhi = __builtin_bit_cast(bf16x8_t, bt[s * 64]);
+        const bf16x8_t blo = __builtin_bit_cast(bf16x8_t, bt[nfrag + s * 64]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[s], bhi, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[s], blo, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[s], bhi, acc, 0, 0, 0);
+      }
+      if (!colok) continue;
+      float v[4] = {acc[0], acc[1], acc[2], acc[3]};
+      const float ep[4] = {epv.x, epv.y, epv.z, epv.w};
+      if (EPI == EPI_FILTER) {
+        float eps[4] = {0.f, 0.f, 0.f, 0.f};
+        if (a.noise) {
+          const u32x4 r = philox4x32_10(
+              u32x4{(uint32_t)orow, (uint32_t)kc, a.sweep,
+                    ((uint32_t)comp << 16) | a.stream},
+              a.k0, a.k1);
+          normals4(r, eps);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (orow + i >= a.R) break;
+          const float Dv = ep[i];
+          const float den = 1.0f + gs2 * Dv;
+          float p = v[i] * (1.0f / den);
+          if (a.noise) p += eps[i] * sqrtf(a.sigma2 / den);
+          const double pd = (double)p;
+          cost += 0.5 * g * (double)Dv * pd * pd;
+          if (a.raw) raw += a.etarho[comp] * (double)Dv * pd * pd;
+          v[i] = p;
+        }
+      } else if (EPI == EPI_ADDX) {
+        if (a.xepi) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) v[i] += ep[i];
+        }
+      }
+      if (VEC && orow + 3 < a.R) {
+        *reinterpret_cast<float4 *>(out + obase) = make_float4(v[0], v[1], v[2], v[3]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (orow + i < a.R) out[obase + i] = v[i];
+      }
+    }
+  }
+  if (EPI == EPI_FILTER) {
+    __syncthreads();
+    double *red = reinterpret_cast<double *>(bsh);  // matrix no longer needed
+    const int t = threadIdx.x;
+    red[t] = cost;
+    red[kPersistThreads + t] = raw;
+    __syncthreads();
+    for (int st = kPersistThreads / 2; st > 0; st >>= 1) {
+      if (t < st) {
+        red[t] += red[t + st];
+        red[kPersistThreads + t] += red[kPersistThreads + t + st];
+      }
+      __syncthreads();
+    }
+    if (t == 0) {
+      a.cost_part[blockIdx.x] = red[0];
+      if (a.raw) a.raw_part[blockIdx.x] = red[kPersistThreads];
+    }
+  }
+}
+
+int g_num_cus = 0;
 
 // Deterministic sum of `count` partials (fixed assignment and tree order).
 __global__ __launch_bounds__(256) void k_sum_partials(const double *part,
@@ -257,9 +454,48 @@ __global__ void k_draw_nu(const double *raw, double *gamma, int N, double nv,
   gamma[2] = nu * rho_prod * e2 * e2 * nv;
 }
 
+template <int KS, bool VEC, bool PRO, int EPI>
+void launch_ks(dim3 grid, hipStream_t s, const PassArgs &a) {
+  hipLaunchKernelGGL((k_contract<KS, VEC, PRO, EPI>), grid, dim3(256), 0, s, a);
+}
+
+template <int KS, bool VEC, bool PRO, int EPI>
+void launch_persist(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs &a) {
+  auto k = k_contract_lds<KS, VEC, PRO, EPI>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIP_OK(hipFuncSetAttribute((const void *)k,
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kPersistMaxLds));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(kPersistThreads), lds, s, a, comps);
+}
+
+template <bool VEC, bool PRO, int EPI>
+void launch_lds(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs &a) {
+  switch (a.ks) {
+    case 1: launch_persist<1, VEC, PRO, EPI>(blocks, lds, comps, s, a); break;
+    case 2: launch_persist<2, VEC, PRO, EPI>(blocks, lds, comps, s, a); break;
+    case 3: launch_persist<3, VEC, PRO, EPI>(blocks, lds, comps, s, a); break;
+    case 4: launch_persist<4, VEC, PRO, EPI>(blocks, lds, comps, s, a); break;
+    case 5: launch_persist<5, VEC, PRO, EPI>(blocks, lds, comps, s, a); break;
+    default: launch_persist<6, VEC, PRO, EPI>(blocks, lds, comps, s, a); break;
+  }
+}
+
 template <bool VEC, bool PRO, int EPI>
 void launch(dim3 grid, hipStream_t s, const PassArgs &a) {
-  hipLaunchKernelGGL((k_contract<VEC, PRO, EPI>), grid, dim3(256), 0, s, a);
+  switch (a.ks) {
+    case 1: launch_ks<1, VEC, PRO, EPI>(grid, s, a); break;
+    case 2: launch_ks<2, VEC, PRO, EPI>(grid, s, a); break;
+    case 3: launch_ks<3, VEC, PRO, EPI>(grid, s, a); break;
+    case 4: launch_ks<4, VEC, PRO, EPI>(grid, s, a); break;
+    case 5: launch_ks<5, VEC, PRO, EPI>(grid, s, a); break;
+    case 6: launch_ks<6, VEC, PRO, EPI>(grid, s, a); break;
+    case 7: launch_ks<7, VEC, PRO, EPI>(grid, s, a); break;
+    default: launch_ks<8, VEC, PRO, EPI>(grid, s, a); break;
+  }
 }
 
 uint16_t host_bf16(float x) {
@@ -333,10 +569,10 @@ const PassMatrix &MatrixCache::get(int kind, int n_in, int n_out,
   return ref;
 }
 
-void contract_pass(hipStream_t stream, const PassMatrix &m, const float *in,
-                   float *out, int comps, long long cin, long long cout,
-                   long long R, const float *xpro, const float *xepi, int epi,
-                   const FilterArgs *f, KernelTimer *timer, const char *name) {
+long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in,
+                        float *out, int comps, long long cin, long long cout,
+                        long long R, const float *xpro, const float *xepi, int epi,
+                        const FilterArgs *f, KernelTimer *timer, const char *name) {
   BINDER_REQUIRE(m.n_in <= 32 * kMaxKSteps, "contraction length above 256");
   PassArgs a{};
   a.in = in;
@@ -370,6 +606,28 @@ void contract_pass(hipStream_t stream, const PassMatrix &m, const float *in,
   dim3 grid((unsigned)((R + kRowsPerBlock - 1) / kRowsPerBlock), comps);
   TimedLaunch tl(timer, name, stream);
   const bool pro = xpro != nullptr;
+  const size_t mat_lds = (size_t)((m.n_out + 15) / 16) * a.ks * 64 * 16 * 2;
+  // the filter epilogue reuses the matrix area for a 2 x 1024 double reduction
+  const size_t lds = mat_lds > 16384 ? mat_lds : 16384;
+  const long long tiles = ((R + 15) / 16) * comps;
+  if (a.ks <= 6 && mat_lds <= kPersistMaxLds && tiles >= 2048) {
+    const int blocks = persistent_blocks();
+#define BINDER_LAUNCH_LDS(V)                                                         \
+  if (pro) {                                                                        \
+    if (epi == EPI_FILTER) launch_lds<V, true, EPI_FILTER>(blocks, lds, comps, stream, a); \
+    else launch_lds<V, true, EPI_STORE>(blocks, lds, comps, stream, a);             \
+  } else if (epi == EPI_FILTER) launch_lds<V, false, EPI_FILTER>(blocks, lds, comps, stream, a); \
+  else if (epi == EPI_ADDX) launch_lds<V, false, EPI_ADDX>(blocks, lds, comps, stream, a); \
+  else launch_lds<V, false, EPI_STORE>(blocks, lds, comps, stream, a);
+    if (vec) {
+      BINDER_LAUNCH_LDS(true)
+    } else {
+      BINDER_LAUNCH_LDS(false)
+    }
+#undef BINDER_LAUNCH_LDS
+    HIP_LAUNCH_OK();
+    return blocks;
+  }
 #define BINDER_LAUNCH(V)                                               \
   if (pro) {                                                           \
     if (epi == EPI_FILTER) launch<V, true, EPI_FILTER>(grid, stream, a); \
@@ -384,10 +642,23 @@ void contract_pass(hipStream_t stream, const PassMatrix &m, const float *in,
   }
 #undef BINDER_LAUNCH
   HIP_LAUNCH_OK();
+  return (long long)grid.x * grid.y;
+}
+
+int persistent_blocks() {
+  if (g_num_cus == 0) {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    HIP_OK(hipDeviceGetAttribute(&g_num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (g_num_cus <= 0) g_num_cus = 256;
+  }
+  return g_num_cus;
 }
 
 long long contract_blocks(long long R, int comps) {
-  return ((R + kRowsPerBlock - 1) / kRowsPerBlock) * comps;
+  const long long nonpersist = ((R + kRowsPerBlock - 1) / kRowsPerBlock) * comps;
+  const long long p = persistent_blocks();
+  return nonpersist > p ? nonpersist : p;
 }
 
 Smoother::Smoother(int nx, int ny, int nz, const double gamma[3],
@@ -472,17 +743,19 @@ void Smoother::run(const float *dream, const float *x, float *out,
   float *bufs[2] = {bufA_.get(), bufB_.get()};
   int bi = 0;
   const float *src = dream;
+  long long nparts = 0;  // partial-sum slots written by the filter pass
   for (int i = 0; i < na; ++i) {
     const int n = dims[i];
     const PassMatrix &m = mats_.get(MatrixCache::kDctForward, n, n, stream_);
     const bool last = (i == na - 1);
-    contract_pass(stream_, m, src, bufs[bi], N_, nv_, nv_, nv_ / n,
-                  i == 0 ? x : nullptr, nullptr, last ? EPI_FILTER : EPI_STORE,
-                  last ? &f : nullptr, timer, fwd_names[i]);
+    const long long written =
+        contract_pass(stream_, m, src, bufs[bi], N_, nv_, nv_, nv_ / n,
+                      i == 0 ? x : nullptr, nullptr, last ? EPI_FILTER : EPI_STORE,
+                      last ? &f : nullptr, timer, fwd_names[i]);
+    if (last) nparts = written;
     src = bufs[bi];
     bi ^= 1;
   }
-  const long long nparts = contract_blocks(nv_ / dims[na - 1], N_);
   {
     TimedLaunch tl(timer, "sum_partials", stream_);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, stream_,

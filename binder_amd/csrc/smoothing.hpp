@@ -43,11 +43,14 @@ struct FilterArgs {
 
 // Out[c][k'][r] = sum_k In[c][r][k] M[k][k'] for r < R, c < comps.
 // epi: 0 store, 1 filter (needs f), 2 add xepi.  xpro: In - xpro prologue.
-void contract_pass(hipStream_t stream, const PassMatrix &m, const float *in,
-                   float *out, int comps, long long cin, long long cout,
-                   long long R, const float *xpro, const float *xepi, int epi,
-                   const FilterArgs *f, KernelTimer *timer, const char *name);
+// Returns the number of partial-sum slots written by a filter pass.
+long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in,
+                        float *out, int comps, long long cin, long long cout,
+                        long long R, const float *xpro, const float *xepi, int epi,
+                        const FilterArgs *f, KernelTimer *timer, const char *name);
+// Upper bound on the partial-sum slots a filter pass over R rows writes.
 long long contract_blocks(long long R, int comps);
+int persistent_blocks();
 
 void up_sample_field(const float *in_soa, int nx, int ny, int nz, int nxo,
                      int nyo, int nzo, float *out_soa, hipStream_t stream,

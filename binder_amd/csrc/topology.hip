@@ -89,9 +89,12 @@ __global__ __launch_bounds__(256) void k_jacobian3d(Field w, int Nx, int Ny,
                                                     double *part) {
   const long long nv = (long long)Nx * Ny * Nz;
   double mn = INFINITY, mx = -INFINITY, lo = 0, hi = 0;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
-       i += stride) {
+  // contiguous voxel range per logical block, logical blocks contiguous per
+  // XCD: the 7-point stencil re-reads stay in one L2
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const long long per = (nv + gridDim.x - 1) / gridDim.x;
+  const long long vend = ((long long)lb + 1) * per < nv ? ((long long)lb + 1) * per : nv;
+  for (long long i = (long long)lb * per + threadIdx.x; i < vend; i += blockDim.x) {
     const int z = (int)(i % Nz);
     const long long r = i / Nz;
     const int y = (int)(r % Ny), x = (int)(r / Ny);
@@ -120,7 +123,7 @@ __global__ __launch_bounds__(256) void k_jacobian3d(Field w, int Nx, int Ny,
     __syncthreads();
   }
   if (t == 0)
-    for (int k = 0; k < 4; ++k) part[blockIdx.x * 4 + k] = red[k][0];
+    for (int k = 0; k < 4; ++k) part[lb * 4 + k] = red[k][0];
 }
 
 // 2D Jacobian (utils.cpp:10-98): corners ff, bf, fb, bb.
