@@ -52,6 +52,13 @@ void orc_topology_3d(const double *h_init, double e1, double e2, int max_outer,
 void orc_corner_jacobians_3d(const double *field, int Nx, int Ny, int Nz,
                              const double *vs, double e1, double e2,
                              double *Jc_out, long long *counts);
+// One inner topology iteration's steps (physical-unit fields).
+void orc_topo_gradients(const double *w_phys, int Nx, int Ny, int Nz,
+                        const double *vs, double *g_out);
+void orc_topo_limit(double *g, const double *Jc, int Nx, int Ny, int Nz,
+                    double e1, double e2);
+void orc_topo_integrate(const double *g, const double *mean3, int Nx, int Ny,
+                        int Nz, const double *vs, double *w_out);
 #ifdef __cplusplus
 }
 #endif

@@ -393,4 +393,47 @@ void orc_corner_jacobians_3d(const double *field, int Nx, int Ny, int Nz,
   counts[1] = hi;
 }
 
+// The three steps of one inner iteration, exposed separately so that tests
+// can pin each one (fields here are in physical units, as inside the loop).
+static Topo make_topo(int Nx, int Ny, int Nz, const double *vs, double e1,
+                      double e2) {
+  Topo t;
+  t.Nx = Nx;
+  t.Ny = Ny;
+  t.Nz = Nz;
+  t.nv = (long long)Nx * Ny * Nz;
+  t.e1 = e1;
+  t.e2 = e2;
+  for (int k = 0; k < 3; ++k) t.vs[k] = vs[k];
+  return t;
+}
+
+void orc_topo_gradients(const double *w_phys, int Nx, int Ny, int Nz,
+                        const double *vs, double *g_out) {
+  Topo t = make_topo(Nx, Ny, Nz, vs, 0.0, 0.0);
+  t.w.assign(w_phys, w_phys + t.nv * 3);
+  std::vector<double> g;
+  t.gradients(g);
+  std::memcpy(g_out, g.data(), sizeof(double) * g.size());
+}
+
+void orc_topo_limit(double *g, const double *Jc, int Nx, int Ny, int Nz,
+                    double e1, double e2) {
+  const double ones[3] = {1, 1, 1};
+  Topo t = make_topo(Nx, Ny, Nz, ones, e1, e2);
+  std::vector<double> gv(g, g + t.nv * 9);
+  std::vector<double> jc(Jc, Jc + (long long)(Nx - 1) * (Ny - 1) * (Nz - 1) * 8);
+  t.limit(gv, jc);
+  std::memcpy(g, gv.data(), sizeof(double) * gv.size());
+}
+
+void orc_topo_integrate(const double *g, const double *mean3, int Nx, int Ny,
+                        int Nz, const double *vs, double *w_out) {
+  Topo t = make_topo(Nx, Ny, Nz, vs, 0.0, 0.0);
+  t.w.assign(t.nv * 3, 0.0);
+  std::vector<double> gv(g, g + t.nv * 9);
+  t.integrate(gv, mean3);
+  std::memcpy(w_out, t.w.data(), sizeof(double) * t.nv * 3);
+}
+
 }  // extern "C"

@@ -66,6 +66,9 @@ _jac3 = _sig("orc_jacobian_3d", None, _vp, _i, _i, _i, _d, _d, _d, _vp)
 _jac2 = _sig("orc_jacobian_2d", None, _vp, _i, _i, _d, _d, _vp)
 _topo = _sig("orc_topology_3d", None, _vp, _d, _d, _i, _i, _vp, _i, _i, _i, _vp, _vp)
 _corner = _sig("orc_corner_jacobians_3d", None, _vp, _i, _i, _i, _vp, _d, _d, _vp, _vp)
+_tgrad = _sig("orc_topo_gradients", None, _vp, _i, _i, _i, _vp, _vp)
+_tlimit = _sig("orc_topo_limit", None, _vp, _vp, _i, _i, _i, _d, _d)
+_tint = _sig("orc_topo_integrate", None, _vp, _vp, _i, _i, _i, _vp, _vp)
 _spline = _sig("orc_spline", None, _i, _d, _vp)
 _dct3 = _sig("orc_dct3", None, _vp, _i, _i, _i, _i, _i)
 
@@ -242,6 +245,28 @@ def corner_jacobians_3D(field, Nx, Ny, Nz, voxel_sizes, e1, e2):
     cnt = np.zeros(2, np.int64)
     _corner(_p(f), Nx, Ny, Nz, _p(vs), float(e1), float(e2), _p(Jc), _p(cnt))
     return Jc, cnt
+
+
+def topo_gradients(w_phys, shape, voxel_sizes):
+    """Periodic forward differences [Nx,Ny,Nz,3(component),3(axis)] / vs[component]."""
+    w = _a(w_phys)
+    g = np.empty(tuple(shape) + (3, 3))
+    _tgrad(_p(w), *map(int, shape), _p(_a(voxel_sizes)), _p(g))
+    return g
+
+
+def topo_limit(g, Jc, shape, e1, e2):
+    """Sequential in-place Jacobian limiting of gradient field g (copy returned)."""
+    out = _a(g).copy()
+    _tlimit(_p(out), _p(_a(Jc)), *map(int, shape), float(e1), float(e2))
+    return out
+
+
+def topo_integrate(g, mean3, shape, voxel_sizes):
+    """Periodic least-squares integration of g, scaled by vs, plus mean3."""
+    out = np.empty(tuple(shape) + (3,))
+    _tint(_p(_a(g)), _p(_a(mean3)), *map(int, shape), _p(_a(voxel_sizes)), _p(out))
+    return out
 
 
 def spline(order, d):

@@ -1,0 +1,150 @@
+"""Statistical parity of the Gibbs sampler (BINDER/src/cpu/Registration.cpp:
+515-642, MILikelihood.cpp:132-169, NonLinearTransformation.cpp:79-126).
+The random streams differ from the reference by design (Philox instead of
+mt19937), so draws are checked against the exact distributions and against
+independent replicate chains of the FP64 oracle within Monte Carlo error."""
+import numpy as np
+import pytest
+from scipy import stats
+
+from binder_amd import synth
+from binder_amd.utils import initialize_fourier_domain_smoothing
+
+
+def _small_problem(shape=(8, 7, 6), K=6, L=5, seed=4, gamma_per_voxel=0.05):
+    rng = np.random.default_rng(seed)
+    pos = np.stack(np.indices(shape, dtype=np.float64), -1)
+    d = pos + rng.normal(0, 0.8, pos.shape)
+    bn = rng.integers(0, K, shape).astype(np.int32)
+    bv = rng.integers(0, L, shape).astype(np.int32)
+    theta = rng.random((K, L)) + 0.1
+    theta /= theta.sum(1, keepdims=True)
+    D = initialize_fourier_domain_smoothing(list(shape), True, 2, 3).reshape(shape)
+    g = gamma_per_voxel * np.prod(shape)
+    return synth.Problem(shape=shape, final_locations=d, voxel_pos=pos, binned_nodes=bn,
+                         binned_voxels=bv, theta=theta, D=D, gammas=(g, g, g), K=K, L=L,
+                         alpha=2.0, nnz=int(np.prod(shape)))
+
+
+def _b3(t):
+    return np.array([(1 - t) ** 3 / 6, 2 / 3 - t ** 2 + t ** 3 / 2,
+                     2 / 3 - (1 - t) ** 2 + (1 - t) ** 3 / 2, t ** 3 / 6])
+
+
+def _posterior(prob, idx):
+    """Exact 64-node posterior of voxel idx: cubic B-spline weight x
+    theta[moving bin, fixed bin], nodes outside the grid in moving bin 0."""
+    p = prob.final_locations[idx]
+    base = np.floor(p).astype(int)
+    w = [_b3(p[a] - base[a]) for a in range(3)]
+    nodes, probs = [], []
+    for off in np.ndindex(4, 4, 4):
+        node = base + np.array(off) - 1
+        inside = all(0 <= node[a] < prob.shape[a] for a in range(3))
+        k = prob.binned_nodes[tuple(node)] if inside else 0
+        probs.append(w[0][off[0]] * w[1][off[1]] * w[2][off[2]] *
+                     prob.theta[k, prob.binned_voxels[idx]])
+        nodes.append(tuple(int(v) for v in node))
+    probs = np.array(probs)
+    return nodes, probs / probs.sum()
+
+
+def _check_estep_draws(impl, S):
+    prob = _small_problem()
+    r = synth.build_registration(impl, prob)
+    voxels = [(1, 2, 3), (4, 4, 2), (6, 1, 5), (0, 0, 0)]
+    counts = [dict() for _ in voxels]
+    for s in range(S):
+        # only the categorical node draw: the deformation stays fixed
+        r.sampler(1000 + s, 0, 2, 0, 1, 1, 0, 0)
+        dream = np.asarray(r.get_dream_locations()).reshape(prob.final_locations.shape)
+        for c, v in zip(counts, voxels):
+            key = tuple(int(round(x)) for x in dream[v])
+            c[key] = c.get(key, 0) + 1
+    for c, v in zip(counts, voxels):
+        nodes, probs = _posterior(prob, v)
+        obs = np.array([c.get(n, 0) for n in nodes], float)
+        assert obs.sum() == S, (v, c)  # every draw is one of the 64 neighbours
+        order = np.argsort(probs)[::-1]
+        o, e = obs[order], probs[order] * S
+        # pool the tail cells so every expected count is >= 5
+        k = max(1, min(len(e) - 1, np.count_nonzero(e >= 5)))
+        o = np.append(o[:k], o[k:].sum())
+        e = np.append(e[:k], e[k:].sum())
+        chi2 = float(np.sum((o - e) ** 2 / np.maximum(e, 1e-300)))
+        assert stats.chi2.sf(chi2, len(o) - 1) > 1e-4, (v, chi2, len(o))
+
+
+@pytest.mark.gpu
+def test_estep_draw_frequencies_gpu(core):
+    """HIP categorical node draws follow the exact posterior (chi-square)."""
+    print("E-step draw frequencies (HIP)", flush=True)
+    _check_estep_draws(core, 3000)
+
+
+def test_estep_draw_frequencies_oracle(orc):
+    """The oracle's node draws follow the exact posterior (chi-square)."""
+    _check_estep_draws(orc, 2000)
+
+
+@pytest.mark.gpu
+def test_dirichlet_draw_moments(core):
+    """theta_k ~ Dir(H_k + alpha): mean and variance of repeated draws."""
+    print("Dirichlet moments", flush=True)
+    prob = _small_problem()
+    g = synth.build_registration(core, prob)
+    _, H = g.e_step(return_histogram=True)
+    a = H + prob.alpha
+    S = 4000
+    draws = np.empty((S, prob.K, prob.L))
+    for s in range(S):
+        g.sampler_run(77 + s, 0, 1, 0, 1, 0, 1, 0)
+        draws[s] = np.asarray(g.get_likelihood_parameters()).reshape(prob.K, prob.L)
+    assert np.allclose(draws.sum(2), 1.0, atol=1e-9)
+    a0 = a.sum(1, keepdims=True)
+    mean = a / a0
+    var = a * (a0 - a) / (a0 ** 2 * (a0 + 1))
+    z = (draws.mean(0) - mean) / np.sqrt(var / S)
+    assert np.abs(z).max() < 5.0, np.abs(z).max()
+    assert np.allclose(draws.var(0), var, rtol=0.15)
+
+
+def _replicates(impl, prob, R, T, sample_gamma, seed0):
+    means, vars_ = [], []
+    for r in range(R):
+        reg = synth.build_registration(impl, prob)
+        m, c = reg.sampler(seed0 + r, 2, T, sample_gamma, 2, 1, 1, 1)
+        means.append(m)
+        vars_.append(np.diagonal(c, axis1=-2, axis2=-1))
+    return np.array(means), np.array(vars_)
+
+
+def _two_sample_z(a, b):
+    se = np.sqrt(a.var(0, ddof=1) / len(a) + b.var(0, ddof=1) / len(b))
+    return (a.mean(0) - b.mean(0)) / np.maximum(se, 1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sample_gamma", [0, 1])
+def test_chain_matches_oracle_within_mc_error(core, orc, sample_gamma):
+    """Independent replicate chains (node draw, Dirichlet draw, DCT-domain
+    draw, optionally the nu draw) from the same state: per-voxel posterior
+    mean and variance of the HIP path and of the oracle agree within the
+    two-sample Monte Carlo error."""
+    print(f"chain parity sample_gamma={sample_gamma}", flush=True)
+    prob = _small_problem(shape=(10, 9, 8), seed=11)
+    R, T = 64, 12
+    gm, gv = _replicates(core, prob, R, T, sample_gamma, 500)
+    om, ov = _replicates(orc, prob, R, T, sample_gamma, 900)
+    for name, z in (("mean", _two_sample_z(gm, om)), ("var", _two_sample_z(gv, ov))):
+        # ~2000 statistics with ~126 degrees of freedom each
+        assert np.mean(np.abs(z) > 4) < 0.005 and np.abs(z).max() < 6.5, \
+            (name, np.abs(z).max(), np.mean(np.abs(z) > 4))
+    # pooled functionals per replicate (mean variance, mean squared and mean
+    # displacement): a 30% error in the prior precision moves the first by
+    # about 7 standard errors at this size
+    def pooled(m, v):
+        disp = (m - prob.voxel_pos).reshape(len(m), -1)
+        return np.stack([v.reshape(len(v), -1).mean(1), (disp ** 2).mean(1), disp.mean(1)], 1)
+    zp = _two_sample_z(pooled(gm, gv), pooled(om, ov))
+    assert np.abs(zp).max() < 4.5, zp

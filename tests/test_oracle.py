@@ -382,3 +382,96 @@ def test_oracle_topology_invariants(orc):
     assert np.allclose(out.reshape(-1, 3).mean(0), f.reshape(-1, 3).mean(0), atol=1e-10)
     same = t.correct_topology(pos.ravel(), 0.01, 100, 10, 10, 1, np.ones(3), *shape)
     assert np.array_equal(same, pos) and t.last_stats[0] == 0
+
+
+# ---------------------------------------------------------------------------
+# topology correction inner steps (topology_correction_3D.cpp:172-449)
+# ---------------------------------------------------------------------------
+def test_topology_integrate_matches_fft_closed_form(orc):
+    """Periodic least-squares integration = numpy FFT of
+    sum_a conj(e^{iw_a}-1) g_a / sum_a (2 - 2 cos w_a), DC zero, x vs + mean."""
+    rng = np.random.default_rng(71)
+    shape = (6, 5, 7)
+    vs = np.array([1.3, 0.8, 1.1])
+    g = rng.standard_normal(shape + (3, 3))
+    m = np.array([3.0, -2.0, 0.5])
+    got = orc.topo_integrate(g, m, shape, vs)
+    w = [np.exp(2j * np.pi * np.fft.fftfreq(n)) for n in shape]
+    E = [w[0][:, None, None], w[1][None, :, None], w[2][None, None, :]]
+    den = sum(2 - 2 * np.real(e) for e in E)
+    den[0, 0, 0] = 1.0
+    for n in range(3):
+        U = sum(np.conj(E[a] - 1) * np.fft.fftn(g[..., n, a]) for a in range(3)) / den
+        U[0, 0, 0] = 0.0
+        exp = np.real(np.fft.ifftn(U)) * vs[n] + m[n]
+        assert np.allclose(got[..., n], exp, atol=1e-12)
+
+
+def test_topology_gradients_then_integrate_roundtrip(orc):
+    """Exact periodic gradients integrate back to the field (mean restored);
+    affine fields have constant forward differences A[n, a] / vs[n]."""
+    rng = np.random.default_rng(72)
+    shape = (7, 6, 5)
+    vs = np.array([0.9, 1.2, 1.05])
+    w = rng.standard_normal(shape + (3,)) * 4
+    g = orc.topo_gradients(w, shape, vs)
+    back = orc.topo_integrate(g, w.reshape(-1, 3).mean(0), shape, vs)
+    assert np.allclose(back, w, atol=1e-10)
+    A = rng.standard_normal((3, 3))
+    pos = np.stack(np.indices(shape, dtype=np.float64), -1)
+    g = orc.topo_gradients(pos @ A.T, shape, vs)
+    inner = g[:-1, :-1, :-1]
+    assert np.allclose(inner, (A / vs[:, None])[None, None, None], atol=1e-12)
+
+
+def _corner_columns(cell, c_vertex):
+    """Voxels holding the three edge differences of the corner at vertex
+    cell + c_vertex: the edge along axis a starts at the cell's lower face."""
+    locs = []
+    for a in range(3):
+        v = [cell[k] + c_vertex[k] for k in range(3)]
+        v[a] = cell[a]
+        locs.append(tuple(v))
+    return locs
+
+
+@pytest.mark.parametrize("fold", [False, True])
+def test_topology_limit_single_corner(orc, fold):
+    """Limiting one flagged corner rewrites exactly its three edge-difference
+    columns, whose determinant is that corner's Jacobian, as (1-a)J + aI with
+    a one 0.1 step past the first blend whose determinant is in [e1, e2]."""
+    rng = np.random.default_rng(73)
+    shape = (5, 4, 6)
+    vs = np.ones(3)
+    e1, e2 = 0.01, 100.0
+    pos = np.stack(np.indices(shape, dtype=np.float64), -1)
+    w = pos + rng.normal(0, 0.05, pos.shape)
+    cell = (2, 1, 3)
+    if fold:
+        w[3, 2, 4, 0] -= 1.6  # pull a vertex of the cell back past its neighbour
+    Jc, _ = orc.corner_jacobians_3D(w.ravel(), *shape, vs, e1, e2)
+    Jc = Jc.reshape(shape[0] - 1, shape[1] - 1, shape[2] - 1, 8)
+    g0 = orc.topo_gradients(w, shape, vs)
+    vertices = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (0, 1, 1), (1, 0, 1), (1, 1, 0),
+                (1, 1, 1)]
+    corners = [int(np.argmin(Jc[cell]))] if fold else range(8)
+    for c in corners:
+        jc = np.full_like(Jc, 1.0)
+        jc[cell + (c,)] = Jc[cell + (c,)] if fold else -1.0
+        g = orc.topo_limit(g0, jc.ravel(), shape, e1, e2)
+        locs = _corner_columns(cell, vertices[c])
+        M = np.stack([g0[locs[a]][:, a] for a in range(3)], 1)
+        assert np.isclose(np.linalg.det(M), Jc[cell + (c,)], rtol=1e-12, atol=1e-14)
+        expect = g0.copy()
+        alpha = 0.0
+        det = np.linalg.det(M)
+        while det < e1 or det > e2:
+            alpha = min(alpha + 0.1, 1.0)
+            det = np.linalg.det((1 - alpha) * M + alpha * np.eye(3))
+        alpha = min(alpha + 0.1, 1.0)
+        new = (1 - alpha) * M + alpha * np.eye(3)
+        for a in range(3):
+            expect[locs[a]][:, a] = new[:, a]
+        assert np.allclose(g, expect, atol=1e-14, rtol=0)
+        if fold:
+            assert alpha > 0.15 and e1 <= np.linalg.det(new) <= e2
