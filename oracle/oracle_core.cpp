@@ -6,7 +6,11 @@
 // __graft_entry__.smoke(), and the CPU baseline timed by bench.py.  The
 // product library (binder_amd) never loads it.
 //
-// Parity status of each function is listed in oracle/README.md.
+// Each function is pinned by tests/test_oracle.py: golden vectors and
+// fresh-input comparisons against the reference's own FFTW/Eigen-free sources
+// compiled into oracle/_ref (Jacobians, spline, MI histogram/theta update),
+// closed forms (DCT, smoothing and cost, up_sample, topology integration),
+// brute force on tiny grids (E-step), and distribution tests (draws).
 //
 // Build: oracle/Makefile (g++ -O2 -fopenmp -ffp-contract=off).
 #include <omp.h>

@@ -87,19 +87,16 @@ def test_estep_draw_frequencies_oracle(orc):
     _check_estep_draws(orc, 2000)
 
 
-@pytest.mark.gpu
-def test_dirichlet_draw_moments(core):
-    """theta_k ~ Dir(H_k + alpha): mean and variance of repeated draws."""
-    print("Dirichlet moments", flush=True)
+def _check_dirichlet(impl, S):
     prob = _small_problem()
-    g = synth.build_registration(core, prob)
-    _, H = g.e_step(return_histogram=True)
+    r = synth.build_registration(impl, prob)
+    _, H = r.e_step(return_histogram=True)
     a = H + prob.alpha
-    S = 4000
     draws = np.empty((S, prob.K, prob.L))
     for s in range(S):
-        g.sampler_run(77 + s, 0, 1, 0, 1, 0, 1, 0)
-        draws[s] = np.asarray(g.get_likelihood_parameters()).reshape(prob.K, prob.L)
+        # only the theta block is sampled, from the histogram of the E-step
+        r.sampler(77 + s, 0, 2, 0, 1, 0, 1, 0)
+        draws[s] = np.asarray(r.get_likelihood_parameters()).reshape(prob.K, prob.L)
     assert np.allclose(draws.sum(2), 1.0, atol=1e-9)
     a0 = a.sum(1, keepdims=True)
     mean = a / a0
@@ -107,6 +104,18 @@ def test_dirichlet_draw_moments(core):
     z = (draws.mean(0) - mean) / np.sqrt(var / S)
     assert np.abs(z).max() < 5.0, np.abs(z).max()
     assert np.allclose(draws.var(0), var, rtol=0.15)
+
+
+@pytest.mark.gpu
+def test_dirichlet_draw_moments_gpu(core):
+    """theta_k ~ Dir(H_k + alpha) (MILikelihood.cpp:132-169): moments."""
+    print("Dirichlet moments (HIP)", flush=True)
+    _check_dirichlet(core, 4000)
+
+
+def test_dirichlet_draw_moments_oracle(orc):
+    """The oracle's Dirichlet draw has the same moments."""
+    _check_dirichlet(orc, 4000)
 
 
 def _replicates(impl, prob, R, T, sample_gamma, seed0):

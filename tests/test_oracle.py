@@ -475,3 +475,21 @@ def test_topology_limit_single_corner(orc, fold):
         assert np.allclose(g, expect, atol=1e-14, rtol=0)
         if fold:
             assert alpha > 0.15 and e1 <= np.linalg.det(new) <= e2
+
+
+@pytest.mark.parametrize("shape_in,shape_out", [((6, 5, 4), (12, 10, 8)),
+                                                ((7, 9, 1), (14, 18, 1)),
+                                                ((5, 6, 7), (9, 11, 13))])
+def test_oracle_up_sample_closed_form(orc, shape_in, shape_out):
+    """NonLinearTransformation::up_sample (NonLinearTransformation.cpp:366-430):
+    ortho DCT-II of each component at the input size, coefficients copied
+    into the low corner of a zero array of the output size, ortho DCT-III."""
+    from scipy.fft import dctn, idctn
+    rng = np.random.default_rng(sum(shape_out))
+    N = 2 if shape_out[2] == 1 else 3
+    f = rng.standard_normal(shape_in + (N,))
+    got = orc.NonLinearTransformation.up_sample(f.ravel(), *shape_in, *shape_out, 1)
+    for n in range(N):
+        c = np.zeros(shape_out)
+        c[:shape_in[0], :shape_in[1], :shape_in[2]] = dctn(f[..., n], norm="ortho")
+        assert np.allclose(got[..., n], idctn(c, norm="ortho"), atol=1e-12)
