@@ -23,6 +23,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("3D reg time-to-tolerance (warm-start vs plain) + Gibbs voxel-updates/s, 1 GPU")
 HBM_PEAK_GBS = 8000.0
+WELFORD_BATCH = 8  # samples folded per accumulator pass (kWelfordBatch)
 FP32_PEAK_TFLOPS = 157.3
 
 # HBM bytes per launch at the 160x192x160 workload, from two rocprofv3 PMC
@@ -33,7 +34,6 @@ MEASURED_TRAFFIC = {
     (160, 192, 160): {
         "estep_sample": (46214.5 * 2.0 + 74000.0) * 1024,
         "estep": (46547.9 * 2.0 + 74000.0) * 1024,
-        "welford": (201608.7 * 2.0 + 345600.0) * 1024,
     },
 }
 
@@ -52,8 +52,10 @@ def kernel_model(name, nv, nn, N, dims):
             extra = nv * N * f32  # x read by the first / last pass
         return nv * N * 2 * f32 + extra, 2.0 * axis * nv * N
     if name == "welford":
+        # one launch folds WELFORD_BATCH sweeps' samples into the FP64
+        # accumulators: read the samples, read and write mean and M2 once
         npair = N * (N + 1) // 2
-        return nv * (N * f32 + 2 * 8 * N + 2 * 8 * npair), 0.0
+        return nv * (WELFORD_BATCH * N * f32 + 2 * 8 * N + 2 * 8 * npair), 0.0
     return None, None
 
 
@@ -109,7 +111,7 @@ def run(args):
 
     # ---- per-kernel times from HIP events on the handle's stream -----------
     s.set_profiling(True)
-    prof_sweeps = max(1, min(args.steps, 3))
+    prof_sweeps = WELFORD_BATCH  # one full accumulator batch
     s.sampler_run(args.seed + 2, 0, prof_sweeps)
     ktimes = s.get_kernel_times()
     s.set_profiling(False)

@@ -21,8 +21,17 @@
 // serves a whole 4-node z-row and no bounds test is needed after clamping.
 // The likelihood column theta[:, u] and the block histogram live in LDS; the
 // histogram is a 64-bit fixed-point array (exact, order-independent sums).
-// Consecutive neighbours with the same moving bin are merged in registers
-// before the LDS atomic, which removes most atomics in homogeneous regions.
+// EM histogram, K <= 32 (k_estep_em32): the posterior mass of moving bin v
+// in voxel i is theta[v, u_i] * S_v / Z_i, where S_v is the spline weight of
+// the neighbours whose bin is v.  Each lane adds its nodes' spline weights,
+// as 32-bit fixed point (scale 2^31, S_v <= 1), into a private per-bin LDS
+// column laid out bin-major / lane-minor (bank = lane: the integer adds never
+// conflict), marks the bins it touched in a 32-bit mask, and after the voxel
+// flushes only those bins (about 6 of 32 on noisy images, where runs of equal
+// neighbouring bins are short) into the block histogram.  Larger K uses
+// k_estep, which merges runs of equal bins in registers before each LDS
+// atomic.  The sampling E-step counts its one drawn
+// node per voxel in a 32-bit LDS count array.
 #include "estep.hpp"
 
 namespace binder {
@@ -124,6 +133,9 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int KL = a.K * a.L;
   unsigned long long *hist = reinterpret_cast<unsigned long long *>(smem);
+  // sampling: one drawn node per voxel, counted in 32 bits (the first half of
+  // the same zeroed LDS range)
+  unsigned int *cnt = reinterpret_cast<unsigned int *>(smem);
   // likelihood columns theta[:, u] with a row stride of K+1 so that lanes with
   // different fixed bins reading the same moving bin hit different banks
   float *th = reinterpret_cast<float *>(hist + KL);
@@ -214,7 +226,7 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
     unsigned long long hval = 0ull;
     if (!SAMPLE) {
       if (valid) {
-        mll -= log((double)Z);
+        mll += (double)__log2f(Z);
         const float invZ = 1.0f / Z;
         a.dream[i] = (float)ix + sx * invZ;
         a.dream[a.nv + i] = (float)iy + sy * invZ;
@@ -335,12 +347,17 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
       }
     }
     if (hval == 0ull) hidx = -1;
-    wave_hist_add(hist, hidx, hval);
+    if constexpr (SAMPLE) {
+      if (hidx >= 0) atomicAdd(cnt + hidx, 1u);
+    } else {
+      wave_hist_add(hist, hidx, hval);
+    }
   }
 
   __syncthreads();
   unsigned long long *out = a.hpart + (long long)lb * KL;
-  for (int i = threadIdx.x; i < KL; i += blockDim.x) out[i] = hist[i];
+  for (int i = threadIdx.x; i < KL; i += blockDim.x)
+    out[i] = SAMPLE ? (unsigned long long)cnt[i] << kHistShift : hist[i];
   // deterministic block reduction of mll
   __shared__ double red[256];
   red[threadIdx.x] = mll;
@@ -349,7 +366,132 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) a.mllpart[lb] = red[0];
+  if (threadIdx.x == 0) a.mllpart[lb] = -kLn2 * red[0];
+}
+
+// EM E-step for K <= 32 (same outputs as k_estep<IS2D, false>).  LDS:
+// block histogram (u64 fixed point) | theta columns | private per-lane bin
+// columns priv[wave][bin][lane] (u32 fixed-point spline mass).
+constexpr int kEm32Threads = 512;
+constexpr int kEm32MaxK = 32;
+constexpr float kSplineScale = 2147483648.0f;  // 2^31
+
+template <bool IS2D>
+__global__ __launch_bounds__(kEm32Threads) void k_estep_em32(EStepArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int KL = a.K * a.L;
+  unsigned long long *hist = reinterpret_cast<unsigned long long *>(smem);
+  float *th = reinterpret_cast<float *>(hist + KL);
+  const int Kp = a.K + 1;
+  uint32_t *priv = reinterpret_cast<uint32_t *>(th + ((a.L * Kp + 3) & ~3));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t *mine = priv + wave * (kEm32MaxK * 64) + lane;  // bin v at mine[64 * v]
+  for (int i = threadIdx.x; i < KL; i += blockDim.x) {
+    hist[i] = 0ull;
+    th[(i / a.K) * Kp + (i % a.K)] = a.thT[i];
+  }
+  for (int v = 0; v < kEm32MaxK; ++v) mine[64 * v] = 0u;
+  __syncthreads();
+
+  constexpr int NZ = IS2D ? 1 : 4;
+  double l2sum = 0.0;  // sum of log2 Z_i
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const long long per =
+      ((a.nv + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
+  const long long vbeg = (long long)lb * per;
+  const long long vend = vbeg + per < a.nv ? vbeg + per : a.nv;
+  // rotation of this lane's flush order, so lanes of one wave that share a
+  // histogram row flush different bins in the same step
+  const int rot = lane & 31;
+  for (long long base = vbeg; base < vend; base += blockDim.x) {
+    const long long i = base + threadIdx.x;
+    if (i >= vend) break;
+    const int u = a.bv[i];
+    const float *thu = th + u * Kp;
+    float tx, ty, tz = 0.f;
+    const int ix = safe_floor(a.d[i], tx);
+    const int iy = safe_floor(a.d[a.nv + i], ty);
+    int iz = 0;
+    if (!IS2D) iz = safe_floor(a.d[2 * a.nv + i], tz);
+    float wx[4], wy[4], wz[4] = {1.f, 0.f, 0.f, 0.f};
+    spline3(tx, wx);
+    spline3(ty, wy);
+    if (!IS2D) spline3(tz, wz);
+    uint32_t qall[16];
+    load_columns<IS2D>(a, ix, iy, iz, qall);
+
+    float Z = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+    uint32_t mask = 0u;
+#pragma unroll
+    for (int xa = 0; xa < 4; ++xa) {
+      // one x-slice: all theta reads first, then the private adds, so the
+      // reads are not serialised behind the adds (LDS completes in order)
+      uint32_t w[4 * NZ];
+      int vv[4 * NZ];
+      float zx = 0.f;
+#pragma unroll
+      for (int yb = 0; yb < 4; ++yb) {
+        const float wxy = wx[xa] * wy[yb];
+        const uint32_t qq = qall[xa * 4 + yb];
+        float zy = 0.f;
+#pragma unroll
+        for (int zc = 0; zc < NZ; ++zc) {
+          const int v = (qq >> (8 * zc)) & 0xFF;
+          const float sw = wxy * wz[zc];
+          const float wj = sw * thu[v];
+          w[yb * NZ + zc] = __float2uint_rn(sw * kSplineScale);
+          vv[yb * NZ + zc] = v;
+          zy += wj;
+          if (!IS2D) sz += wj * (float)(zc - 1);
+        }
+        zx += zy;
+        sy += zy * (float)(yb - 1);
+      }
+      Z += zx;
+      sx += zx * (float)(xa - 1);
+#pragma unroll
+      for (int j = 0; j < 4 * NZ; ++j) {
+        // private column: only this lane touches it; an integer LDS add with
+        // no return value, landing in program order
+        atomicAdd(mine + 64 * vv[j], w[j]);
+        mask |= 1u << vv[j];
+      }
+    }
+
+    l2sum += (double)__log2f(Z);
+    const float invZ = 1.0f / Z;
+    a.dream[i] = (float)ix + sx * invZ;
+    a.dream[a.nv + i] = (float)iy + sy * invZ;
+    if (!IS2D) a.dream[2 * a.nv + i] = (float)iz + sz * invZ;
+    const bool add = Z > 0.f;
+    uint32_t m = __builtin_amdgcn_alignbit(mask, mask, rot);  // rotate right
+    while (m) {
+      const int b = __builtin_ctz(m);
+      m &= m - 1u;
+      const int v = (b + rot) & 31;
+      const uint32_t sv = mine[64 * v];
+      mine[64 * v] = 0u;
+      if (add) {
+        const float mass = thu[v] * ((float)sv * (1.0f / kSplineScale)) * invZ;
+        const unsigned long long f = to_fixed(mass);
+        if (f) atomicAdd(hist + v * a.L + u, f);
+      }
+    }
+  }
+
+  __syncthreads();
+  unsigned long long *out = a.hpart + (long long)lb * KL;
+  for (int i = threadIdx.x; i < KL; i += blockDim.x) out[i] = hist[i];
+  // the private columns (all zero again) are free now: reuse them for the
+  // deterministic reduction of the log sums, so two blocks fit in one CU
+  double *red = reinterpret_cast<double *>(priv);
+  red[threadIdx.x] = l2sum;
+  __syncthreads();
+  for (int s2 = kEm32Threads / 2; s2 > 0; s2 >>= 1) {
+    if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.mllpart[lb] = -kLn2 * red[0];
 }
 
 constexpr int kRowsPerReduce = 16;
@@ -446,24 +588,34 @@ __global__ __launch_bounds__(1024) void k_theta(ThetaArgs a) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_welford(const float *d, double *mean,
+__global__ __launch_bounds__(256) void k_welford(WelfordBatch b, double *mean,
                                                  double *m2, long long nv,
-                                                 int N, double inv_n) {
+                                                 int N) {
   const long long stride = (long long)gridDim.x * blockDim.x;
+  const int np = N * (N + 1) / 2;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
        i += stride) {
-    double d1[3], d2[3];
-    for (int c = 0; c < N; ++c) {
-      const double val = (double)d[c * nv + i];
-      const double mo = mean[c * nv + i];
-      d1[c] = val - mo;
-      const double mn = mo + d1[c] * inv_n;
-      mean[c * nv + i] = mn;
-      d2[c] = val - mn;
+    double mu[3] = {0.0, 0.0, 0.0}, M[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < N; ++c) mu[c] = mean[c * nv + i];
+    for (int p = 0; p < np; ++p) M[p] = m2[p * nv + i];
+    for (int k = 0; k < b.count; ++k) {
+      // Registration.cpp:593-623: d1 = x - mu_old, mu += d1 / n,
+      // d2 = x - mu_new, M2 += d1 d2^T
+      const double inv_n = 1.0 / ((double)(b.n0 + k) + 1.0);
+      const float *x = b.x[k];
+      double d1[3] = {0.0, 0.0, 0.0}, d2[3] = {0.0, 0.0, 0.0};
+      for (int c = 0; c < N; ++c) {
+        const double val = (double)x[c * nv + i];
+        d1[c] = val - mu[c];
+        mu[c] = mu[c] + d1[c] * inv_n;
+        d2[c] = val - mu[c];
+      }
+      int p = 0;
+      for (int r = 0; r < N; ++r)
+        for (int c = r; c < N; ++c, ++p) M[p] += d1[r] * d2[c];
     }
-    int p = 0;
-    for (int r = 0; r < N; ++r)
-      for (int c = r; c < N; ++c, ++p) m2[p * nv + i] += d1[r] * d2[c];
+    for (int c = 0; c < N; ++c) mean[c * nv + i] = mu[c];
+    for (int p = 0; p < np; ++p) m2[p * nv + i] = M[p];
   }
 }
 
@@ -474,6 +626,22 @@ void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
   const size_t lds = (size_t)a.K * a.L * sizeof(unsigned long long) +
                      (size_t)(a.K + 1) * a.L * sizeof(float);
   dim3 g(blocks), b(256);
+  if (!sample && a.K <= kEm32MaxK) {
+    const size_t lds32 = (size_t)a.K * a.L * sizeof(unsigned long long) +
+                         (size_t)(((a.K + 1) * a.L + 3) & ~3) * sizeof(float) +
+                         (size_t)(kEm32Threads / 64) * kEm32MaxK * 64 * sizeof(float);
+    static bool attr_set[2] = {false, false};
+    auto k = is2d ? k_estep_em32<true> : k_estep_em32<false>;
+    if (!attr_set[is2d]) {
+      HIP_OK(hipFuncSetAttribute((const void *)k,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 160 * 1024));
+      attr_set[is2d] = true;
+    }
+    hipLaunchKernelGGL(k, g, dim3(kEm32Threads), lds32, s, a);
+    HIP_LAUNCH_OK();
+    return;
+  }
   if (is2d) {
     if (sample) hipLaunchKernelGGL((k_estep<true, true>), g, b, lds, s, a);
     else hipLaunchKernelGGL((k_estep<true, false>), g, b, lds, s, a);
@@ -500,12 +668,13 @@ void launch_theta(const ThetaArgs &a, hipStream_t s) {
   HIP_LAUNCH_OK();
 }
 
-void launch_welford(const float *d, double *mean, double *m2, long long nv,
-                    int N, double inv_n, hipStream_t s) {
+void launch_welford(const WelfordBatch &b, double *mean, double *m2,
+                    long long nv, int N, hipStream_t s) {
+  if (b.count <= 0) return;
   long long blocks = (nv + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(k_welford, dim3((unsigned)blocks), dim3(256), 0, s, d,
-                     mean, m2, nv, N, inv_n);
+  hipLaunchKernelGGL(k_welford, dim3((unsigned)blocks), dim3(256), 0, s, b,
+                     mean, m2, nv, N);
   HIP_LAUNCH_OK();
 }
 

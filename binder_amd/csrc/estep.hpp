@@ -9,6 +9,8 @@ namespace binder {
 // block and grid reductions are exact and the histogram is independent of
 // scheduling.
 constexpr double kHistScale = 2147483648.0;
+constexpr int kHistShift = 31;  // kHistScale == 2^kHistShift
+constexpr double kLn2 = 0.693147180559945309417232121458;
 
 struct EStepArgs {
   const float *d;           // [N][nv] current deformed positions (SoA)
@@ -51,8 +53,18 @@ struct ThetaArgs {
 
 void launch_theta(const ThetaArgs &a, hipStream_t s);
 
-// FP64 Welford update of mean [N][nv] and upper-triangular M2 [N(N+1)/2][nv].
-void launch_welford(const float *d, double *mean, double *m2, long long nv,
-                    int N, double inv_n, hipStream_t s);
+// FP64 Welford update of mean [N][nv] and upper-triangular M2 [N(N+1)/2][nv]
+// with `count` samples x[0..count) (each [N][nv]) taken in that order, n0
+// samples having been folded in before: per voxel the accumulators are
+// loaded once, updated sample by sample exactly as one update per sweep
+// would, and stored once.
+constexpr int kWelfordBatch = 8;
+struct WelfordBatch {
+  const float *x[kWelfordBatch];
+  int count;
+  long long n0;
+};
+void launch_welford(const WelfordBatch &b, double *mean, double *m2,
+                    long long nv, int N, hipStream_t s);
 
 }  // namespace binder

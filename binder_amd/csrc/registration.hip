@@ -65,7 +65,17 @@ struct binder_registration {
   long long nv, nn;
   double nnz;
   bool is2d;
-  DBuf<float> d, x, dream;
+  // d: the current field lives in slot dslot of a ring of kWelfordBatch
+  // buffers (slot 0 = d, slots 1.. = dextra, allocated by the first sampler
+  // call): each sampling sweep writes the next slot, so the last
+  // kWelfordBatch samples stay resident and the FP64 accumulators are
+  // updated once per batch.
+  DBuf<float> d, dextra, x, dream;
+  int dslot = 0;
+  float *slot(int k) {
+    return k == 0 ? d.get() : dextra.get() + (size_t)(k - 1) * N * nv;
+  }
+  float *dcur() { return slot(dslot); }
   // likelihood
   bool has_mi = false;
   int K = 0, L = 0;
@@ -95,7 +105,7 @@ struct binder_registration {
 
   EStepArgs estep_args(bool sample, uint32_t k0, uint32_t k1, uint32_t sweep) {
     EStepArgs a{};
-    a.d = d.get();
+    a.d = dcur();
     a.bv = bv.get();
     a.quads = quads.get();
     a.bins2d = bins2d.get();
@@ -341,7 +351,7 @@ int binder_registration_em(binder_registration *r, int max_it, double th,
     opt.prior_out = r->stats.get() + 2;
     while (it < max_it && !converged) {
       r->e_and_theta(false, update ? kThetaMap : kThetaKeep, 0, 0, 0);
-      r->sm->run(r->dream.get(), r->x.get(), r->d.get(), opt, r->tm());
+      r->sm->run(r->dream.get(), r->x.get(), r->dcur(), opt, r->tm());
       HIP_OK(hipMemcpyAsync(r->host_stats.p, r->stats.get(), 3 * sizeof(double),
                             hipMemcpyDeviceToHost, r->s()));
       HIP_OK(hipStreamSynchronize(r->s()));
@@ -392,6 +402,9 @@ int binder_registration_sampler_run(binder_registration *r, int seed, int N_b,
     r->samples = N_s;
     const uint32_t k0 = (uint32_t)seed;
     const uint32_t k1 = 0x85EBCA6Bu ^ (uint32_t)((long long)seed >> 32);
+    if (st && !r->dextra.get()) r->dextra.alloc((size_t)(kWelfordBatch - 1) * N * r->nv);
+    WelfordBatch wb{};
+    long long folded = 0;  // samples already in mean / m2
     for (int sweep = 0; sweep < N_b + N_s; ++sweep) {
       if (sp) r->e_and_theta(true, -1, k0, k1, (uint32_t)sweep);
       if (sl) r->run_theta(kThetaDirichlet, k0, k1, (uint32_t)sweep);
@@ -404,13 +417,21 @@ int binder_registration_sampler_run(binder_registration *r, int seed, int N_b,
         o.sweep = (uint32_t)sweep;
         o.stream = kStreamDeform;
         o.prior_out = r->stats.get() + 3;
-        r->sm->run(r->dream.get(), r->x.get(), r->d.get(), o, r->tm());
+        const int next = (r->dslot + 1) % kWelfordBatch;
+        r->sm->run(r->dream.get(), r->x.get(), r->slot(next), o, r->tm());
+        r->dslot = next;
       }
       if (sweep >= N_b) {
-        const double inv_n = 1.0 / ((double)(sweep - N_b) + 1.0);
-        TimedLaunch t(r->tm(), "welford", r->s());
-        launch_welford(r->d.get(), r->mean.get(), r->m2.get(), r->nv, N, inv_n,
-                       r->s());
+        // the sample stays in its ring slot until the batch is folded in,
+        // which happens before the ring wraps onto it
+        wb.x[wb.count++] = r->dcur();
+        if (wb.count == kWelfordBatch || sweep == N_b + N_s - 1) {
+          wb.n0 = folded;
+          TimedLaunch t(r->tm(), "welford", r->s());
+          launch_welford(wb, r->mean.get(), r->m2.get(), r->nv, N, r->s());
+          folded += wb.count;
+          wb.count = 0;
+        }
       }
     }
     HIP_OK(hipStreamSynchronize(r->s()));
@@ -462,7 +483,8 @@ int binder_registration_get_final_locations(binder_registration *r,
   return guarded([&] {
     BINDER_REQUIRE(r && out, "null argument");
     std::vector<float> h((size_t)r->N * r->nv);
-    r->d.download(h.data(), h.size(), r->s());
+    HIP_OK(hipMemcpyAsync(h.data(), r->dcur(), sizeof(float) * h.size(),
+                          hipMemcpyDeviceToHost, r->s()));
     HIP_OK(hipStreamSynchronize(r->s()));
     soa_to_aos(h.data(), out, r->nv, r->N);
   });
