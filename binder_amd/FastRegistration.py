@@ -1,0 +1,311 @@
+"""Multi-resolution driver of the registration path (the reference's
+``BINDER.FastRegistration``, BINDER/FastRegistration.py:26-750), on the
+gfx950 backend ``binder_amd._core``.
+
+Same constructor keywords, ``run(resolutions, max_number_of_iterations)``,
+``move_resolution(resolution)`` and ``sample_param(...)`` as the reference,
+for the MI metric with the non-linear (DCT-prior) transformation, which is
+the path this backend implements.  The reference's SSD / feature likelihoods,
+affine / rigid transformations and smoothed pyramids are not part of it:
+asking for them raises NotImplementedError instead of silently running
+something else.  nibabel and scikit-image are replaced by
+``binder_amd.nifti`` and ``binder_amd.resample``.
+
+Every registration and sampling call runs on the GPU through ``_core``;
+only the pyramid bookkeeping (down-sampling the bins, resizing the field,
+gamma and D per level) is host NumPy, as in the reference.
+"""
+import numpy as np
+
+from . import _core as Registration
+from . import nifti, resample, utils
+
+
+def _apply_affine(aff, pts):
+    return nifti.apply_affine(aff, pts)
+
+
+class FastRegistration:
+    """Registers ``voxels`` (the fixed image) into ``nodes`` (the moving
+    image).  See BINDER/FastRegistration.py:27-440 for the meaning of each
+    keyword."""
+
+    def __init__(self, voxels, nodes, voxels_matrix=np.eye(4), nodes_matrix=np.eye(4),
+                 mask_voxels=None, mask_nodes=None, use_masks=False, mask_threshold=0,
+                 num_voxel_intesity_levels=64, num_node_intensity_levels=64, alpha=2.0,
+                 alpha_0=1, beta_0=1, gamma=5e-7, metric='MI', histogram_matching=False,
+                 scale_shift=True, scale=1, shift=0, spline_order=3,
+                 finite_difference_order=2, topology_correction=True, visualizer=False,
+                 feature_moving=False, feature_fixed=False, precomputed_features_moving=None,
+                 precomputed_features_fixed=None, number_of_filters_moving=1,
+                 number_of_filters_fixed=1, fixed_features_reg=False, affine=False, rigid=False,
+                 smooth=False, upsample_in_frequency_domain=False, convergence_th=5e-5,
+                 e1=0.01, e2=100, max_outer_it=10, max_inner_it=10, threads=1, seed=12345,
+                 debug=0, update_likelihood_parameters=1, initial_transformation=None,
+                 initialize_with_center_of_mass=True, bin_robust=True,
+                 include_image_as_feature=True, increase_gamma_at_DS_resolution=False):
+        unsupported = []
+        if metric != 'MI':
+            unsupported.append(f"metric={metric!r}")
+        if feature_moving or feature_fixed or precomputed_features_moving is not None \
+                or precomputed_features_fixed is not None:
+            unsupported.append("feature likelihoods")
+        if affine or rigid:
+            unsupported.append("affine/rigid transformations")
+        if smooth:
+            unsupported.append("smooth=True (spline-smoothed pyramid)")
+        if visualizer:
+            unsupported.append("visualizer=True")
+        if unsupported:
+            raise NotImplementedError(
+                "the gfx950 backend implements the MI metric with the non-linear DCT-prior "
+                "transformation; not available: " + ", ".join(unsupported))
+        if spline_order != 3:
+            raise ValueError("the gfx950 backend supports spline_order=3 only")
+
+        self.registratorObj = None
+        self.metric = metric
+        self.affine = False
+        self.rigid = False
+        self.smooth = False
+        self.visualizer = False
+        self.feature_moving = False
+        self.feature_fixed = False
+        self.convergence_th = convergence_th
+        self.debug = debug
+        self.update_likelihood_parameters = update_likelihood_parameters
+        self.upsample_in_frequency_domain = upsample_in_frequency_domain
+        self.increase_gamma_at_DS_resolution = increase_gamma_at_DS_resolution
+        self.threads = threads
+        self.seed = seed
+        self.D = None
+
+        voxels = np.asarray(voxels, dtype=np.float64)
+        nodes = np.asarray(nodes, dtype=np.float64)
+        self.number_of_dimensions = nodes.ndim
+        if self.number_of_dimensions == 2:
+            voxels = voxels[:, :, None]
+            nodes = nodes[:, :, None]
+
+        if topology_correction:
+            name = "topology_correction_2D" if self.number_of_dimensions == 2 else \
+                "topology_correction_3D"
+            if not hasattr(Registration, name):
+                raise RuntimeError(
+                    f"topology_correction=True was requested, but this backend has no {name}; "
+                    "pass topology_correction=False")
+            self.top_cor_obj = getattr(Registration, name)()
+            self.e1, self.e2 = e1, e2
+            self.max_inner_it, self.max_outer_it = max_inner_it, max_outer_it
+        else:
+            self.top_cor_obj = 0
+
+        self.voxels = voxels
+        self.nodes = nodes
+        vm = np.asarray(voxels_matrix, dtype=np.float64)
+        nm = np.asarray(nodes_matrix, dtype=np.float64)
+        # voxel size = norm of each column of the linear part (FastRegistration.py:154-155)
+        self.voxels_voxel_size = np.sqrt(np.sum(vm[0:3, 0:3] ** 2, axis=0))
+        self.nodes_voxel_size = np.sqrt(np.sum(nm[0:3, 0:3] ** 2, axis=0))
+        self.current_voxels_voxel_size = self.voxels_voxel_size
+        self.current_nodes_voxel_size = self.nodes_voxel_size
+        self.gamma_init = gamma
+
+        if histogram_matching:
+            nodes = utils.hist_norm(nodes, voxels)
+
+        if mask_nodes is None:
+            self.full_mask_nodes = utils.maskImage(nodes, th_value=mask_threshold) if use_masks \
+                else np.ones(nodes.shape)
+        else:
+            self.full_mask_nodes = np.asarray(mask_nodes).reshape(nodes.shape)
+        if mask_voxels is None:
+            self.full_mask_voxels = utils.maskImage(voxels, th_value=mask_threshold) if use_masks \
+                else np.ones(voxels.shape)
+        else:
+            self.full_mask_voxels = np.asarray(mask_voxels).reshape(voxels.shape)
+
+        self.finite_difference_order = finite_difference_order
+        self.orig_voxels_shape = voxels.shape
+        self.orig_nodes_shape = nodes.shape
+        self.num_voxel_intensity_levels = num_voxel_intesity_levels
+        self.num_node_intensity_levels = num_node_intensity_levels
+
+        def _bin(img, mask, levels):
+            if bin_robust:
+                b, _ = utils.bin_signal_robust(img * mask, levels)
+            else:
+                b = utils.bin_signal(img * mask, levels)
+            b[np.logical_not(mask)] = 0
+            return b
+
+        self.full_binned_voxels = _bin(voxels, self.full_mask_voxels,
+                                       self.num_voxel_intensity_levels)
+        self.full_binned_nodes = _bin(nodes, self.full_mask_nodes,
+                                      self.num_node_intensity_levels)
+        self.alpha = alpha
+        # theta initialised to 1/L for all k, l (FastRegistration.py:224-225)
+        self.theta = np.ones([self.num_node_intensity_levels, self.num_voxel_intensity_levels]) \
+            / self.num_voxel_intensity_levels
+
+        grid_v = np.stack(np.indices(voxels.shape, dtype=np.float64), axis=-1)
+        grid_n = np.stack(np.indices(nodes.shape, dtype=np.float64), axis=-1)
+        if self.number_of_dimensions == 2:
+            grid_v, grid_n = grid_v[..., 0:2], grid_n[..., 0:2]
+        self.full_voxels_pos = grid_v
+        self.full_nodes_pos = grid_n
+
+        # voxel positions expressed in node space (FastRegistration.py:375-395)
+        if initial_transformation is not None:
+            self.full_voxels_pos = _apply_affine(initial_transformation, self.full_voxels_pos)
+        elif not np.allclose(nm, vm):
+            M = np.linalg.inv(nm) @ vm
+            self.full_voxels_pos = _apply_affine(M, self.full_voxels_pos)
+            if initialize_with_center_of_mass:
+                from scipy.ndimage import center_of_mass
+                com_nodes = np.array(center_of_mass(self.full_binned_nodes > 0))
+                com_voxels = _apply_affine(M, np.array(center_of_mass(self.full_binned_voxels > 0)))
+                self.full_voxels_pos = self.full_voxels_pos + (com_nodes - com_voxels)
+
+        self.current_resolution = 1
+        self.voxels_pos = self.full_voxels_pos
+        self.nodes_pos = self.full_nodes_pos
+        self.binned_voxels = self.full_binned_voxels
+        self.binned_nodes = self.full_binned_nodes
+        self.nodes_shape = self.orig_nodes_shape
+        self.voxels_shape = self.orig_voxels_shape
+        self.non_zero_voxels = np.count_nonzero(self.binned_voxels)
+        self.deformedVoxelPositions = self.full_voxels_pos
+        self.dreamLocations = np.zeros_like(self.deformedVoxelPositions)
+        self.spline_order = spline_order
+        self._set_gammas(self.voxels_shape)
+        # per-level record: (resolution, EM iterations, seconds)
+        self.level_log = []
+
+    def _set_gammas(self, shape_for_scaling):
+        # gamma per voxel at 1 mm, scaled by voxel volume, voxel count and the
+        # squared node size of each axis (FastRegistration.py:411-417)
+        base = self.gamma_init * np.prod(self.current_voxels_voxel_size) * \
+            np.prod(shape_for_scaling)
+        self.gamma_x = base * self.current_nodes_voxel_size[0] ** 2
+        self.gamma_y = base * self.current_nodes_voxel_size[1] ** 2
+        self.gamma_z = base * self.current_nodes_voxel_size[2] ** 2
+
+    def run(self, resolutions, max_number_of_iterations):
+        """EM at each resolution of the pyramid, then (optionally) topology
+        correction (FastRegistration.py:443-546)."""
+        import time
+        for resolution in resolutions:
+            self.move_resolution(resolution)
+            self.deformedVoxelPositions = np.ascontiguousarray(self.deformedVoxelPositions)
+            t0 = time.perf_counter()
+            self.registratorObj = self._new_registration(self.deformedVoxelPositions)
+            self.registratorObj.setMILikelihood(self.alpha, np.ascontiguousarray(self.theta).ravel(),
+                                                self.num_node_intensity_levels,
+                                                self.num_voxel_intensity_levels, self.threads,
+                                                np.ascontiguousarray(self.binned_nodes).ravel(),
+                                                np.ascontiguousarray(self.binned_voxels).ravel())
+            self.registratorObj.EM(max_number_of_iterations * self.current_resolution,
+                                   self.convergence_th,
+                                   update_likelihood_parameters=self.update_likelihood_parameters,
+                                   debug=self.debug)
+            self.deformedVoxelPositions = self.registratorObj.get_final_locations_as_numpy_array()
+            self.theta = np.ascontiguousarray(np.array(
+                self.registratorObj.get_likelihood_parameters()).reshape(
+                    self.num_node_intensity_levels, self.num_voxel_intensity_levels))
+            iters = len(self.registratorObj.get_objective_trace())
+            if self.top_cor_obj:
+                vs = self.current_voxels_voxel_size
+                if self.number_of_dimensions == 2:
+                    f = np.ascontiguousarray(self.deformedVoxelPositions[:, :, 0])
+                    out = self.top_cor_obj.correct_topology(
+                        f.ravel(), self.e1, self.e2, self.max_outer_it, self.max_inner_it,
+                        self.threads, vs, self.voxels_shape[0], self.voxels_shape[1])
+                    d = np.array(self.deformedVoxelPositions, dtype=np.float64)
+                    d[:, :, 0] = np.asarray(out).reshape(d[:, :, 0].shape)
+                    self.deformedVoxelPositions = d
+                else:
+                    self.deformedVoxelPositions = self.top_cor_obj.correct_topology(
+                        np.ascontiguousarray(self.deformedVoxelPositions).ravel(), self.e1,
+                        self.e2, self.max_outer_it, self.max_inner_it, self.threads, vs,
+                        self.voxels_shape[0], self.voxels_shape[1], self.voxels_shape[2])
+            self.level_log.append((resolution, iters, time.perf_counter() - t0))
+
+    def _new_registration(self, positions):
+        vp = np.ascontiguousarray(self.voxels_pos)
+        r = Registration.Registration(np.ascontiguousarray(positions).ravel(), self.threads,
+                                      vp.ravel(), np.ascontiguousarray(self.nodes_pos).ravel(),
+                                      vp.shape[0], vp.shape[1], vp.shape[2],
+                                      self.nodes_shape[0], self.nodes_shape[1],
+                                      self.nodes_shape[2], self.spline_order,
+                                      self.non_zero_voxels)
+        self.D = np.ascontiguousarray(self.D)
+        r.setNonLinearTransformation(self.gamma_x, self.gamma_y, self.gamma_z,
+                                     self.current_voxels_voxel_size[0],
+                                     self.current_voxels_voxel_size[1],
+                                     self.current_voxels_voxel_size[2],
+                                     self.current_nodes_voxel_size[0],
+                                     self.current_nodes_voxel_size[1],
+                                     self.current_nodes_voxel_size[2], self.D.ravel())
+        return r
+
+    def move_resolution(self, resolution):
+        """Down-sample the bins and positions by `resolution`, resample the
+        current field to the new grid and rescale it, recompute D and gamma
+        (FastRegistration.py:550-650)."""
+        res = int(resolution)
+        self.current_voxels_voxel_size = self.voxels_voxel_size * res
+        self.current_nodes_voxel_size = self.nodes_voxel_size * res
+        self.voxels_shape = tuple(int(np.ceil(x / res)) for x in self.orig_voxels_shape)
+        self.nodes_shape = np.array([int(np.ceil(x / res)) for x in self.orig_nodes_shape])
+        self.voxels_pos = self.full_voxels_pos[::res, ::res, ::res, :] / res
+        self.nodes_pos = self.full_nodes_pos[::res, ::res, ::res, :] / res
+
+        cur = self.current_resolution
+        if not self.upsample_in_frequency_domain or cur - res < 0:
+            self.deformedVoxelPositions = resample.resize(
+                self.deformedVoxelPositions, self.voxels_shape, order=1) * (cur / res)
+        elif res != cur:
+            full = self.full_voxels_pos[::cur, ::cur, ::cur, :] / cur
+            delta = np.ascontiguousarray(self.deformedVoxelPositions - full)
+            sh = self.deformedVoxelPositions.shape
+            tmp = Registration.NonLinearTransformation.up_sample(
+                delta.ravel(), sh[0], sh[1], sh[2], self.voxels_shape[0], self.voxels_shape[1],
+                self.voxels_shape[2], self.threads)
+            self.deformedVoxelPositions = tmp * cur / res + self.voxels_pos
+
+        self.binned_voxels = self.full_binned_voxels[::res, ::res, ::res]
+        self.binned_nodes = self.full_binned_nodes[::res, ::res, ::res]
+        self.D = utils.initialize_fourier_domain_smoothing(
+            self.voxels_shape, True, self.finite_difference_order, self.number_of_dimensions,
+            voxel_size=self.current_voxels_voxel_size)
+        self.non_zero_voxels = np.count_nonzero(self.binned_voxels)
+        self._set_gammas(self.orig_voxels_shape if self.increase_gamma_at_DS_resolution
+                         else self.voxels_shape)
+        self.current_resolution = res
+
+    def sample_param(self, burnin, samples, sample_gamma=0, sample_gamma_every=1,
+                     sample_posteriors=1, sample_likelihood_parameters=1,
+                     sample_transformation_parameters=1):
+        """Gibbs sampler from the current registration state; returns the
+        mean field and the voxel-wise covariance (FastRegistration.py:652-746)."""
+        self.deformedVoxelPositions = np.ascontiguousarray(self.deformedVoxelPositions)
+        if sample_posteriors == 0:
+            self.dreamLocations = np.ascontiguousarray(
+                self.registratorObj.get_dream_locations())
+        self.registratorObj = self._new_registration(self.deformedVoxelPositions)
+        if sample_posteriors == 0:
+            self.registratorObj.set_dream_locations(np.asarray(self.dreamLocations).ravel())
+        self.registratorObj.setMILikelihood(self.alpha, np.ascontiguousarray(self.theta).ravel(),
+                                            self.num_node_intensity_levels,
+                                            self.num_voxel_intensity_levels, self.threads,
+                                            np.ascontiguousarray(self.binned_nodes).ravel(),
+                                            np.ascontiguousarray(self.binned_voxels).ravel())
+        return self.registratorObj.sampler(self.seed, burnin, samples, sample_gamma,
+                                           sample_gamma_every, sample_posteriors,
+                                           sample_likelihood_parameters,
+                                           sample_transformation_parameters)
+
+    def visualize_output(self, positions):
+        """The reference's matplotlib view is not part of this backend."""
+        return None

@@ -19,5 +19,10 @@ def __getattr__(name):
         core = importlib.import_module("._core", __name__)
         return {"_core": core, "Registration": core, "_backend": core.backend}[name]
     if name == "FastRegistration":
-        return importlib.import_module(".FastRegistration", __name__).FastRegistration
+        # importing the submodule binds the package attribute to the module;
+        # rebind it to the class, as `from .FastRegistration import
+        # FastRegistration` does in the reference package
+        cls = importlib.import_module(".FastRegistration", __name__).FastRegistration
+        globals()["FastRegistration"] = cls
+        return cls
     raise AttributeError(name)

@@ -80,3 +80,75 @@ def initialize_fourier_domain_smoothing(size, DCT=True, finite_difference_order=
     if return_full_matrix:
         return D[..., :half], D
     return D[..., :half]
+
+
+def hist_norm(source, template):
+    """Histogram matching of `source` to `template` by quantile mapping
+    (BINDER/utils.py:84-100)."""
+    src = np.asarray(source)
+    s_vals, s_idx, s_cnt = np.unique(src.ravel(), return_inverse=True, return_counts=True)
+    t_vals, t_cnt = np.unique(np.asarray(template).ravel(), return_counts=True)
+    s_q = np.cumsum(s_cnt).astype(np.float64)
+    s_q /= s_q[-1]
+    t_q = np.cumsum(t_cnt).astype(np.float64)
+    t_q /= t_q[-1]
+    mapped = np.interp(s_q, t_q, t_vals).astype(src.dtype)
+    return mapped[s_idx].reshape(src.shape)
+
+
+def maskImage(image, th_value=0, use_robust_minimum=False):
+    """Foreground mask: threshold (optionally at the 10th percentile of a
+    1000-bin histogram), one binary closing, hole filling
+    (BINDER/utils.py:242-267).  int32."""
+    from scipy import ndimage
+    image = np.asarray(image)
+    if use_robust_minimum:
+        hist, centers = np.histogram(image.ravel(), 1000)
+        cdf = np.cumsum(hist / np.sum(hist))
+        p = 0.1
+        idx = np.where(cdf <= p)[0]
+        while len(idx) == 0:
+            p += 0.01
+            idx = np.where(cdf <= p)[0]
+        th_value = centers[idx[-1]]
+    mask = ndimage.binary_closing(image > th_value, iterations=1)
+    mask = ndimage.binary_fill_holes(mask)
+    return mask.astype(np.int32)
+
+
+def save_warped_image(deformed_position, movingImage, outputDir, order=1, affine=np.eye(4),
+                      outputName="warped.nii.gz"):
+    """Moving image resampled at the deformed positions, saved as NIfTI
+    (BINDER/utils.py:222-227)."""
+    import os
+    from . import nifti, resample
+    pos = np.asarray(deformed_position, dtype=np.float64)
+    img = resample.warp(movingImage, np.moveaxis(pos, -1, 0), order=order)
+    nifti.save(nifti.NiftiImage(img, affine), os.path.join(outputDir, outputName))
+    print("Warped image saved")
+
+
+def save_warp(deformations, outputDir, affine=np.eye(4), outputName="warp.nii.gz"):
+    """BINDER/utils.py:230-233."""
+    import os
+    from . import nifti
+    nifti.save(nifti.NiftiImage(np.asarray(deformations), affine),
+               os.path.join(outputDir, outputName))
+    print("Warp saved")
+
+
+def save_image(generalImage, outputDir, outputName, affine=np.eye(4)):
+    """BINDER/utils.py:236-239."""
+    import os
+    from . import nifti
+    nifti.save(nifti.NiftiImage(np.asarray(generalImage), affine),
+               os.path.join(outputDir, outputName))
+    print("Image saved")
+
+
+def save_likelihood_parameters(registrator, outputDir):
+    """MI likelihood parameters to likelihood_parameters.npz
+    (BINDER/utils.py:443-472, MI branch)."""
+    import os
+    np.savez(os.path.join(outputDir, "likelihood_parameters.npz"), metric=registrator.metric,
+             theta=registrator.theta, alpha=registrator.alpha)

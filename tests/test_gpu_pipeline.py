@@ -1,0 +1,55 @@
+"""The reference's registration tests and a full pyramid on the HIP backend
+(BINDER/tests/cpu/test_2D_MI_registration.py, test_3D_MI_registration.py,
+BINDER/FastRegistration.py:443-650)."""
+import numpy as np
+import pytest
+
+from test_host_pipeline import FIXED_2D, t1_registration, toy_2d_registration
+
+pytestmark = pytest.mark.gpu
+
+
+def test_toy_2d_registration_gpu(core, monkeypatch):
+    """Exact recovery of the toy deformation (EM and sample mean)."""
+    print("toy 2D registration (HIP)", flush=True)
+    em, mcmc = toy_2d_registration(core, monkeypatch)
+    assert np.allclose(em, FIXED_2D)
+    assert np.allclose(mcmc, FIXED_2D)
+
+
+def test_t1_registration_gpu(core, monkeypatch, tmp_path):
+    """EM and sampling both lower the SSD of the 8 mm T1 pair; sampling
+    lowers it below the 3-iteration EM result."""
+    print("T1 8mm registration (HIP)", flush=True)
+    start, em, mc = t1_registration(core, monkeypatch, tmp_path)
+    assert em < start and mc < start and mc < em
+
+
+def _pyramid(backend_module, monkeypatch, pair):
+    import binder_amd.FastRegistration as fr
+    monkeypatch.setattr(fr, "Registration", backend_module)
+    reg = fr.FastRegistration(voxels=pair.fixed, nodes=pair.moving,
+                              num_voxel_intesity_levels=32, num_node_intensity_levels=32,
+                              gamma=1e-6, convergence_th=2.5e-5, topology_correction=True,
+                              increase_gamma_at_DS_resolution=True)
+    reg.run(resolutions=[4, 2, 1], max_number_of_iterations=10)
+    return reg
+
+
+def test_pyramid_matches_oracle(core, orc, monkeypatch):
+    """Three-level pyramid with topology correction on a 48x40x44 synthetic
+    pair: same EM iteration counts per level as the FP64 oracle and a final
+    field within 1e-2 voxel RMS of it (the tolerance of the time-to-tolerance
+    target is 0.1)."""
+    from binder_amd import synth
+    print("pyramid 48x40x44 (HIP vs oracle)", flush=True)
+    pair = synth.make_pair((48, 40, 44), 5, 2.0, seed=3, semi=(3.0, 12.0))
+    g = _pyramid(core, monkeypatch, pair)
+    o = _pyramid(orc, monkeypatch, pair)
+    gi = [lv[1] for lv in g.level_log]
+    oi = [lv[1] for lv in o.level_log]
+    diff = np.asarray(g.deformedVoxelPositions, np.float64) - o.deformedVoxelPositions
+    rms = float(np.sqrt(np.mean(np.sum(diff ** 2, -1))))
+    print(f"iterations HIP {gi} oracle {oi}; field RMS diff {rms:.2e}, max {np.abs(diff).max():.2e}",
+          flush=True)
+    assert rms < 1e-2
