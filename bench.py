@@ -59,8 +59,118 @@ def kernel_model(name, nv, nn, N, dims):
     return None, None
 
 
+def host_threads():
+    """Host cores this process may use (affinity, capped by OMP_NUM_THREADS)."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return min(n, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else n
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+REG_KW = dict(num_voxel_intesity_levels=32, num_node_intensity_levels=32, gamma=1e-6,
+              topology_correction=True, increase_gamma_at_DS_resolution=True, alpha=2.0)
+TOLERANCE_VOX_RMS = 0.1
+
+
+def _rms(a, b):
+    return float(np.sqrt(np.mean(np.sum((np.asarray(a, np.float64) - b) ** 2, -1))))
+
+
+def registration(args, pair):
+    """Plain (cold-start) pyramid registration as run_BINDER runs it, then
+    its time to tolerance: the wall-clock of the same pipeline stopped at
+    the first last-level EM iteration whose field is within 0.1 voxel RMS of
+    the converged field (the pipeline is deterministic, so the truncated run
+    retraces the converged one)."""
+    from binder_amd.FastRegistration import FastRegistration
+
+    levels = [2, 1] if args.quick else list(args.resolutions)
+    iters = 3 if args.quick else args.em_iters
+    kw = dict(REG_KW, convergence_th=args.convergence_th)
+    if args.quick:
+        kw["max_outer_it"] = 0
+    # A: to convergence (the reference's "Time for EM")
+    A = FastRegistration(voxels=pair.fixed, nodes=pair.moving, **kw)
+    t0 = time.perf_counter()
+    A.run(levels, iters)
+    conv_s = time.perf_counter() - t0
+    F = np.asarray(A.deformedVoxelPositions, np.float64)
+    counts = [int(lv[1]) for lv in A.level_log]
+    log(f"pyramid {levels}: iterations {counts} in {conv_s:.3f}s")
+    out = {"resolutions": levels, "converged_s": round(conv_s, 4),
+           "converged_iterations": counts,
+           "level_seconds": [round(lv[2], 4) for lv in A.level_log]}
+    if args.quick:
+        return A, F, out
+    # B: retrace the last level in chunks to find the first iteration within
+    # tolerance (untimed; the field is read back after every chunk)
+    B = FastRegistration(voxels=pair.fixed, nodes=pair.moving, **kw)
+    B.run(levels[:-1], iters, level_iterations=counts[:-1])
+    B.move_resolution(levels[-1])
+    reg = B._new_registration(np.ascontiguousarray(B.deformedVoxelPositions))
+    reg.setMILikelihood(B.alpha, np.ascontiguousarray(B.theta).ravel(),
+                        B.num_node_intensity_levels, B.num_voxel_intensity_levels, 1,
+                        np.ascontiguousarray(B.binned_nodes).ravel(),
+                        np.ascontiguousarray(B.binned_voxels).ravel())
+    k, chunk, k_tol = 0, 5, counts[-1]
+    while k < counts[-1]:
+        step = min(chunk, counts[-1] - k)
+        reg.EM(step, -1e30, 1, 0)
+        k += step
+        if _rms(reg.get_final_locations_as_numpy_array(), F) < TOLERANCE_VOX_RMS:
+            k_tol = k
+            break
+    # C: the timed pipeline, stopped there (topology correction included)
+    C = FastRegistration(voxels=pair.fixed, nodes=pair.moving, **kw)
+    t0 = time.perf_counter()
+    C.run(levels, iters, level_iterations=counts[:-1] + [k_tol])
+    tt_s = time.perf_counter() - t0
+    rms_c = _rms(C.deformedVoxelPositions, F)
+    log(f"time to tolerance: {tt_s:.3f}s ({k_tol} of {counts[-1]} last-level iterations, "
+        f"RMS {rms_c:.3f} vox)")
+    out.update({"tolerance_vox_rms": TOLERANCE_VOX_RMS,
+                "reference_field": "this pipeline's converged field (its FP64-oracle "
+                                   "agreement is a GPU test on a reduced grid)",
+                "gpu_s": round(tt_s, 4), "iterations": counts[:-1] + [k_tol],
+                "rms_vox_at_stop": round(rms_c, 4)})
+    return A, F, out
+
+
+def cpu_em_estimate(pair, args, counts, levels):
+    """FP64 oracle: seconds per EM iteration at each pyramid level (all host
+    cores), times the GPU run's iteration counts."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pycore
+    import binder_amd.FastRegistration as frm
+
+    th = host_threads()
+    saved = frm.Registration
+    frm.Registration = pycore
+    try:
+        per_level = []
+        for res in levels:
+            O = frm.FastRegistration(voxels=pair.fixed, nodes=pair.moving, threads=th,
+                                     **dict(REG_KW, topology_correction=False))
+            O.move_resolution(res)
+            r = O._new_registration(np.ascontiguousarray(O.deformedVoxelPositions))
+            r.setMILikelihood(O.alpha, np.ascontiguousarray(O.theta).ravel(), 32, 32, th,
+                              np.ascontiguousarray(O.binned_nodes).ravel(),
+                              np.ascontiguousarray(O.binned_voxels).ravel())
+            r.EM(1, -1e30, 1, 0)
+            n = 2
+            t0 = time.perf_counter()
+            r.EM(n, -1e30, 1, 0)
+            per_level.append((time.perf_counter() - t0) / n)
+    finally:
+        frm.Registration = saved
+    total = sum(c * t for c, t in zip(counts, per_level))
+    return {"seconds": round(total, 3), "cores": th, "kind": "port",
+            "per_iteration_s": [round(t, 5) for t in per_level],
+            "sample": "FP64 oracle EM: 3 iterations timed per pyramid level, times the "
+                      "GPU run's iteration counts (topology correction not included)"}
 
 
 def run(args):
@@ -74,33 +184,25 @@ def run(args):
     gen_s = time.perf_counter() - t0
     log(f"inputs {Nx}x{Ny}x{Nz} generated in {gen_s:.1f}s")
 
-    # ---- EM registration, cold start, to convergence ----------------------
-    reg = synth.build_registration(_core, prob)
+    # ---- plain pyramid registration and its time to tolerance -------------
+    A, F, reg_out = registration(args, pair)
     t0 = time.perf_counter()
-    reg.EM(3 if args.quick else args.em_iters, args.convergence_th, 1, 0)
-    em_s = time.perf_counter() - t0
-    trace = reg.get_objective_trace()
-    d_em = reg.get_final_locations_as_numpy_array().astype(np.float64)
-    theta = np.asarray(reg.get_likelihood_parameters())
-    disp_rms = float(np.sqrt(np.mean(np.sum((d_em - prob.voxel_pos) ** 2, -1))))
-    log(f"EM: {len(trace)} iterations in {em_s:.3f}s")
-
-    # ---- fold detection + topology correction ------------------------------
-    t0 = time.perf_counter()
-    _, folds_before = _core.fold_mask_3D(d_em.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0, e1=0.01)
+    _, folds = _core.fold_mask_3D(F.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0, e1=0.01)
     fold_s = time.perf_counter() - t0
-    topo = _core.topology_correction_3D()
-    t0 = time.perf_counter()
-    d_tc = topo.correct_topology(d_em.ravel(), 0.01, 100, 0 if args.quick else 10, 10, 1,
-                                 np.ones(3), Nx, Ny, Nz)
-    topo_s = time.perf_counter() - t0
-    _, folds_after = _core.fold_mask_3D(d_tc.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0, e1=0.01)
-    log(f"topology: {folds_before} -> {folds_after} folded voxels in {topo_s:.3f}s")
+    reg_out["folds_after"] = folds
+    reg_out["fold_detect_s"] = round(fold_s, 5)
+    disp_rms = float(np.sqrt(np.mean(np.sum((F - prob.voxel_pos) ** 2, -1))))
+    reg_out["displacement_rms_vox"] = round(disp_rms, 4)
 
-    # ---- Gibbs sweeps -------------------------------------------------------
-    prob.final_locations = d_tc
-    prob.theta = theta.reshape(prob.K, prob.L)
-    s = synth.build_registration(_core, prob)
+    def full_res_handle():
+        h = A._new_registration(np.ascontiguousarray(F))
+        h.setMILikelihood(A.alpha, np.ascontiguousarray(A.theta).ravel(), 32, 32, 1,
+                          np.ascontiguousarray(A.binned_nodes).ravel(),
+                          np.ascontiguousarray(A.binned_voxels).ravel())
+        return h
+
+    # ---- Gibbs sweeps from the registration result --------------------------
+    s = full_res_handle()
     s.sampler_run(args.seed, 0, max(args.warmup, 0) or 0)  # untimed warmup sweeps
     t0 = time.perf_counter()
     s.sampler_run(args.seed + 1, 0, args.steps)
@@ -140,7 +242,12 @@ def run(args):
     roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
     roof["traffic"] = MEASURED_TRAFFIC.get((Nx, Ny, Nz), {}).get(dom)
 
-    # EM iteration kernel times
+    # EM iteration at full resolution: wall-clock and kernel times
+    reg = full_res_handle()
+    reg.EM(2, -1e30, 1, 0)
+    t0 = time.perf_counter()
+    reg.EM(20, -1e30, 1, 0)
+    em_it_ms = 1e3 * (time.perf_counter() - t0) / 20
     reg.set_profiling(True)
     reg.EM(2, -1e30, 1, 0)
     em_k = {k: round(1e3 * v[0] / max(v[1], 1), 2) for k, v in reg.get_kernel_times().items()}
@@ -160,23 +267,22 @@ def run(args):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "Gibbs sweep, 3D MI registration 160x192x160, 32 bins, "
-                               "after cold-start EM + topology correction",
+                               "after cold-start pyramid EM + topology correction",
                    "shape": [Nx, Ny, Nz], "bins": prob.K, "gamma_per_voxel": 1e-6,
                    "alpha": prob.alpha, "seed": args.seed},
         "roofline": roof,
-        "em": {"cold_start_s": round(em_s, 4), "iterations": int(len(trace)),
-               "ms_per_iteration": round(1e3 * em_s / max(len(trace), 1), 3),
-               "convergence_th": args.convergence_th, "final_mlp": float(trace[-1, 3]),
-               "displacement_rms_vox": round(disp_rms, 4), "kernel_avg_us": em_k},
-        "topology": {"seconds": round(topo_s, 4), "folds_before": folds_before,
-                     "folds_after": folds_after, "fold_detect_s": round(fold_s, 5),
-                     "outer_inner": [int(topo.last_stats[0]), int(topo.last_stats[1])]},
+        "registration": reg_out,
+        "em_full_res": {"ms_per_iteration": round(em_it_ms, 4), "kernel_avg_us": em_k},
         "kernels": kernels,
         "input_gen_s": round(gen_s, 2),
     }
     if not args.no_cpu:
         log("CPU baseline (oracle, all host cores)")
         out["cpu_baseline"] = cpu_baseline(prob, nv)
+        if "gpu_s" in reg_out:
+            est = cpu_em_estimate(pair, args, reg_out["iterations"], reg_out["resolutions"])
+            reg_out["cpu_oracle_estimate"] = est
+            reg_out["speedup_vs_cpu"] = round(est["seconds"] / reg_out["gpu_s"], 1)
     return out
 
 
@@ -187,7 +293,7 @@ def cpu_baseline(prob, nv):
     import pycore
     from binder_amd import synth
 
-    th = os.cpu_count() or 1
+    th = host_threads()
     r = synth.build_registration(pycore, prob, threads=th)
     t0 = time.perf_counter()
     r.sampler(1, 0, 1, 0, 1, 1, 1, 1)  # probe: one sweep
@@ -222,7 +328,10 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--em-iters", type=int, default=200)
+    ap.add_argument("--em-iters", type=int, default=1000,
+                    help="EM iterations per level, times the level's resolution "
+                         "(run_BINDER --iterations)")
+    ap.add_argument("--resolutions", type=int, nargs="+", default=[8, 4, 2, 1])
     ap.add_argument("--convergence-th", type=float, default=2.5e-5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--quick", action="store_true",

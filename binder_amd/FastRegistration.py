@@ -18,7 +18,7 @@ gamma and D per level) is host NumPy, as in the reference.
 import numpy as np
 
 from . import _core as Registration
-from . import nifti, resample, utils
+from . import nifti, utils
 
 
 def _apply_affine(aff, pts):
@@ -191,11 +191,13 @@ class FastRegistration:
         self.gamma_y = base * self.current_nodes_voxel_size[1] ** 2
         self.gamma_z = base * self.current_nodes_voxel_size[2] ** 2
 
-    def run(self, resolutions, max_number_of_iterations):
+    def run(self, resolutions, max_number_of_iterations, level_iterations=None):
         """EM at each resolution of the pyramid, then (optionally) topology
-        correction (FastRegistration.py:443-546)."""
+        correction (FastRegistration.py:443-546).  `level_iterations` (not in
+        the reference) replaces the per-level cap
+        max_number_of_iterations * resolution with explicit counts."""
         import time
-        for resolution in resolutions:
+        for li, resolution in enumerate(resolutions):
             self.move_resolution(resolution)
             self.deformedVoxelPositions = np.ascontiguousarray(self.deformedVoxelPositions)
             t0 = time.perf_counter()
@@ -205,7 +207,10 @@ class FastRegistration:
                                                 self.num_voxel_intensity_levels, self.threads,
                                                 np.ascontiguousarray(self.binned_nodes).ravel(),
                                                 np.ascontiguousarray(self.binned_voxels).ravel())
-            self.registratorObj.EM(max_number_of_iterations * self.current_resolution,
+            cap = max_number_of_iterations * self.current_resolution
+            if level_iterations is not None:
+                cap = int(level_iterations[li])
+            self.registratorObj.EM(cap,
                                    self.convergence_th,
                                    update_likelihood_parameters=self.update_likelihood_parameters,
                                    debug=self.debug)
@@ -263,8 +268,9 @@ class FastRegistration:
 
         cur = self.current_resolution
         if not self.upsample_in_frequency_domain or cur - res < 0:
-            self.deformedVoxelPositions = resample.resize(
-                self.deformedVoxelPositions, self.voxels_shape, order=1) * (cur / res)
+            # skimage resize(order=1) semantics, on the GPU
+            self.deformedVoxelPositions = Registration.resize_field(
+                self.deformedVoxelPositions, self.voxels_shape, cur / res)
         elif res != cur:
             full = self.full_voxels_pos[::cur, ::cur, ::cur, :] / cur
             delta = np.ascontiguousarray(self.deformedVoxelPositions - full)

@@ -368,10 +368,29 @@ def fold_mask_3D(field, Nx, Ny, Nz, voxel_size_x, voxel_size_y, voxel_size_z, e1
     return mask.astype(bool), int(cnt.value)
 
 
+_resize = _sig("binder_resize_field", _c_int, _vp, _c_int, _c_int, _c_int, _c_int, _c_int,
+               _c_int, _c_int, _c_double, _vp)
+
+
+def resize_field(field, output_shape, scale=1.0):
+    """Field [nx,ny,nz(,comps)] resampled to output_shape (3 ints) on the GPU
+    as skimage.transform.resize(order=1) does, times `scale`
+    (BINDER/FastRegistration.py:566-568).  float64."""
+    f = _arr(field, np.float64)
+    if f.ndim not in (3, 4) or len(output_shape) != 3:
+        raise ValueError("field must be [nx,ny,nz] or [nx,ny,nz,comps], shape 3 ints")
+    comps = 1 if f.ndim == 3 else f.shape[3]
+    shp = tuple(int(v) for v in output_shape)
+    out = np.empty(shp + f.shape[3:], np.float64)
+    _check(_resize(_ptr(f), f.shape[0], f.shape[1], f.shape[2], comps, *shp, float(scale),
+                   _ptr(out)))
+    return out
+
+
 def exported_symbols():
     """Names of the C entry points this module binds (for ABI checks)."""
     return [
-        "binder_last_error", "binder_backend", "binder_philox4x32_10",
+        "binder_last_error", "binder_backend", "binder_philox4x32_10", "binder_resize_field",
         "binder_registration_create",
         "binder_registration_destroy", "binder_registration_set_mi_likelihood",
         "binder_registration_set_nonlinear_transformation", "binder_registration_em",

@@ -26,6 +26,8 @@ SOURCES = {
     "smoothing.hip": [],
     "registration.hip": [],
     "topology.hip": ["-ffp-contract=off"],
+    # FP64 resampling: same operation order as the host reference restatement
+    "resample.hip": ["-ffp-contract=off"],
 }
 
 

@@ -62,12 +62,16 @@ def initialize_fourier_domain_smoothing(size, DCT=True, finite_difference_order=
         half = int(np.floor(len(D) / 2)) + 1
         return (D[:half], D) if return_full_matrix else D[:half]
     dims = list(size[:number_of_dimensions])
+    nd = len(dims)
     if DCT:
-        full = [np.arange(2 * n) for n in dims]
-        grids = np.meshgrid(*full, indexing="ij") if return_full_matrix else \
-            np.meshgrid(*[np.arange(n) for n in dims], indexing="ij")
-        D = sum((2.0 - 2 * np.cos(np.pi * g / n)) / vs[a] ** 2
-                for a, (g, n) in enumerate(zip(grids, dims)))
+        # separable: the per-axis terms are 1-D and broadcast (same values and
+        # the same x + y + z summation order as evaluating on a full grid)
+        D = None
+        for a, n in enumerate(dims):
+            k = np.arange(2 * n if return_full_matrix else n, dtype=np.float64)
+            t = (2.0 - 2 * np.cos(np.pi * k / n)) / vs[a] ** 2
+            t = t.reshape([-1 if b == a else 1 for b in range(nd)])
+            D = t if D is None else D + t
         D = D ** order
         if return_full_matrix:
             return D[tuple(slice(0, n) for n in dims)], D

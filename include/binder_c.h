@@ -245,6 +245,15 @@ int binder_topology_correction_3d(const double *h_init, double e1, double e2,
                                   int Ny_v, int Nz_v, double *out,
                                   double *stats);
 
+/* Pyramid resampling of a field: skimage.transform.resize(field, shape,
+ * order=1) as FastRegistration.move_resolution calls it
+ * (BINDER/FastRegistration.py:566-568), times `scale`.  in: [nx,ny,nz,comps],
+ * out: [nxo,nyo,nzo,comps], FP64, host buffers.  Gaussian anti-aliasing on
+ * shrinking axes, linear interpolation on the pixel extent, mirrored edges,
+ * clipped to the input's range. */
+int binder_resize_field(const double *in, int nx, int ny, int nz, int comps,
+                        int nxo, int nyo, int nzo, double scale, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -269,6 +269,14 @@ def topo_integrate(g, mean3, shape, voxel_sizes):
     return out
 
 
+def resize_field(field, output_shape, scale=1.0):
+    """Host restatement of the pyramid resampling (same surface as _core)."""
+    import sys
+    sys.path.insert(0, os.path.dirname(HERE))
+    from binder_amd import resample
+    return resample.resize(field, output_shape, order=1) * scale
+
+
 def spline(order, d):
     out = np.empty(4)
     _spline(int(order), float(d), _p(out))

@@ -177,3 +177,35 @@ def t1_registration(backend_module, monkeypatch, tmp_path):
 def test_t1_registration_oracle(orc, monkeypatch, tmp_path):
     start, em, mc = t1_registration(orc, monkeypatch, tmp_path)
     assert em < start and mc < start and mc < em
+
+
+def run_cli(tmp_path, extra):
+    from binder_amd.cli import run_binder
+    out = tmp_path / "out"
+    args = ["-f", os.path.join(GOLDEN, "T1w_1_8mm.nii.gz"),
+            "-m", os.path.join(GOLDEN, "T1w_2_8mm.nii.gz"), "-o", str(out),
+            "--resolutions", "2", "1", "--iterations", "4", "--bins-fixed", "32",
+            "--bins-moving", "32", "--sampler", "--number-of-burnin", "2",
+            "--number-of-samples", "4", "--save-warped-image", "--save-binned-images",
+            "--save-masks", "--save-likelihood-parameters"] + extra
+    assert run_binder.main(args) == 0
+    shape = (22, 24, 24)
+    for name, shp in [("warp.nii.gz", shape + (3,)), ("deformation_field.nii.gz", shape + (3,)),
+                      ("warped.nii.gz", shape), ("binned_warped.nii.gz", shape),
+                      ("binned_fixed.nii.gz", shape), ("mean_warp_sample.nii.gz", shape + (3,)),
+                      ("mean_deformation_field_sample.nii.gz", shape + (3,)),
+                      ("covariance_field_sample.nii.gz", shape + (3, 3)),
+                      ("warped_sample.nii.gz", shape), ("mask_fixed.nii.gz", shape),
+                      ("mask_moving.nii.gz", shape)]:
+        img = nifti.load(str(out / name))
+        assert img.shape == shp, name
+        assert np.all(np.isfinite(img.get_fdata())), name
+    assert (out / "likelihood_parameters.npz").exists()
+    return out
+
+
+def test_cli_oracle(orc, monkeypatch, tmp_path):
+    """run_BINDER's flags and outputs (BINDER/cli/run_BINDER.py), oracle backend."""
+    import binder_amd.FastRegistration as fr
+    monkeypatch.setattr(fr, "Registration", orc)
+    run_cli(tmp_path, ["--no-topology-correction"])
