@@ -32,8 +32,7 @@ FP32_PEAK_TFLOPS = 157.3
 # per-kernel table: profiles/gibbs_sweep_160x192x160_hbm_traffic.md.
 MEASURED_TRAFFIC = {
     (160, 192, 160): {
-        "estep_sample": (46214.5 * 2.0 + 74000.0) * 1024,
-        "estep": (46547.9 * 2.0 + 74000.0) * 1024,
+        "estep_sample": (46934.6 * 2.0 + 74000.0) * 1024,
     },
 }
 
