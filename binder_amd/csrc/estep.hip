@@ -117,6 +117,35 @@ __device__ __forceinline__ void load_columns(const EStepArgs &a, int ix, int iy,
   }
 }
 
+// Same words as load_columns, with 32-bit 24-bit-multiply indexing (the
+// host guarantees (Ny+2)(Nz+5) < 2^24 and the padded grid < 2^31 words).
+template <bool IS2D>
+__device__ __forceinline__ void load_columns32(const EStepArgs &a, int ix, int iy,
+                                               int iz, uint32_t q[16]) {
+  const uint32_t SX = IS2D ? (uint32_t)a.Yp : (uint32_t)a.Yp * (uint32_t)a.Zq;
+  const uint32_t SY = IS2D ? 1u : (uint32_t)a.Zq;
+  const uint32_t pz = IS2D ? 0u : (uint32_t)(clampi(iz - 1, -4, a.nzn) + 4);
+  uint32_t ox[4], oy[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    ox[k] = __umul24((uint32_t)(clampi(ix - 1 + k, -1, a.nxn) + 1), SX);
+    oy[k] = __umul24((uint32_t)(clampi(iy - 1 + k, -1, a.nyn) + 1), SY) + pz;
+  }
+#pragma unroll
+  for (int xa = 0; xa < 4; ++xa)
+#pragma unroll
+    for (int yb = 0; yb < 4; ++yb) {
+      const uint32_t idx = ox[xa] + oy[yb];
+      q[xa * 4 + yb] = IS2D ? (uint32_t)a.bins2d[idx] : a.quads[idx];
+    }
+}
+
+// Uniform in (0, 1): the 24 high bits of a Philox word, centred in their
+// interval, so a draw never lands exactly on 0 (a zero-weight node) or 1.
+__device__ __forceinline__ float u01_mid(uint32_t v) {
+  return ((float)(v >> 8) + 0.5f) * (1.0f / 16777216.0f);
+}
+
 // q[xa*4 + yb] for a runtime slice index xa, without dynamic register indexing
 __device__ __forceinline__ void slice_columns(const uint32_t q[16], int xa,
                                               uint32_t s[4]) {
@@ -367,6 +396,112 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
     __syncthreads();
   }
   if (threadIdx.x == 0) a.mllpart[lb] = -kLn2 * red[0];
+}
+
+// Sampling E-step (any K): one categorical node draw per voxel from the
+// posterior w_j / Z over the 64 (3D) or 16 (2D) neighbours, in the node
+// order of the reference's CDF scan (x, then y, then z;
+// Registration.cpp:278-307).  Column sums col = (w_x w_y) * sum_z w_z theta
+// are accumulated in that order into cum_0..cum_15, Z = cum_15; the draw
+// target r Z (r in (0, 1)) selects the first column with cum >= target by
+// counting columns below it, then the node inside the column with partial
+// sums formed by the same operations, so the scan always ends inside the
+// selected column.  LDS: theta columns (stride K+1) | 32-bit counts [K*L].
+template <bool IS2D>
+__global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int KL = a.K * a.L;
+  const int Kp = a.K + 1;
+  float *th = reinterpret_cast<float *>(smem);
+  unsigned int *cnt = reinterpret_cast<unsigned int *>(th + ((a.L * Kp + 3) & ~3));
+  for (int i = threadIdx.x; i < KL; i += blockDim.x) {
+    cnt[i] = 0u;
+    th[(i / a.K) * Kp + (i % a.K)] = a.thT[i];
+  }
+  __syncthreads();
+
+  constexpr int NZ = IS2D ? 1 : 4;
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const long long per =
+      ((a.nv + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
+  const long long vbeg = (long long)lb * per;
+  const long long vend = vbeg + per < a.nv ? vbeg + per : a.nv;
+  for (long long i = vbeg + threadIdx.x; i < vend; i += blockDim.x) {
+    const int u = a.bv[i];
+    const float *thu = th + u * Kp;
+    float tx, ty, tz = 0.f;
+    const int ix = safe_floor(a.d[i], tx);
+    const int iy = safe_floor(a.d[a.nv + i], ty);
+    int iz = 0;
+    if (!IS2D) iz = safe_floor(a.d[2 * a.nv + i], tz);
+    float wx[4], wy[4], wz[4] = {1.f, 0.f, 0.f, 0.f};
+    spline3(tx, wx);
+    spline3(ty, wy);
+    if (!IS2D) spline3(tz, wz);
+    uint32_t q[16];
+    load_columns32<IS2D>(a, ix, iy, iz, q);
+
+    float cum[16];
+    float acc = 0.f;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      float zy = wz[0] * thu[q[c] & 0xFF];
+#pragma unroll
+      for (int zc = 1; zc < NZ; ++zc) zy = fmaf(wz[zc], thu[(q[c] >> (8 * zc)) & 0xFF], zy);
+      acc = fmaf(wx[c >> 2] * wy[c & 3], zy, acc);
+      cum[c] = acc;
+    }
+    const float Z = acc;
+    const u32x4 rr = philox4x32_10(
+        u32x4{(uint32_t)i, (uint32_t)(i >> 32), a.sweep, kStreamEStep}, a.k0, a.k1);
+    const float target = u01_mid(rr.x) * Z;
+    // column: the first whose cumulative sum reaches the target.  One pass
+    // over the (non-decreasing) cumulative sums carries the count of columns
+    // below the target, the last sum below it and the next column's word and
+    // weight (running selects: no indexed access to the arrays)
+    int sel = 0;
+    float before = 0.f;
+    uint32_t qs = q[0];
+    float wxy = wx[0] * wy[0];
+#pragma unroll
+    for (int c = 0; c < 15; ++c) {
+      const bool below = cum[c] < target;
+      sel += below ? 1 : 0;
+      before = below ? cum[c] : before;
+      qs = below ? q[c + 1] : qs;
+      wxy = below ? wx[(c + 1) >> 2] * wy[(c + 1) & 3] : wxy;
+    }
+    const int xa = sel >> 2, yb = sel & 3;
+    // node inside the column, partial sums with the column's own operations
+    int zsel = 0;
+    if (!IS2D) {
+      float zy = wz[0] * thu[qs & 0xFF];
+#pragma unroll
+      for (int zc = 1; zc < NZ; ++zc) {
+        zsel += fmaf(wxy, zy, before) < target ? 1 : 0;
+        zy = fmaf(wz[zc], thu[(qs >> (8 * zc)) & 0xFF], zy);
+      }
+    }
+    if (Z > 0.f) {
+      a.dream[i] = (float)(ix - 1 + xa);
+      a.dream[a.nv + i] = (float)(iy - 1 + yb);
+      if (!IS2D) a.dream[2 * a.nv + i] = (float)(iz - 1 + zsel);
+      const int v = (qs >> (8 * zsel)) & 0xFF;
+      atomicAdd(cnt + v * a.L + u, 1u);
+    } else {
+      // no draw (all weights zero): the reference leaves the voxel's reset
+      // dream location (0) and adds nothing to the histogram
+      a.dream[i] = 0.f;
+      a.dream[a.nv + i] = 0.f;
+      if (!IS2D) a.dream[2 * a.nv + i] = 0.f;
+    }
+  }
+
+  __syncthreads();
+  unsigned long long *out = a.hpart + (long long)lb * KL;
+  for (int i = threadIdx.x; i < KL; i += blockDim.x)
+    out[i] = (unsigned long long)cnt[i] << kHistShift;
+  if (threadIdx.x == 0) a.mllpart[lb] = 0.0;
 }
 
 // EM E-step for K <= 32 (same outputs as k_estep<IS2D, false>).  LDS:
@@ -626,7 +761,15 @@ void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
   const size_t lds = (size_t)a.K * a.L * sizeof(unsigned long long) +
                      (size_t)(a.K + 1) * a.L * sizeof(float);
   dim3 g(blocks), b(256);
-  if (!sample && a.K <= kEm32MaxK) {
+  if (sample) {
+    const size_t ldss = (size_t)(((a.K + 1) * a.L + 3) & ~3) * sizeof(float) +
+                        (size_t)a.K * a.L * sizeof(unsigned int);
+    if (is2d) hipLaunchKernelGGL(k_esample<true>, g, b, ldss, s, a);
+    else hipLaunchKernelGGL(k_esample<false>, g, b, ldss, s, a);
+    HIP_LAUNCH_OK();
+    return;
+  }
+  if (a.K <= kEm32MaxK) {
     const size_t lds32 = (size_t)a.K * a.L * sizeof(unsigned long long) +
                          (size_t)(((a.K + 1) * a.L + 3) & ~3) * sizeof(float) +
                          (size_t)(kEm32Threads / 64) * kEm32MaxK * 64 * sizeof(float);

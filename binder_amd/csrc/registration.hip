@@ -281,6 +281,11 @@ int binder_registration_set_mi_likelihood(binder_registration *r, double alpha,
     r->alpha = alpha;
     r->bv.upload(v8.data(), v8.size(), r->s());
     const int Xp = r->nxn + 2, Yp = r->nyn + 2;
+    // the E-step indexes the padded moving grid with 32-bit offsets and
+    // 24-bit multiplies
+    const long long slab = r->is2d ? (long long)Yp : (long long)Yp * (r->nzn + 5);
+    BINDER_REQUIRE(slab < (1ll << 24) && (long long)Xp * slab < (1ll << 31),
+                   "moving grid too large for the E-step's 32-bit indexing");
     if (r->is2d) {
       std::vector<uint8_t> p((size_t)Xp * Yp, 0);
       for (int x = 0; x < r->nxn; ++x)
