@@ -334,6 +334,32 @@ class topology_correction_3D:
         return out
 
 
+_topo2 = _sig("binder_topology_correction_2d", _c_int, _vp, _c_double, _c_double, _c_int,
+              _c_int, _c_int, _vp, _c_int, _c_int, _vp, _vp)
+
+
+class topology_correction_2D:
+    """topology_correction_2D().correct_topology(h_init, e1, e2, max_outer_it,
+    max_it, threads, voxel_sizes, Nx_v, Ny_v)
+    (BINDER/src/cpu/bindings.cpp:449-478); FP64 on the GPU."""
+
+    def __init__(self):
+        self.last_stats = None
+
+    def correct_topology(self, h_init, e1, e2, max_outer_it, max_it, threads, voxel_sizes,
+                         Nx_v, Ny_v):
+        h = _arr(h_init, np.float64)
+        vs = _arr(voxel_sizes, np.float64)
+        if h.size != int(Nx_v) * int(Ny_v) * 2 or vs.size < 2:
+            raise ValueError("h_init / voxel_sizes have the wrong size")
+        out = np.empty((int(Nx_v), int(Ny_v), 2), np.float64)
+        stats = np.zeros(6, np.float64)
+        _check(_topo2(_ptr(h), float(e1), float(e2), int(max_outer_it), int(max_it),
+                      int(threads), _ptr(vs), int(Nx_v), int(Ny_v), _ptr(out), _ptr(stats)))
+        self.last_stats = stats
+        return out
+
+
 def compute_jacobian_determinant_3D(field, Nx, Ny, Nz, voxel_size_x, voxel_size_y,
                                     voxel_size_z):
     """BINDER/src/cpu/bindings.cpp:564-593; FP64, bit-exact with the CPU."""
@@ -391,6 +417,7 @@ def exported_symbols():
     """Names of the C entry points this module binds (for ABI checks)."""
     return [
         "binder_last_error", "binder_backend", "binder_philox4x32_10", "binder_resize_field",
+        "binder_topology_correction_2d",
         "binder_registration_create",
         "binder_registration_destroy", "binder_registration_set_mi_likelihood",
         "binder_registration_set_nonlinear_transformation", "binder_registration_em",

@@ -127,13 +127,8 @@ __global__ __launch_bounds__(256) void k_jacobian3d(Field w, int Nx, int Ny,
 }
 
 // 2D Jacobian (utils.cpp:10-98): corners ff, bf, fb, bb.
-__global__ __launch_bounds__(256) void k_jacobian2d(const double *wx,
-                                                    const double *wy, int Nx,
-                                                    int Ny, double vol_scale,
-                                                    double *J) {
-  const long long nv = (long long)Nx * Ny;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
+__device__ double voxel_jacobian2d(const double *wx, const double *wy, long long i,
+                                   int Nx, int Ny, double vol_scale) {
   const int y = (int)(i % Ny), x = (int)(i / Ny);
   const double px = wx[i], py = wy[i];
   double total = 0.0;
@@ -153,7 +148,54 @@ __global__ __launch_bounds__(256) void k_jacobian2d(const double *wx,
     total += (X1 * Y2 - X2 * Y1) * vol_scale;
     valid++;
   }
-  J[i] = valid > 0 ? total / valid : 1.0;
+  return valid > 0 ? total / valid : 1.0;
+}
+
+__global__ __launch_bounds__(256) void k_jacobian2d(const double *wx,
+                                                    const double *wy, int Nx,
+                                                    int Ny, double vol_scale,
+                                                    double *J) {
+  const long long nv = (long long)Nx * Ny;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  J[i] = voxel_jacobian2d(wx, wy, i, Nx, Ny, vol_scale);
+}
+
+// Block partials of the 2D Jacobian: [0] min, [1] max, [2] #(J<e1), [3] #(J>e2).
+__global__ __launch_bounds__(256) void k_jacobian2d_stats(const double *wx,
+                                                          const double *wy, int Nx,
+                                                          int Ny, double vol_scale,
+                                                          double e1, double e2,
+                                                          double *part) {
+  const long long nv = (long long)Nx * Ny;
+  double mn = INFINITY, mx = -INFINITY, lo = 0, hi = 0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
+       i += stride) {
+    const double v = voxel_jacobian2d(wx, wy, i, Nx, Ny, vol_scale);
+    mn = fmin(mn, v);
+    mx = fmax(mx, v);
+    lo += v < e1 ? 1.0 : 0.0;
+    hi += v > e2 ? 1.0 : 0.0;
+  }
+  __shared__ double red[4][256];
+  const int t = threadIdx.x;
+  red[0][t] = mn;
+  red[1][t] = mx;
+  red[2][t] = lo;
+  red[3][t] = hi;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (t < st) {
+      red[0][t] = fmin(red[0][t], red[0][t + st]);
+      red[1][t] = fmax(red[1][t], red[1][t + st]);
+      red[2][t] += red[2][t + st];
+      red[3][t] += red[3][t + st];
+    }
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int k = 0; k < 4; ++k) part[blockIdx.x * 4 + k] = red[k][0];
 }
 
 // Vertex of each corner Jacobian of a cell in J_corner order fff, bff, fbf,
@@ -410,6 +452,173 @@ __global__ __launch_bounds__(256) void k_divergence(const double *g, int n,
                      (gy[((long long)x * Ny + ym) * Nz + z] - gy[i]) +
                      (gz[((long long)x * Ny + y) * Nz + zm] - gz[i]);
     s[i] = v;
+  }
+}
+
+// ---- 2D topology correction (topology_correction_2D.cpp:17-430) -------------
+// Corner Jacobians Jff, Jbf, Jfb, Jbb of each cell (:156-200) -> 4-bit
+// out-of-range mask; block partial counts of cells with any corner below e1
+// / above e2.
+__global__ __launch_bounds__(256) void k_corner_mask2d(const double *wx,
+                                                       const double *wy, int Nx,
+                                                       int Ny, double vol_scale,
+                                                       double e1, double e2,
+                                                       uint8_t *cmask,
+                                                       unsigned long long *part) {
+  const int Cy = Ny - 1;
+  const long long ncell = (long long)(Nx - 1) * Cy;
+  unsigned long long lo = 0, hi = 0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+       cell < ncell; cell += stride) {
+    const int y = (int)(cell % Cy), x = (int)(cell / Cy);
+    const long long i00 = (long long)x * Ny + y, i01 = i00 + 1, i10 = i00 + Ny,
+                    i11 = i10 + 1;
+    const double wx00 = wx[i00], wx01 = wx[i01], wx10 = wx[i10], wx11 = wx[i11];
+    const double wy00 = wy[i00], wy01 = wy[i01], wy10 = wy[i10], wy11 = wy[i11];
+    const double J[4] = {
+        ((wx10 - wx00) * (wy01 - wy00) - (wx01 - wx00) * (wy10 - wy00)) * vol_scale,
+        ((wx10 - wx00) * (wy11 - wy10) - (wx11 - wx10) * (wy10 - wy00)) * vol_scale,
+        ((wx11 - wx01) * (wy01 - wy00) - (wx01 - wx00) * (wy11 - wy01)) * vol_scale,
+        ((wx11 - wx01) * (wy11 - wy10) - (wx11 - wx10) * (wy11 - wy01)) * vol_scale};
+    uint8_t m = 0;
+    bool l = false, h = false;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      l = l || J[c] < e1;
+      h = h || J[c] > e2;
+      if (J[c] < e1 || J[c] > e2) m |= (uint8_t)(1u << c);
+    }
+    lo += l ? 1 : 0;
+    hi += h ? 1 : 0;
+    cmask[cell] = m;
+  }
+  __shared__ unsigned long long red[2][256];
+  const int t = threadIdx.x;
+  red[0][t] = lo;
+  red[1][t] = hi;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (t < st) {
+      red[0][t] += red[0][t + st];
+      red[1][t] += red[1][t + st];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    part[blockIdx.x * 2 + 0] = red[0][0];
+    part[blockIdx.x * 2 + 1] = red[1][0];
+  }
+}
+
+// compute_gradients_2D (:202-245): g[(n*2+a)][p] = (w_n(p+e_a) - w_n(p))/vs_n,
+// periodic.
+__global__ __launch_bounds__(256) void k_gradients2d(const double *wx,
+                                                     const double *wy, int Nx,
+                                                     int Ny, double vs0,
+                                                     double vs1, double *g) {
+  const long long nv = (long long)Nx * Ny;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
+       i += stride) {
+    const int y = (int)(i % Ny), x = (int)(i / Ny);
+    const int x2 = x + 1 > Nx - 1 ? x + 1 - Nx : x + 1;
+    const int y2 = y + 1 > Ny - 1 ? y + 1 - Ny : y + 1;
+    const long long ix = (long long)x2 * Ny + y, iy = (long long)x * Ny + y2;
+    g[0 * nv + i] = (wx[ix] - wx[i]) / vs0;
+    g[1 * nv + i] = (wx[iy] - wx[i]) / vs0;
+    g[2 * nv + i] = (wy[ix] - wy[i]) / vs1;
+    g[3 * nv + i] = (wy[iy] - wy[i]) / vs1;
+  }
+}
+
+// Flagged cells -> hyperplane buckets t = 2x + y: every lexicographically
+// earlier cell within {-1,0,1}^2 has a smaller t and two cells of one
+// hyperplane never share a gradient entry.
+__global__ void k_bucket2d(const uint8_t *cmask, int Nx, int Ny, int *cnt,
+                           int *cursor, int *list, int scatter) {
+  const int Cy = Ny - 1;
+  const long long ncell = (long long)(Nx - 1) * Cy;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+       c += stride) {
+    if (!cmask[c]) continue;
+    const int y = (int)(c % Cy), x = (int)(c / Cy);
+    if (scatter) list[atomicAdd(&cursor[2 * x + y], 1)] = (int)c;
+    else atomicAdd(&cnt[2 * x + y], 1);
+  }
+}
+
+// Column offsets of each 2D corner (get_jac_off_2D, :394-410).
+__constant__ int kOff2[4][2][2] = {{{0, 0}, {0, 0}}, {{0, 0}, {1, 0}},
+                                   {{0, 1}, {0, 0}}, {{0, 1}, {1, 0}}};
+
+// limit_gradients_2D (:247-318) in hyperplane order, one workgroup.
+__global__ __launch_bounds__(1024) void k_limit2d(const uint8_t *cmask,
+                                                  const int *start, int T,
+                                                  const int *list, int Nx,
+                                                  int Ny, double e1, double e2,
+                                                  double *g) {
+  const long long nv = (long long)Nx * Ny;
+  const int Cy = Ny - 1;
+  for (int t = 0; t < T; ++t) {
+    const int b0 = start[t], b1 = start[t + 1];
+    if (b0 == b1) continue;
+    for (int k = b0 + (int)threadIdx.x; k < b1; k += blockDim.x) {
+      const int cell = list[k];
+      const int y = cell % Cy, x = cell / Cy;
+      const uint8_t m = cmask[cell];
+      for (int c = 0; c < 4; ++c) {
+        if (!((m >> c) & 1)) continue;
+        long long loc[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+          loc[a] = (long long)(x + kOff2[c][a][0]) * Ny + (y + kOff2[c][a][1]);
+        double J[4];
+#pragma unroll
+        for (int n1 = 0; n1 < 2; ++n1)
+#pragma unroll
+          for (int n2 = 0; n2 < 2; ++n2) J[n1 * 2 + n2] = g[(n1 * 2 + n2) * nv + loc[n2]];
+        const double I[4] = {1, 0, 0, 1};
+        double alpha = 0.0;
+        double det = J[0] * J[3] - J[1] * J[2];
+        int guard = 0;
+        while ((det > e2 || det < e1) && guard < 64) {
+          alpha += 0.1;
+          if (alpha > 1.0) alpha = 1.0;
+          double Jn[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) Jn[q] = (1 - alpha) * J[q] + alpha * I[q];
+          det = Jn[0] * Jn[3] - Jn[1] * Jn[2];
+          if (alpha >= 1.0) ++guard;
+        }
+        alpha += 0.1;
+        if (alpha > 1.0) alpha = 1.0;
+#pragma unroll
+        for (int n1 = 0; n1 < 2; ++n1)
+#pragma unroll
+          for (int n2 = 0; n2 < 2; ++n2)
+            g[(n1 * 2 + n2) * nv + loc[n2]] =
+                (1 - alpha) * J[n1 * 2 + n2] + alpha * I[n1 * 2 + n2];
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+// s(p) = sum_a g_{n,a}(p - e_a) - g_{n,a}(p) over a = x, y, periodic.
+__global__ __launch_bounds__(256) void k_divergence2d(const double *g, int n,
+                                                      int Nx, int Ny, double *s) {
+  const long long nv = (long long)Nx * Ny;
+  const double *gx = g + (n * 2 + 0) * nv, *gy = g + (n * 2 + 1) * nv;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
+       i += stride) {
+    const int y = (int)(i % Ny), x = (int)(i / Ny);
+    const int xm = x == 0 ? Nx - 1 : x - 1;
+    const int ym = y == 0 ? Ny - 1 : y - 1;
+    s[i] = (gx[(long long)xm * Ny + y] - gx[i]) + (gy[(long long)x * Ny + ym] - gy[i]);
   }
 }
 
@@ -865,6 +1074,167 @@ int binder_topology_correction_3d(const double *h_init, double e1, double e2,
     HIP_OK(hipStreamSynchronize(c.s));
     for (long long i = 0; i < c.nv; ++i)
       for (int k = 0; k < 3; ++k) out[i * 3 + k] = h[k * c.nv + i];
+    if (stats) {
+      stats[0] = outer;
+      stats[1] = inner_total;
+      stats[2] = jmin;
+      stats[3] = jmax;
+      stats[4] = (double)nb;
+      stats[5] = (double)na;
+    }
+  });
+}
+
+int binder_topology_correction_2d(const double *h_init, double e1, double e2,
+                                  int max_outer_it, int max_it, int threads,
+                                  const double *voxel_sizes, int Nx_v, int Ny_v,
+                                  double *out, double *stats) {
+  (void)threads;
+  return guarded([&] {
+    BINDER_REQUIRE(h_init && voxel_sizes && out, "null argument");
+    BINDER_REQUIRE(Nx_v >= 2 && Ny_v >= 2,
+                   "topology correction needs at least 2 voxels per axis");
+    OwnedStream st;
+    // the Poisson solve reuses the 3D machinery on the grid (Nx, 1, Ny):
+    // the unit axis adds 2 - 2 cos 0 = 0 to every eigenvalue
+    TopoCtx c;
+    c.s = st.s;
+    c.Nx = Nx_v;
+    c.Ny = 1;
+    c.Nz = Ny_v;
+    c.nv = (long long)Nx_v * Ny_v;
+    const long long nv = c.nv;
+    const long long ncell = (long long)(Nx_v - 1) * (Ny_v - 1);
+    const double vs0 = voxel_sizes[0], vs1 = voxel_sizes[1];
+    const double vol_scale = 1.0 / (vs0 * vs1);
+    c.blocks = (int)std::min<long long>((nv + 255) / 256, 4096);
+    c.part.alloc((size_t)c.blocks * 4);
+    c.cpart.alloc((size_t)c.blocks * 2);
+    {
+      std::vector<double> h((size_t)3 * nv, 0.0);
+      for (long long i = 0; i < nv; ++i) {
+        h[i] = h_init[2 * i];
+        h[nv + i] = h_init[2 * i + 1];
+      }
+      c.w.upload(h.data(), h.size(), c.s);
+      HIP_OK(hipStreamSynchronize(c.s));
+    }
+    double *wx = c.w.get(), *wy = c.w.get() + nv;
+    hipLaunchKernelGGL(k_scale_planes, dim3(c.blocks), dim3(256), 0, c.s, c.w.get(),
+                       nv, vs0, vs1, 1.0, 0);
+    HIP_LAUNCH_OK();
+    auto jstats = [&](double &jmin, double &jmax, long long &nb, long long &na) {
+      hipLaunchKernelGGL(k_jacobian2d_stats, dim3(c.blocks), dim3(256), 0, c.s, wx, wy,
+                         Nx_v, Ny_v, vol_scale, e1, e2, c.part.get());
+      HIP_LAUNCH_OK();
+      c.hpart.resize((size_t)c.blocks * 4);
+      c.part.download(c.hpart.data(), c.hpart.size(), c.s);
+      HIP_OK(hipStreamSynchronize(c.s));
+      jmin = INFINITY;
+      jmax = -INFINITY;
+      double lo = 0, hi = 0;
+      for (int b = 0; b < c.blocks; ++b) {
+        jmin = std::min(jmin, c.hpart[b * 4 + 0]);
+        jmax = std::max(jmax, c.hpart[b * 4 + 1]);
+        lo += c.hpart[b * 4 + 2];
+        hi += c.hpart[b * 4 + 3];
+      }
+      nb = (long long)lo;
+      na = (long long)hi;
+    };
+    auto corner = [&](long long &lo, long long &hi) {
+      hipLaunchKernelGGL(k_corner_mask2d, dim3(c.blocks), dim3(256), 0, c.s, wx, wy,
+                         Nx_v, Ny_v, vol_scale, e1, e2, c.cmask.get(), c.cpart.get());
+      HIP_LAUNCH_OK();
+      c.hcpart.resize((size_t)c.blocks * 2);
+      c.cpart.download(c.hcpart.data(), c.hcpart.size(), c.s);
+      HIP_OK(hipStreamSynchronize(c.s));
+      lo = hi = 0;
+      for (int b = 0; b < c.blocks; ++b) {
+        lo += (long long)c.hcpart[b * 2];
+        hi += (long long)c.hcpart[b * 2 + 1];
+      }
+    };
+    double jmin, jmax;
+    long long nb, na;
+    jstats(jmin, jmax, nb, na);
+    int outer = 0, inner_total = 0;
+    if (!(jmin >= e1 && jmax <= e2)) {
+      c.cmask.alloc(ncell);
+      c.g.alloc((size_t)4 * nv);
+      c.sbuf.alloc(nv);
+      c.tbuf.alloc(nv);
+      c.ubuf.alloc(nv);
+      c.T = 2 * (Nx_v - 2) + (Ny_v - 2) + 1;
+      c.cnt.alloc(c.T);
+      c.start.alloc(c.T + 1);
+      c.cursor.alloc(c.T);
+      c.list.alloc(ncell);
+      c.dht_matrix(Nx_v, c.Jmat[0]);
+      c.dht_matrix(Ny_v, c.Jmat[2]);
+      const int gb = (int)std::min<long long>((ncell + 255) / 256, 4096);
+      while ((jmin < e1 || jmax > e2) && outer < max_outer_it) {
+        long long lo, hi;
+        corner(lo, hi);
+        int inner = 0;
+        while ((lo > 0 || hi > 0) && inner < max_it) {
+          hipLaunchKernelGGL(k_gradients2d, dim3(c.blocks), dim3(256), 0, c.s, wx, wy,
+                             Nx_v, Ny_v, vs0, vs1, c.g.get());
+          HIP_LAUNCH_OK();
+          // limit_gradients_2D in hyperplane order
+          c.cnt.zero(c.s);
+          hipLaunchKernelGGL(k_bucket2d, dim3(gb), dim3(256), 0, c.s, c.cmask.get(), Nx_v,
+                             Ny_v, c.cnt.get(), (int *)nullptr, (int *)nullptr, 0);
+          HIP_LAUNCH_OK();
+          hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, c.s, c.cnt.get(), c.T,
+                             c.start.get(), c.cursor.get());
+          HIP_LAUNCH_OK();
+          hipLaunchKernelGGL(k_bucket2d, dim3(gb), dim3(256), 0, c.s, c.cmask.get(), Nx_v,
+                             Ny_v, (int *)nullptr, c.cursor.get(), c.list.get(), 1);
+          HIP_LAUNCH_OK();
+          hipLaunchKernelGGL(k_limit2d, dim3(1), dim3(1024), 0, c.s, c.cmask.get(),
+                             c.start.get(), c.T, c.list.get(), Nx_v, Ny_v, e1, e2,
+                             c.g.get());
+          HIP_LAUNCH_OK();
+          // compute_mean_coordinates, then integrate_gradient_field_2D
+          hipLaunchKernelGGL(k_plane_sums, dim3(c.blocks), dim3(256), 0, c.s,
+                             Field{wx, wy, wy}, nv, c.part.get());
+          HIP_LAUNCH_OK();
+          c.hpart.resize((size_t)c.blocks * 3);
+          c.part.download(c.hpart.data(), c.hpart.size(), c.s);
+          HIP_OK(hipStreamSynchronize(c.s));
+          double mean[2] = {0, 0};
+          for (int b = 0; b < c.blocks; ++b)
+            for (int k = 0; k < 2; ++k) mean[k] += c.hpart[b * 3 + k];
+          for (int k = 0; k < 2; ++k) mean[k] /= (double)nv;
+          const double vsn[2] = {vs0, vs1};
+          for (int n = 0; n < 2; ++n) {
+            hipLaunchKernelGGL(k_divergence2d, dim3(c.blocks), dim3(256), 0, c.s,
+                               c.g.get(), n, Nx_v, Ny_v, c.sbuf.get());
+            HIP_LAUNCH_OK();
+            c.dpass(2, c.sbuf.get(), c.tbuf.get(), TEPI_STORE, 1, 0);
+            c.dpass(0, c.tbuf.get(), c.ubuf.get(), TEPI_INVLAMBDA, 1, 0);
+            c.dpass(0, c.ubuf.get(), c.tbuf.get(), TEPI_STORE, 1, 0);
+            c.dpass(2, c.tbuf.get(), c.w.get() + n * nv, TEPI_FINAL, vsn[n], mean[n]);
+          }
+          corner(lo, hi);
+          inner++;
+        }
+        inner_total += inner;
+        jstats(jmin, jmax, nb, na);
+        outer++;
+      }
+    }
+    hipLaunchKernelGGL(k_scale_planes, dim3(c.blocks), dim3(256), 0, c.s, c.w.get(), nv,
+                       vs0, vs1, 1.0, 1);
+    HIP_LAUNCH_OK();
+    std::vector<double> h((size_t)2 * nv);
+    c.w.download(h.data(), h.size(), c.s);
+    HIP_OK(hipStreamSynchronize(c.s));
+    for (long long i = 0; i < nv; ++i) {
+      out[2 * i] = h[i];
+      out[2 * i + 1] = h[nv + i];
+    }
     if (stats) {
       stats[0] = outer;
       stats[1] = inner_total;

@@ -245,6 +245,17 @@ int binder_topology_correction_3d(const double *h_init, double e1, double e2,
                                   int Ny_v, int Nz_v, double *out,
                                   double *stats);
 
+/* topology_correction_2D().correct_topology(h_init, e1, e2, max_outer_it,
+ *   max_it, threads, voxel_sizes, Nx_v, Ny_v)
+ * -- BINDER/src/cpu/bindings.cpp:449-478,
+ *    BINDER/src/cpu/topology_correction_2D.cpp:17-430.
+ * h_init/out: [Nx,Ny,2] grid-unit positions; voxel_sizes: 2 values; FP64.
+ * stats as for binder_topology_correction_3d. */
+int binder_topology_correction_2d(const double *h_init, double e1, double e2,
+                                  int max_outer_it, int max_it, int threads,
+                                  const double *voxel_sizes, int Nx_v, int Ny_v,
+                                  double *out, double *stats);
+
 /* Pyramid resampling of a field: skimage.transform.resize(field, shape,
  * order=1) as FastRegistration.move_resolution calls it
  * (BINDER/FastRegistration.py:566-568), times `scale`.  in: [nx,ny,nz,comps],

@@ -59,6 +59,17 @@ void orc_topo_limit(double *g, const double *Jc, int Nx, int Ny, int Nz,
                     double e1, double e2);
 void orc_topo_integrate(const double *g, const double *mean3, int Nx, int Ny,
                         int Nz, const double *vs, double *w_out);
+// 2D topology correction and its steps (topology_correction_2D.cpp).
+void orc_topology_2d(const double *h_init, double e1, double e2, int max_outer,
+                     int max_inner, const double *vs, int Nx, int Ny,
+                     double *out, double *stats);
+void orc_corner_jacobians_2d(const double *field, int Nx, int Ny, const double *vs,
+                             double e1, double e2, double *Jc_out, long long *counts);
+void orc_topo2_gradients(const double *w_phys, int Nx, int Ny, const double *vs,
+                         double *g_out);
+void orc_topo2_limit(double *g, const double *Jc, int Nx, int Ny, double e1, double e2);
+void orc_topo2_integrate(const double *g, const double *mean2, int Nx, int Ny,
+                         const double *vs, double *w_out);
 #ifdef __cplusplus
 }
 #endif

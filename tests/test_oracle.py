@@ -493,3 +493,94 @@ def test_oracle_up_sample_closed_form(orc, shape_in, shape_out):
         c = np.zeros(shape_out)
         c[:shape_in[0], :shape_in[1], :shape_in[2]] = dctn(f[..., n], norm="ortho")
         assert np.allclose(got[..., n], idctn(c, norm="ortho"), atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# 2D topology correction (topology_correction_2D.cpp:17-430)
+# ---------------------------------------------------------------------------
+def test_topology2d_integrate_matches_fft_closed_form(orc):
+    rng = np.random.default_rng(81)
+    shape = (7, 6)
+    vs = np.array([1.2, 0.9])
+    g = rng.standard_normal(shape + (2, 2))
+    m = np.array([2.0, -1.5])
+    got = orc.topo2_integrate(g, m, shape, vs)
+    E = [np.exp(2j * np.pi * np.fft.fftfreq(shape[0]))[:, None],
+         np.exp(2j * np.pi * np.fft.fftfreq(shape[1]))[None, :]]
+    den = sum(2 - 2 * np.real(e) for e in E)
+    den[0, 0] = 1.0
+    for n in range(2):
+        U = sum(np.conj(E[a] - 1) * np.fft.fft2(g[..., n, a]) for a in range(2)) / den
+        U[0, 0] = 0.0
+        exp = np.real(np.fft.ifft2(U)) * vs[n] + m[n]
+        assert np.allclose(got[..., n], exp, atol=1e-12)
+
+
+def test_topology2d_gradients_then_integrate_roundtrip(orc):
+    rng = np.random.default_rng(82)
+    shape = (9, 8)
+    vs = np.array([0.8, 1.3])
+    w = rng.standard_normal(shape + (2,)) * 3
+    g = orc.topo2_gradients(w, shape, vs)
+    back = orc.topo2_integrate(g, w.reshape(-1, 2).mean(0), shape, vs)
+    assert np.allclose(back, w, atol=1e-10)
+
+
+@pytest.mark.parametrize("fold", [False, True])
+def test_topology2d_limit_single_corner(orc, fold):
+    """As in 3D: the limited corner's two columns are the edge differences
+    that start at the cell's lower face, their determinant is the corner
+    Jacobian, and they become (1-a)J + aI one step past the first blend in
+    range."""
+    rng = np.random.default_rng(83)
+    shape = (6, 7)
+    vs = np.ones(2)
+    e1, e2 = 0.01, 100.0
+    pos = np.stack(np.indices(shape, dtype=np.float64), -1)
+    w = pos + rng.normal(0, 0.05, pos.shape)
+    cell = (2, 3)
+    if fold:
+        w[3, 4, 0] -= 1.6
+    Jc, _ = orc.corner_jacobians_2D(w.ravel(), *shape, vs, e1, e2)
+    Jc = Jc.reshape(shape[0] - 1, shape[1] - 1, 4)
+    g0 = orc.topo2_gradients(w, shape, vs)
+    vertices = [(0, 0), (1, 0), (0, 1), (1, 1)]
+    corners = [int(np.argmin(Jc[cell]))] if fold else range(4)
+    for c in corners:
+        jc = np.full_like(Jc, 1.0)
+        jc[cell + (c,)] = Jc[cell + (c,)] if fold else -1.0
+        g = orc.topo2_limit(g0, jc.ravel(), shape, e1, e2)
+        locs = []
+        for a in range(2):
+            v = [cell[k] + vertices[c][k] for k in range(2)]
+            v[a] = cell[a]
+            locs.append(tuple(v))
+        M = np.stack([g0[locs[a]][:, a] for a in range(2)], 1)
+        assert np.isclose(np.linalg.det(M), Jc[cell + (c,)], rtol=1e-12, atol=1e-14)
+        alpha, det = 0.0, np.linalg.det(M)
+        while det < e1 or det > e2:
+            alpha = min(alpha + 0.1, 1.0)
+            det = np.linalg.det((1 - alpha) * M + alpha * np.eye(2))
+        alpha = min(alpha + 0.1, 1.0)
+        new = (1 - alpha) * M + alpha * np.eye(2)
+        expect = g0.copy()
+        for a in range(2):
+            expect[locs[a]][:, a] = new[:, a]
+        assert np.allclose(g, expect, atol=1e-14, rtol=0)
+
+
+def test_topology2d_invariants(orc):
+    from binder_amd import synth
+    shape = (24, 20)
+    rng = np.random.default_rng(84)
+    pos = np.stack(np.indices(shape, dtype=np.float64), -1)
+    f = pos + synth.dct_displacement(shape, 2.5, rng, ncoef=4)
+    t = orc.topology_correction_2D()
+    Jc0, cnt0 = orc.corner_jacobians_2D(f.ravel(), *shape, np.ones(2), 0.01, 100)
+    assert cnt0[0] > 0  # the field folds
+    out = t.correct_topology(f.ravel(), 0.01, 100, 10, 10, 1, np.ones(2), *shape)
+    Jc, cnt = orc.corner_jacobians_2D(out.ravel(), *shape, np.ones(2), 0.01, 100)
+    assert cnt[0] == 0 and cnt[1] == 0 and Jc.min() >= 0.01
+    assert np.allclose(out.reshape(-1, 2).mean(0), f.reshape(-1, 2).mean(0), atol=1e-10)
+    same = t.correct_topology(pos.ravel(), 0.01, 100, 10, 10, 1, np.ones(2), *shape)
+    assert np.array_equal(same, pos) and t.last_stats[0] == 0
