@@ -334,6 +334,32 @@ class topology_correction_3D:
         return out
 
 
+_splw = _sig("binder_spline_weights", _c_int, _c_int, _c_double, _c_int, _vp)
+
+
+class Spline:
+    """Spline(order, is2D) with .spline(d) and .log_spline(d)
+    (BINDER/src/cpu/bindings.cpp:436-445): the 4 B-spline weights (or their
+    logs) of order 1, 2 or 3 at fractional distance d, as a list."""
+
+    def __init__(self, order, is2D):
+        if int(order) not in (1, 2, 3):
+            raise ValueError("spline order must be 1, 2 or 3")
+        self.order = int(order)
+        self.is2D = bool(is2D)
+
+    def _eval(self, d, log_space):
+        out = np.empty(4, np.float64)
+        _check(_splw(self.order, float(d), int(log_space), _ptr(out)))
+        return out.tolist()
+
+    def spline(self, d):
+        return self._eval(d, 0)
+
+    def log_spline(self, d):
+        return self._eval(d, 1)
+
+
 _topo2 = _sig("binder_topology_correction_2d", _c_int, _vp, _c_double, _c_double, _c_int,
               _c_int, _c_int, _vp, _c_int, _c_int, _vp, _vp)
 
@@ -417,7 +443,7 @@ def exported_symbols():
     """Names of the C entry points this module binds (for ABI checks)."""
     return [
         "binder_last_error", "binder_backend", "binder_philox4x32_10", "binder_resize_field",
-        "binder_topology_correction_2d",
+        "binder_topology_correction_2d", "binder_spline_weights",
         "binder_registration_create",
         "binder_registration_destroy", "binder_registration_set_mi_likelihood",
         "binder_registration_set_nonlinear_transformation", "binder_registration_em",

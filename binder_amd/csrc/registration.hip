@@ -483,6 +483,56 @@ int binder_registration_sampler(binder_registration *r, int seed, int N_b,
   return binder_registration_sampler_result(r, mean_out, cov_out);
 }
 
+// Spline(order, is2D).spline(d) / .log_spline(d)
+// (BINDER/src/cpu/Spline.cpp:44-145, BINDER/src/cpu/bindings.cpp:436-445):
+// B-spline weights of order 1, 2 or 3 at the order's node offsets for the
+// fractional distance d, and their logarithms with -1e100 for a weight that
+// is exactly zero.  Unused trailing entries are 0 (weights) / -1e100 (logs).
+int binder_spline_weights(int order, double d, int log_space, double *out4) {
+  return guarded([&] {
+    BINDER_REQUIRE(out4, "null argument");
+    BINDER_REQUIRE(order >= 1 && order <= 3, "spline order must be 1, 2 or 3");
+    const double kNeg = -1e100;
+    const double e = 1.0 - d;
+    if (!log_space) {
+      if (order == 3) {
+        out4[0] = e * e * e / 6.0;
+        out4[1] = 2.0 / 3.0 - d * d + d * d * d * 0.5;
+        out4[2] = 2.0 / 3.0 - e * e + e * e * e * 0.5;
+        out4[3] = d * d * d / 6.0;
+      } else if (order == 2) {
+        const double a = 1.0 - 2.0 * d, b = -2.0 * d - 1.0;
+        out4[0] = 0.125 * a * a;
+        out4[1] = 0.75 - d * d;
+        out4[2] = 0.125 * b * b;
+        out4[3] = 0.0;
+      } else {
+        out4[0] = e;
+        out4[1] = d;
+        out4[2] = out4[3] = 0.0;
+      }
+      return;
+    }
+    out4[0] = out4[1] = out4[2] = out4[3] = kNeg;
+    if (order == 3) {
+      out4[0] = 3.0 * std::log(e) - std::log(6.0);
+      out4[1] = std::log(2.0 / 3.0 - d * d + d * d * d / 2.0);
+      out4[2] = std::log(2.0 / 3.0 - e * e + e * e * e / 2.0);
+      out4[3] = d < 1e-300 ? kNeg : 3.0 * std::log(d) - std::log(6.0);
+    } else if (order == 2) {
+      if (d <= 0.5) {
+        const double a = 1.0 - 2.0 * d, b = -2.0 * d - 1.0;
+        out4[0] = std::log(0.125 * a * a);
+        out4[1] = std::log(0.75 - d * d);
+        out4[2] = std::log(0.125 * b * b);
+      }
+    } else {
+      out4[0] = d >= 1.0 ? kNeg : std::log(e);
+      out4[1] = d < 1e-300 ? kNeg : std::log(d);
+    }
+  });
+}
+
 int binder_registration_get_final_locations(binder_registration *r,
                                             float *out) {
   return guarded([&] {

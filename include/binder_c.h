@@ -245,6 +245,12 @@ int binder_topology_correction_3d(const double *h_init, double e1, double e2,
                                   int Ny_v, int Nz_v, double *out,
                                   double *stats);
 
+/* Spline(order, is2D).spline(d) / .log_spline(d)
+ * -- BINDER/src/cpu/bindings.cpp:436-445, BINDER/src/cpu/Spline.cpp:44-145.
+ * Writes the 4 weights (log_space = 0) or their logs (log_space = 1) of the
+ * order-1/2/3 B-spline at fractional distance d; host computation. */
+int binder_spline_weights(int order, double d, int log_space, double *out4);
+
 /* topology_correction_2D().correct_topology(h_init, e1, e2, max_outer_it,
  *   max_it, threads, voxel_sizes, Nx_v, Ny_v)
  * -- BINDER/src/cpu/bindings.cpp:449-478,

@@ -91,3 +91,34 @@ def test_dct_eigenvalues_closed_form():
            (2 - 2 * np.cos(np.pi * k[2] / 6)) / 0.25) ** 2
     assert np.allclose(D, exp)
     assert D[0, 0, 0] == 0.0
+
+
+def test_spline_binding(orc):
+    """Spline(order, is2D).spline / log_spline of the HIP library
+    (BINDER/tests/cpu/test_check_spline.py:5-76 properties, host code, no
+    GPU): partition of unity, exp(log) = weights, no NaN at the range edges,
+    and the oracle's weights (pinned to the reference build)."""
+    from binder_amd import _core
+    for dim in (2, 3):
+        for order in (1, 2, 3):
+            sp = _core.Spline(order, 1 if dim == 2 else 0)
+            ds = [0.4, 0.3] if dim == 2 else [0.2, 0.4, 0.1]
+            ws = [np.array(sp.spline(d))[:order + 1] for d in ds]
+            ls = [np.array(sp.log_spline(d))[:order + 1] for d in ds]
+            prod = ws[0][:, None, None] * ws[1][None, :, None]
+            if dim == 3:
+                prod = prod * ws[2][None, None, :]
+            assert np.isclose(prod.sum(), 1.0)
+            for w, lw in zip(ws, ls):
+                assert np.allclose(w, np.exp(lw))
+            for d in ds:
+                assert np.allclose(np.array(sp.spline(d))[:order + 1],
+                                   orc.spline(order, d)[:order + 1], rtol=0, atol=1e-15)
+            edge = [0.0, 1.0] if dim == 2 else [0.0, 1.0, -0.3]
+            ws = [np.array(sp.spline(d))[:order + 1] for d in edge]
+            prod = ws[0][:, None, None] * ws[1][None, :, None]
+            if dim == 3:
+                prod = prod * ws[2][None, None, :]
+            assert np.isclose(prod.sum(), 1.0)
+            for d in edge:
+                assert not np.any(np.isnan(np.array(sp.log_spline(d))[:order + 1]))
