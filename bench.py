@@ -32,7 +32,7 @@ FP32_PEAK_TFLOPS = 157.3
 # per-kernel table: profiles/gibbs_sweep_160x192x160_hbm_traffic.md.
 MEASURED_TRAFFIC = {
     (160, 192, 160): {
-        "estep_sample": (46934.6 * 2.0 + 74000.0) * 1024,
+        "estep_sample": (46422.6 * 2.0 + 74000.7) * 1024,
     },
 }
 
@@ -103,7 +103,7 @@ def registration(args, pair):
            "converged_iterations": counts,
            "level_seconds": [round(lv[2], 4) for lv in A.level_log]}
     if args.quick:
-        return A, F, out
+        return A, F, out, F
     # B: retrace the last level in chunks to find the first iteration within
     # tolerance (untimed; the field is read back after every chunk)
     B = FastRegistration(voxels=pair.fixed, nodes=pair.moving, **kw)
@@ -119,7 +119,10 @@ def registration(args, pair):
         step = min(chunk, counts[-1] - k)
         reg.EM(step, -1e30, 1, 0)
         k += step
-        if _rms(reg.get_final_locations_as_numpy_array(), F) < TOLERANCE_VOX_RMS:
+        f = reg.get_final_locations_as_numpy_array()
+        if B.top_cor_obj:  # compare what the pipeline would return here
+            f = B.correct_topology(f)
+        if _rms(f, F) < TOLERANCE_VOX_RMS:
             k_tol = k
             break
     # C: the timed pipeline, stopped there (topology correction included)
@@ -135,7 +138,7 @@ def registration(args, pair):
                                    "agreement is a GPU test on a reduced grid)",
                 "gpu_s": round(tt_s, 4), "iterations": counts[:-1] + [k_tol],
                 "rms_vox_at_stop": round(rms_c, 4)})
-    return A, F, out
+    return A, F, out, np.asarray(C.deformedVoxelPositions, np.float64)
 
 
 def cpu_em_estimate(pair, args, counts, levels):
@@ -184,7 +187,7 @@ def run(args):
     log(f"inputs {Nx}x{Ny}x{Nz} generated in {gen_s:.1f}s")
 
     # ---- plain pyramid registration and its time to tolerance -------------
-    A, F, reg_out = registration(args, pair)
+    A, F, reg_out, _ = registration(args, pair)
     t0 = time.perf_counter()
     _, folds = _core.fold_mask_3D(F.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0, e1=0.01)
     fold_s = time.perf_counter() - t0
@@ -307,6 +310,143 @@ def cpu_baseline(prob, nv):
                                       f"Welford) after a 1-sweep probe, {t:.2f} s"}
 
 
+def _dct_matrix(n):
+    k = np.arange(n)[:, None]
+    j = np.arange(n)[None, :]
+    C = np.sqrt(2.0 / n) * np.cos(np.pi * (2 * j + 1) * k / (2 * n))
+    C[0] /= np.sqrt(2.0)
+    return C  # C[k][j], orthonormal DCT-II
+
+
+def _mi(H):
+    p = H / H.sum()
+    pk = p.sum(1, keepdims=True)
+    pl = p.sum(0, keepdims=True)
+    nz = p > 0
+    return float(np.sum(p[nz] * np.log(p[nz] / (pk @ pl)[nz])))
+
+
+def c1_report(args):
+    """The 2D configuration (192x160, 32 bins): plain pyramid time to
+    tolerance on the GPU and the FP64 oracle running the same pipeline with
+    the same per-level iteration counts on all host cores."""
+    from binder_amd import synth
+    import binder_amd.FastRegistration as frm
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pycore
+
+    pair, _ = synth.config_problem("c1", seed=args.seed)
+    A, F, reg, stop_field = registration(args, pair)
+    th = host_threads()
+    saved = frm.Registration
+    frm.Registration = pycore
+    try:
+        O = frm.FastRegistration(voxels=pair.fixed, nodes=pair.moving, threads=th,
+                                 **dict(REG_KW, convergence_th=args.convergence_th))
+        t0 = time.perf_counter()
+        O.run(reg["resolutions"], args.em_iters, level_iterations=reg["iterations"])
+        cpu_s = time.perf_counter() - t0
+    finally:
+        frm.Registration = saved
+    reg["cpu_oracle"] = {"seconds": round(cpu_s, 3), "cores": th, "kind": "port",
+                         "sample": "FP64 oracle pipeline (pyramid EM + 2D topology "
+                                   "correction), same per-level iterations as the GPU run"}
+    reg["speedup_vs_cpu"] = round(cpu_s / reg["gpu_s"], 1)
+    reg["field_rms_vs_oracle_vox"] = round(_rms(stop_field, O.deformedVoxelPositions), 5)
+    # conditioning of the pipeline itself: the FP64 oracle against itself
+    # with the initial positions moved by N(0, 1e-4) voxel
+    frm.Registration = pycore
+    try:
+        P = frm.FastRegistration(voxels=pair.fixed, nodes=pair.moving, threads=th,
+                                 **dict(REG_KW, convergence_th=args.convergence_th))
+        P.full_voxels_pos = P.full_voxels_pos + np.random.default_rng(1).normal(
+            0.0, 1e-4, P.full_voxels_pos.shape)
+        P.deformedVoxelPositions = P.full_voxels_pos
+        P.run(reg["resolutions"], args.em_iters, level_iterations=reg["iterations"])
+    finally:
+        frm.Registration = saved
+    reg["oracle_self_rms_vox_at_1e-4_start_perturbation"] = round(
+        _rms(P.deformedVoxelPositions, O.deformedVoxelPositions), 5)
+    return {"report": "c1", "config": {"shape": list(pair.fixed.shape), "bins": 32,
+                                       "gamma_per_voxel": 1e-6, "seed": args.seed},
+            "registration": reg}
+
+
+def precision_report(args):
+    """Error of the GPU path against the FP64 oracle at 256x256x192 (the
+    precision variant): field, warped image, MI and objective after the same
+    EM iterations from the same start; fold counts on the same field; the
+    sampler's voxel-wise variance against its closed form."""
+    from binder_amd import _core, resample, synth
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pycore
+
+    th = host_threads()
+    t0 = time.perf_counter()
+    pair, prob = synth.config_problem("c3", seed=args.seed)
+    Nx, Ny, Nz = prob.shape
+    log(f"c3 inputs {prob.shape} in {time.perf_counter() - t0:.1f}s")
+    iters = args.precision_iters
+    g = synth.build_registration(_core, prob)
+    o = synth.build_registration(pycore, prob, threads=th)
+    g.EM(iters, -1e30, 1, 0)
+    t0 = time.perf_counter()
+    o.EM(iters, -1e30, 1, 0)
+    log(f"oracle EM {iters} iterations in {time.perf_counter() - t0:.1f}s")
+    fg = g.get_final_locations_as_numpy_array().astype(np.float64)
+    fo = o.get_final_locations_as_numpy_array()
+    d = fg - fo
+    field = {"rms_vox": float(np.sqrt(np.mean(np.sum(d ** 2, -1)))),
+             "max_vox": float(np.abs(d).max())}
+    tg, to = g.get_objective_trace(), o.get_objective_trace()
+    objective = {"mll_rel": float(abs(tg[-1, 0] - to[-1, 0]) / abs(to[-1, 0])),
+                 "prior_d_rel": float(abs(tg[-1, 2] - to[-1, 2]) / max(abs(to[-1, 2]), 1e-300)),
+                 "mlp_rel": float(abs(tg[-1, 3] - to[-1, 3]) / abs(to[-1, 3]))}
+    _, Hg = g.e_step(return_histogram=True)
+    _, Ho = o.e_step(return_histogram=True)
+    mi = {"gpu": _mi(Hg), "oracle": _mi(Ho)}
+    mi["rel"] = abs(mi["gpu"] - mi["oracle"]) / abs(mi["oracle"])
+    wg = resample.warp(pair.moving, np.moveaxis(fg, -1, 0), order=1)
+    wo = resample.warp(pair.moving, np.moveaxis(fo, -1, 0), order=1)
+    rng_int = float(pair.moving.max() - pair.moving.min())
+    warped = {"rms_rel_range": float(np.sqrt(np.mean((wg - wo) ** 2)) / rng_int),
+              "max_rel_range": float(np.abs(wg - wo).max() / rng_int)}
+    # folds: GPU and oracle Jacobians of the same (GPU) field
+    _, cg = _core.fold_mask_3D(fg.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0, e1=0.0)
+    Jo = pycore.compute_jacobian_determinant_3D(fg.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0)
+    Jg = _core.compute_jacobian_determinant_3D(fg.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0)
+    folds = {"gpu": int(cg), "oracle": int(np.count_nonzero(Jo < 0.0)),
+             "jacobian_bitexact": bool(np.array_equal(Jg, Jo))}
+    # covariance: deformation block only, dream = x, so each component is
+    # x + DCT-III(sqrt(sigma^2 / (1 + gamma sigma^2 D)) eps): voxel variance
+    # sum_k C[k, i]^2 sigma^2 / (1 + gamma sigma^2 D_k), separable
+    sig2 = 9.0 / (8.0 * np.pi)
+    gam = prob.gammas[0]
+    V = sig2 / (1.0 + gam * sig2 * prob.D)
+    var = V
+    for ax, n in enumerate(prob.shape):
+        C2 = _dct_matrix(n) ** 2
+        var = np.moveaxis(np.tensordot(C2.T, np.moveaxis(var, ax, 0), axes=1), 0, ax)
+    s = synth.build_registration(_core, prob)
+    s.set_dream_locations(prob.voxel_pos.ravel())
+    ns = args.precision_sweeps
+    _, cov = s.sampler(args.seed, 0, ns, 0, 1, 0, 0, 1)
+    vd = np.stack([cov[..., k, k] for k in range(3)], -1)
+    ratio = vd / var[..., None]
+    se = np.sqrt(2.0 / (ns - 1))  # relative standard error of a Gaussian variance
+    covariance = {"sweeps": ns, "mean_ratio": float(ratio.mean()),
+                  "rms_rel_err": float(np.sqrt(np.mean((ratio - 1) ** 2))),
+                  "mc_rel_se": float(se),
+                  "frac_beyond_4se": float(np.mean(np.abs(ratio - 1) > 4 * se))}
+    return {"report": "precision", "config": {"shape": [Nx, Ny, Nz], "bins": prob.K,
+                                              "em_iterations": iters,
+                                              "gamma_per_voxel": 1e-6, "seed": args.seed},
+            "dct": "split-BF16 MFMA (3 products, FP32 accumulation); FP64 cost, Welford, "
+                   "Jacobians",
+            "field": field, "objective": objective, "mi": mi, "warped_image": warped,
+            "folds": folds, "covariance": covariance, "oracle_threads": th}
+
+
 def copy_calibration(mib=1024):
     """A plain device-to-device copy of known size (torch elementwise copy),
     used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
@@ -337,7 +477,19 @@ def main():
                     help="3 EM iterations and no topology iterations (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the copy kernel used to calibrate HBM counters")
+    ap.add_argument("--precision", action="store_true",
+                    help="only the 256x256x192 precision report against the FP64 oracle")
+    ap.add_argument("--precision-iters", type=int, default=3)
+    ap.add_argument("--c1", action="store_true",
+                    help="only the 2D 192x160 registration report (GPU vs FP64 oracle)")
+    ap.add_argument("--precision-sweeps", type=int, default=200)
     args = ap.parse_args()
+    if args.precision:
+        print(json.dumps(precision_report(args)), flush=True)
+        return
+    if args.c1:
+        print(json.dumps(c1_report(args)), flush=True)
+        return
     if args.copy_calibration:
         copy_calibration()
         return

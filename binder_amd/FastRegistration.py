@@ -220,21 +220,25 @@ class FastRegistration:
                     self.num_node_intensity_levels, self.num_voxel_intensity_levels))
             iters = len(self.registratorObj.get_objective_trace())
             if self.top_cor_obj:
-                vs = self.current_voxels_voxel_size
-                if self.number_of_dimensions == 2:
-                    f = np.ascontiguousarray(self.deformedVoxelPositions[:, :, 0])
-                    out = self.top_cor_obj.correct_topology(
-                        f.ravel(), self.e1, self.e2, self.max_outer_it, self.max_inner_it,
-                        self.threads, vs, self.voxels_shape[0], self.voxels_shape[1])
-                    d = np.array(self.deformedVoxelPositions, dtype=np.float64)
-                    d[:, :, 0] = np.asarray(out).reshape(d[:, :, 0].shape)
-                    self.deformedVoxelPositions = d
-                else:
-                    self.deformedVoxelPositions = self.top_cor_obj.correct_topology(
-                        np.ascontiguousarray(self.deformedVoxelPositions).ravel(), self.e1,
-                        self.e2, self.max_outer_it, self.max_inner_it, self.threads, vs,
-                        self.voxels_shape[0], self.voxels_shape[1], self.voxels_shape[2])
+                self.deformedVoxelPositions = self.correct_topology(self.deformedVoxelPositions)
             self.level_log.append((resolution, iters, time.perf_counter() - t0))
+
+    def correct_topology(self, positions):
+        """Topology correction of a field at the current resolution, as run()
+        applies it after each level (FastRegistration.py:521-544)."""
+        vs = self.current_voxels_voxel_size
+        if self.number_of_dimensions == 2:
+            f = np.ascontiguousarray(np.asarray(positions)[:, :, 0])
+            out = self.top_cor_obj.correct_topology(
+                f.ravel(), self.e1, self.e2, self.max_outer_it, self.max_inner_it,
+                self.threads, vs, self.voxels_shape[0], self.voxels_shape[1])
+            d = np.array(positions, dtype=np.float64)
+            d[:, :, 0] = np.asarray(out).reshape(d[:, :, 0].shape)
+            return d
+        return self.top_cor_obj.correct_topology(
+            np.ascontiguousarray(positions).ravel(), self.e1, self.e2, self.max_outer_it,
+            self.max_inner_it, self.threads, vs, self.voxels_shape[0], self.voxels_shape[1],
+            self.voxels_shape[2])
 
     def _new_registration(self, positions):
         vp = np.ascontiguousarray(self.voxels_pos)

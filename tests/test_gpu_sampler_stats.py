@@ -157,3 +157,41 @@ def test_chain_matches_oracle_within_mc_error(core, orc, sample_gamma):
         return np.stack([v.reshape(len(v), -1).mean(1), (disp ** 2).mean(1), disp.mean(1)], 1)
     zp = _two_sample_z(pooled(gm, gv), pooled(om, ov))
     assert np.abs(zp).max() < 4.5, zp
+
+
+@pytest.mark.gpu
+def test_sampler_variance_closed_form(core):
+    """Deformation block only, dream = x: each component is x plus the DCT-III
+    of N(0, sigma^2 / (1 + gamma sigma^2 D)) coefficients, so the voxel-wise
+    variance is sum_k C[k, i]^2 sigma^2 / (1 + gamma sigma^2 D_k)
+    (NonLinearTransformation.cpp:79-126 with the sampler's Welford,
+    Registration.cpp:593-636)."""
+    print("sampler variance closed form", flush=True)
+    shape = (24, 20, 16)
+    pos = np.stack(np.indices(shape, dtype=np.float64), -1)
+    D = initialize_fourier_domain_smoothing(list(shape), True, 2, 3).reshape(shape)
+    g = 0.02 * np.prod(shape)
+    prob = synth.Problem(shape=shape, final_locations=pos.copy(), voxel_pos=pos,
+                         binned_nodes=np.zeros(shape, np.int32),
+                         binned_voxels=np.zeros(shape, np.int32), theta=np.ones((4, 4)) / 4,
+                         D=D, gammas=(g, g, g), K=4, L=4, alpha=2.0, nnz=int(np.prod(shape)))
+    sig2 = 9.0 / (8.0 * np.pi)
+    var = sig2 / (1.0 + g * sig2 * D)
+    for ax, n in enumerate(shape):
+        k = np.arange(n)[:, None]
+        j = np.arange(n)[None, :]
+        C = np.sqrt(2.0 / n) * np.cos(np.pi * (2 * j + 1) * k / (2 * n))
+        C[0] /= np.sqrt(2.0)
+        var = np.moveaxis(np.tensordot((C ** 2).T, np.moveaxis(var, ax, 0), axes=1), 0, ax)
+    r = synth.build_registration(core, prob)
+    r.set_dream_locations(pos.ravel())
+    ns = 4000
+    mean, cov = r.sampler(9, 0, ns, 0, 1, 0, 0, 1)
+    se = np.sqrt(2.0 / (ns - 1))
+    for c in range(3):
+        ratio = cov[..., c, c] / var
+        assert np.mean(np.abs(ratio - 1) > 4 * se) < 1e-3, c
+        assert abs(ratio.mean() - 1) < 4 * se / np.sqrt(ratio.size) * 10
+    # off-diagonal components are independent draws
+    assert np.abs(cov[..., 0, 1]).mean() < 4 * np.sqrt(var.mean() ** 2 / ns)
+    assert np.abs(mean - pos).max() < 6 * np.sqrt(var.max() / ns)
