@@ -214,12 +214,20 @@ class FastRegistration:
                                    self.convergence_th,
                                    update_likelihood_parameters=self.update_likelihood_parameters,
                                    debug=self.debug)
+            on_device = bool(self.top_cor_obj) and hasattr(self.registratorObj,
+                                                          "correct_topology")
+            if on_device:
+                # the same correction on the field where it lives
+                n = 2 if self.number_of_dimensions == 2 else 3
+                self.registratorObj.correct_topology(
+                    self.e1, self.e2, self.max_outer_it, self.max_inner_it,
+                    self.current_voxels_voxel_size[:n])
             self.deformedVoxelPositions = self.registratorObj.get_final_locations_as_numpy_array()
             self.theta = np.ascontiguousarray(np.array(
                 self.registratorObj.get_likelihood_parameters()).reshape(
                     self.num_node_intensity_levels, self.num_voxel_intensity_levels))
             iters = len(self.registratorObj.get_objective_trace())
-            if self.top_cor_obj:
+            if self.top_cor_obj and not on_device:
                 self.deformedVoxelPositions = self.correct_topology(self.deformedVoxelPositions)
             self.level_log.append((resolution, iters, time.perf_counter() - t0))
 

@@ -57,6 +57,8 @@ _reg_get_dream = _sig("binder_registration_get_dream_locations", _c_int, _vp, _v
 _reg_set_dream = _sig("binder_registration_set_dream_locations", _c_int, _vp, _vp)
 _reg_get_theta = _sig("binder_registration_get_likelihood_parameters", _c_int, _vp, _vp)
 _reg_get_shape = _sig("binder_registration_get_shape", _c_int, _vp, _vp)
+_reg_topo = _sig("binder_registration_correct_topology", _c_int, _vp, _c_double, _c_double,
+                 _c_int, _c_int, _vp, _vp)
 _reg_get_trace = _sig("binder_registration_get_objective_trace", _c_int, _vp, _vp, _P(_c_int))
 _reg_e_step = _sig("binder_registration_e_step", _c_int, _vp, _P(_c_double), _vp)
 _reg_set_prof = _sig("binder_registration_set_profiling", _c_int, _vp, _c_int)
@@ -236,6 +238,18 @@ class Registration:
             hist = np.empty(self._KL, np.float64)
         _check(_reg_e_step(self._h, ctypes.byref(mll), _ptr(hist) if hist is not None else None))
         return (mll.value, hist) if return_histogram else mll.value
+
+    def correct_topology(self, e1, e2, max_outer_it, max_it, voxel_sizes):
+        """Topology correction of the current field in place on the GPU (the
+        computation of topology_correction_3D / _2D.correct_topology without
+        the host round trip); returns the stats of the last call."""
+        vs = _arr(voxel_sizes, np.float64)
+        if vs.size < self._shape[3]:
+            raise ValueError("voxel_sizes needs one value per component")
+        stats = np.zeros(6, np.float64)
+        _check(_reg_topo(self._h, float(e1), float(e2), int(max_outer_it), int(max_it),
+                         _ptr(vs), _ptr(stats)))
+        return stats
 
     def set_profiling(self, enable):
         _check(_reg_set_prof(self._h, int(bool(enable))))
@@ -444,6 +458,7 @@ def exported_symbols():
     return [
         "binder_last_error", "binder_backend", "binder_philox4x32_10", "binder_resize_field",
         "binder_topology_correction_2d", "binder_spline_weights",
+        "binder_registration_correct_topology",
         "binder_registration_create",
         "binder_registration_destroy", "binder_registration_set_mi_likelihood",
         "binder_registration_set_nonlinear_transformation", "binder_registration_em",

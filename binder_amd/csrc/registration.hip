@@ -16,6 +16,7 @@
 
 #include "estep.hpp"
 #include "smoothing.hpp"
+#include "topology.hpp"
 
 namespace binder {
 
@@ -530,6 +531,16 @@ int binder_spline_weights(int order, double d, int log_space, double *out4) {
       out4[0] = d >= 1.0 ? kNeg : std::log(e);
       out4[1] = d < 1e-300 ? kNeg : std::log(d);
     }
+  });
+}
+
+int binder_registration_correct_topology(binder_registration *r, double e1,
+                                         double e2, int max_outer_it, int max_it,
+                                         const double *voxel_sizes, double *stats) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && voxel_sizes, "null argument");
+    topology_on_device(r->dcur(), r->N, r->nxv, r->nyv, r->nzv, voxel_sizes, e1, e2,
+                       max_outer_it, max_it, r->s(), stats);
   });
 }
 

@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "topology.hpp"
 
 #pragma clang fp contract(off)
 
@@ -916,7 +917,249 @@ void upload_soa3(const double *aos, long long nv, DBuf<double> &dst,
   HIP_OK(hipStreamSynchronize(s));
 }
 
+void topo_init(TopoCtx &c, hipStream_t s, int Nx, int Ny, int Nz, const double vs[3],
+               double e1, double e2) {
+  c.s = s;
+  c.Nx = Nx;
+  c.Ny = Ny;
+  c.Nz = Nz;
+  c.nv = (long long)Nx * Ny * Nz;
+  c.ncell = (long long)(Nx - 1) * (Ny - 1) * (Nz - 1);
+  for (int k = 0; k < 3; ++k) c.vs[k] = vs[k];
+  c.e1 = e1;
+  c.e2 = e2;
+  c.vol_scale = 1.0 / (c.vs[0] * c.vs[1] * c.vs[2]);
+  c.blocks = (int)std::min<long long>((c.nv + 255) / 256, 4096);
+  c.part.alloc((size_t)c.blocks * 4);
+  c.cpart.alloc((size_t)c.blocks * 2);
+}
+
+// 3D correction of the grid-unit field c.w (three f64 planes, on the
+// device): to physical units, the outer/inner loop of correct_topology
+// (topology_correction_3D.cpp:17-143), back to grid units.
+void topo3_run(TopoCtx &c, int max_outer, int max_inner, double *stats) {
+  hipLaunchKernelGGL(k_scale_planes, dim3(c.blocks), dim3(256), 0, c.s,
+                     c.w.get(), c.nv, c.vs[0], c.vs[1], c.vs[2], 0);
+  HIP_LAUNCH_OK();
+  double jmin, jmax;
+  long long nb, na;
+  c.jstats(jmin, jmax, nb, na);
+  int outer = 0, inner_total = 0;
+  if (!(jmin >= c.e1 && jmax <= c.e2)) {
+    c.cmask.alloc(c.ncell);
+    c.g.alloc((size_t)9 * c.nv);
+    c.sbuf.alloc(c.nv);
+    c.tbuf.alloc(c.nv);
+    c.ubuf.alloc(c.nv);
+    c.T = 4 * (c.Nx - 2) + 2 * (c.Ny - 2) + (c.Nz - 2) + 1;
+    c.cnt.alloc(c.T);
+    c.start.alloc(c.T + 1);
+    c.cursor.alloc(c.T);
+    c.list.alloc(c.ncell);
+    c.dht_matrix(c.Nx, c.Jmat[0]);
+    c.dht_matrix(c.Ny, c.Jmat[1]);
+    c.dht_matrix(c.Nz, c.Jmat[2]);
+    while ((jmin < c.e1 || jmax > c.e2) && outer < max_outer) {
+      long long lo, hi;
+      c.corner(lo, hi);
+      int inner = 0;
+      while ((lo > 0 || hi > 0) && inner < max_inner) {
+        hipLaunchKernelGGL(k_gradients, dim3(c.blocks), dim3(256), 0, c.s,
+                           c.field(), c.Nx, c.Ny, c.Nz, c.vs[0], c.vs[1],
+                           c.vs[2], c.g.get());
+        HIP_LAUNCH_OK();
+        c.limit();
+        c.integrate();
+        c.corner(lo, hi);
+        inner++;
+      }
+      inner_total += inner;
+      c.jstats(jmin, jmax, nb, na);
+      outer++;
+    }
+  }
+  hipLaunchKernelGGL(k_scale_planes, dim3(c.blocks), dim3(256), 0, c.s,
+                     c.w.get(), c.nv, c.vs[0], c.vs[1], c.vs[2], 1);
+  HIP_LAUNCH_OK();
+  if (stats) {
+    stats[0] = outer;
+    stats[1] = inner_total;
+    stats[2] = jmin;
+    stats[3] = jmax;
+    stats[4] = (double)nb;
+    stats[5] = (double)na;
+  }
+}
+
+// 2D correction of the grid-unit field c.w (planes x, y, and a zero third
+// plane, on the device), c set up on the grid (Nx, 1, Ny)
+// (topology_correction_2D.cpp:17-140).
+void topo2_run(TopoCtx &c, int Nx_v, int Ny_v, double vs0, double vs1, double e1,
+             double e2, int max_outer, int max_inner, double *stats) {
+  const long long nv = c.nv;
+  const long long ncell = (long long)(Nx_v - 1) * (Ny_v - 1);
+  const double vol_scale = 1.0 / (vs0 * vs1);
+  double *wx = c.w.get(), *wy = c.w.get() + nv;
+  hipLaunchKernelGGL(k_scale_planes, dim3(c.blocks), dim3(256), 0, c.s, c.w.get(),
+                     nv, vs0, vs1, 1.0, 0);
+  HIP_LAUNCH_OK();
+  auto jstats = [&](double &jmin, double &jmax, long long &nb, long long &na) {
+    hipLaunchKernelGGL(k_jacobian2d_stats, dim3(c.blocks), dim3(256), 0, c.s, wx, wy,
+                       Nx_v, Ny_v, vol_scale, e1, e2, c.part.get());
+    HIP_LAUNCH_OK();
+    c.hpart.resize((size_t)c.blocks * 4);
+    c.part.download(c.hpart.data(), c.hpart.size(), c.s);
+    HIP_OK(hipStreamSynchronize(c.s));
+    jmin = INFINITY;
+    jmax = -INFINITY;
+    double lo = 0, hi = 0;
+    for (int b = 0; b < c.blocks; ++b) {
+      jmin = std::min(jmin, c.hpart[b * 4 + 0]);
+      jmax = std::max(jmax, c.hpart[b * 4 + 1]);
+      lo += c.hpart[b * 4 + 2];
+      hi += c.hpart[b * 4 + 3];
+    }
+    nb = (long long)lo;
+    na = (long long)hi;
+  };
+  auto corner = [&](long long &lo, long long &hi) {
+    hipLaunchKernelGGL(k_corner_mask2d, dim3(c.blocks), dim3(256), 0, c.s, wx, wy,
+                       Nx_v, Ny_v, vol_scale, e1, e2, c.cmask.get(), c.cpart.get());
+    HIP_LAUNCH_OK();
+    c.hcpart.resize((size_t)c.blocks * 2);
+    c.cpart.download(c.hcpart.data(), c.hcpart.size(), c.s);
+    HIP_OK(hipStreamSynchronize(c.s));
+    lo = hi = 0;
+    for (int b = 0; b < c.blocks; ++b) {
+      lo += (long long)c.hcpart[b * 2];
+      hi += (long long)c.hcpart[b * 2 + 1];
+    }
+  };
+  double jmin, jmax;
+  long long nb, na;
+  jstats(jmin, jmax, nb, na);
+  int outer = 0, inner_total = 0;
+  if (!(jmin >= e1 && jmax <= e2)) {
+    c.cmask.alloc(ncell);
+    c.g.alloc((size_t)4 * nv);
+    c.sbuf.alloc(nv);
+    c.tbuf.alloc(nv);
+    c.ubuf.alloc(nv);
+    c.T = 2 * (Nx_v - 2) + (Ny_v - 2) + 1;
+    c.cnt.alloc(c.T);
+    c.start.alloc(c.T + 1);
+    c.cursor.alloc(c.T);
+    c.list.alloc(ncell);
+    c.dht_matrix(Nx_v, c.Jmat[0]);
+    c.dht_matrix(Ny_v, c.Jmat[2]);
+    const int gb = (int)std::min<long long>((ncell + 255) / 256, 4096);
+    while ((jmin < e1 || jmax > e2) && outer < max_outer) {
+      long long lo, hi;
+      corner(lo, hi);
+      int inner = 0;
+      while ((lo > 0 || hi > 0) && inner < max_inner) {
+        hipLaunchKernelGGL(k_gradients2d, dim3(c.blocks), dim3(256), 0, c.s, wx, wy,
+                           Nx_v, Ny_v, vs0, vs1, c.g.get());
+        HIP_LAUNCH_OK();
+        // limit_gradients_2D in hyperplane order
+        c.cnt.zero(c.s);
+        hipLaunchKernelGGL(k_bucket2d, dim3(gb), dim3(256), 0, c.s, c.cmask.get(), Nx_v,
+                           Ny_v, c.cnt.get(), (int *)nullptr, (int *)nullptr, 0);
+        HIP_LAUNCH_OK();
+        hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, c.s, c.cnt.get(), c.T,
+                           c.start.get(), c.cursor.get());
+        HIP_LAUNCH_OK();
+        hipLaunchKernelGGL(k_bucket2d, dim3(gb), dim3(256), 0, c.s, c.cmask.get(), Nx_v,
+                           Ny_v, (int *)nullptr, c.cursor.get(), c.list.get(), 1);
+        HIP_LAUNCH_OK();
+        hipLaunchKernelGGL(k_limit2d, dim3(1), dim3(1024), 0, c.s, c.cmask.get(),
+                           c.start.get(), c.T, c.list.get(), Nx_v, Ny_v, e1, e2,
+                           c.g.get());
+        HIP_LAUNCH_OK();
+        // compute_mean_coordinates, then integrate_gradient_field_2D
+        hipLaunchKernelGGL(k_plane_sums, dim3(c.blocks), dim3(256), 0, c.s,
+                           Field{wx, wy, wy}, nv, c.part.get());
+        HIP_LAUNCH_OK();
+        c.hpart.resize((size_t)c.blocks * 3);
+        c.part.download(c.hpart.data(), c.hpart.size(), c.s);
+        HIP_OK(hipStreamSynchronize(c.s));
+        double mean[2] = {0, 0};
+        for (int b = 0; b < c.blocks; ++b)
+          for (int k = 0; k < 2; ++k) mean[k] += c.hpart[b * 3 + k];
+        for (int k = 0; k < 2; ++k) mean[k] /= (double)nv;
+        const double vsn[2] = {vs0, vs1};
+        for (int n = 0; n < 2; ++n) {
+          hipLaunchKernelGGL(k_divergence2d, dim3(c.blocks), dim3(256), 0, c.s,
+                             c.g.get(), n, Nx_v, Ny_v, c.sbuf.get());
+          HIP_LAUNCH_OK();
+          c.dpass(2, c.sbuf.get(), c.tbuf.get(), TEPI_STORE, 1, 0);
+          c.dpass(0, c.tbuf.get(), c.ubuf.get(), TEPI_INVLAMBDA, 1, 0);
+          c.dpass(0, c.ubuf.get(), c.tbuf.get(), TEPI_STORE, 1, 0);
+          c.dpass(2, c.tbuf.get(), c.w.get() + n * nv, TEPI_FINAL, vsn[n], mean[n]);
+        }
+        corner(lo, hi);
+        inner++;
+      }
+      inner_total += inner;
+      jstats(jmin, jmax, nb, na);
+      outer++;
+    }
+  }
+  hipLaunchKernelGGL(k_scale_planes, dim3(c.blocks), dim3(256), 0, c.s, c.w.get(), nv,
+                     vs0, vs1, 1.0, 1);
+  HIP_LAUNCH_OK();
+  if (stats) {
+    stats[0] = outer;
+    stats[1] = inner_total;
+    stats[2] = jmin;
+    stats[3] = jmax;
+    stats[4] = (double)nb;
+    stats[5] = (double)na;
+  }
+}
+
+__global__ void k_planes_f32_to_f64(const float *in, double *out, long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = (double)in[i];
+}
+
+__global__ void k_planes_f64_to_f32(const double *in, float *out, long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = (float)in[i];
+}
+
 }  // namespace
+
+void topology_on_device(float *d_soa, int N, int Nx, int Ny, int Nz, const double *vs,
+                        double e1, double e2, int max_outer, int max_inner, hipStream_t s,
+                        double *stats) {
+  TopoCtx c;
+  if (N == 3) {
+    BINDER_REQUIRE(Nx >= 2 && Ny >= 2 && Nz >= 2,
+                   "topology correction needs at least 2 voxels per axis");
+    topo_init(c, s, Nx, Ny, Nz, vs, e1, e2);
+  } else {
+    BINDER_REQUIRE(Nx >= 2 && Ny >= 2, "topology correction needs at least 2 voxels per axis");
+    const double vs3[3] = {vs[0], 1.0, vs[1]};
+    topo_init(c, s, Nx, 1, Ny, vs3, e1, e2);
+  }
+  const long long nv = c.nv;
+  c.w.alloc((size_t)3 * nv);
+  if (N == 2) HIP_OK(hipMemsetAsync(c.w.get() + 2 * nv, 0, sizeof(double) * nv, s));
+  const int gb = (int)std::min<long long>((N * nv + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_planes_f32_to_f64, dim3(gb), dim3(256), 0, s, d_soa, c.w.get(),
+                     (long long)N * nv);
+  HIP_LAUNCH_OK();
+  if (N == 3) topo3_run(c, max_outer, max_inner, stats);
+  else topo2_run(c, Nx, Ny, vs[0], vs[1], e1, e2, max_outer, max_inner, stats);
+  hipLaunchKernelGGL(k_planes_f64_to_f32, dim3(gb), dim3(256), 0, s, c.w.get(), d_soa,
+                     (long long)N * nv);
+  HIP_LAUNCH_OK();
+  HIP_OK(hipStreamSynchronize(s));
+}
+
 }  // namespace binder
 
 using namespace binder;
@@ -1012,76 +1255,14 @@ int binder_topology_correction_3d(const double *h_init, double e1, double e2,
                    "topology correction needs at least 2 voxels per axis");
     OwnedStream st;
     TopoCtx c;
-    c.s = st.s;
-    c.Nx = Nx_v;
-    c.Ny = Ny_v;
-    c.Nz = Nz_v;
-    c.nv = (long long)Nx_v * Ny_v * Nz_v;
-    c.ncell = (long long)(Nx_v - 1) * (Ny_v - 1) * (Nz_v - 1);
-    for (int k = 0; k < 3; ++k) c.vs[k] = voxel_sizes[k];
-    c.e1 = e1;
-    c.e2 = e2;
-    c.vol_scale = 1.0 / (c.vs[0] * c.vs[1] * c.vs[2]);
-    c.blocks = (int)std::min<long long>((c.nv + 255) / 256, 4096);
-    c.part.alloc((size_t)c.blocks * 4);
-    c.cpart.alloc((size_t)c.blocks * 2);
+    topo_init(c, st.s, Nx_v, Ny_v, Nz_v, voxel_sizes, e1, e2);
     upload_soa3(h_init, c.nv, c.w, c.s);
-    hipLaunchKernelGGL(k_scale_planes, dim3(c.blocks), dim3(256), 0, c.s,
-                       c.w.get(), c.nv, c.vs[0], c.vs[1], c.vs[2], 0);
-    HIP_LAUNCH_OK();
-    double jmin, jmax;
-    long long nb, na;
-    c.jstats(jmin, jmax, nb, na);
-    int outer = 0, inner_total = 0;
-    if (!(jmin >= e1 && jmax <= e2)) {
-      c.cmask.alloc(c.ncell);
-      c.g.alloc((size_t)9 * c.nv);
-      c.sbuf.alloc(c.nv);
-      c.tbuf.alloc(c.nv);
-      c.ubuf.alloc(c.nv);
-      c.T = 4 * (Nx_v - 2) + 2 * (Ny_v - 2) + (Nz_v - 2) + 1;
-      c.cnt.alloc(c.T);
-      c.start.alloc(c.T + 1);
-      c.cursor.alloc(c.T);
-      c.list.alloc(c.ncell);
-      c.dht_matrix(Nx_v, c.Jmat[0]);
-      c.dht_matrix(Ny_v, c.Jmat[1]);
-      c.dht_matrix(Nz_v, c.Jmat[2]);
-      while ((jmin < e1 || jmax > e2) && outer < max_outer_it) {
-        long long lo, hi;
-        c.corner(lo, hi);
-        int inner = 0;
-        while ((lo > 0 || hi > 0) && inner < max_it) {
-          hipLaunchKernelGGL(k_gradients, dim3(c.blocks), dim3(256), 0, c.s,
-                             c.field(), Nx_v, Ny_v, Nz_v, c.vs[0], c.vs[1],
-                             c.vs[2], c.g.get());
-          HIP_LAUNCH_OK();
-          c.limit();
-          c.integrate();
-          c.corner(lo, hi);
-          inner++;
-        }
-        inner_total += inner;
-        c.jstats(jmin, jmax, nb, na);
-        outer++;
-      }
-    }
-    hipLaunchKernelGGL(k_scale_planes, dim3(c.blocks), dim3(256), 0, c.s,
-                       c.w.get(), c.nv, c.vs[0], c.vs[1], c.vs[2], 1);
-    HIP_LAUNCH_OK();
+    topo3_run(c, max_outer_it, max_it, stats);
     std::vector<double> h((size_t)3 * c.nv);
     c.w.download(h.data(), h.size(), c.s);
     HIP_OK(hipStreamSynchronize(c.s));
     for (long long i = 0; i < c.nv; ++i)
       for (int k = 0; k < 3; ++k) out[i * 3 + k] = h[k * c.nv + i];
-    if (stats) {
-      stats[0] = outer;
-      stats[1] = inner_total;
-      stats[2] = jmin;
-      stats[3] = jmax;
-      stats[4] = (double)nb;
-      stats[5] = (double)na;
-    }
   });
 }
 
@@ -1095,21 +1276,10 @@ int binder_topology_correction_2d(const double *h_init, double e1, double e2,
     BINDER_REQUIRE(Nx_v >= 2 && Ny_v >= 2,
                    "topology correction needs at least 2 voxels per axis");
     OwnedStream st;
-    // the Poisson solve reuses the 3D machinery on the grid (Nx, 1, Ny):
-    // the unit axis adds 2 - 2 cos 0 = 0 to every eigenvalue
     TopoCtx c;
-    c.s = st.s;
-    c.Nx = Nx_v;
-    c.Ny = 1;
-    c.Nz = Ny_v;
-    c.nv = (long long)Nx_v * Ny_v;
+    const double vs3[3] = {voxel_sizes[0], 1.0, voxel_sizes[1]};
+    topo_init(c, st.s, Nx_v, 1, Ny_v, vs3, e1, e2);
     const long long nv = c.nv;
-    const long long ncell = (long long)(Nx_v - 1) * (Ny_v - 1);
-    const double vs0 = voxel_sizes[0], vs1 = voxel_sizes[1];
-    const double vol_scale = 1.0 / (vs0 * vs1);
-    c.blocks = (int)std::min<long long>((nv + 255) / 256, 4096);
-    c.part.alloc((size_t)c.blocks * 4);
-    c.cpart.alloc((size_t)c.blocks * 2);
     {
       std::vector<double> h((size_t)3 * nv, 0.0);
       for (long long i = 0; i < nv; ++i) {
@@ -1119,129 +1289,14 @@ int binder_topology_correction_2d(const double *h_init, double e1, double e2,
       c.w.upload(h.data(), h.size(), c.s);
       HIP_OK(hipStreamSynchronize(c.s));
     }
-    double *wx = c.w.get(), *wy = c.w.get() + nv;
-    hipLaunchKernelGGL(k_scale_planes, dim3(c.blocks), dim3(256), 0, c.s, c.w.get(),
-                       nv, vs0, vs1, 1.0, 0);
-    HIP_LAUNCH_OK();
-    auto jstats = [&](double &jmin, double &jmax, long long &nb, long long &na) {
-      hipLaunchKernelGGL(k_jacobian2d_stats, dim3(c.blocks), dim3(256), 0, c.s, wx, wy,
-                         Nx_v, Ny_v, vol_scale, e1, e2, c.part.get());
-      HIP_LAUNCH_OK();
-      c.hpart.resize((size_t)c.blocks * 4);
-      c.part.download(c.hpart.data(), c.hpart.size(), c.s);
-      HIP_OK(hipStreamSynchronize(c.s));
-      jmin = INFINITY;
-      jmax = -INFINITY;
-      double lo = 0, hi = 0;
-      for (int b = 0; b < c.blocks; ++b) {
-        jmin = std::min(jmin, c.hpart[b * 4 + 0]);
-        jmax = std::max(jmax, c.hpart[b * 4 + 1]);
-        lo += c.hpart[b * 4 + 2];
-        hi += c.hpart[b * 4 + 3];
-      }
-      nb = (long long)lo;
-      na = (long long)hi;
-    };
-    auto corner = [&](long long &lo, long long &hi) {
-      hipLaunchKernelGGL(k_corner_mask2d, dim3(c.blocks), dim3(256), 0, c.s, wx, wy,
-                         Nx_v, Ny_v, vol_scale, e1, e2, c.cmask.get(), c.cpart.get());
-      HIP_LAUNCH_OK();
-      c.hcpart.resize((size_t)c.blocks * 2);
-      c.cpart.download(c.hcpart.data(), c.hcpart.size(), c.s);
-      HIP_OK(hipStreamSynchronize(c.s));
-      lo = hi = 0;
-      for (int b = 0; b < c.blocks; ++b) {
-        lo += (long long)c.hcpart[b * 2];
-        hi += (long long)c.hcpart[b * 2 + 1];
-      }
-    };
-    double jmin, jmax;
-    long long nb, na;
-    jstats(jmin, jmax, nb, na);
-    int outer = 0, inner_total = 0;
-    if (!(jmin >= e1 && jmax <= e2)) {
-      c.cmask.alloc(ncell);
-      c.g.alloc((size_t)4 * nv);
-      c.sbuf.alloc(nv);
-      c.tbuf.alloc(nv);
-      c.ubuf.alloc(nv);
-      c.T = 2 * (Nx_v - 2) + (Ny_v - 2) + 1;
-      c.cnt.alloc(c.T);
-      c.start.alloc(c.T + 1);
-      c.cursor.alloc(c.T);
-      c.list.alloc(ncell);
-      c.dht_matrix(Nx_v, c.Jmat[0]);
-      c.dht_matrix(Ny_v, c.Jmat[2]);
-      const int gb = (int)std::min<long long>((ncell + 255) / 256, 4096);
-      while ((jmin < e1 || jmax > e2) && outer < max_outer_it) {
-        long long lo, hi;
-        corner(lo, hi);
-        int inner = 0;
-        while ((lo > 0 || hi > 0) && inner < max_it) {
-          hipLaunchKernelGGL(k_gradients2d, dim3(c.blocks), dim3(256), 0, c.s, wx, wy,
-                             Nx_v, Ny_v, vs0, vs1, c.g.get());
-          HIP_LAUNCH_OK();
-          // limit_gradients_2D in hyperplane order
-          c.cnt.zero(c.s);
-          hipLaunchKernelGGL(k_bucket2d, dim3(gb), dim3(256), 0, c.s, c.cmask.get(), Nx_v,
-                             Ny_v, c.cnt.get(), (int *)nullptr, (int *)nullptr, 0);
-          HIP_LAUNCH_OK();
-          hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, c.s, c.cnt.get(), c.T,
-                             c.start.get(), c.cursor.get());
-          HIP_LAUNCH_OK();
-          hipLaunchKernelGGL(k_bucket2d, dim3(gb), dim3(256), 0, c.s, c.cmask.get(), Nx_v,
-                             Ny_v, (int *)nullptr, c.cursor.get(), c.list.get(), 1);
-          HIP_LAUNCH_OK();
-          hipLaunchKernelGGL(k_limit2d, dim3(1), dim3(1024), 0, c.s, c.cmask.get(),
-                             c.start.get(), c.T, c.list.get(), Nx_v, Ny_v, e1, e2,
-                             c.g.get());
-          HIP_LAUNCH_OK();
-          // compute_mean_coordinates, then integrate_gradient_field_2D
-          hipLaunchKernelGGL(k_plane_sums, dim3(c.blocks), dim3(256), 0, c.s,
-                             Field{wx, wy, wy}, nv, c.part.get());
-          HIP_LAUNCH_OK();
-          c.hpart.resize((size_t)c.blocks * 3);
-          c.part.download(c.hpart.data(), c.hpart.size(), c.s);
-          HIP_OK(hipStreamSynchronize(c.s));
-          double mean[2] = {0, 0};
-          for (int b = 0; b < c.blocks; ++b)
-            for (int k = 0; k < 2; ++k) mean[k] += c.hpart[b * 3 + k];
-          for (int k = 0; k < 2; ++k) mean[k] /= (double)nv;
-          const double vsn[2] = {vs0, vs1};
-          for (int n = 0; n < 2; ++n) {
-            hipLaunchKernelGGL(k_divergence2d, dim3(c.blocks), dim3(256), 0, c.s,
-                               c.g.get(), n, Nx_v, Ny_v, c.sbuf.get());
-            HIP_LAUNCH_OK();
-            c.dpass(2, c.sbuf.get(), c.tbuf.get(), TEPI_STORE, 1, 0);
-            c.dpass(0, c.tbuf.get(), c.ubuf.get(), TEPI_INVLAMBDA, 1, 0);
-            c.dpass(0, c.ubuf.get(), c.tbuf.get(), TEPI_STORE, 1, 0);
-            c.dpass(2, c.tbuf.get(), c.w.get() + n * nv, TEPI_FINAL, vsn[n], mean[n]);
-          }
-          corner(lo, hi);
-          inner++;
-        }
-        inner_total += inner;
-        jstats(jmin, jmax, nb, na);
-        outer++;
-      }
-    }
-    hipLaunchKernelGGL(k_scale_planes, dim3(c.blocks), dim3(256), 0, c.s, c.w.get(), nv,
-                       vs0, vs1, 1.0, 1);
-    HIP_LAUNCH_OK();
+    topo2_run(c, Nx_v, Ny_v, voxel_sizes[0], voxel_sizes[1], e1, e2, max_outer_it, max_it,
+              stats);
     std::vector<double> h((size_t)2 * nv);
     c.w.download(h.data(), h.size(), c.s);
     HIP_OK(hipStreamSynchronize(c.s));
     for (long long i = 0; i < nv; ++i) {
       out[2 * i] = h[i];
       out[2 * i + 1] = h[nv + i];
-    }
-    if (stats) {
-      stats[0] = outer;
-      stats[1] = inner_total;
-      stats[2] = jmin;
-      stats[3] = jmax;
-      stats[4] = (double)nb;
-      stats[5] = (double)na;
     }
   });
 }

@@ -245,6 +245,16 @@ int binder_topology_correction_3d(const double *h_init, double e1, double e2,
                                   int Ny_v, int Nz_v, double *out,
                                   double *stats);
 
+/* Topology correction of the handle's current field in place, on the
+ * device: the same computation as binder_topology_correction_3d (N = 3) or
+ * _2d (N = 2) applied to the field get_final_locations would return, written
+ * back rounded to float32.  No reference counterpart: it lets the
+ * FastRegistration pyramid (BINDER/FastRegistration.py:521-544) skip the
+ * field's round trip through host memory.  voxel_sizes: N values. */
+int binder_registration_correct_topology(binder_registration *reg, double e1,
+                                         double e2, int max_outer_it, int max_it,
+                                         const double *voxel_sizes, double *stats);
+
 /* Spline(order, is2D).spline(d) / .log_spline(d)
  * -- BINDER/src/cpu/bindings.cpp:436-445, BINDER/src/cpu/Spline.cpp:44-145.
  * Writes the 4 weights (log_space = 0) or their logs (log_space = 1) of the
