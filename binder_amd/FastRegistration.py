@@ -197,11 +197,31 @@ class FastRegistration:
         the reference) replaces the per-level cap
         max_number_of_iterations * resolution with explicit counts."""
         import time
+        cls = Registration.Registration
+        # with this backend the field stays on the GPU from level to level:
+        # device resize into the next level's registration, device topology
+        # correction; other backends (the oracle) take the host path
+        chain = hasattr(cls, "resampled_from") and \
+            (not self.top_cor_obj or hasattr(cls, "correct_topology"))
+        prev = None  # the previous level's registration, whose field is on the GPU
         for li, resolution in enumerate(resolutions):
-            self.move_resolution(resolution)
-            self.deformedVoxelPositions = np.ascontiguousarray(self.deformedVoxelPositions)
+            res, cur = int(resolution), self.current_resolution
+            # ... unless the frequency-domain up-sampling is asked for
+            # (FastRegistration.py:566-580)
+            on_gpu = chain and prev is not None and \
+                not (self.upsample_in_frequency_domain and cur - res >= 0 and res != cur)
+            self.move_resolution(resolution, resize_field=not on_gpu)
             t0 = time.perf_counter()
-            self.registratorObj = self._new_registration(self.deformedVoxelPositions)
+            if on_gpu:
+                vp = np.ascontiguousarray(self.voxels_pos)
+                self.registratorObj = Registration.Registration.resampled_from(
+                    prev, cur / res, vp.ravel(), vp.shape[0], vp.shape[1], vp.shape[2],
+                    self.nodes_shape[0], self.nodes_shape[1], self.nodes_shape[2],
+                    self.spline_order, self.non_zero_voxels)
+                self._set_nonlinear(self.registratorObj)
+            else:
+                self.deformedVoxelPositions = np.ascontiguousarray(self.deformedVoxelPositions)
+                self.registratorObj = self._new_registration(self.deformedVoxelPositions)
             self.registratorObj.setMILikelihood(self.alpha, np.ascontiguousarray(self.theta).ravel(),
                                                 self.num_node_intensity_levels,
                                                 self.num_voxel_intensity_levels, self.threads,
@@ -214,21 +234,28 @@ class FastRegistration:
                                    self.convergence_th,
                                    update_likelihood_parameters=self.update_likelihood_parameters,
                                    debug=self.debug)
-            on_device = bool(self.top_cor_obj) and hasattr(self.registratorObj,
-                                                          "correct_topology")
-            if on_device:
-                # the same correction on the field where it lives
+            if self.top_cor_obj and chain:
+                # the same correction as correct_topology(), on the field where it lives
                 n = 2 if self.number_of_dimensions == 2 else 3
                 self.registratorObj.correct_topology(
                     self.e1, self.e2, self.max_outer_it, self.max_inner_it,
                     self.current_voxels_voxel_size[:n])
-            self.deformedVoxelPositions = self.registratorObj.get_final_locations_as_numpy_array()
+            last = li == len(resolutions) - 1
+            nxt = None if last else int(resolutions[li + 1])
+            if not chain or last or (self.upsample_in_frequency_domain and res - nxt >= 0
+                                     and nxt != res):
+                # the host copy is needed: end of the pyramid, a host-side
+                # backend, or a frequency-domain up-sampling next
+                self.deformedVoxelPositions = \
+                    self.registratorObj.get_final_locations_as_numpy_array()
+                if self.top_cor_obj and not chain:
+                    self.deformedVoxelPositions = self.correct_topology(
+                        self.deformedVoxelPositions)
             self.theta = np.ascontiguousarray(np.array(
                 self.registratorObj.get_likelihood_parameters()).reshape(
                     self.num_node_intensity_levels, self.num_voxel_intensity_levels))
             iters = len(self.registratorObj.get_objective_trace())
-            if self.top_cor_obj and not on_device:
-                self.deformedVoxelPositions = self.correct_topology(self.deformedVoxelPositions)
+            prev = self.registratorObj
             self.level_log.append((resolution, iters, time.perf_counter() - t0))
 
     def correct_topology(self, positions):
@@ -256,6 +283,10 @@ class FastRegistration:
                                       self.nodes_shape[0], self.nodes_shape[1],
                                       self.nodes_shape[2], self.spline_order,
                                       self.non_zero_voxels)
+        self._set_nonlinear(r)
+        return r
+
+    def _set_nonlinear(self, r):
         self.D = np.ascontiguousarray(self.D)
         r.setNonLinearTransformation(self.gamma_x, self.gamma_y, self.gamma_z,
                                      self.current_voxels_voxel_size[0],
@@ -264,22 +295,27 @@ class FastRegistration:
                                      self.current_nodes_voxel_size[0],
                                      self.current_nodes_voxel_size[1],
                                      self.current_nodes_voxel_size[2], self.D.ravel())
-        return r
 
-    def move_resolution(self, resolution):
+    def move_resolution(self, resolution, resize_field=True):
         """Down-sample the bins and positions by `resolution`, resample the
         current field to the new grid and rescale it, recompute D and gamma
-        (FastRegistration.py:550-650)."""
+        (FastRegistration.py:550-650).  resize_field=False (not in the
+        reference) leaves the field to run(), which resizes it on the GPU."""
         res = int(resolution)
         self.current_voxels_voxel_size = self.voxels_voxel_size * res
         self.current_nodes_voxel_size = self.nodes_voxel_size * res
         self.voxels_shape = tuple(int(np.ceil(x / res)) for x in self.orig_voxels_shape)
         self.nodes_shape = np.array([int(np.ceil(x / res)) for x in self.orig_nodes_shape])
-        self.voxels_pos = self.full_voxels_pos[::res, ::res, ::res, :] / res
-        self.nodes_pos = self.full_nodes_pos[::res, ::res, ::res, :] / res
+        if res == 1:  # x / 1 == x: no copy of the full-resolution grids
+            self.voxels_pos, self.nodes_pos = self.full_voxels_pos, self.full_nodes_pos
+        else:
+            self.voxels_pos = self.full_voxels_pos[::res, ::res, ::res, :] / res
+            self.nodes_pos = self.full_nodes_pos[::res, ::res, ::res, :] / res
 
         cur = self.current_resolution
-        if not self.upsample_in_frequency_domain or cur - res < 0:
+        if not resize_field:
+            pass
+        elif not self.upsample_in_frequency_domain or cur - res < 0:
             # skimage resize(order=1) semantics, on the GPU
             self.deformedVoxelPositions = Registration.resize_field(
                 self.deformedVoxelPositions, self.voxels_shape, cur / res)
@@ -294,9 +330,15 @@ class FastRegistration:
 
         self.binned_voxels = self.full_binned_voxels[::res, ::res, ::res]
         self.binned_nodes = self.full_binned_nodes[::res, ::res, ::res]
-        self.D = utils.initialize_fourier_domain_smoothing(
-            self.voxels_shape, True, self.finite_difference_order, self.number_of_dimensions,
-            voxel_size=self.current_voxels_voxel_size)
+        if hasattr(Registration, "dct_prior_eigenvalues"):
+            # the float32 values this backend keeps (utils.py:331-349 rounded)
+            self.D = Registration.dct_prior_eigenvalues(
+                self.voxels_shape, self.finite_difference_order, self.number_of_dimensions,
+                voxel_size=self.current_voxels_voxel_size)
+        else:
+            self.D = utils.initialize_fourier_domain_smoothing(
+                self.voxels_shape, True, self.finite_difference_order,
+                self.number_of_dimensions, voxel_size=self.current_voxels_voxel_size)
         self.non_zero_voxels = np.count_nonzero(self.binned_voxels)
         self._set_gammas(self.orig_voxels_shape if self.increase_gamma_at_DS_resolution
                          else self.voxels_shape)

@@ -57,6 +57,9 @@ _reg_get_dream = _sig("binder_registration_get_dream_locations", _c_int, _vp, _v
 _reg_set_dream = _sig("binder_registration_set_dream_locations", _c_int, _vp, _vp)
 _reg_get_theta = _sig("binder_registration_get_likelihood_parameters", _c_int, _vp, _vp)
 _reg_get_shape = _sig("binder_registration_get_shape", _c_int, _vp, _vp)
+_reg_create_resampled = _sig("binder_registration_create_resampled", _c_int, _P(_vp), _vp,
+                             _c_double, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                             _c_int, _c_double)
 _reg_topo = _sig("binder_registration_correct_topology", _c_int, _vp, _c_double, _c_double,
                  _c_int, _c_int, _vp, _vp)
 _reg_get_trace = _sig("binder_registration_get_objective_trace", _c_int, _vp, _vp, _P(_c_int))
@@ -82,6 +85,25 @@ _topo3 = _sig("binder_topology_correction_3d", _c_int, _vp, _c_double, _c_double
 
 
 _philox = _sig("binder_philox4x32_10", _c_int, _vp, _vp, _vp)
+
+
+_dct_prior = _sig("binder_dct_prior_eigenvalues", _c_int, _c_int, _c_int, _c_int, _c_int,
+                  _c_int, _vp, _vp)
+
+
+def dct_prior_eigenvalues(size, finite_difference_order=2, number_of_dimensions=3,
+                          voxel_size=(1.0, 1.0, 1.0)):
+    """float32 values of utils.initialize_fourier_domain_smoothing(size, True,
+    order, nd, voxel_size=...) (BINDER/utils.py:331-349), computed by a host
+    loop over the separable per-axis terms."""
+    nd = int(number_of_dimensions)
+    dims = [int(n) for n in list(size)[:nd]]
+    vs = np.ascontiguousarray(np.resize(np.asarray(voxel_size, np.float64), 3))
+    out = np.empty(dims, np.float32)
+    full = dims + [1] * (3 - nd)
+    _check(_dct_prior(full[0], full[1], full[2], nd, int(finite_difference_order), _ptr(vs),
+                      _ptr(out)))
+    return out
 
 
 def philox4x32_10(ctr, key):
@@ -135,6 +157,29 @@ class Registration:
         self._shape = (int(Nx_v), int(Ny_v), int(Nz_v), N)
         self._nn = int(Nx_n) * int(Ny_n) * int(Nz_n)
         self._KL = None
+
+    @classmethod
+    def resampled_from(cls, src, scale, voxel_pos, Nx_v, Ny_v, Nz_v, Nx_n, Ny_n, Nz_n,
+                       spline_order, non_zero_voxels):
+        """A Registration on a new grid whose field is `src`'s current field
+        resized on the GPU (the pyramid's resize, times `scale`) -- what
+        Registration(resize(src.get_final_locations_as_numpy_array()), ...)
+        computes, without the host round trip."""
+        self = cls.__new__(cls)
+        self._h = None
+        vp = _arr(voxel_pos, np.float32)
+        N = 2 if int(Nz_v) == 1 else 3
+        if vp.size != int(Nx_v) * int(Ny_v) * int(Nz_v) * N:
+            raise ValueError("voxel_pos has the wrong size")
+        h = _vp()
+        _check(_reg_create_resampled(ctypes.byref(h), src._h, float(scale), _ptr(vp),
+                                     int(Nx_v), int(Ny_v), int(Nz_v), int(Nx_n), int(Ny_n),
+                                     int(Nz_n), int(spline_order), float(non_zero_voxels)))
+        self._h = h
+        self._shape = (int(Nx_v), int(Ny_v), int(Nz_v), N)
+        self._nn = int(Nx_n) * int(Ny_n) * int(Nz_n)
+        self._KL = None
+        return self
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -458,7 +503,8 @@ def exported_symbols():
     return [
         "binder_last_error", "binder_backend", "binder_philox4x32_10", "binder_resize_field",
         "binder_topology_correction_2d", "binder_spline_weights",
-        "binder_registration_correct_topology",
+        "binder_registration_correct_topology", "binder_registration_create_resampled",
+        "binder_dct_prior_eigenvalues",
         "binder_registration_create",
         "binder_registration_destroy", "binder_registration_set_mi_likelihood",
         "binder_registration_set_nonlinear_transformation", "binder_registration_em",

@@ -271,7 +271,11 @@ __host__ __device__ inline double gamma_mt(double shape, u32x4 c, uint32_t k0,
     const double v = v1 * v1 * v1;
     const double u = u01_open(r.z);
     last = d * v;
-    if (log(u) < 0.5 * n0 * n0 + d - d * v + d * log(v)) return d * v * boost;
+    // the paper's squeeze u < 1 - 0.0331 x^4 accepts most draws without the
+    // two logarithms; it only accepts draws the exact test accepts
+    const double x2 = n0 * n0;
+    if (u < 1.0 - 0.0331 * x2 * x2) return d * v * boost;
+    if (log(u) < 0.5 * x2 + d - d * v + d * log(v)) return d * v * boost;
   }
   return last * boost;
 }

@@ -119,6 +119,32 @@ __device__ __forceinline__ void load_columns(const EStepArgs &a, int ix, int iy,
 
 // Same words as load_columns, with 32-bit 24-bit-multiply indexing (the
 // host guarantees (Ny+2)(Nz+5) < 2^24 and the padded grid < 2^31 words).
+// Element `idx` of a device array addressed with a 32-bit byte offset from
+// the (wavefront-uniform) base, so the load takes the scalar-base form; the
+// host bounds every array read this way below 2^32 bytes.
+template <typename T>
+__device__ __forceinline__ T ld_off32(const T *base, uint32_t idx) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                      idx * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st_off32(T *base, uint32_t idx, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(T)) = v;
+}
+
+// Word of one column (xa, yb) of the 4 x 4 neighbourhood, indexed as in
+// load_columns32.
+template <bool IS2D>
+__device__ __forceinline__ uint32_t load_column32(const EStepArgs &a, int ix, int iy,
+                                                  int iz, int xa, int yb) {
+  const uint32_t SX = IS2D ? (uint32_t)a.Yp : (uint32_t)a.Yp * (uint32_t)a.Zq;
+  const uint32_t SY = IS2D ? 1u : (uint32_t)a.Zq;
+  const uint32_t pz = IS2D ? 0u : (uint32_t)(clampi(iz - 1, -4, a.nzn) + 4);
+  const uint32_t idx = __umul24((uint32_t)(clampi(ix - 1 + xa, -1, a.nxn) + 1), SX) +
+                       __umul24((uint32_t)(clampi(iy - 1 + yb, -1, a.nyn) + 1), SY) + pz;
+  return IS2D ? (uint32_t)ld_off32(a.bins2d, idx) : ld_off32(a.quads, idx);
+}
+
 template <bool IS2D>
 __device__ __forceinline__ void load_columns32(const EStepArgs &a, int ix, int iy,
                                                int iz, uint32_t q[16]) {
@@ -136,7 +162,7 @@ __device__ __forceinline__ void load_columns32(const EStepArgs &a, int ix, int i
 #pragma unroll
     for (int yb = 0; yb < 4; ++yb) {
       const uint32_t idx = ox[xa] + oy[yb];
-      q[xa * 4 + yb] = IS2D ? (uint32_t)a.bins2d[idx] : a.quads[idx];
+      q[xa * 4 + yb] = IS2D ? (uint32_t)ld_off32(a.bins2d, idx) : ld_off32(a.quads, idx);
     }
 }
 
@@ -165,10 +191,9 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
   // sampling: one drawn node per voxel, counted in 32 bits (the first half of
   // the same zeroed LDS range)
   unsigned int *cnt = reinterpret_cast<unsigned int *>(smem);
-  // likelihood columns theta[:, u] with a row stride of K+1 so that lanes with
-  // different fixed bins reading the same moving bin hit different banks
+  // likelihood rows theta[u][:] at a padded stride (bank spread)
   float *th = reinterpret_cast<float *>(hist + KL);
-  const int Kp = a.K + 1;
+  const int Kp = theta_row_stride(a.K, a.L);
   for (int i = threadIdx.x; i < KL; i += blockDim.x) {
     hist[i] = 0ull;
     th[(i / a.K) * Kp + (i % a.K)] = a.thT[i];
@@ -406,12 +431,12 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
 // target r Z (r in (0, 1)) selects the first column with cum >= target by
 // counting columns below it, then the node inside the column with partial
 // sums formed by the same operations, so the scan always ends inside the
-// selected column.  LDS: theta columns (stride K+1) | 32-bit counts [K*L].
+// selected column.  LDS: theta rows (padded stride) | 32-bit counts [K*L].
 template <bool IS2D>
 __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int KL = a.K * a.L;
-  const int Kp = a.K + 1;
+  const int Kp = theta_row_stride(a.K, a.L);
   float *th = reinterpret_cast<float *>(smem);
   unsigned int *cnt = reinterpret_cast<unsigned int *>(th + ((a.L * Kp + 3) & ~3));
   for (int i = threadIdx.x; i < KL; i += blockDim.x) {
@@ -422,78 +447,95 @@ __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
 
   constexpr int NZ = IS2D ? 1 : 4;
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-  const long long per =
-      ((a.nv + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
-  const long long vbeg = (long long)lb * per;
-  const long long vend = vbeg + per < a.nv ? vbeg + per : a.nv;
-  for (long long i = vbeg + threadIdx.x; i < vend; i += blockDim.x) {
-    const int u = a.bv[i];
-    const float *thu = th + u * Kp;
-    float tx, ty, tz = 0.f;
-    const int ix = safe_floor(a.d[i], tx);
-    const int iy = safe_floor(a.d[a.nv + i], ty);
-    int iz = 0;
-    if (!IS2D) iz = safe_floor(a.d[2 * a.nv + i], tz);
-    float wx[4], wy[4], wz[4] = {1.f, 0.f, 0.f, 0.f};
-    spline3(tx, wx);
-    spline3(ty, wy);
-    if (!IS2D) spline3(tz, wz);
-    uint32_t q[16];
-    load_columns32<IS2D>(a, ix, iy, iz, q);
+  // Voxels go in chunks of 256, one chunk per wavefront at a time: lane l
+  // takes voxels 256 c + 64 q + l (q = 0..3) and one Philox block, counter
+  // 64 c + l, supplies their four uniforms (word q for voxel q).  The stream
+  // therefore depends on the voxel index only.  Each logical block owns a
+  // contiguous run of chunks (XCD-local gathers).
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const long long nch = (a.nv + 255) >> 8;
+  const long long cbeg = (long long)lb * nch / gridDim.x;
+  const long long cend = (long long)(lb + 1) * nch / gridDim.x;
+  for (long long ch = cbeg + wave; ch < cend; ch += nwaves) {
+    const long long g = ch * 64 + lane;
+    u32x4 rr = philox4x32_10(u32x4{(uint32_t)g, (uint32_t)(g >> 32), a.sweep, kStreamEStep},
+                             a.k0, a.k1);
+#pragma unroll 1
+    for (int qv = 0; qv < 4; ++qv) {
+      const long long i = ch * 256 + qv * 64 + lane;
+      if (i >= a.nv) break;  // only the last chunk is partial
+      const uint32_t word = rr.x;
+      rr = u32x4{rr.y, rr.z, rr.w, rr.x};
+      // 32-bit offsets: the host keeps N * nv below 2^30
+      const uint32_t iu = (uint32_t)i, nv32 = (uint32_t)a.nv;
+      const int u = ld_off32(a.bv, iu);
+      const float *thu = th + u * Kp;
+      float tx, ty, tz = 0.f;
+      const int ix = safe_floor(ld_off32(a.d, iu), tx);
+      const int iy = safe_floor(ld_off32(a.d, nv32 + iu), ty);
+      int iz = 0;
+      if (!IS2D) iz = safe_floor(ld_off32(a.d, 2 * nv32 + iu), tz);
+      float wx[4], wy[4], wz[4] = {1.f, 0.f, 0.f, 0.f};
+      spline3(tx, wx);
+      spline3(ty, wy);
+      if (!IS2D) spline3(tz, wz);
+      uint32_t q[16];
+      load_columns32<IS2D>(a, ix, iy, iz, q);
 
-    float cum[16];
-    float acc = 0.f;
+      float cum[16];
+      float acc = 0.f;
 #pragma unroll
-    for (int c = 0; c < 16; ++c) {
-      float zy = wz[0] * thu[q[c] & 0xFF];
+      for (int c = 0; c < 16; ++c) {
+        float zy = wz[0] * thu[q[c] & 0xFF];
 #pragma unroll
-      for (int zc = 1; zc < NZ; ++zc) zy = fmaf(wz[zc], thu[(q[c] >> (8 * zc)) & 0xFF], zy);
-      acc = fmaf(wx[c >> 2] * wy[c & 3], zy, acc);
-      cum[c] = acc;
-    }
-    const float Z = acc;
-    const u32x4 rr = philox4x32_10(
-        u32x4{(uint32_t)i, (uint32_t)(i >> 32), a.sweep, kStreamEStep}, a.k0, a.k1);
-    const float target = u01_mid(rr.x) * Z;
-    // column: the first whose cumulative sum reaches the target.  One pass
-    // over the (non-decreasing) cumulative sums carries the count of columns
-    // below the target, the last sum below it and the next column's word and
-    // weight (running selects: no indexed access to the arrays)
-    int sel = 0;
-    float before = 0.f;
-    uint32_t qs = q[0];
-    float wxy = wx[0] * wy[0];
-#pragma unroll
-    for (int c = 0; c < 15; ++c) {
-      const bool below = cum[c] < target;
-      sel += below ? 1 : 0;
-      before = below ? cum[c] : before;
-      qs = below ? q[c + 1] : qs;
-      wxy = below ? wx[(c + 1) >> 2] * wy[(c + 1) & 3] : wxy;
-    }
-    const int xa = sel >> 2, yb = sel & 3;
-    // node inside the column, partial sums with the column's own operations
-    int zsel = 0;
-    if (!IS2D) {
-      float zy = wz[0] * thu[qs & 0xFF];
-#pragma unroll
-      for (int zc = 1; zc < NZ; ++zc) {
-        zsel += fmaf(wxy, zy, before) < target ? 1 : 0;
-        zy = fmaf(wz[zc], thu[(qs >> (8 * zc)) & 0xFF], zy);
+        for (int zc = 1; zc < NZ; ++zc) zy = fmaf(wz[zc], thu[(q[c] >> (8 * zc)) & 0xFF], zy);
+        acc = fmaf(wx[c >> 2] * wy[c & 3], zy, acc);
+        cum[c] = acc;
       }
-    }
-    if (Z > 0.f) {
-      a.dream[i] = (float)(ix - 1 + xa);
-      a.dream[a.nv + i] = (float)(iy - 1 + yb);
-      if (!IS2D) a.dream[2 * a.nv + i] = (float)(iz - 1 + zsel);
-      const int v = (qs >> (8 * zsel)) & 0xFF;
-      atomicAdd(cnt + v * a.L + u, 1u);
-    } else {
-      // no draw (all weights zero): the reference leaves the voxel's reset
-      // dream location (0) and adds nothing to the histogram
-      a.dream[i] = 0.f;
-      a.dream[a.nv + i] = 0.f;
-      if (!IS2D) a.dream[2 * a.nv + i] = 0.f;
+      const float Z = acc;
+      const float target = u01_mid(word) * Z;
+      // column: the first whose cumulative sum reaches the target.  One pass
+      // over the (non-decreasing) cumulative sums counts the columns below the
+      // target and keeps the last sum below it (running selects: no indexed
+      // access to the arrays); the selected column's word is then read again
+      // (a cache hit) rather than kept live through the scan
+      int sel = 0;
+      float before = 0.f;
+#pragma unroll
+      for (int c = 0; c < 15; ++c) {
+        const bool below = cum[c] < target;
+        sel += below ? 1 : 0;
+        before = below ? cum[c] : before;
+      }
+      const int xa = sel >> 2, yb = sel & 3;
+      const uint32_t qs = load_column32<IS2D>(a, ix, iy, iz, xa, yb);
+      const float wxa = xa == 0 ? wx[0] : (xa == 1 ? wx[1] : (xa == 2 ? wx[2] : wx[3]));
+      const float wyb = yb == 0 ? wy[0] : (yb == 1 ? wy[1] : (yb == 2 ? wy[2] : wy[3]));
+      const float wxy = wxa * wyb;
+      // node inside the column, partial sums with the column's own operations
+      int zsel = 0;
+      if (!IS2D) {
+        float zy = wz[0] * thu[qs & 0xFF];
+#pragma unroll
+        for (int zc = 1; zc < NZ; ++zc) {
+          zsel += fmaf(wxy, zy, before) < target ? 1 : 0;
+          zy = fmaf(wz[zc], thu[(qs >> (8 * zc)) & 0xFF], zy);
+        }
+      }
+      if (Z > 0.f) {
+        st_off32(a.dream, iu, (float)(ix - 1 + xa));
+        st_off32(a.dream, nv32 + iu, (float)(iy - 1 + yb));
+        if (!IS2D) st_off32(a.dream, 2 * nv32 + iu, (float)(iz - 1 + zsel));
+        const int v = (qs >> (8 * zsel)) & 0xFF;
+        atomicAdd(cnt + v * a.L + u, 1u);
+      } else {
+        // no draw (all weights zero): the reference leaves the voxel's reset
+        // dream location (0) and adds nothing to the histogram
+        st_off32(a.dream, iu, 0.f);
+        st_off32(a.dream, nv32 + iu, 0.f);
+        if (!IS2D) st_off32(a.dream, 2 * nv32 + iu, 0.f);
+      }
     }
   }
 
@@ -517,7 +559,7 @@ __global__ __launch_bounds__(kEm32Threads) void k_estep_em32(EStepArgs a) {
   const int KL = a.K * a.L;
   unsigned long long *hist = reinterpret_cast<unsigned long long *>(smem);
   float *th = reinterpret_cast<float *>(hist + KL);
-  const int Kp = a.K + 1;
+  const int Kp = theta_row_stride(a.K, a.L);
   uint32_t *priv = reinterpret_cast<uint32_t *>(th + ((a.L * Kp + 3) & ~3));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint32_t *mine = priv + wave * (kEm32MaxK * 64) + lane;  // bin v at mine[64 * v]
@@ -703,16 +745,23 @@ __global__ __launch_bounds__(1024) void k_theta(ThetaArgs a) {
     for (int kl = tid; kl < KL; kl += 1024) sh[kl] = sh[kl] / rowc[kl / a.L];
     __syncthreads();
   }
-  // -log Dir(theta_k; alpha) per row, then summed in row order
+  // -log Dir(theta_k; alpha) per row: the K*L terms (alpha - 1) log theta in
+  // parallel (after theta is written out), then each row summed in index
+  // order, then the rows in row order
+  for (int kl = tid; kl < KL; kl += 1024) {
+    const double t = sh[kl];
+    a.theta[kl] = t;
+    a.thT[(kl % a.L) * a.K + kl / a.L] = (float)t;
+    sh[kl] = (a.alpha - 1.0) * log(t);
+  }
+  __syncthreads();
   for (int k = tid; k < a.K; k += 1024) {
     double lp = a.lg_sum_alpha;
     for (int l = 0; l < a.L; ++l) lp -= a.lg_alpha;
-    for (int l = 0; l < a.L; ++l) lp += (a.alpha - 1.0) * log(sh[k * a.L + l]);
+    const double *terms = sh + k * a.L;
+#pragma unroll 8
+    for (int l = 0; l < a.L; ++l) lp += terms[l];
     rowc[k] = -lp;
-  }
-  for (int kl = tid; kl < KL; kl += 1024) {
-    a.theta[kl] = sh[kl];
-    a.thT[(kl % a.L) * a.K + kl / a.L] = (float)sh[kl];
   }
   __syncthreads();
   if (tid == 0) {
@@ -758,11 +807,12 @@ __global__ __launch_bounds__(256) void k_welford(WelfordBatch b, double *mean,
 
 void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
                   hipStream_t s) {
+  const int Kp = theta_row_stride(a.K, a.L);
   const size_t lds = (size_t)a.K * a.L * sizeof(unsigned long long) +
-                     (size_t)(a.K + 1) * a.L * sizeof(float);
+                     (size_t)Kp * a.L * sizeof(float);
   dim3 g(blocks), b(256);
   if (sample) {
-    const size_t ldss = (size_t)(((a.K + 1) * a.L + 3) & ~3) * sizeof(float) +
+    const size_t ldss = (size_t)((Kp * a.L + 3) & ~3) * sizeof(float) +
                         (size_t)a.K * a.L * sizeof(unsigned int);
     if (is2d) hipLaunchKernelGGL(k_esample<true>, g, b, ldss, s, a);
     else hipLaunchKernelGGL(k_esample<false>, g, b, ldss, s, a);
@@ -771,7 +821,7 @@ void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
   }
   if (a.K <= kEm32MaxK) {
     const size_t lds32 = (size_t)a.K * a.L * sizeof(unsigned long long) +
-                         (size_t)(((a.K + 1) * a.L + 3) & ~3) * sizeof(float) +
+                         (size_t)((Kp * a.L + 3) & ~3) * sizeof(float) +
                          (size_t)(kEm32Threads / 64) * kEm32MaxK * 64 * sizeof(float);
     static bool attr_set[2] = {false, false};
     auto k = is2d ? k_estep_em32<true> : k_estep_em32<false>;

@@ -17,6 +17,7 @@
 #include "estep.hpp"
 #include "smoothing.hpp"
 #include "topology.hpp"
+#include "resample.hpp"
 
 namespace binder {
 
@@ -199,6 +200,60 @@ int binder_philox4x32_10(const unsigned int *ctr4, const unsigned int *key2,
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Shape checks and buffers shared by the two constructors; the caller fills
+// the current field r->d.
+std::unique_ptr<binder_registration> new_registration(const float *voxel_pos, int Nx_v,
+                                                      int Ny_v, int Nz_v, int Nx_n, int Ny_n,
+                                                      int Nz_n, int spline_order,
+                                                      double non_zero_voxels) {
+  BINDER_REQUIRE(voxel_pos, "null input array");
+  BINDER_REQUIRE(Nx_v >= 2 && Ny_v >= 2 && Nz_v >= 1, "bad fixed-grid shape");
+  BINDER_REQUIRE(Nx_n >= 1 && Ny_n >= 1 && Nz_n >= 1, "bad moving-grid shape");
+  BINDER_REQUIRE(spline_order == 3,
+                 "the HIP backend implements the cubic B-spline only "
+                 "(spline_order must be 3)");
+  BINDER_REQUIRE((Nz_v == 1) == (Nz_n == 1),
+                 "fixed and moving images must both be 2D or both 3D");
+  auto r = std::make_unique<binder_registration>();
+  r->nxv = Nx_v;
+  r->nyv = Ny_v;
+  r->nzv = Nz_v;
+  r->nxn = Nx_n;
+  r->nyn = Ny_n;
+  r->nzn = Nz_n;
+  r->N = Nz_v == 1 ? 2 : 3;
+  r->is2d = Nz_v == 1;
+  r->nv = (long long)Nx_v * Ny_v * Nz_v;
+  // the sampling E-step addresses the [N][nv] fields with 32-bit offsets
+  BINDER_REQUIRE(r->nv * 3 < (1ll << 30), "fixed grid too large (N * Nv must be below 2^30)");
+  r->nn = (long long)Nx_n * Ny_n * Nz_n;
+  r->nnz = non_zero_voxels;
+  std::vector<float> tmp((size_t)r->N * r->nv);
+  aos_to_soa(voxel_pos, tmp.data(), r->nv, r->N);
+  r->x.upload(tmp.data(), tmp.size(), r->s());
+  r->d.alloc(tmp.size());
+  r->dream.alloc(tmp.size());
+  r->dream.zero(r->s());
+  r->stats.alloc(16);
+  r->stats.zero(r->s());
+  // 8 blocks of 4 wavefronts per CU on 256 CUs; each block keeps one
+  // partial histogram row
+  long long blocks = (r->nv + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  r->estep_blocks = (int)blocks;
+  r->mllpart.alloc(blocks);
+  HIP_OK(hipStreamSynchronize(r->s()));
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
 int binder_registration_create(binder_registration **out,
                                const float *final_locations, int threads,
                                const float *voxel_pos, const float *node_pos,
@@ -210,43 +265,32 @@ int binder_registration_create(binder_registration **out,
   return guarded([&] {
     BINDER_REQUIRE(out != nullptr, "out handle pointer is null");
     *out = nullptr;
-    BINDER_REQUIRE(final_locations && voxel_pos, "null input array");
-    BINDER_REQUIRE(Nx_v >= 2 && Ny_v >= 2 && Nz_v >= 1, "bad fixed-grid shape");
-    BINDER_REQUIRE(Nx_n >= 1 && Ny_n >= 1 && Nz_n >= 1, "bad moving-grid shape");
-    BINDER_REQUIRE(spline_order == 3,
-                   "the HIP backend implements the cubic B-spline only "
-                   "(spline_order must be 3)");
-    BINDER_REQUIRE((Nz_v == 1) == (Nz_n == 1),
-                   "fixed and moving images must both be 2D or both 3D");
-    auto r = std::make_unique<binder_registration>();
-    r->nxv = Nx_v;
-    r->nyv = Ny_v;
-    r->nzv = Nz_v;
-    r->nxn = Nx_n;
-    r->nyn = Ny_n;
-    r->nzn = Nz_n;
-    r->N = Nz_v == 1 ? 2 : 3;
-    r->is2d = Nz_v == 1;
-    r->nv = (long long)Nx_v * Ny_v * Nz_v;
-    r->nn = (long long)Nx_n * Ny_n * Nz_n;
-    r->nnz = non_zero_voxels;
+    BINDER_REQUIRE(final_locations, "null input array");
+    auto r = new_registration(voxel_pos, Nx_v, Ny_v, Nz_v, Nx_n, Ny_n, Nz_n, spline_order,
+                              non_zero_voxels);
     std::vector<float> tmp((size_t)r->N * r->nv);
     aos_to_soa(final_locations, tmp.data(), r->nv, r->N);
     r->d.upload(tmp.data(), tmp.size(), r->s());
     HIP_OK(hipStreamSynchronize(r->s()));
-    aos_to_soa(voxel_pos, tmp.data(), r->nv, r->N);
-    r->x.upload(tmp.data(), tmp.size(), r->s());
-    r->dream.alloc(tmp.size());
-    r->dream.zero(r->s());
-    r->stats.alloc(16);
-    r->stats.zero(r->s());
-    // 8 blocks of 4 wavefronts per CU on 256 CUs; each block keeps one
-    // partial histogram row
-    long long blocks = (r->nv + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
-    r->estep_blocks = (int)blocks;
-    r->mllpart.alloc(blocks);
-    HIP_OK(hipStreamSynchronize(r->s()));
+    *out = r.release();
+  });
+}
+
+int binder_registration_create_resampled(binder_registration **out,
+                                         binder_registration *src, double scale,
+                                         const float *voxel_pos, int Nx_v, int Ny_v,
+                                         int Nz_v, int Nx_n, int Ny_n, int Nz_n,
+                                         int spline_order, double non_zero_voxels) {
+  return guarded([&] {
+    BINDER_REQUIRE(out != nullptr, "out handle pointer is null");
+    *out = nullptr;
+    BINDER_REQUIRE(src != nullptr, "null source handle");
+    auto r = new_registration(voxel_pos, Nx_v, Ny_v, Nz_v, Nx_n, Ny_n, Nz_n, spline_order,
+                              non_zero_voxels);
+    BINDER_REQUIRE(r->N == src->N, "source and new grid differ in dimensionality");
+    HIP_OK(hipStreamSynchronize(src->s()));
+    resize_planes_device(src->dcur(), src->nxv, src->nyv, src->nzv, src->N, r->d.get(),
+                         r->nxv, r->nyv, r->nzv, scale, r->s());
     *out = r.release();
   });
 }
@@ -285,7 +329,7 @@ int binder_registration_set_mi_likelihood(binder_registration *r, double alpha,
     // the E-step indexes the padded moving grid with 32-bit offsets and
     // 24-bit multiplies
     const long long slab = r->is2d ? (long long)Yp : (long long)Yp * (r->nzn + 5);
-    BINDER_REQUIRE(slab < (1ll << 24) && (long long)Xp * slab < (1ll << 31),
+    BINDER_REQUIRE(slab < (1ll << 24) && (long long)Xp * slab < (1ll << 30),
                    "moving grid too large for the E-step's 32-bit indexing");
     if (r->is2d) {
       std::vector<uint8_t> p((size_t)Xp * Yp, 0);
@@ -486,6 +530,41 @@ int binder_registration_sampler(binder_registration *r, int seed, int N_b,
 
 // Spline(order, is2D).spline(d) / .log_spline(d)
 // (BINDER/src/cpu/Spline.cpp:44-145, BINDER/src/cpu/bindings.cpp:436-445):
+// Eigenvalues of the finite-difference prior in the DCT domain,
+// D = (sum_a (2 - 2 cos(pi k_a / N_a)) / vs_a^2) ^ order over the first nd
+// axes of (nx, ny, nz), summed x + y (+ z) in FP64 and rounded to float32:
+// the values of BINDER/utils.py:331-349 (initialize_fourier_domain_smoothing,
+// DCT branch) as the handle stores them, from a host loop over the separable
+// terms instead of a full-grid NumPy evaluation.
+int binder_dct_prior_eigenvalues(int nx, int ny, int nz, int nd, int order,
+                                 const double *voxel_size, float *out) {
+  return guarded([&] {
+    BINDER_REQUIRE(out && voxel_size, "null argument");
+    BINDER_REQUIRE(nd >= 1 && nd <= 3, "nd must be 1, 2 or 3");
+    BINDER_REQUIRE(order >= 1, "order must be positive");
+    const int dims[3] = {nx, nd >= 2 ? ny : 1, nd >= 3 ? nz : 1};
+    BINDER_REQUIRE(dims[0] > 0 && dims[1] > 0 && dims[2] > 0, "sizes must be positive");
+    std::vector<double> t[3];
+    for (int a = 0; a < 3; ++a) {
+      t[a].assign(dims[a], 0.0);
+      if (a >= nd) continue;
+      const double v2 = voxel_size[a] * voxel_size[a];
+      for (int k = 0; k < dims[a]; ++k)
+        t[a][k] = (2.0 - 2.0 * std::cos(M_PI * (double)k / (double)dims[a])) / v2;
+    }
+    auto power = [order](double s) {
+      return order == 1 ? s : (order == 2 ? s * s : std::pow(s, (double)order));
+    };
+    size_t o = 0;
+    for (int x = 0; x < dims[0]; ++x)
+      for (int y = 0; y < dims[1]; ++y) {
+        const double sxy = nd >= 2 ? t[0][x] + t[1][y] : t[0][x];
+        for (int z = 0; z < dims[2]; ++z)
+          out[o++] = (float)power(nd >= 3 ? sxy + t[2][z] : sxy);
+      }
+  });
+}
+
 // B-spline weights of order 1, 2 or 3 at the order's node offsets for the
 // fractional distance d, and their logarithms with -1e100 for a weight that
 // is exactly zero.  Unused trailing entries are 0 (weights) / -1e100 (logs).

@@ -11,12 +11,16 @@
 //      input coordinate (o + 0.5) * n_in / n_out - 0.5;
 //   3. both steps extend the input by mirroring about the edge samples
 //      (... c b | a b c d | c b ...);
-//   4. the result is clipped to the input's [min, max] and multiplied by a
-//      scale factor (the pyramid's current / new resolution).
-// The field is [nx][ny][nz][comps] (components innermost), as at the C ABI.
+//   4. the result is clipped to the input's [min, max] (over all components)
+//      and multiplied by a scale factor (the pyramid's current / new
+//      resolution).
+// Two layouts share the kernels: the C ABI's [nx][ny][nz][comps] doubles
+// (components innermost) and a registration handle's [comps][nx][ny][nz]
+// float planes.  Element (voxel v, component c) sits at c * cs + v * vs.
 #include <algorithm>
 
 #include "common.hpp"
+#include "resample.hpp"
 
 namespace binder {
 namespace {
@@ -30,28 +34,48 @@ __device__ __forceinline__ int mirror_index(int i, int n) {
   return i < n ? i : period - i;
 }
 
+struct Layout {
+  long long cs, vs;  // component and voxel strides
+};
+
+// Element number e of a grid-stride loop -> (voxel, component), ordered as
+// the layout stores them.
+__device__ __forceinline__ void split_element(long long e, int comps, long long nvox,
+                                              Layout L, long long &v, int &c) {
+  if (L.vs == 1) {  // planes
+    c = (int)(e / nvox);
+    v = e - (long long)c * nvox;
+  } else {  // components innermost
+    v = e / comps;
+    c = (int)(e - v * comps);
+  }
+}
+
 // One axis of the separable Gaussian: out = sum_t w[t] in[mirror(p + t - r)]
-// along `axis` of a [nx][ny][nz][comps] array.
+// along `axis`.  Same layout for in and out.
 __global__ __launch_bounds__(256) void k_gauss_axis(const double *__restrict__ in,
                                                     double *__restrict__ out, int nx,
-                                                    int ny, int nz, int comps, int axis,
-                                                    const double *__restrict__ w,
+                                                    int ny, int nz, int comps, Layout L,
+                                                    int axis, const double *__restrict__ w,
                                                     int radius) {
   const long long total = (long long)nx * ny * nz * comps;
   const long long stride = (long long)gridDim.x * blockDim.x;
   const int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
-  const long long step = axis == 0 ? (long long)ny * nz * comps
-                                   : (axis == 1 ? (long long)nz * comps : (long long)comps);
+  const long long vstep = axis == 0 ? (long long)ny * nz : (axis == 1 ? (long long)nz : 1ll);
+  const long long nvox = (long long)nx * ny * nz;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += stride) {
-    const long long v = e / comps;
+    // consecutive threads take consecutive addresses in either layout
+    long long v;
+    int c;
+    split_element(e, comps, nvox, L, v, c);
     const int z = (int)(v % nz), y = (int)((v / nz) % ny), x = (int)(v / nz / ny);
     const int p = axis == 0 ? x : (axis == 1 ? y : z);
-    const long long base = e - (long long)p * step;
+    const long long base = c * L.cs + (v - (long long)p * vstep) * L.vs;
     double acc = 0.0;
     for (int t = -radius; t <= radius; ++t)
-      acc += w[t + radius] * in[base + (long long)mirror_index(p + t, n) * step];
-    out[e] = acc;
+      acc += w[t + radius] * in[base + (long long)mirror_index(p + t, n) * vstep * L.vs];
+    out[c * L.cs + v * L.vs] = acc;
   }
 }
 
@@ -70,21 +94,28 @@ __device__ __forceinline__ Lerp lerp_at(int o, int n_in, int n_out) {
   return l;
 }
 
-__global__ __launch_bounds__(256) void k_zoom(const double *__restrict__ in, int nx,
-                                              int ny, int nz, int comps,
-                                              double *__restrict__ out, int nxo,
-                                              int nyo, int nzo, double lo, double hi,
+// Linear zoom, clip to [range[0], range[1]], scale; OUT is double or float.
+template <typename OUT>
+__global__ __launch_bounds__(256) void k_zoom(const double *__restrict__ in, int nx, int ny,
+                                              int nz, int comps, Layout Li,
+                                              OUT *__restrict__ out, int nxo, int nyo,
+                                              int nzo, Layout Lo,
+                                              const double *__restrict__ range,
                                               double scale) {
   const long long total = (long long)nxo * nyo * nzo * comps;
   const long long stride = (long long)gridDim.x * blockDim.x;
+  const double lo = range[0], hi = range[1];
+  const long long nvox = (long long)nxo * nyo * nzo;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += stride) {
-    const int c = (int)(e % comps);
-    const long long v = e / comps;
+    long long v;
+    int c;
+    split_element(e, comps, nvox, Lo, v, c);
     const int zo = (int)(v % nzo), yo = (int)((v / nzo) % nyo), xo = (int)(v / nzo / nyo);
     const Lerp lx = lerp_at(xo, nx, nxo), ly = lerp_at(yo, ny, nyo), lz = lerp_at(zo, nz, nzo);
+    const double *inc = in + c * Li.cs;
     auto at = [&](int x, int y, int z) {
-      return in[(((long long)x * ny + y) * nz + z) * comps + c];
+      return inc[(((long long)x * ny + y) * nz + z) * Li.vs];
     };
     const double z00 = (1.0 - lz.w) * at(lx.i0, ly.i0, lz.i0) + lz.w * at(lx.i0, ly.i0, lz.i1);
     const double z01 = (1.0 - lz.w) * at(lx.i0, ly.i1, lz.i0) + lz.w * at(lx.i0, ly.i1, lz.i1);
@@ -94,7 +125,65 @@ __global__ __launch_bounds__(256) void k_zoom(const double *__restrict__ in, int
     const double y1 = (1.0 - ly.w) * z10 + ly.w * z11;
     double r = (1.0 - lx.w) * y0 + lx.w * y1;
     r = r < lo ? lo : (r > hi ? hi : r);
-    out[e] = r * scale;
+    out[c * Lo.cs + v * Lo.vs] = (OUT)(r * scale);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_f32_to_f64(const float *__restrict__ in,
+                                                    double *__restrict__ out, long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = (double)in[i];
+}
+
+// Min and max of n doubles: per-block partials, then one block folds them.
+__global__ __launch_bounds__(256) void k_minmax_part(const double *__restrict__ in,
+                                                     long long n, double *__restrict__ part) {
+  __shared__ double lo[256], hi[256];
+  double a = INFINITY, b = -INFINITY;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double v = in[i];
+    a = v < a ? v : a;
+    b = v > b ? v : b;
+  }
+  lo[threadIdx.x] = a;
+  hi[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + s]);
+      hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = lo[0];
+    part[2 * blockIdx.x + 1] = hi[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_minmax_final(const double *__restrict__ part,
+                                                      int nparts, double *__restrict__ range) {
+  __shared__ double lo[256], hi[256];
+  double a = INFINITY, b = -INFINITY;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    a = fmin(a, part[2 * i]);
+    b = fmax(b, part[2 * i + 1]);
+  }
+  lo[threadIdx.x] = a;
+  hi[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + s]);
+      hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    range[0] = lo[0];
+    range[1] = hi[0];
   }
 }
 
@@ -103,7 +192,74 @@ int grid_for(long long total) {
   return (int)std::min<long long>(std::max<long long>(b, 1), 8192);
 }
 
+std::vector<double> gauss_weights(int n_in, int n_out, int &radius) {
+  const double sigma = ((double)n_in / n_out - 1.0) / 2.0;
+  radius = (int)(4.0 * sigma + 0.5);
+  std::vector<double> wh(2 * radius + 1);
+  double sum = 0.0;
+  for (int t = -radius; t <= radius; ++t) {
+    wh[t + radius] = std::exp(-0.5 / (sigma * sigma) * (double)t * t);
+    sum += wh[t + radius];
+  }
+  for (double &v : wh) v /= sum;
+  return wh;
+}
+
+// Gaussian passes on `a` (layout L, scratch b), then the zoom into `out`.
+// `range` is a device pointer to {min, max}.  Synchronises the stream.
+template <typename OUT>
+void resize_device(DBuf<double> &a, DBuf<double> &b, Layout L, int nx, int ny, int nz,
+                   int comps, OUT *out, Layout Lo, int nxo, int nyo, int nzo,
+                   const double *range, double scale, hipStream_t s) {
+  const long long nin = (long long)nx * ny * nz * comps;
+  const long long nout = (long long)nxo * nyo * nzo * comps;
+  const int nin_ax[3] = {nx, ny, nz}, nout_ax[3] = {nxo, nyo, nzo};
+  DBuf<double> w;
+  double *cur = a.get();
+  for (int axis = 0; axis < 3; ++axis) {
+    if (nin_ax[axis] <= nout_ax[axis]) continue;
+    int radius = 0;
+    const std::vector<double> wh = gauss_weights(nin_ax[axis], nout_ax[axis], radius);
+    w.upload(wh.data(), wh.size(), s);
+    if (!b.get()) b.alloc((size_t)nin);
+    double *dst = cur == a.get() ? b.get() : a.get();
+    hipLaunchKernelGGL(k_gauss_axis, dim3(grid_for(nin)), dim3(256), 0, s, cur, dst, nx, ny,
+                       nz, comps, L, axis, w.get(), radius);
+    HIP_LAUNCH_OK();
+    // the weights buffer is re-uploaded for the next axis: wait first
+    HIP_OK(hipStreamSynchronize(s));
+    cur = dst;
+  }
+  hipLaunchKernelGGL(k_zoom<OUT>, dim3(grid_for(nout)), dim3(256), 0, s, cur, nx, ny, nz,
+                     comps, L, out, nxo, nyo, nzo, Lo, range, scale);
+  HIP_LAUNCH_OK();
+  HIP_OK(hipStreamSynchronize(s));
+}
+
 }  // namespace
+
+void resize_planes_device(const float *src, int nx, int ny, int nz, int comps, float *dst,
+                          int nxo, int nyo, int nzo, double scale, hipStream_t s) {
+  const long long nin = (long long)nx * ny * nz;
+  const long long nout = (long long)nxo * nyo * nzo;
+  DBuf<double> a, b, part, range;
+  a.alloc((size_t)comps * nin);
+  hipLaunchKernelGGL(k_f32_to_f64, dim3(grid_for(comps * nin)), dim3(256), 0, s, src, a.get(),
+                     comps * nin);
+  HIP_LAUNCH_OK();
+  const int nparts = std::min(grid_for(comps * nin), 1024);
+  part.alloc((size_t)2 * nparts);
+  range.alloc(2);
+  hipLaunchKernelGGL(k_minmax_part, dim3(nparts), dim3(256), 0, s, a.get(), comps * nin,
+                     part.get());
+  HIP_LAUNCH_OK();
+  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(256), 0, s, part.get(), nparts,
+                     range.get());
+  HIP_LAUNCH_OK();
+  resize_device<float>(a, b, Layout{nin, 1}, nx, ny, nz, comps, dst, Layout{nout, 1}, nxo,
+                       nyo, nzo, range.get(), scale, s);
+}
+
 }  // namespace binder
 
 using namespace binder;
@@ -118,10 +274,10 @@ extern "C" int binder_resize_field(const double *in, int nx, int ny, int nz,
                    "sizes must be positive");
     const long long nin = (long long)nx * ny * nz * comps;
     const long long nout = (long long)nxo * nyo * nzo * comps;
-    double lo = in[0], hi = in[0];
+    double lohi[2] = {in[0], in[0]};
     for (long long i = 1; i < nin; ++i) {
-      lo = std::min(lo, in[i]);
-      hi = std::max(hi, in[i]);
+      lohi[0] = std::min(lohi[0], in[i]);
+      lohi[1] = std::max(lohi[1], in[i]);
     }
     hipStream_t s;
     HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -129,35 +285,12 @@ extern "C" int binder_resize_field(const double *in, int nx, int ny, int nz,
       hipStream_t s;
       ~StreamGuard() { (void)hipStreamDestroy(s); }
     } sg{s};
-    DBuf<double> a, b, o, w;
+    DBuf<double> a, b, o, range;
     a.upload(in, (size_t)nin, s);
-    const int nin_ax[3] = {nx, ny, nz}, nout_ax[3] = {nxo, nyo, nzo};
-    double *cur = a.get();
-    for (int axis = 0; axis < 3; ++axis) {
-      if (nin_ax[axis] <= nout_ax[axis]) continue;
-      const double sigma = ((double)nin_ax[axis] / nout_ax[axis] - 1.0) / 2.0;
-      const int radius = (int)(4.0 * sigma + 0.5);
-      std::vector<double> wh(2 * radius + 1);
-      double sum = 0.0;
-      for (int t = -radius; t <= radius; ++t) {
-        wh[t + radius] = std::exp(-0.5 / (sigma * sigma) * (double)t * t);
-        sum += wh[t + radius];
-      }
-      for (double &v : wh) v /= sum;
-      w.upload(wh.data(), wh.size(), s);
-      if (!b.get()) b.alloc((size_t)nin);
-      double *dst = cur == a.get() ? b.get() : a.get();
-      hipLaunchKernelGGL(k_gauss_axis, dim3(grid_for(nin)), dim3(256), 0, s, cur, dst,
-                         nx, ny, nz, comps, axis, w.get(), radius);
-      HIP_LAUNCH_OK();
-      // the weights buffer is re-uploaded for the next axis: wait first
-      HIP_OK(hipStreamSynchronize(s));
-      cur = dst;
-    }
+    range.upload(lohi, 2, s);
     o.alloc((size_t)nout);
-    hipLaunchKernelGGL(k_zoom, dim3(grid_for(nout)), dim3(256), 0, s, cur, nx, ny, nz,
-                       comps, o.get(), nxo, nyo, nzo, lo, hi, scale);
-    HIP_LAUNCH_OK();
+    resize_device<double>(a, b, Layout{1, comps}, nx, ny, nz, comps, o.get(),
+                          Layout{1, comps}, nxo, nyo, nzo, range.get(), scale, s);
     o.download(out, (size_t)nout, s);
     HIP_OK(hipStreamSynchronize(s));
   });

@@ -44,6 +44,9 @@ struct PassArgs {
   const uint32_t *bhi;   // swizzled matrix, bf16 hi / lo
   const uint32_t *blo;
   long long R;
+  // output row blocks: row r = b * rin + i writes Out[b][k'][i]; rin == 0
+  // writes Out[k'][r] (the whole row range rotates behind the new axis)
+  long long rin;
   long long cin, cout;   // component strides of in / out
   int n_in, n_out, ks;   // ks = ceil(n_in / 32)
   // filter epilogue
@@ -75,6 +78,14 @@ __device__ __forceinline__ void split8(const float v[8], bf16x8_t &hi,
   }
   hi = __builtin_bit_cast(bf16x8_t, h);
   lo = __builtin_bit_cast(bf16x8_t, l);
+}
+
+// Offset of output element (k' = kc, row) in one component (see rin).
+__device__ __forceinline__ long long out_offset(const PassArgs &a, int kc, long long row) {
+  if (a.rin == 0) return (long long)kc * a.R + row;
+  const uint32_t b = (uint32_t)row / (uint32_t)a.rin;
+  const uint32_t i = (uint32_t)row - b * (uint32_t)a.rin;
+  return ((long long)b * a.n_out + kc) * a.rin + i;
 }
 
 // Four standard normals from one Philox block (Box-Muller, FP32).
@@ -168,7 +179,7 @@ __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
     }
     // epilogue operands of this tile (D or x), loaded before the MFMAs
     const int kc = nt * 16 + (lane & 15);
-    const long long obase = (long long)kc * a.R + orow;
+    const long long obase = out_offset(a, kc, orow);
     const bool colok = kc < a.n_out;
     float4 epv = make_float4(0.f, 0.f, 0.f, 0.f);
     if (EPI == EPI_FILTER || (EPI == EPI_ADDX && a.xepi)) {
@@ -179,7 +190,7 @@ __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
         float t4[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          if (orow + i < a.R) t4[i] = src[obase + i];
+          if (orow + i < a.R) t4[i] = src[out_offset(a, kc, orow + i)];
         epv = make_float4(t4[0], t4[1], t4[2], t4[3]);
       }
     }
@@ -234,7 +245,7 @@ __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
     } else {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        if (orow + i < a.R) out[obase + i] = v[i];
+        if (orow + i < a.R) out[out_offset(a, kc, orow + i)] = v[i];
     }
   }
 
@@ -335,7 +346,7 @@ __global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
     const long long orow = row0 + 4 * (lane >> 4);
     for (int nt = 0; nt < nt_count; ++nt) {
       const int kc = nt * 16 + (lane & 15);
-      const long long obase = (long long)kc * a.R + orow;
+      const long long obase = out_offset(a, kc, orow);
       const bool colok = kc < a.n_out;
       float4 epv = make_float4(0.f, 0.f, 0.f, 0.f);
       if (EPI == EPI_FILTER || (EPI == EPI_ADDX && a.xepi)) {
@@ -346,7 +357,7 @@ __global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
           float t4[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
           for (int i = 0; i < 4; ++i)
-            if (orow + i < a.R) t4[i] = src[obase + i];
+            if (orow + i < a.R) t4[i] = src[out_offset(a, kc, orow + i)];
           epv = make_float4(t4[0], t4[1], t4[2], t4[3]);
         }
       }
@@ -395,7 +406,7 @@ __global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
       } else {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          if (orow + i < a.R) out[obase + i] = v[i];
+          if (orow + i < a.R) out[out_offset(a, kc, orow + i)] = v[i];
       }
     }
   }
@@ -572,7 +583,8 @@ const PassMatrix &MatrixCache::get(int kind, int n_in, int n_out,
 long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in,
                         float *out, int comps, long long cin, long long cout,
                         long long R, const float *xpro, const float *xepi, int epi,
-                        const FilterArgs *f, KernelTimer *timer, const char *name) {
+                        const FilterArgs *f, KernelTimer *timer, const char *name,
+                        long long rin) {
   BINDER_REQUIRE(m.n_in <= 32 * kMaxKSteps, "contraction length above 256");
   PassArgs a{};
   a.in = in;
@@ -582,6 +594,7 @@ long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in
   a.bhi = m.hi.get();
   a.blo = m.lo.get();
   a.R = R;
+  a.rin = rin;
   a.cin = cin;
   a.cout = cout;
   a.n_in = m.n_in;
@@ -602,7 +615,7 @@ long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in
     a.raw_part = f->raw_part;
   }
   const bool vec = (m.n_in % 8 == 0) && (R % 4 == 0) && (cin % 4 == 0) &&
-                   (cout % 4 == 0);
+                   (cout % 4 == 0) && (rin % 4 == 0);
   dim3 grid((unsigned)((R + kRowsPerBlock - 1) / kRowsPerBlock), comps);
   TimedLaunch tl(timer, name, stream);
   const bool pro = xpro != nullptr;

@@ -41,13 +41,16 @@ struct FilterArgs {
   double *cost_part, *raw_part;
 };
 
-// Out[c][k'][r] = sum_k In[c][r][k] M[k][k'] for r < R, c < comps.
-// epi: 0 store, 1 filter (needs f), 2 add xepi.  xpro: In - xpro prologue.
-// Returns the number of partial-sum slots written by a filter pass.
+// Out[c][k'][r] = sum_k In[c][r][k] M[k][k'] for r < R, c < comps; with
+// rin > 0 the rows form blocks of rin and row r = b * rin + i goes to
+// Out[c][b][k'][i] instead.  epi: 0 store, 1 filter (needs f), 2 add xepi
+// (laid out as Out).  xpro: In - xpro prologue.  Returns the number of
+// partial-sum slots written by a filter pass.
 long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in,
                         float *out, int comps, long long cin, long long cout,
                         long long R, const float *xpro, const float *xepi, int epi,
-                        const FilterArgs *f, KernelTimer *timer, const char *name);
+                        const FilterArgs *f, KernelTimer *timer, const char *name,
+                        long long rin = 0);
 // Upper bound on the partial-sum slots a filter pass over R rows writes.
 long long contract_blocks(long long R, int comps);
 int persistent_blocks();

@@ -67,6 +67,19 @@ int binder_registration_create(binder_registration **out,
                                int Nx_v, int Ny_v, int Nz_v,
                                int Nx_n, int Ny_n, int Nz_n,
                                int spline_order, double non_zero_voxels);
+
+/* A new Registration on a new grid whose current field is `src`'s current
+ * field resampled on the device: Gaussian anti-aliasing where an axis
+ * shrinks, linear zoom, clip to the source range, times `scale` -- the
+ * resize that BINDER/FastRegistration.py:566-568 applies between pyramid
+ * levels followed by the per-level constructor (:443-450), without the
+ * field's round trip through host memory.  No reference counterpart; other
+ * arguments as binder_registration_create.  `src` stays valid. */
+int binder_registration_create_resampled(binder_registration **out,
+                                         binder_registration *src, double scale,
+                                         const float *voxel_pos, int Nx_v, int Ny_v,
+                                         int Nz_v, int Nx_n, int Ny_n, int Nz_n,
+                                         int spline_order, double non_zero_voxels);
 void binder_registration_destroy(binder_registration *reg);
 
 /* setMILikelihood(alpha, theta, K, L, threads, binned_nodes, binned_voxels)
@@ -254,6 +267,12 @@ int binder_topology_correction_3d(const double *h_init, double e1, double e2,
 int binder_registration_correct_topology(binder_registration *reg, double e1,
                                          double e2, int max_outer_it, int max_it,
                                          const double *voxel_sizes, double *stats);
+
+/* initialize_fourier_domain_smoothing(size, DCT=True, order, nd, voxel_size)
+ * rounded to float32 -- BINDER/utils.py:331-349.  Writes the nx[*ny[*nz]]
+ * values (the first nd of the three sizes), C order; host computation. */
+int binder_dct_prior_eigenvalues(int nx, int ny, int nz, int nd, int order,
+                                 const double *voxel_size, float *out);
 
 /* Spline(order, is2D).spline(d) / .log_spline(d)
  * -- BINDER/src/cpu/bindings.cpp:436-445, BINDER/src/cpu/Spline.cpp:44-145.

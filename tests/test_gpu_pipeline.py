@@ -105,3 +105,57 @@ def test_pyramid_2d_matches_oracle(core, orc, monkeypatch):
     print(f"iterations HIP {gi} oracle {oi}; field RMS diff {rms:.2e}", flush=True)
     assert gi == oi
     assert rms < 0.025
+
+
+@pytest.mark.parametrize("shape_in,shape_out,scale", [
+    ((20, 24, 20), (40, 48, 40), 2.0),
+    ((80, 96, 80), (160, 192, 160), 2.0),
+    ((17, 9, 13), (9, 18, 5), 0.5),
+    ((30, 22, 1), (15, 44, 1), 1.0)])
+def test_resampled_registration_matches_host_path(core, shape_in, shape_out, scale):
+    """Registration.resampled_from (resize on the GPU, straight into the new
+    level's registration) holds exactly the field of the host path:
+    Registration(resize_field(src.get_final_locations_as_numpy_array()))."""
+    from binder_amd import synth
+    print(f"resampled registration {shape_in} -> {shape_out}", flush=True)
+    N = 2 if shape_in[2] == 1 else 3
+    rng = np.random.default_rng(sum(shape_in) + sum(shape_out))
+    vin = np.stack(np.indices(shape_in, dtype=np.float64), -1)[..., :N]
+    f = vin + rng.standard_normal(vin.shape) * 1.5
+    src = core.Registration(f.ravel(), 1, vin.ravel(), None, *shape_in, *shape_in, 3,
+                            float(np.prod(shape_in)))
+    vout = np.stack(np.indices(shape_out, dtype=np.float64), -1)[..., :N]
+    a = core.Registration.resampled_from(src, scale, vout.ravel(), *shape_out, *shape_out, 3,
+                                         float(np.prod(shape_out)))
+    host = core.resize_field(np.asarray(src.get_final_locations_as_numpy_array(), np.float64),
+                             shape_out, scale)
+    b = core.Registration(host.ravel(), 1, vout.ravel(), None, *shape_out, *shape_out, 3,
+                          float(np.prod(shape_out)))
+    ga = a.get_final_locations_as_numpy_array()
+    gb = b.get_final_locations_as_numpy_array()
+    assert ga.shape == gb.shape == tuple(shape_out) + (N,)
+    assert np.array_equal(ga, gb)
+
+
+def test_pyramid_device_chain_matches_host_chain(core, monkeypatch):
+    """The pyramid with the field kept on the GPU between levels (device
+    resize and topology correction) against the same pyramid with a host
+    round trip at every level: same iteration counts; fields equal up to the
+    float32 rounding of the topology-corrected field that the device chain
+    keeps (the host chain carries it in float64 into the next resize)."""
+    from binder_amd import synth
+    import binder_amd.FastRegistration as fr
+    print("pyramid 48x40x44 device chain vs host chain", flush=True)
+    pair = synth.make_pair((48, 40, 44), 5, 2.0, seed=3, semi=(3.0, 12.0))
+    g = _pyramid(core, monkeypatch, pair)
+    monkeypatch.delattr(core.Registration, "resampled_from")
+    h = _pyramid(core, monkeypatch, pair)
+    gi = [lv[1] for lv in g.level_log]
+    hi = [lv[1] for lv in h.level_log]
+    diff = np.asarray(g.deformedVoxelPositions, np.float64) - \
+        np.asarray(h.deformedVoxelPositions, np.float64)
+    rms = float(np.sqrt(np.mean(np.sum(diff ** 2, -1))))
+    print(f"iterations {gi} / {hi}; RMS diff {rms:.2e}, max {np.abs(diff).max():.2e}",
+          flush=True)
+    assert gi == hi
+    assert rms < 0.01

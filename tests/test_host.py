@@ -93,6 +93,25 @@ def test_dct_eigenvalues_closed_form():
     assert D[0, 0, 0] == 0.0
 
 
+@pytest.mark.parametrize("shape,nd,vs,order", [
+    ((160, 192, 160), 3, (1.0, 1.0, 1.0), 2),
+    ((20, 24, 20), 3, (8.0, 8.0, 8.0), 2),
+    ((40, 33, 7), 3, (1.2, 0.8, 2.0), 3),
+    ((192, 160), 2, (1.0, 1.0, 1.0), 2),
+    ((17, 9), 2, (2.0, 3.0, 1.0), 1),
+    ((13,), 1, (1.5, 1.0, 1.0), 2)])
+def test_dct_prior_eigenvalues_binding(shape, nd, vs, order):
+    """The library's host loop gives exactly the float32 rounding of the
+    NumPy restatement of BINDER/utils.py:331-349 (host code, no GPU)."""
+    from binder_amd import _core
+    from binder_amd.utils import initialize_fourier_domain_smoothing
+    got = _core.dct_prior_eigenvalues(shape, order, nd, voxel_size=np.array(vs))
+    exp = np.asarray(initialize_fourier_domain_smoothing(list(shape), True, order, nd,
+                                                         voxel_size=np.array(vs)), np.float32)
+    assert got.shape == exp.shape and got.dtype == np.float32
+    assert np.array_equal(got, exp)
+
+
 def test_spline_binding(orc):
     """Spline(order, is2D).spline / log_spline of the HIP library
     (BINDER/tests/cpu/test_check_spline.py:5-76 properties, host code, no
