@@ -32,7 +32,7 @@ FP32_PEAK_TFLOPS = 157.3
 # per-kernel table: profiles/gibbs_sweep_160x192x160_hbm_traffic.md.
 MEASURED_TRAFFIC = {
     (160, 192, 160): {
-        "estep_sample": (46422.6 * 2.0 + 74000.7) * 1024,
+        "estep_sample": (46559.7 * 2.0 + 74002.8) * 1024,
     },
 }
 
@@ -44,11 +44,15 @@ def kernel_model(name, nv, nn, N, dims):
     if name in ("estep", "estep_sample"):
         # read d (N f32) + fixed bin (u8) + moving bins (u8 per node), write dream
         return nv * (N * f32 + 1 + N * f32) + nn * 1, 0.0
+    if name == "dct_x_fused":
+        # read the coefficients and D (once per component), write the field;
+        # forward and inverse transform along x
+        return nv * N * 3 * f32, 2 * 2.0 * nx * nv * N
     if name.startswith("dct_"):
+        # five-pass schedule (3D): z forward reads dream and x, the last
+        # pass (y inverse) reads x for the final addition
         axis = {"x": nx, "y": ny, "z": nz}[name[4]]
-        extra = 0
-        if name in ("dct_z_fwd", "dct_z_inv"):
-            extra = nv * N * f32  # x read by the first / last pass
+        extra = nv * N * f32 if name in ("dct_z_fwd", "dct_y_inv") else 0
         return nv * N * 2 * f32 + extra, 2.0 * axis * nv * N
     if name == "welford":
         # one launch folds WELFORD_BATCH sweeps' samples into the FP64

@@ -132,19 +132,6 @@ __device__ __forceinline__ void st_off32(T *base, uint32_t idx, T v) {
   *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(T)) = v;
 }
 
-// Word of one column (xa, yb) of the 4 x 4 neighbourhood, indexed as in
-// load_columns32.
-template <bool IS2D>
-__device__ __forceinline__ uint32_t load_column32(const EStepArgs &a, int ix, int iy,
-                                                  int iz, int xa, int yb) {
-  const uint32_t SX = IS2D ? (uint32_t)a.Yp : (uint32_t)a.Yp * (uint32_t)a.Zq;
-  const uint32_t SY = IS2D ? 1u : (uint32_t)a.Zq;
-  const uint32_t pz = IS2D ? 0u : (uint32_t)(clampi(iz - 1, -4, a.nzn) + 4);
-  const uint32_t idx = __umul24((uint32_t)(clampi(ix - 1 + xa, -1, a.nxn) + 1), SX) +
-                       __umul24((uint32_t)(clampi(iy - 1 + yb, -1, a.nyn) + 1), SY) + pz;
-  return IS2D ? (uint32_t)ld_off32(a.bins2d, idx) : ld_off32(a.quads, idx);
-}
-
 template <bool IS2D>
 __device__ __forceinline__ void load_columns32(const EStepArgs &a, int ix, int iy,
                                                int iz, uint32_t q[16]) {
@@ -447,24 +434,30 @@ __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
 
   constexpr int NZ = IS2D ? 1 : 4;
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-  // Voxels go in chunks of 256, one chunk per wavefront at a time: lane l
-  // takes voxels 256 c + 64 q + l (q = 0..3) and one Philox block, counter
-  // 64 c + l, supplies their four uniforms (word q for voxel q).  The stream
-  // therefore depends on the voxel index only.  Each logical block owns a
-  // contiguous run of chunks (XCD-local gathers).
+  // Voxels go in rows of 64, one row per wavefront step (lane l takes voxel
+  // 64 row + l).  Four consecutive rows 4c..4c+3 share one Philox block per
+  // lane, counter 64 c + l, whose word (row & 3) is the voxel's uniform: the
+  // stream depends on the voxel index only.  Every wavefront owns a
+  // contiguous run of rows, balanced to within one row, and a logical
+  // block's wavefronts own adjacent runs (XCD-local gathers).
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
-  const long long nch = (a.nv + 255) >> 8;
-  const long long cbeg = (long long)lb * nch / gridDim.x;
-  const long long cend = (long long)(lb + 1) * nch / gridDim.x;
-  for (long long ch = cbeg + wave; ch < cend; ch += nwaves) {
-    const long long g = ch * 64 + lane;
-    u32x4 rr = philox4x32_10(u32x4{(uint32_t)g, (uint32_t)(g >> 32), a.sweep, kStreamEStep},
-                             a.k0, a.k1);
-#pragma unroll 1
-    for (int qv = 0; qv < 4; ++qv) {
-      const long long i = ch * 256 + qv * 64 + lane;
-      if (i >= a.nv) break;  // only the last chunk is partial
+  const long long nrows = (a.nv + 63) >> 6;
+  const long long wtotal = (long long)gridDim.x * nwaves;
+  const long long wid = (long long)lb * nwaves + wave;
+  const long long rbeg = wid * nrows / wtotal, rend = (wid + 1) * nrows / wtotal;
+  u32x4 rr{0u, 0u, 0u, 0u};
+  for (long long row = rbeg; row < rend; ++row) {
+    {
+      const int qv = (int)(row & 3);
+      if (qv == 0 || row == rbeg) {  // wavefront-uniform
+        const long long g = (row >> 2) * 64 + lane;
+        rr = philox4x32_10(u32x4{(uint32_t)g, (uint32_t)(g >> 32), a.sweep, kStreamEStep},
+                           a.k0, a.k1);
+        for (int k = 0; k < qv; ++k) rr = u32x4{rr.y, rr.z, rr.w, rr.x};
+      }
+      const long long i = row * 64 + lane;
+      if (i >= a.nv) break;  // only the last row is partial
       const uint32_t word = rr.x;
       rr = u32x4{rr.y, rr.z, rr.w, rr.x};
       // 32-bit offsets: the host keeps N * nv below 2^30
@@ -497,19 +490,20 @@ __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
       const float target = u01_mid(word) * Z;
       // column: the first whose cumulative sum reaches the target.  One pass
       // over the (non-decreasing) cumulative sums counts the columns below the
-      // target and keeps the last sum below it (running selects: no indexed
-      // access to the arrays); the selected column's word is then read again
-      // (a cache hit) rather than kept live through the scan
+      // target and carries the last sum below it and the next column's word
+      // (running selects: no indexed access to the arrays); the weight of the
+      // selected column comes from its (x, y) indices afterwards
       int sel = 0;
       float before = 0.f;
+      uint32_t qs = q[0];
 #pragma unroll
       for (int c = 0; c < 15; ++c) {
         const bool below = cum[c] < target;
         sel += below ? 1 : 0;
         before = below ? cum[c] : before;
+        qs = below ? q[c + 1] : qs;
       }
       const int xa = sel >> 2, yb = sel & 3;
-      const uint32_t qs = load_column32<IS2D>(a, ix, iy, iz, xa, yb);
       const float wxa = xa == 0 ? wx[0] : (xa == 1 ? wx[1] : (xa == 2 ? wx[2] : wx[3]));
       const float wyb = yb == 0 ? wy[0] : (yb == 1 ? wy[1] : (yb == 2 ? wy[2] : wy[3]));
       const float wxy = wxa * wyb;

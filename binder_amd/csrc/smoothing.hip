@@ -431,6 +431,245 @@ __global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
   }
 }
 
+// ---------------------------------------------------------------------------
+// x axis in one pass: forward DCT-II, prior filter (+ noise, cost), inverse
+// DCT-III.  The data is [X][R] per component (x outermost, the R = Ny*Nz
+// frequency columns contiguous); a wavefront takes 16 columns and all x.
+//
+// The ortho DCT matrix has C[k][N-1-x] = (-1)^k C[k][x], so with N even and
+// H = N/2 the forward splits into even rows on S[x] = U[x] + U[N-1-x] and odd
+// rows on D[x] = U[x] - U[N-1-x] (x < H), and the inverse into
+// Ve[x] = sum_m C[2m][x] P[2m], Vo[x] = sum_m C[2m+1][x] P[2m+1] with
+// V[x] = Ve + Vo, V[N-1-x] = Ve - Vo: four H x H matrices, half the MFMA
+// work, and all four fit in LDS for N <= 192.
+//
+// Forward: F = E . S with the matrix as the A operand (rows m, 16 per
+// tile), the data columns as B, so each lane ends with 4 consecutive m of
+// its own column.  Two such tiles are exactly the B operand of the inverse
+// (V = E^T . P) once the inverse matrix's k order is permuted to match:
+// k-slot 8g + j of k-step s holds m = 32 s + 4 g + j (j < 4) or
+// 32 s + 16 + 4 g + j - 4 (j >= 4).  The filtered coefficients never leave
+// registers.
+// LDS: [AfE | AfO | AiE | AiO] hi, then the same lo; each [tile][kstep][lane]
+// uint4 (8 bf16).
+struct FusedArgs {
+  const float *in;  // [comp][N][R]
+  float *out;       // [comp][N][R]
+  long long R, cin, cout;
+  int n;  // N = contraction length (even)
+  const uint32_t *mhi, *mlo;
+  const float *D;  // [N][R]
+  const double *gamma;
+  float sigma2;
+  double etarho[3];
+  int noise, raw;
+  uint32_t k0, k1, sweep, stream;
+  double *cost_part, *raw_part;
+};
+
+// A zero the compiler cannot see through.  Added to the LDS matrix base in
+// every tile iteration, it keeps the fragment reads inside the loop instead
+// of being hoisted out of it (all of them at once would not fit the
+// register file).
+__device__ __forceinline__ int opaque_zero() {
+  int z;
+  asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+  return z;
+}
+
+// Float at element idx of base, addressed by a 32-bit byte offset from the
+// (uniform) base: one VGPR per address and the scalar-base load form.
+__device__ __forceinline__ float ldf(const float *base, uint32_t idx) {
+  return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + idx * 4u);
+}
+__device__ __forceinline__ void stf(float *base, uint32_t idx, float v) {
+  *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + idx * 4u) = v;
+}
+
+__device__ __forceinline__ f32x4_t mfma3(const uint4 &ah, const uint4 &al, const bf16x8_t &bh,
+                                         const bf16x8_t &bl, f32x4_t acc) {
+  const bf16x8_t h = __builtin_bit_cast(bf16x8_t, ah);
+  const bf16x8_t l = __builtin_bit_cast(bf16x8_t, al);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h, bl, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h, bh, acc, 0, 0, 0);
+  return acc;
+}
+
+// Forward rows of one parity (PAR 0: even k on sums, 1: odd k on
+// differences), prior filter, and the split of the filtered coefficients
+// into the inverse's B operand.
+template <int NTH, int KSH, int PAR>
+__device__ __forceinline__ void fused_forward(const FusedArgs &a, const uint4 *mh,
+                                              const uint4 *ml, const bf16x8_t *bh,
+                                              const bf16x8_t *bl, const float (*dv)[4],
+                                              bf16x8_t *ph, bf16x8_t *pl, int lane, int comp,
+                                              uint32_t c, bool colok, int H, double gm,
+                                              float gs2, double &cost, double &raw) {
+  constexpr int NF = NTH * KSH * 64;
+  const int g = lane >> 4;
+  const float sigma = sqrtf(a.sigma2);
+#pragma unroll
+  for (int sp = 0; sp < KSH; ++sp) {
+    float pv[8];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int t = 2 * sp + h;
+      if (t < NTH) {
+        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+        const int fb = PAR * NF + t * KSH * 64 + lane;
+#pragma unroll
+        for (int s = 0; s < KSH; ++s)
+          acc = mfma3(mh[fb + s * 64], ml[fb + s * 64], bh[s], bl[s], acc);
+        const int m0 = 16 * t + 4 * g;
+        float eps[4] = {0.f, 0.f, 0.f, 0.f};
+        if (a.noise) {
+          // one block of four normals per (column, 4 consecutive k of one parity)
+          const u32x4 r = philox4x32_10(
+              u32x4{c, ((uint32_t)m0 << 1) | (uint32_t)PAR, a.sweep,
+                    ((uint32_t)comp << 16) | a.stream},
+              a.k0, a.k1);
+          normals4(r, eps);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = m0 + r;
+          float p = 0.f;
+          if (colok && m < H) {
+            // 1 / den and sigma / sqrt(den) from the hardware reciprocal and
+            // reciprocal square root (1 ulp; the contraction itself carries
+            // about 2^-16 relative error)
+            const float Dv = dv[t][r];
+            const float den = fmaf(gs2, Dv, 1.0f);
+            p = acc[r] * __builtin_amdgcn_rcpf(den);
+            if (a.noise) p = fmaf(eps[r] * sigma, __builtin_amdgcn_rsqf(den), p);
+            const double pd = (double)p;
+            cost += 0.5 * gm * (double)Dv * pd * pd;
+            if (a.raw) raw += a.etarho[comp] * (double)Dv * pd * pd;
+          }
+          pv[4 * h + r] = p;
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) pv[4 * h + r] = 0.f;
+      }
+    }
+    split8(pv, ph[sp], pl[sp]);
+  }
+}
+
+constexpr int kFusedThreads = 512;  // 2 wavefronts per SIMD: 256 VGPRs each
+
+template <int NTH>
+__global__ __launch_bounds__(kFusedThreads) void k_dct_x_fused(FusedArgs a, int comps) {
+  constexpr int KSH = (NTH + 1) / 2;
+  constexpr int NF = NTH * KSH * 64;
+  extern __shared__ __align__(16) uint4 msh[];
+  {
+    const uint4 *gh = reinterpret_cast<const uint4 *>(a.mhi);
+    const uint4 *gl = reinterpret_cast<const uint4 *>(a.mlo);
+    for (int i = threadIdx.x; i < 4 * NF; i += blockDim.x) {
+      msh[i] = gh[i];
+      msh[4 * NF + i] = gl[i];
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
+  const int waves_per_block = blockDim.x >> 6;
+  const int N = a.n, H = N >> 1;
+  const long long tiles_per_comp = (a.R + 15) >> 4;
+  const long long total = tiles_per_comp * comps;
+  double cost = 0.0, raw = 0.0;
+  for (long long tile = (long long)blockIdx.x * waves_per_block + wave; tile < total;
+       tile += (long long)gridDim.x * waves_per_block) {
+    const int comp = (int)(tile / tiles_per_comp);
+    const uint32_t c = (uint32_t)((tile % tiles_per_comp) * 16) + (uint32_t)(lane & 15);
+    const bool colok = c < (uint32_t)a.R;
+    const float *in = a.in + comp * a.cin;
+    float *out = a.out + comp * a.cout;
+    const double gm = a.gamma[comp];
+    const float gs2 = (float)gm * a.sigma2;
+    // per-iteration (opaque) copies of the loop invariants: the row offsets
+    // x * R are recomputed in 32 bits each tile instead of being hoisted
+    // out of the loop as dozens of live 64-bit values
+    const int z0 = opaque_zero();
+    const uint4 *mh = msh + z0, *ml = msh + 4 * NF + z0;
+    const uint32_t Rz = (uint32_t)a.R + (uint32_t)z0;
+    // B operands of the forward: sums and differences of mirrored rows
+    bf16x8_t sh[KSH], sl[KSH], dh[KSH], dl[KSH];
+#pragma unroll
+    for (int s = 0; s < KSH; ++s) {
+      float sv[8], dv[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int x = 32 * s + 8 * g + j;
+        float u1 = 0.f, u2 = 0.f;
+        if (colok && x < H) {
+          u1 = ldf(in, (uint32_t)x * Rz + c);
+          u2 = ldf(in, (uint32_t)(N - 1 - x) * Rz + c);
+        }
+        sv[j] = u1 + u2;
+        dv[j] = u1 - u2;
+      }
+      split8(sv, sh[s], sl[s]);
+      split8(dv, dh[s], dl[s]);
+    }
+    // the prior eigenvalues of this lane's coefficients, loaded with the
+    // data so that the filter does not wait on memory tile by tile
+    float dve[NTH][4], dvo[NTH][4];
+#pragma unroll
+    for (int t = 0; t < NTH; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = 16 * t + 4 * g + r;
+        const bool ok = colok && m < H;
+        dve[t][r] = ok ? ldf(a.D, (uint32_t)(2 * m) * Rz + c) : 0.f;
+        dvo[t][r] = ok ? ldf(a.D, (uint32_t)(2 * m + 1) * Rz + c) : 0.f;
+      }
+    bf16x8_t peh[KSH], pel[KSH], poh[KSH], pol[KSH];
+    fused_forward<NTH, KSH, 0>(a, mh, ml, sh, sl, dve, peh, pel, lane, comp, c, colok, H, gm,
+                               gs2, cost, raw);
+    fused_forward<NTH, KSH, 1>(a, mh, ml, dh, dl, dvo, poh, pol, lane, comp, c, colok, H, gm,
+                               gs2, cost, raw);
+    // inverse: Ve and Vo per 16-row x tile, then the mirrored pair of rows
+#pragma unroll
+    for (int tp = 0; tp < NTH; ++tp) {
+      f32x4_t ae = {0.f, 0.f, 0.f, 0.f}, ao = {0.f, 0.f, 0.f, 0.f};
+      const int fe = 2 * NF + tp * KSH * 64 + lane, fo = 3 * NF + tp * KSH * 64 + lane;
+#pragma unroll
+      for (int s = 0; s < KSH; ++s) {
+        ae = mfma3(mh[fe + s * 64], ml[fe + s * 64], peh[s], pel[s], ae);
+        ao = mfma3(mh[fo + s * 64], ml[fo + s * 64], poh[s], pol[s], ao);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int x = 16 * tp + 4 * g + r;
+        if (colok && x < H) {
+          stf(out, (uint32_t)x * Rz + c, ae[r] + ao[r]);
+          stf(out, (uint32_t)(N - 1 - x) * Rz + c, ae[r] - ao[r]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  double *red = reinterpret_cast<double *>(msh);  // matrices no longer needed
+  const int t = threadIdx.x;
+  red[t] = cost;
+  red[kFusedThreads + t] = raw;
+  __syncthreads();
+  for (int st = kFusedThreads / 2; st > 0; st >>= 1) {
+    if (t < st) {
+      red[t] += red[t + st];
+      red[kFusedThreads + t] += red[kFusedThreads + t + st];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    a.cost_part[blockIdx.x] = red[0];
+    if (a.raw) a.raw_part[blockIdx.x] = red[kFusedThreads];
+  }
+}
+
 int g_num_cus = 0;
 
 // Deterministic sum of `count` partials (fixed assignment and tree order).
@@ -551,11 +790,59 @@ static double dct_entry(int n, int k, int j) {
   return s * std::cos(M_PI * (2.0 * j + 1.0) * k / (2.0 * n));
 }
 
+// The four H x H matrices of the fused x pass (N = n even, H = N / 2), in
+// the A-operand order of k_dct_x_fused: E[m][x] = C_N[2m][x] and
+// O[m][x] = C_N[2m+1][x] for the forward (row m = 16 t + lane%16, k = x =
+// 32 s + 8 (lane/16) + j), then E^T and O^T for the inverse (row x, k = m in
+// the permuted order 32 s + 4 (lane/16) + j, j < 4, or
+// 32 s + 16 + 4 (lane/16) + j - 4).  Entries outside H x H are zero.
+static void build_fused_x(int n, std::vector<uint32_t> &hi, std::vector<uint32_t> &lo) {
+  const int H = n / 2, nt = (H + 15) / 16, ks = (nt + 1) / 2;
+  const size_t nfrag = (size_t)nt * ks * 64;
+  hi.assign(4 * nfrag * 4, 0u);
+  lo.assign(4 * nfrag * 4, 0u);
+  auto E = [&](int m, int x) { return m < H && x < H ? dct_entry(n, 2 * m, x) : 0.0; };
+  auto O = [&](int m, int x) { return m < H && x < H ? dct_entry(n, 2 * m + 1, x) : 0.0; };
+  auto put = [&](int arr, int t, int s, int l, int j, double v) {
+    const float f = (float)v;
+    const uint16_t h = host_bf16(f);
+    const uint16_t r = host_bf16(f - host_bf16_to_float(h));
+    const size_t w = ((size_t)arr * nfrag + ((size_t)t * ks + s) * 64 + l) * 4 + j / 2;
+    hi[w] |= (uint32_t)h << (16 * (j & 1));
+    lo[w] |= (uint32_t)r << (16 * (j & 1));
+  };
+  for (int t = 0; t < nt; ++t)
+    for (int s = 0; s < ks; ++s)
+      for (int l = 0; l < 64; ++l)
+        for (int j = 0; j < 8; ++j) {
+          const int row = 16 * t + (l % 16), g = l / 16;
+          const int xf = 32 * s + 8 * g + j;
+          const int mp = 32 * s + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+          put(0, t, s, l, j, E(row, xf));
+          put(1, t, s, l, j, O(row, xf));
+          put(2, t, s, l, j, E(mp, row));
+          put(3, t, s, l, j, O(mp, row));
+        }
+}
+
 const PassMatrix &MatrixCache::get(int kind, int n_in, int n_out,
                                    hipStream_t stream) {
   const long long key = ((long long)kind << 40) | ((long long)n_in << 20) | n_out;
   auto it = cache_.find(key);
   if (it != cache_.end()) return *it->second;
+  if (kind == kDctFusedX) {
+    std::vector<uint32_t> hi, lo;
+    build_fused_x(n_in, hi, lo);
+    auto pm = std::make_unique<PassMatrix>();
+    pm->n_in = n_in;
+    pm->n_out = n_out;
+    pm->hi.upload(hi.data(), hi.size(), stream);
+    pm->lo.upload(lo.data(), lo.size(), stream);
+    HIP_OK(hipStreamSynchronize(stream));
+    auto &ref = *pm;
+    cache_[key] = std::move(pm);
+    return ref;
+  }
   std::vector<double> M((size_t)n_in * n_out);
   for (int k = 0; k < n_in; ++k)
     for (int c = 0; c < n_out; ++c) {
@@ -658,6 +945,65 @@ long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in
   return (long long)grid.x * grid.y;
 }
 
+bool fused_x_supported(int n) { return n >= 2 && n % 2 == 0 && n <= 192; }
+
+template <int NTH>
+void launch_fused(int blocks, size_t lds, int comps, hipStream_t s, const FusedArgs &a) {
+  auto k = k_dct_x_fused<NTH>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIP_OK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kPersistMaxLds));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(kFusedThreads), lds, s, a, comps);
+}
+
+long long fused_x_pass(hipStream_t stream, const PassMatrix &m, const float *in, float *out,
+                       int comps, long long cstride, long long R, const FilterArgs &f,
+                       KernelTimer *timer, const char *name) {
+  const int n = m.n_in;
+  BINDER_REQUIRE(fused_x_supported(n), "fused x pass needs an even length <= 192");
+  FusedArgs a{};
+  a.in = in;
+  a.out = out;
+  a.R = R;
+  a.cin = cstride;
+  a.cout = cstride;
+  a.n = n;
+  a.mhi = m.hi.get();
+  a.mlo = m.lo.get();
+  a.D = f.D;
+  a.gamma = f.gamma;
+  a.sigma2 = (float)f.sigma2;
+  for (int i = 0; i < 3; ++i) a.etarho[i] = f.etarho[i];
+  a.noise = f.noise;
+  a.raw = f.raw;
+  a.k0 = f.k0;
+  a.k1 = f.k1;
+  a.sweep = f.sweep;
+  a.stream = f.stream;
+  a.cost_part = f.cost_part;
+  a.raw_part = f.raw_part;
+  const int H = n / 2, nt = (H + 15) / 16, ks = (nt + 1) / 2;
+  const size_t mat = (size_t)8 * nt * ks * 64 * 16;
+  const size_t lds = mat > 16384 ? mat : 16384;  // the cost reduction reuses it
+  const long long tiles = ((R + 15) / 16) * comps;
+  long long blocks = (tiles + kFusedThreads / 64 - 1) / (kFusedThreads / 64);
+  if (blocks > persistent_blocks()) blocks = persistent_blocks();
+  TimedLaunch tl(timer, name, stream);
+  switch (nt) {
+    case 1: launch_fused<1>((int)blocks, lds, comps, stream, a); break;
+    case 2: launch_fused<2>((int)blocks, lds, comps, stream, a); break;
+    case 3: launch_fused<3>((int)blocks, lds, comps, stream, a); break;
+    case 4: launch_fused<4>((int)blocks, lds, comps, stream, a); break;
+    case 5: launch_fused<5>((int)blocks, lds, comps, stream, a); break;
+    default: launch_fused<6>((int)blocks, lds, comps, stream, a); break;
+  }
+  HIP_LAUNCH_OK();
+  return blocks;
+}
+
 int persistent_blocks() {
   if (g_num_cus == 0) {
     int dev = 0;
@@ -722,22 +1068,6 @@ double Smoother::flops() const {
 
 void Smoother::run(const float *dream, const float *x, float *out,
                    const SmoothOptions &opt, KernelTimer *timer) {
-  // axes contracted innermost first; each pass rotates the layout, so after
-  // one pass per axis the data is back in [X][Y][Z] order
-  int dims[3], na = 0;
-  const char *fwd_names[3], *inv_names[3];
-  if (nz_ > 1) {
-    dims[na] = nz_;
-    fwd_names[na] = "dct_z_fwd";
-    inv_names[na++] = "dct_z_inv";
-  }
-  dims[na] = ny_;
-  fwd_names[na] = "dct_y_fwd";
-  inv_names[na++] = "dct_y_inv";
-  dims[na] = nx_;
-  fwd_names[na] = "dct_x_fwd";
-  inv_names[na++] = "dct_x_inv";
-
   FilterArgs f{};
   f.D = D_.get();
   f.gamma = gamma_.get();
@@ -754,43 +1084,96 @@ void Smoother::run(const float *dream, const float *x, float *out,
   f.raw_part = part_raw_.get();
 
   float *bufs[2] = {bufA_.get(), bufB_.get()};
-  int bi = 0;
-  const float *src = dream;
   long long nparts = 0;  // partial-sum slots written by the filter pass
-  for (int i = 0; i < na; ++i) {
-    const int n = dims[i];
-    const PassMatrix &m = mats_.get(MatrixCache::kDctForward, n, n, stream_);
-    const bool last = (i == na - 1);
-    const long long written =
-        contract_pass(stream_, m, src, bufs[bi], N_, nv_, nv_, nv_ / n,
-                      i == 0 ? x : nullptr, nullptr, last ? EPI_FILTER : EPI_STORE,
-                      last ? &f : nullptr, timer, fwd_names[i]);
-    if (last) nparts = written;
-    src = bufs[bi];
-    bi ^= 1;
-  }
-  {
-    TimedLaunch tl(timer, "sum_partials", stream_);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, stream_,
-                       part_cost_.get(), nparts,
-                       opt.prior_out ? opt.prior_out : scal_.get());
-    HIP_LAUNCH_OK();
-  }
-  if (opt.sample && opt.sample_gamma) {
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, stream_,
-                       part_raw_.get(), nparts, scal_.get() + 1);
-    HIP_LAUNCH_OK();
-  }
-  for (int i = 0; i < na; ++i) {
-    const int n = dims[i];
-    const PassMatrix &m = mats_.get(MatrixCache::kDctInverse, n, n, stream_);
-    const bool last = (i == na - 1);
-    float *dst = last ? out : bufs[bi];
-    contract_pass(stream_, m, src, dst, N_, nv_, nv_, nv_ / n, nullptr,
-                  last ? x : nullptr, last ? EPI_ADDX : EPI_STORE, nullptr, timer,
-                  inv_names[i]);
-    src = dst;
-    bi ^= 1;
+  auto sum_partials = [&]() {
+    {
+      TimedLaunch tl(timer, "sum_partials", stream_);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, stream_,
+                         part_cost_.get(), nparts,
+                         opt.prior_out ? opt.prior_out : scal_.get());
+      HIP_LAUNCH_OK();
+    }
+    if (opt.sample && opt.sample_gamma) {
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, stream_,
+                         part_raw_.get(), nparts, scal_.get() + 1);
+      HIP_LAUNCH_OK();
+    }
+  };
+  using MC = MatrixCache;
+  if (fused_x_supported(nx_)) {
+    // Five passes (3D) or three (2D).  The y and z passes contract their
+    // contiguous axis and move it behind x ([X][Y][Z] -> [X][KZ][Y] ->
+    // [X][KY][KZ] and back), so x stays outermost for the fused x pass,
+    // which transforms, filters and transforms back in one read and write.
+    const PassMatrix &mx = mats_.get(MC::kDctFusedX, nx_, nx_, stream_);
+    const PassMatrix &fy = mats_.get(MC::kDctForward, ny_, ny_, stream_);
+    const PassMatrix &iy = mats_.get(MC::kDctInverse, ny_, ny_, stream_);
+    if (nz_ > 1) {
+      const PassMatrix &fz = mats_.get(MC::kDctForward, nz_, nz_, stream_);
+      const PassMatrix &iz = mats_.get(MC::kDctInverse, nz_, nz_, stream_);
+      contract_pass(stream_, fz, dream, bufs[0], N_, nv_, nv_, nv_ / nz_, x, nullptr,
+                    EPI_STORE, nullptr, timer, "dct_z_fwd", ny_);
+      contract_pass(stream_, fy, bufs[0], bufs[1], N_, nv_, nv_, nv_ / ny_, nullptr, nullptr,
+                    EPI_STORE, nullptr, timer, "dct_y_fwd", nz_);
+      nparts = fused_x_pass(stream_, mx, bufs[1], bufs[0], N_, nv_, (long long)ny_ * nz_, f,
+                            timer, "dct_x_fused");
+      sum_partials();
+      contract_pass(stream_, iz, bufs[0], bufs[1], N_, nv_, nv_, nv_ / nz_, nullptr, nullptr,
+                    EPI_STORE, nullptr, timer, "dct_z_inv", ny_);
+      contract_pass(stream_, iy, bufs[1], out, N_, nv_, nv_, nv_ / ny_, nullptr, x, EPI_ADDX,
+                    nullptr, timer, "dct_y_inv", nz_);
+    } else {
+      contract_pass(stream_, fy, dream, bufs[0], N_, nv_, nv_, nx_, x, nullptr, EPI_STORE,
+                    nullptr, timer, "dct_y_fwd", 1);
+      nparts = fused_x_pass(stream_, mx, bufs[0], bufs[1], N_, nv_, ny_, f, timer,
+                            "dct_x_fused");
+      sum_partials();
+      contract_pass(stream_, iy, bufs[1], out, N_, nv_, nv_, nx_, nullptr, x, EPI_ADDX,
+                    nullptr, timer, "dct_y_inv", 1);
+    }
+  } else {
+    // Six rotating passes: axes contracted innermost first, each pass
+    // rotating the layout, so after one pass per axis the data is back in
+    // [X][Y][Z] order; the filter rides on the last forward pass.
+    int dims[3], na = 0;
+    const char *fwd_names[3], *inv_names[3];
+    if (nz_ > 1) {
+      dims[na] = nz_;
+      fwd_names[na] = "dct_z_fwd";
+      inv_names[na++] = "dct_z_inv";
+    }
+    dims[na] = ny_;
+    fwd_names[na] = "dct_y_fwd";
+    inv_names[na++] = "dct_y_inv";
+    dims[na] = nx_;
+    fwd_names[na] = "dct_x_fwd";
+    inv_names[na++] = "dct_x_inv";
+    int bi = 0;
+    const float *src = dream;
+    for (int i = 0; i < na; ++i) {
+      const int n = dims[i];
+      const PassMatrix &m = mats_.get(MC::kDctForward, n, n, stream_);
+      const bool last = (i == na - 1);
+      const long long written =
+          contract_pass(stream_, m, src, bufs[bi], N_, nv_, nv_, nv_ / n,
+                        i == 0 ? x : nullptr, nullptr, last ? EPI_FILTER : EPI_STORE,
+                        last ? &f : nullptr, timer, fwd_names[i]);
+      if (last) nparts = written;
+      src = bufs[bi];
+      bi ^= 1;
+    }
+    sum_partials();
+    for (int i = 0; i < na; ++i) {
+      const int n = dims[i];
+      const PassMatrix &m = mats_.get(MC::kDctInverse, n, n, stream_);
+      const bool last = (i == na - 1);
+      float *dst = last ? out : bufs[bi];
+      contract_pass(stream_, m, src, dst, N_, nv_, nv_, nv_ / n, nullptr,
+                    last ? x : nullptr, last ? EPI_ADDX : EPI_STORE, nullptr, timer,
+                    inv_names[i]);
+      src = dst;
+      bi ^= 1;
+    }
   }
   if (opt.sample && opt.sample_gamma) {
     hipLaunchKernelGGL(k_draw_nu, dim3(1), dim3(64), 0, stream_,

@@ -22,7 +22,8 @@ struct PassMatrix {
 
 class MatrixCache {
  public:
-  enum Kind { kDctForward = 1, kDctInverse = 2 };
+  // kDctFusedX: the four half-size matrices of the fused x pass (n_in = n_out = N).
+  enum Kind { kDctForward = 1, kDctInverse = 2, kDctFusedX = 3 };
   // kDctForward: M[k][c] = C_{n}[c][k] (n = n_in = n_out);
   // kDctInverse: M[k][c] = C_{n_out}[k][c] (first n_in coefficients).
   const PassMatrix &get(int kind, int n_in, int n_out, hipStream_t stream);
@@ -51,6 +52,13 @@ long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in
                         long long R, const float *xpro, const float *xepi, int epi,
                         const FilterArgs *f, KernelTimer *timer, const char *name,
                         long long rin = 0);
+// x axis of a [comp][N][R] field in one pass: forward DCT-II, prior filter
+// with noise and cost (f), inverse DCT-III; N even, N <= 192 (m from
+// MatrixCache kDctFusedX).  Returns the partial-sum slots written.
+bool fused_x_supported(int n);
+long long fused_x_pass(hipStream_t stream, const PassMatrix &m, const float *in, float *out,
+                       int comps, long long cstride, long long R, const FilterArgs &f,
+                       KernelTimer *timer, const char *name);
 // Upper bound on the partial-sum slots a filter pass over R rows writes.
 long long contract_blocks(long long R, int comps);
 int persistent_blocks();
