@@ -155,7 +155,9 @@ class FastRegistration:
         self.full_voxels_pos = grid_v
         self.full_nodes_pos = grid_n
 
-        # voxel positions expressed in node space (FastRegistration.py:375-395)
+        # voxel positions expressed in node space (FastRegistration.py:375-395);
+        # without a transformation they stay the grid indices at every level
+        self._index_grid = initial_transformation is None and np.allclose(nm, vm)
         if initial_transformation is not None:
             self.full_voxels_pos = _apply_affine(initial_transformation, self.full_voxels_pos)
         elif not np.allclose(nm, vm):
@@ -214,8 +216,10 @@ class FastRegistration:
             t0 = time.perf_counter()
             if on_gpu:
                 vp = np.ascontiguousarray(self.voxels_pos)
+                # full[::res] / res of the index grid is the index grid again
                 self.registratorObj = Registration.Registration.resampled_from(
-                    prev, cur / res, vp.ravel(), vp.shape[0], vp.shape[1], vp.shape[2],
+                    prev, cur / res, None if self._index_grid else vp.ravel(),
+                    vp.shape[0], vp.shape[1], vp.shape[2],
                     self.nodes_shape[0], self.nodes_shape[1], self.nodes_shape[2],
                     self.spline_order, self.non_zero_voxels)
                 self._set_nonlinear(self.registratorObj)

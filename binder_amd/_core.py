@@ -164,15 +164,19 @@ class Registration:
         """A Registration on a new grid whose field is `src`'s current field
         resized on the GPU (the pyramid's resize, times `scale`) -- what
         Registration(resize(src.get_final_locations_as_numpy_array()), ...)
-        computes, without the host round trip."""
+        computes, without the host round trip.  voxel_pos=None stands for
+        the grid indices."""
         self = cls.__new__(cls)
         self._h = None
-        vp = _arr(voxel_pos, np.float32)
         N = 2 if int(Nz_v) == 1 else 3
-        if vp.size != int(Nx_v) * int(Ny_v) * int(Nz_v) * N:
-            raise ValueError("voxel_pos has the wrong size")
+        vp = None
+        if voxel_pos is not None:  # None: the grid indices, generated on the GPU
+            vp = _arr(voxel_pos, np.float32)
+            if vp.size != int(Nx_v) * int(Ny_v) * int(Nz_v) * N:
+                raise ValueError("voxel_pos has the wrong size")
         h = _vp()
-        _check(_reg_create_resampled(ctypes.byref(h), src._h, float(scale), _ptr(vp),
+        _check(_reg_create_resampled(ctypes.byref(h), src._h, float(scale),
+                                     None if vp is None else _ptr(vp),
                                      int(Nx_v), int(Ny_v), int(Nz_v), int(Nx_n), int(Ny_n),
                                      int(Nz_n), int(spline_order), float(non_zero_voxels)))
         self._h = h

@@ -38,6 +38,60 @@ void soa_to_aos(const T *in, U *out, long long nv, int N) {
     for (int c = 0; c < N; ++c) out[i * N + c] = (U)in[c * nv + i];
 }
 
+// Device-side layout changes: [nv][N] interleaved <-> [N][nv] planes, and
+// the index grid (i, j[, k]) as planes.
+__global__ __launch_bounds__(256) void k_interleaved_to_planes(const float *__restrict__ in,
+                                                               float *__restrict__ out,
+                                                               long long nv, int N) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride)
+    for (int c = 0; c < N; ++c) out[c * nv + i] = in[i * N + c];
+}
+__global__ __launch_bounds__(256) void k_planes_to_interleaved(const float *__restrict__ in,
+                                                               float *__restrict__ out,
+                                                               long long nv, int N) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride)
+    for (int c = 0; c < N; ++c) out[i * N + c] = in[c * nv + i];
+}
+__global__ __launch_bounds__(256) void k_index_grid(float *__restrict__ out, int nx, int ny,
+                                                    int nz, int N) {
+  const long long nv = (long long)nx * ny * nz;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
+    const long long yz = (long long)ny * nz;
+    out[i] = (float)(i / yz);
+    out[nv + i] = (float)((i / nz) % ny);
+    if (N == 3) out[2 * nv + i] = (float)(i % nz);
+  }
+}
+
+int grid_blocks(long long n) {
+  const long long b = (n + 255) / 256;
+  return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+}
+
+// Host [nv][N] floats -> device planes (staged through `tmp`, nv * N floats).
+void upload_planes(const float *host, DBuf<float> &dst, DBuf<float> &tmp, long long nv, int N,
+                   hipStream_t s) {
+  if (!tmp.get() || tmp.n < (size_t)(nv * N)) tmp.alloc((size_t)(nv * N));
+  HIP_OK(hipMemcpyAsync(tmp.get(), host, sizeof(float) * nv * N, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_interleaved_to_planes, dim3(grid_blocks(nv)), dim3(256), 0, s,
+                     tmp.get(), dst.get(), nv, N);
+  HIP_LAUNCH_OK();
+}
+
+// Device planes -> host [nv][N] floats (staged through `tmp`); synchronises.
+void download_planes(const float *src, float *host, DBuf<float> &tmp, long long nv, int N,
+                     hipStream_t s) {
+  if (!tmp.get() || tmp.n < (size_t)(nv * N)) tmp.alloc((size_t)(nv * N));
+  hipLaunchKernelGGL(k_planes_to_interleaved, dim3(grid_blocks(nv)), dim3(256), 0, s, src,
+                     tmp.get(), nv, N);
+  HIP_LAUNCH_OK();
+  HIP_OK(hipMemcpyAsync(host, tmp.get(), sizeof(float) * nv * N, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+}
+
 struct Stream {
   hipStream_t s = nullptr;
   Stream() { HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
@@ -73,6 +127,7 @@ struct binder_registration {
   // kWelfordBatch samples stay resident and the FP64 accumulators are
   // updated once per batch.
   DBuf<float> d, dextra, x, dream;
+  DBuf<float> stage;  // [nv][N] staging for host transfers
   int dslot = 0;
   float *slot(int k) {
     return k == 0 ? d.get() : dextra.get() + (size_t)(k - 1) * N * nv;
@@ -206,11 +261,11 @@ namespace {
 
 // Shape checks and buffers shared by the two constructors; the caller fills
 // the current field r->d.
+// voxel_pos == nullptr: the voxel positions are the grid indices (i, j[, k]).
 std::unique_ptr<binder_registration> new_registration(const float *voxel_pos, int Nx_v,
                                                       int Ny_v, int Nz_v, int Nx_n, int Ny_n,
                                                       int Nz_n, int spline_order,
                                                       double non_zero_voxels) {
-  BINDER_REQUIRE(voxel_pos, "null input array");
   BINDER_REQUIRE(Nx_v >= 2 && Ny_v >= 2 && Nz_v >= 1, "bad fixed-grid shape");
   BINDER_REQUIRE(Nx_n >= 1 && Ny_n >= 1 && Nz_n >= 1, "bad moving-grid shape");
   BINDER_REQUIRE(spline_order == 3,
@@ -232,11 +287,17 @@ std::unique_ptr<binder_registration> new_registration(const float *voxel_pos, in
   BINDER_REQUIRE(r->nv * 3 < (1ll << 30), "fixed grid too large (N * Nv must be below 2^30)");
   r->nn = (long long)Nx_n * Ny_n * Nz_n;
   r->nnz = non_zero_voxels;
-  std::vector<float> tmp((size_t)r->N * r->nv);
-  aos_to_soa(voxel_pos, tmp.data(), r->nv, r->N);
-  r->x.upload(tmp.data(), tmp.size(), r->s());
-  r->d.alloc(tmp.size());
-  r->dream.alloc(tmp.size());
+  const size_t nf = (size_t)r->N * r->nv;
+  r->x.alloc(nf);
+  if (voxel_pos) {
+    upload_planes(voxel_pos, r->x, r->stage, r->nv, r->N, r->s());
+  } else {
+    hipLaunchKernelGGL(k_index_grid, dim3(grid_blocks(r->nv)), dim3(256), 0, r->s(),
+                       r->x.get(), Nx_v, Ny_v, Nz_v, r->N);
+    HIP_LAUNCH_OK();
+  }
+  r->d.alloc(nf);
+  r->dream.alloc(nf);
   r->dream.zero(r->s());
   r->stats.alloc(16);
   r->stats.zero(r->s());
@@ -265,12 +326,10 @@ int binder_registration_create(binder_registration **out,
   return guarded([&] {
     BINDER_REQUIRE(out != nullptr, "out handle pointer is null");
     *out = nullptr;
-    BINDER_REQUIRE(final_locations, "null input array");
+    BINDER_REQUIRE(final_locations && voxel_pos, "null input array");
     auto r = new_registration(voxel_pos, Nx_v, Ny_v, Nz_v, Nx_n, Ny_n, Nz_n, spline_order,
                               non_zero_voxels);
-    std::vector<float> tmp((size_t)r->N * r->nv);
-    aos_to_soa(final_locations, tmp.data(), r->nv, r->N);
-    r->d.upload(tmp.data(), tmp.size(), r->s());
+    upload_planes(final_locations, r->d, r->stage, r->nv, r->N, r->s());
     HIP_OK(hipStreamSynchronize(r->s()));
     *out = r.release();
   });
@@ -318,9 +377,6 @@ int binder_registration_set_mi_likelihood(binder_registration *r, double alpha,
       BINDER_REQUIRE(b >= 0 && b < L, "binned_voxels value outside [0, L)");
       v8[i] = (uint8_t)b;
     }
-    for (long long i = 0; i < r->nn; ++i)
-      BINDER_REQUIRE(binned_nodes[i] >= 0 && binned_nodes[i] < K,
-                     "binned_nodes value outside [0, K)");
     r->K = K;
     r->L = L;
     r->alpha = alpha;
@@ -331,28 +387,33 @@ int binder_registration_set_mi_likelihood(binder_registration *r, double alpha,
     const long long slab = r->is2d ? (long long)Yp : (long long)Yp * (r->nzn + 5);
     BINDER_REQUIRE(slab < (1ll << 24) && (long long)Xp * slab < (1ll << 30),
                    "moving grid too large for the E-step's 32-bit indexing");
+    auto node_bin = [K](int b) {
+      BINDER_REQUIRE(b >= 0 && b < K, "binned_nodes value outside [0, K)");
+      return (uint8_t)b;
+    };
     if (r->is2d) {
       std::vector<uint8_t> p((size_t)Xp * Yp, 0);
       for (int x = 0; x < r->nxn; ++x)
         for (int y = 0; y < r->nyn; ++y)
-          p[(size_t)(x + 1) * Yp + (y + 1)] = (uint8_t)binned_nodes[(long long)x * r->nyn + y];
+          p[(size_t)(x + 1) * Yp + (y + 1)] = node_bin(binned_nodes[(long long)x * r->nyn + y]);
       r->bins2d.upload(p.data(), p.size(), r->s());
       HIP_OK(hipStreamSynchronize(r->s()));
     } else {
+      // word zq of a column packs the bins of z = zq - 4 .. zq - 1 (byte c =
+      // z - zq + 4), zero outside the grid: four consecutive bytes of the
+      // column's bins behind four zeros
       const int Zq = r->nzn + 5;
       std::vector<uint32_t> q((size_t)Xp * Yp * Zq, 0u);
+      std::vector<uint8_t> pad((size_t)r->nzn + 8, 0);
       for (int x = 0; x < r->nxn; ++x)
         for (int y = 0; y < r->nyn; ++y) {
           const int *row = binned_nodes + ((long long)x * r->nyn + y) * r->nzn;
+          for (int z = 0; z < r->nzn; ++z) pad[z + 4] = node_bin(row[z]);
           uint32_t *qr = &q[((size_t)(x + 1) * Yp + (y + 1)) * Zq];
           for (int zq = 0; zq < Zq; ++zq) {
-            uint32_t w = 0;
-            for (int c = 0; c < 4; ++c) {
-              const int z = zq - 4 + c;
-              const uint32_t b = (z >= 0 && z < r->nzn) ? (uint32_t)row[z] : 0u;
-              w |= b << (8 * c);
-            }
-            qr[zq] = w;
+            const uint8_t *b = &pad[zq];
+            qr[zq] = (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) |
+                     ((uint32_t)b[3] << 24);
           }
         }
       r->quads.upload(q.data(), q.size(), r->s());
@@ -466,7 +527,7 @@ int binder_registration_sampler_run(binder_registration *r, int seed, int N_b,
         o.key1 = k1;
         o.sweep = (uint32_t)sweep;
         o.stream = kStreamDeform;
-        o.prior_out = r->stats.get() + 3;
+        o.want_prior = false;  // the sampler reports no objective
         const int next = (r->dslot + 1) % kWelfordBatch;
         r->sm->run(r->dream.get(), r->x.get(), r->slot(next), o, r->tm());
         r->dslot = next;
@@ -627,11 +688,7 @@ int binder_registration_get_final_locations(binder_registration *r,
                                             float *out) {
   return guarded([&] {
     BINDER_REQUIRE(r && out, "null argument");
-    std::vector<float> h((size_t)r->N * r->nv);
-    HIP_OK(hipMemcpyAsync(h.data(), r->dcur(), sizeof(float) * h.size(),
-                          hipMemcpyDeviceToHost, r->s()));
-    HIP_OK(hipStreamSynchronize(r->s()));
-    soa_to_aos(h.data(), out, r->nv, r->N);
+    download_planes(r->dcur(), out, r->stage, r->nv, r->N, r->s());
   });
 }
 
@@ -639,10 +696,7 @@ int binder_registration_get_dream_locations(binder_registration *r,
                                             float *out) {
   return guarded([&] {
     BINDER_REQUIRE(r && out, "null argument");
-    std::vector<float> h((size_t)r->N * r->nv);
-    r->dream.download(h.data(), h.size(), r->s());
-    HIP_OK(hipStreamSynchronize(r->s()));
-    soa_to_aos(h.data(), out, r->nv, r->N);
+    download_planes(r->dream.get(), out, r->stage, r->nv, r->N, r->s());
   });
 }
 
@@ -650,9 +704,7 @@ int binder_registration_set_dream_locations(binder_registration *r,
                                             const float *in) {
   return guarded([&] {
     BINDER_REQUIRE(r && in, "null argument");
-    std::vector<float> h((size_t)r->N * r->nv);
-    aos_to_soa(in, h.data(), r->nv, r->N);
-    r->dream.upload(h.data(), h.size(), r->s());
+    upload_planes(in, r->dream, r->stage, r->nv, r->N, r->s());
     HIP_OK(hipStreamSynchronize(r->s()));
   });
 }

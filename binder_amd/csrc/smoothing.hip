@@ -1086,7 +1086,7 @@ void Smoother::run(const float *dream, const float *x, float *out,
   float *bufs[2] = {bufA_.get(), bufB_.get()};
   long long nparts = 0;  // partial-sum slots written by the filter pass
   auto sum_partials = [&]() {
-    {
+    if (opt.want_prior) {
       TimedLaunch tl(timer, "sum_partials", stream_);
       hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, stream_,
                          part_cost_.get(), nparts,

@@ -74,6 +74,7 @@ struct SmoothOptions {
   uint32_t sweep = 0;          // Philox counter word
   uint32_t stream = 0;         // Philox stream word
   double *prior_out = nullptr; // device double: receives sum_n 0.5 g_n D p^2
+  bool want_prior = true;      // false: skip the reduction of the prior cost
 };
 
 class Smoother {

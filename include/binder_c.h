@@ -74,7 +74,9 @@ int binder_registration_create(binder_registration **out,
  * resize that BINDER/FastRegistration.py:566-568 applies between pyramid
  * levels followed by the per-level constructor (:443-450), without the
  * field's round trip through host memory.  No reference counterpart; other
- * arguments as binder_registration_create.  `src` stays valid. */
+ * arguments as binder_registration_create, except that voxel_pos may be NULL
+ * for voxel positions equal to the grid indices (i, j[, k]), which are then
+ * generated on the device.  `src` stays valid. */
 int binder_registration_create_resampled(binder_registration **out,
                                          binder_registration *src, double scale,
                                          const float *voxel_pos, int Nx_v, int Ny_v,

@@ -159,3 +159,25 @@ def test_pyramid_device_chain_matches_host_chain(core, monkeypatch):
           flush=True)
     assert gi == hi
     assert rms < 0.01
+
+
+def test_resampled_registration_index_grid(core):
+    """voxel_pos=None (the index grid, generated on the GPU) gives the same
+    EM trajectory as passing the index grid from the host."""
+    from binder_amd import synth
+    print("resampled registration with a generated index grid", flush=True)
+    pair, prob = synth.config_problem("c2", seed=4, scale=0.2)
+    Nx, Ny, Nz = prob.shape
+    src = synth.build_registration(core, prob)
+    a = core.Registration.resampled_from(src, 1.0, prob.voxel_pos.ravel(), Nx, Ny, Nz,
+                                         Nx, Ny, Nz, 3, prob.nnz)
+    b = core.Registration.resampled_from(src, 1.0, None, Nx, Ny, Nz, Nx, Ny, Nz, 3, prob.nnz)
+    for r in (a, b):
+        r.setNonLinearTransformation(*prob.gammas, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, prob.D.ravel())
+        r.setMILikelihood(prob.alpha, prob.theta.ravel(), prob.K, prob.L, 1,
+                          prob.binned_nodes.ravel(), prob.binned_voxels.ravel())
+        r.EM(3, -1e30, 1, 0)
+    assert np.array_equal(a.get_final_locations_as_numpy_array(),
+                          b.get_final_locations_as_numpy_array())
+    assert np.array_equal(src.get_final_locations_as_numpy_array(),
+                          prob.final_locations.astype(np.float32))
