@@ -23,7 +23,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("3D reg time-to-tolerance (warm-start vs plain) + Gibbs voxel-updates/s, 1 GPU")
 HBM_PEAK_GBS = 8000.0
-WELFORD_BATCH = 8  # samples folded per accumulator pass (kWelfordBatch)
+WELFORD_BATCH = 16  # samples folded per accumulator pass (kWelfordBatch)
 FP32_PEAK_TFLOPS = 157.3
 
 # HBM bytes per launch at the 160x192x160 workload, from two rocprofv3 PMC

@@ -69,7 +69,7 @@ void launch_theta(const ThetaArgs &a, hipStream_t s);
 // samples having been folded in before: per voxel the accumulators are
 // loaded once, updated sample by sample exactly as one update per sweep
 // would, and stored once.
-constexpr int kWelfordBatch = 8;
+constexpr int kWelfordBatch = 16;
 struct WelfordBatch {
   const float *x[kWelfordBatch];
   int count;

@@ -66,6 +66,21 @@ __global__ __launch_bounds__(256) void k_index_grid(float *__restrict__ out, int
   }
 }
 
+// flag stays 1 if planes == the index grid (i, j[, k]) exactly
+__global__ __launch_bounds__(256) void k_check_index_grid(const float *__restrict__ x, int nx,
+                                                          int ny, int nz, int N,
+                                                          int *__restrict__ flag) {
+  const long long nv = (long long)nx * ny * nz;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  bool ok = true;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
+    const long long yz = (long long)ny * nz;
+    ok = ok && x[i] == (float)(i / yz) && x[nv + i] == (float)((i / nz) % ny);
+    if (N == 3) ok = ok && x[2 * nv + i] == (float)(i % nz);
+  }
+  if (!ok) *flag = 0;
+}
+
 int grid_blocks(long long n) {
   const long long b = (n + 255) / 256;
   return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
@@ -128,6 +143,7 @@ struct binder_registration {
   // updated once per batch.
   DBuf<float> d, dextra, x, dream;
   DBuf<float> stage;  // [nv][N] staging for host transfers
+  bool x_index_grid = false;  // x holds the grid indices exactly
   int dslot = 0;
   float *slot(int k) {
     return k == 0 ? d.get() : dextra.get() + (size_t)(k - 1) * N * nv;
@@ -291,10 +307,22 @@ std::unique_ptr<binder_registration> new_registration(const float *voxel_pos, in
   r->x.alloc(nf);
   if (voxel_pos) {
     upload_planes(voxel_pos, r->x, r->stage, r->nv, r->N, r->s());
+    // the smoothing passes generate x when it is the index grid
+    DBuf<int> flag;
+    const int one = 1;
+    flag.upload(&one, 1, r->s());
+    hipLaunchKernelGGL(k_check_index_grid, dim3(grid_blocks(r->nv)), dim3(256), 0, r->s(),
+                       r->x.get(), Nx_v, Ny_v, Nz_v, r->N, flag.get());
+    HIP_LAUNCH_OK();
+    int got = 0;
+    flag.download(&got, 1, r->s());
+    HIP_OK(hipStreamSynchronize(r->s()));
+    r->x_index_grid = got == 1;
   } else {
     hipLaunchKernelGGL(k_index_grid, dim3(grid_blocks(r->nv)), dim3(256), 0, r->s(),
                        r->x.get(), Nx_v, Ny_v, Nz_v, r->N);
     HIP_LAUNCH_OK();
+    r->x_index_grid = true;
   }
   r->d.alloc(nf);
   r->dream.alloc(nf);
@@ -446,6 +474,7 @@ int binder_registration_set_nonlinear_transformation(
     const double eta[3] = {nodes_size_x, nodes_size_y, nodes_size_z};
     r->sm = std::make_unique<Smoother>(r->nxv, r->nyv, r->nzv, g, rho, eta, D,
                                        std::sqrt(9.0 / (8.0 * M_PI)), r->s());
+    r->sm->set_index_grid(r->x_index_grid);
   });
 }
 

@@ -34,13 +34,20 @@ typedef __attribute__((ext_vector_type(4))) unsigned int u32x4_t;
 constexpr int kRowsPerBlock = 64;  // 4 wavefronts x 16 rows
 constexpr int kMaxKSteps = 8;      // contraction length <= 256
 
-enum : int { EPI_STORE = 0, EPI_FILTER = 1, EPI_ADDX = 2 };
+enum : int { EPI_STORE = 0, EPI_FILTER = 1, EPI_ADDX = 2, EPI_ADDGRID = 3 };
+// prologue: 0 none, 1 subtract xpro, 2 subtract the generated index grid
+enum : int { PRO_NONE = 0, PRO_X = 1, PRO_GRID = 2 };
 
 struct PassArgs {
   const float *in;       // [comp][R][n_in]
   float *out;            // [comp][n_out][R]
   const float *xpro;     // prologue: in - xpro (same layout as in), may be null
   const float *xepi;     // ADDX epilogue: + xepi (same layout as out)
+  // index grid in place of xpro / xepi (PRO_GRID, EPI_ADDGRID): component
+  // `comp` of the voxel at (row, k) is k when comp == xg_axis, else
+  // (row / xg_div[comp]) (mod xg_mod[comp] unless 0)
+  int xg_axis;
+  int xg_div[3], xg_mod[3];
   const uint32_t *bhi;   // swizzled matrix, bf16 hi / lo
   const uint32_t *blo;
   long long R;
@@ -80,6 +87,13 @@ __device__ __forceinline__ void split8(const float v[8], bf16x8_t &hi,
   lo = __builtin_bit_cast(bf16x8_t, l);
 }
 
+// Index-grid coordinate of component `comp` at `row` (comp != xg_axis).
+__device__ __forceinline__ float grid_coord(const PassArgs &a, int comp, long long row) {
+  uint32_t v = (uint32_t)row / (uint32_t)a.xg_div[comp];
+  if (a.xg_mod[comp]) v %= (uint32_t)a.xg_mod[comp];
+  return (float)v;
+}
+
 // Offset of output element (k' = kc, row) in one component (see rin).
 __device__ __forceinline__ long long out_offset(const PassArgs &a, int kc, long long row) {
   if (a.rin == 0) return (long long)kc * a.R + row;
@@ -106,13 +120,13 @@ __device__ __forceinline__ void normals4(u32x4 r, float n[4]) {
 
 // KS = number of 32-wide k-steps (compile time, so every fragment loop is
 // straight-line code and the B loads of a whole output tile issue together).
-template <int KS, bool VEC, bool PRO, int EPI>
+template <int KS, bool VEC, int PRO, int EPI>
 __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int comp = blockIdx.y;
   const long long row0 = (long long)blockIdx.x * kRowsPerBlock + wave * 16;
   const float *in = a.in + comp * a.cin;
-  const float *xp = PRO ? a.xpro + comp * a.cin : nullptr;
+  const float *xp = PRO == PRO_X ? a.xpro + comp * a.cin : nullptr;
   float *out = a.out + comp * a.cout;
 
   // A fragments: this lane's row and 8 consecutive k per 32-wide k-step
@@ -129,11 +143,16 @@ __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
       const float4 x0 = p[0], x1 = p[1];
       v[0] = x0.x; v[1] = x0.y; v[2] = x0.z; v[3] = x0.w;
       v[4] = x1.x; v[5] = x1.y; v[6] = x1.z; v[7] = x1.w;
-      if (PRO) {
+      if (PRO == PRO_X) {
         const float4 *q = reinterpret_cast<const float4 *>(xp + arow * a.n_in + k0);
         const float4 y0 = q[0], y1 = q[1];
         v[0] -= y0.x; v[1] -= y0.y; v[2] -= y0.z; v[3] -= y0.w;
         v[4] -= y1.x; v[5] -= y1.y; v[6] -= y1.z; v[7] -= y1.w;
+      } else if (PRO == PRO_GRID) {
+        const bool along = comp == a.xg_axis;
+        const float rc = along ? 0.f : grid_coord(a, comp, arow);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] -= along ? (float)(k0 + j) : rc;
       }
     } else {
 #pragma unroll
@@ -142,7 +161,9 @@ __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
         float t = 0.f;
         if (rowok && k < a.n_in) {
           t = in[arow * a.n_in + k];
-          if (PRO) t -= xp[arow * a.n_in + k];
+          if (PRO == PRO_X) t -= xp[arow * a.n_in + k];
+          if (PRO == PRO_GRID)
+            t -= comp == a.xg_axis ? (float)k : grid_coord(a, comp, arow);
         }
         v[j] = t;
       }
@@ -159,6 +180,11 @@ __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
   double cost = 0.0, raw = 0.0;
   const int nt_count = (a.n_out + 15) >> 4;
   const long long orow = row0 + 4 * (lane >> 4);  // first of this lane's 4 rows
+  float grc[4] = {0.f, 0.f, 0.f, 0.f};  // EPI_ADDGRID: this lane's rows' coordinate
+  if (EPI == EPI_ADDGRID && comp != a.xg_axis) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) grc[i] = grid_coord(a, comp, orow + i);
+  }
   const uint4 *bh = reinterpret_cast<const uint4 *>(a.bhi) + lane;
   const uint4 *bl = reinterpret_cast<const uint4 *>(a.blo) + lane;
   // B fragments of the current output tile; the next tile's are prefetched
@@ -239,6 +265,10 @@ __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) v[i] += ep[i];
       }
+    } else if (EPI == EPI_ADDGRID) {
+      const bool along = comp == a.xg_axis;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] += along ? (float)kc : grc[i];
     }
     if (VEC && orow + 3 < a.R) {
       *reinterpret_cast<float4 *>(out + obase) = make_float4(v[0], v[1], v[2], v[3]);
@@ -277,7 +307,7 @@ __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
 constexpr int kPersistThreads = 1024;
 constexpr size_t kPersistMaxLds = 144 * 1024;
 
-template <int KS, bool VEC, bool PRO, int EPI>
+template <int KS, bool VEC, int PRO, int EPI>
 __global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
                                                                  int comps) {
   extern __shared__ __align__(16) uint4 bsh[];  // [nt][KS][64] hi, then lo
@@ -303,7 +333,7 @@ __global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
     const int comp = (int)(tile / tiles_per_comp);
     const long long row0 = (tile % tiles_per_comp) * 16;
     const float *in = a.in + comp * a.cin;
-    const float *xp = PRO ? a.xpro + comp * a.cin : nullptr;
+    const float *xp = PRO == PRO_X ? a.xpro + comp * a.cin : nullptr;
     float *out = a.out + comp * a.cout;
     const long long arow = row0 + (lane & 15);
     const bool rowok = arow < a.R;
@@ -317,11 +347,16 @@ __global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
         const float4 x0 = p[0], x1 = p[1];
         v[0] = x0.x; v[1] = x0.y; v[2] = x0.z; v[3] = x0.w;
         v[4] = x1.x; v[5] = x1.y; v[6] = x1.z; v[7] = x1.w;
-        if (PRO) {
+        if (PRO == PRO_X) {
           const float4 *q = reinterpret_cast<const float4 *>(xp + arow * a.n_in + k0);
           const float4 y0 = q[0], y1 = q[1];
           v[0] -= y0.x; v[1] -= y0.y; v[2] -= y0.z; v[3] -= y0.w;
           v[4] -= y1.x; v[5] -= y1.y; v[6] -= y1.z; v[7] -= y1.w;
+        } else if (PRO == PRO_GRID) {
+          const bool along = comp == a.xg_axis;
+          const float rc = along ? 0.f : grid_coord(a, comp, arow);
+#pragma unroll
+          for (int j = 0; j < 8; ++j) v[j] -= along ? (float)(k0 + j) : rc;
         }
       } else {
 #pragma unroll
@@ -330,7 +365,9 @@ __global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
           float t = 0.f;
           if (rowok && k < a.n_in) {
             t = in[arow * a.n_in + k];
-            if (PRO) t -= xp[arow * a.n_in + k];
+            if (PRO == PRO_X) t -= xp[arow * a.n_in + k];
+            if (PRO == PRO_GRID)
+              t -= comp == a.xg_axis ? (float)k : grid_coord(a, comp, arow);
           }
           v[j] = t;
         }
@@ -344,6 +381,11 @@ __global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
       gs2 = (float)g * a.sigma2;
     }
     const long long orow = row0 + 4 * (lane >> 4);
+    float grc[4] = {0.f, 0.f, 0.f, 0.f};  // EPI_ADDGRID: this lane's rows' coordinate
+    if (EPI == EPI_ADDGRID && comp != a.xg_axis) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) grc[i] = grid_coord(a, comp, orow + i);
+    }
     for (int nt = 0; nt < nt_count; ++nt) {
       const int kc = nt * 16 + (lane & 15);
       const long long obase = out_offset(a, kc, orow);
@@ -400,6 +442,10 @@ __global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
 #pragma unroll
           for (int i = 0; i < 4; ++i) v[i] += ep[i];
         }
+      } else if (EPI == EPI_ADDGRID) {
+        const bool along = comp == a.xg_axis;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] += along ? (float)kc : grc[i];
       }
       if (VEC && orow + 3 < a.R) {
         *reinterpret_cast<float4 *>(out + obase) = make_float4(v[0], v[1], v[2], v[3]);
@@ -704,12 +750,12 @@ __global__ void k_draw_nu(const double *raw, double *gamma, int N, double nv,
   gamma[2] = nu * rho_prod * e2 * e2 * nv;
 }
 
-template <int KS, bool VEC, bool PRO, int EPI>
+template <int KS, bool VEC, int PRO, int EPI>
 void launch_ks(dim3 grid, hipStream_t s, const PassArgs &a) {
   hipLaunchKernelGGL((k_contract<KS, VEC, PRO, EPI>), grid, dim3(256), 0, s, a);
 }
 
-template <int KS, bool VEC, bool PRO, int EPI>
+template <int KS, bool VEC, int PRO, int EPI>
 void launch_persist(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs &a) {
   auto k = k_contract_lds<KS, VEC, PRO, EPI>;
   static bool attr_set = false;
@@ -722,7 +768,7 @@ void launch_persist(int blocks, size_t lds, int comps, hipStream_t s, const Pass
   hipLaunchKernelGGL(k, dim3(blocks), dim3(kPersistThreads), lds, s, a, comps);
 }
 
-template <bool VEC, bool PRO, int EPI>
+template <bool VEC, int PRO, int EPI>
 void launch_lds(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs &a) {
   switch (a.ks) {
     case 1: launch_persist<1, VEC, PRO, EPI>(blocks, lds, comps, s, a); break;
@@ -734,7 +780,7 @@ void launch_lds(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs
   }
 }
 
-template <bool VEC, bool PRO, int EPI>
+template <bool VEC, int PRO, int EPI>
 void launch(dim3 grid, hipStream_t s, const PassArgs &a) {
   switch (a.ks) {
     case 1: launch_ks<1, VEC, PRO, EPI>(grid, s, a); break;
@@ -871,7 +917,7 @@ long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in
                         float *out, int comps, long long cin, long long cout,
                         long long R, const float *xpro, const float *xepi, int epi,
                         const FilterArgs *f, KernelTimer *timer, const char *name,
-                        long long rin) {
+                        long long rin, const GridCoords *xg) {
   BINDER_REQUIRE(m.n_in <= 32 * kMaxKSteps, "contraction length above 256");
   PassArgs a{};
   a.in = in;
@@ -905,44 +951,49 @@ long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in
                    (cout % 4 == 0) && (rin % 4 == 0);
   dim3 grid((unsigned)((R + kRowsPerBlock - 1) / kRowsPerBlock), comps);
   TimedLaunch tl(timer, name, stream);
-  const bool pro = xpro != nullptr;
+  int pro = xpro != nullptr ? PRO_X : PRO_NONE;
+  if (xg) {
+    a.xg_axis = xg->axis;
+    for (int i = 0; i < 3; ++i) {
+      a.xg_div[i] = xg->div[i] > 0 ? xg->div[i] : 1;
+      a.xg_mod[i] = xg->mod[i];
+    }
+    if (pro == PRO_X) pro = PRO_GRID;
+    if (epi == EPI_ADDX) epi = EPI_ADDGRID;
+  }
+  BINDER_REQUIRE(pro == PRO_NONE || epi == EPI_STORE || epi == EPI_FILTER,
+                 "a pass either subtracts x in the prologue or adds it in the epilogue");
   const size_t mat_lds = (size_t)((m.n_out + 15) / 16) * a.ks * 64 * 16 * 2;
   // the filter epilogue reuses the matrix area for a 2 x 1024 double reduction
   const size_t lds = mat_lds > 16384 ? mat_lds : 16384;
   const long long tiles = ((R + 15) / 16) * comps;
-  if (a.ks <= 6 && mat_lds <= kPersistMaxLds && tiles >= 2048) {
-    const int blocks = persistent_blocks();
-#define BINDER_LAUNCH_LDS(V)                                                         \
-  if (pro) {                                                                        \
-    if (epi == EPI_FILTER) launch_lds<V, true, EPI_FILTER>(blocks, lds, comps, stream, a); \
-    else launch_lds<V, true, EPI_STORE>(blocks, lds, comps, stream, a);             \
-  } else if (epi == EPI_FILTER) launch_lds<V, false, EPI_FILTER>(blocks, lds, comps, stream, a); \
-  else if (epi == EPI_ADDX) launch_lds<V, false, EPI_ADDX>(blocks, lds, comps, stream, a); \
-  else launch_lds<V, false, EPI_STORE>(blocks, lds, comps, stream, a);
-    if (vec) {
-      BINDER_LAUNCH_LDS(true)
-    } else {
-      BINDER_LAUNCH_LDS(false)
-    }
-#undef BINDER_LAUNCH_LDS
-    HIP_LAUNCH_OK();
-    return blocks;
-  }
-#define BINDER_LAUNCH(V)                                               \
-  if (pro) {                                                           \
-    if (epi == EPI_FILTER) launch<V, true, EPI_FILTER>(grid, stream, a); \
-    else launch<V, true, EPI_STORE>(grid, stream, a);                  \
-  } else if (epi == EPI_FILTER) launch<V, false, EPI_FILTER>(grid, stream, a); \
-  else if (epi == EPI_ADDX) launch<V, false, EPI_ADDX>(grid, stream, a); \
-  else launch<V, false, EPI_STORE>(grid, stream, a);
+  const bool persist = a.ks <= 6 && mat_lds <= kPersistMaxLds && tiles >= 2048;
+  const int blocks = persistent_blocks();
+#define BINDER_GO(V, P, E)                                       \
+  do {                                                           \
+    if (persist) launch_lds<V, P, E>(blocks, lds, comps, stream, a); \
+    else launch<V, P, E>(grid, stream, a);                       \
+  } while (0)
+#define BINDER_DISPATCH(V)                                                     \
+  if (pro == PRO_X) {                                                          \
+    if (epi == EPI_FILTER) BINDER_GO(V, PRO_X, EPI_FILTER);                    \
+    else BINDER_GO(V, PRO_X, EPI_STORE);                                       \
+  } else if (pro == PRO_GRID) {                                                \
+    if (epi == EPI_FILTER) BINDER_GO(V, PRO_GRID, EPI_FILTER);                 \
+    else BINDER_GO(V, PRO_GRID, EPI_STORE);                                    \
+  } else if (epi == EPI_FILTER) BINDER_GO(V, PRO_NONE, EPI_FILTER);            \
+  else if (epi == EPI_ADDX) BINDER_GO(V, PRO_NONE, EPI_ADDX);                  \
+  else if (epi == EPI_ADDGRID) BINDER_GO(V, PRO_NONE, EPI_ADDGRID);            \
+  else BINDER_GO(V, PRO_NONE, EPI_STORE);
   if (vec) {
-    BINDER_LAUNCH(true)
+    BINDER_DISPATCH(true)
   } else {
-    BINDER_LAUNCH(false)
+    BINDER_DISPATCH(false)
   }
-#undef BINDER_LAUNCH
+#undef BINDER_DISPATCH
+#undef BINDER_GO
   HIP_LAUNCH_OK();
-  return (long long)grid.x * grid.y;
+  return persist ? blocks : (long long)grid.x * grid.y;
 }
 
 bool fused_x_supported(int n) { return n >= 2 && n % 2 == 0 && n <= 192; }
@@ -1108,11 +1159,15 @@ void Smoother::run(const float *dream, const float *x, float *out,
     const PassMatrix &mx = mats_.get(MC::kDctFusedX, nx_, nx_, stream_);
     const PassMatrix &fy = mats_.get(MC::kDctForward, ny_, ny_, stream_);
     const PassMatrix &iy = mats_.get(MC::kDctInverse, ny_, ny_, stream_);
+    const bool gen = index_grid_ && x != nullptr;
     if (nz_ > 1) {
       const PassMatrix &fz = mats_.get(MC::kDctForward, nz_, nz_, stream_);
       const PassMatrix &iz = mats_.get(MC::kDctInverse, nz_, nz_, stream_);
+      // z forward: row = x * Ny + y, k = z; y inverse: row = x * Nz + z, k' = y
+      const GridCoords gz{2, {ny_, 1, 1}, {0, ny_, 0}};
+      const GridCoords gy{1, {nz_, 1, 1}, {0, 0, nz_}};
       contract_pass(stream_, fz, dream, bufs[0], N_, nv_, nv_, nv_ / nz_, x, nullptr,
-                    EPI_STORE, nullptr, timer, "dct_z_fwd", ny_);
+                    EPI_STORE, nullptr, timer, "dct_z_fwd", ny_, gen ? &gz : nullptr);
       contract_pass(stream_, fy, bufs[0], bufs[1], N_, nv_, nv_, nv_ / ny_, nullptr, nullptr,
                     EPI_STORE, nullptr, timer, "dct_y_fwd", nz_);
       nparts = fused_x_pass(stream_, mx, bufs[1], bufs[0], N_, nv_, (long long)ny_ * nz_, f,
@@ -1121,15 +1176,17 @@ void Smoother::run(const float *dream, const float *x, float *out,
       contract_pass(stream_, iz, bufs[0], bufs[1], N_, nv_, nv_, nv_ / nz_, nullptr, nullptr,
                     EPI_STORE, nullptr, timer, "dct_z_inv", ny_);
       contract_pass(stream_, iy, bufs[1], out, N_, nv_, nv_, nv_ / ny_, nullptr, x, EPI_ADDX,
-                    nullptr, timer, "dct_y_inv", nz_);
+                    nullptr, timer, "dct_y_inv", nz_, gen ? &gy : nullptr);
     } else {
+      // rows are x, k (k') is y
+      const GridCoords g2{1, {1, 1, 1}, {0, 0, 0}};
       contract_pass(stream_, fy, dream, bufs[0], N_, nv_, nv_, nx_, x, nullptr, EPI_STORE,
-                    nullptr, timer, "dct_y_fwd", 1);
+                    nullptr, timer, "dct_y_fwd", 1, gen ? &g2 : nullptr);
       nparts = fused_x_pass(stream_, mx, bufs[0], bufs[1], N_, nv_, ny_, f, timer,
                             "dct_x_fused");
       sum_partials();
       contract_pass(stream_, iy, bufs[1], out, N_, nv_, nv_, nx_, nullptr, x, EPI_ADDX,
-                    nullptr, timer, "dct_y_inv", 1);
+                    nullptr, timer, "dct_y_inv", 1, gen ? &g2 : nullptr);
     }
   } else {
     // Six rotating passes: axes contracted innermost first, each pass

@@ -42,16 +42,26 @@ struct FilterArgs {
   double *cost_part, *raw_part;
 };
 
+// The index grid (x = voxel indices) as a pass sees it: component c of the
+// element at (row, k) is k when c == axis, else (row / div[c]) mod mod[c]
+// (no modulo when mod[c] == 0).
+struct GridCoords {
+  int axis;
+  int div[3], mod[3];
+};
+
 // Out[c][k'][r] = sum_k In[c][r][k] M[k][k'] for r < R, c < comps; with
 // rin > 0 the rows form blocks of rin and row r = b * rin + i goes to
 // Out[c][b][k'][i] instead.  epi: 0 store, 1 filter (needs f), 2 add xepi
-// (laid out as Out).  xpro: In - xpro prologue.  Returns the number of
-// partial-sum slots written by a filter pass.
+// (laid out as Out).  xpro: In - xpro prologue.  With xg, the x of the
+// prologue / epilogue is generated from the indices instead of read (xpro /
+// xepi then only say which end applies it).  Returns the number of partial-
+// sum slots written by a filter pass.
 long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in,
                         float *out, int comps, long long cin, long long cout,
                         long long R, const float *xpro, const float *xepi, int epi,
                         const FilterArgs *f, KernelTimer *timer, const char *name,
-                        long long rin = 0);
+                        long long rin = 0, const GridCoords *xg = nullptr);
 // x axis of a [comp][N][R] field in one pass: forward DCT-II, prior filter
 // with noise and cost (f), inverse DCT-III; N even, N <= 192 (m from
 // MatrixCache kDctFusedX).  Returns the partial-sum slots written.
@@ -95,6 +105,8 @@ class Smoother {
   void set_gamma(const double g[3]);
   void get_gamma(double g[3]);
   double nu() const { return nu_; }
+  // x of run() is the index grid: the passes generate it instead of reading it
+  void set_index_grid(bool v) { index_grid_ = v; }
 
   // Flops of one run (forward + inverse contractions over all components).
   double flops() const;
@@ -103,6 +115,7 @@ class Smoother {
   int nx_, ny_, nz_, N_;
   long long nv_;
   double sigma_, rho_[3], eta_[3], nu_;
+  bool index_grid_ = false;
   hipStream_t stream_;
   DBuf<float> D_, bufA_, bufB_;
   DBuf<double> gamma_, part_cost_, part_raw_, scal_;
