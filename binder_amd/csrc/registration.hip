@@ -115,6 +115,14 @@ struct Stream {
   }
 };
 
+struct Event {
+  hipEvent_t e = nullptr;
+  Event() { HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
+  ~Event() {
+    if (e) (void)hipEventDestroy(e);
+  }
+};
+
 struct PinnedStats {
   double *p = nullptr;
   PinnedStats() { HIP_OK(hipHostMalloc((void **)&p, 16 * sizeof(double), 0)); }
@@ -130,6 +138,10 @@ using namespace binder;
 
 struct binder_registration {
   Stream stream;
+  // the sampler's theta chain (histogram reduction, Dirichlet draw) runs on
+  // side_stream, beside the deformation draw, which does not depend on it
+  Stream side_stream;
+  Event estep_done, theta_done;
   KernelTimer timer;
   PinnedStats host_stats;
   int nxv, nyv, nzv, nxn, nyn, nzn, N;
@@ -201,23 +213,28 @@ struct binder_registration {
     return a;
   }
 
+  void estep_only(bool sample, uint32_t k0, uint32_t k1, uint32_t sweep) {
+    EStepArgs a = estep_args(sample, k0, k1, sweep);
+    TimedLaunch t(tm(), sample ? "estep_sample" : "estep", s());
+    launch_estep(a, is2d, sample, estep_blocks, s());
+  }
+
+  void hist_reduce_on(hipStream_t st) {
+    TimedLaunch t(tm(), "hist_reduce", st);
+    launch_hist_reduce(hpart.get(), estep_blocks, K * L, hpart2.get(), Hd.get(), st);
+  }
+
   // E-step + histogram reduction (+ theta step with the given mode).
   void e_and_theta(bool sample, int theta_mode, uint32_t k0, uint32_t k1,
                    uint32_t sweep) {
-    EStepArgs a = estep_args(sample, k0, k1, sweep);
-    {
-      TimedLaunch t(tm(), sample ? "estep_sample" : "estep", s());
-      launch_estep(a, is2d, sample, estep_blocks, s());
-    }
-    {
-      TimedLaunch t(tm(), "hist_reduce", s());
-      launch_hist_reduce(hpart.get(), estep_blocks, K * L, hpart2.get(),
-                         Hd.get(), s());
-    }
+    estep_only(sample, k0, k1, sweep);
+    hist_reduce_on(s());
     if (theta_mode >= 0) run_theta(theta_mode, k0, k1, sweep);
   }
 
-  void run_theta(int mode, uint32_t k0, uint32_t k1, uint32_t sweep) {
+  void run_theta(int mode, uint32_t k0, uint32_t k1, uint32_t sweep,
+                 hipStream_t st = nullptr) {
+    if (!st) st = s();
     ThetaArgs t{};
     t.Hd = Hd.get();
     t.theta = theta.get();
@@ -239,8 +256,8 @@ struct binder_registration {
     t.k0 = k0;
     t.k1 = k1;
     t.sweep = sweep;
-    TimedLaunch tl(tm(), mode == kThetaDirichlet ? "dirichlet" : "theta", s());
-    launch_theta(t, s());
+    TimedLaunch tl(tm(), mode == kThetaDirichlet ? "dirichlet" : "theta", st);
+    launch_theta(t, st);
   }
 };
 
@@ -545,9 +562,23 @@ int binder_registration_sampler_run(binder_registration *r, int seed, int N_b,
     if (st && !r->dextra.get()) r->dextra.alloc((size_t)(kWelfordBatch - 1) * N * r->nv);
     WelfordBatch wb{};
     long long folded = 0;  // samples already in mean / m2
+    const bool overlap = sp && sl && st;
     for (int sweep = 0; sweep < N_b + N_s; ++sweep) {
-      if (sp) r->e_and_theta(true, -1, k0, k1, (uint32_t)sweep);
-      if (sl) r->run_theta(kThetaDirichlet, k0, k1, (uint32_t)sweep);
+      if (overlap) {
+        // E-draw; then the histogram reduction and the Dirichlet draw on the
+        // side stream while the main stream draws the deformation from the
+        // E-draw's dream locations; the next E-draw waits for theta
+        if (sweep > 0) HIP_OK(hipStreamWaitEvent(r->s(), r->theta_done.e, 0));
+        r->estep_only(true, k0, k1, (uint32_t)sweep);
+        HIP_OK(hipEventRecord(r->estep_done.e, r->s()));
+        HIP_OK(hipStreamWaitEvent(r->side_stream.s, r->estep_done.e, 0));
+        r->hist_reduce_on(r->side_stream.s);
+        r->run_theta(kThetaDirichlet, k0, k1, (uint32_t)sweep, r->side_stream.s);
+        HIP_OK(hipEventRecord(r->theta_done.e, r->side_stream.s));
+      } else {
+        if (sp) r->e_and_theta(true, -1, k0, k1, (uint32_t)sweep);
+        if (sl) r->run_theta(kThetaDirichlet, k0, k1, (uint32_t)sweep);
+      }
       if (st) {
         SmoothOptions o;
         o.sample = true;
@@ -574,6 +605,7 @@ int binder_registration_sampler_run(binder_registration *r, int seed, int N_b,
         }
       }
     }
+    if (overlap) HIP_OK(hipStreamWaitEvent(r->s(), r->theta_done.e, 0));
     HIP_OK(hipStreamSynchronize(r->s()));
   });
 }
