@@ -66,25 +66,17 @@ struct PassArgs {
   double *cost_part, *raw_part;
 };
 
-__device__ __forceinline__ uint32_t bf16_bits(float x) {
-  const uint32_t u = __float_as_uint(x);
-  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;  // round to nearest even
-}
 
-// Split 8 floats into bf16 hi and lo vectors (element j in bits 16j..16j+15).
+// Split 8 floats into bf16 hi and lo vectors: hi = bf16(v), lo = bf16(v - hi),
+// both rounded to nearest even (v_cvt_pk_bf16_f32).
 __device__ __forceinline__ void split8(const float v[8], bf16x8_t &hi,
                                        bf16x8_t &lo) {
-  u32x4_t h, l;
 #pragma unroll
-  for (int p = 0; p < 4; ++p) {
-    const uint32_t h0 = bf16_bits(v[2 * p]), h1 = bf16_bits(v[2 * p + 1]);
-    const float r0 = v[2 * p] - __uint_as_float(h0 << 16);
-    const float r1 = v[2 * p + 1] - __uint_as_float(h1 << 16);
-    h[p] = h0 | (h1 << 16);
-    l[p] = bf16_bits(r0) | (bf16_bits(r1) << 16);
+  for (int j = 0; j < 8; ++j) {
+    const __bf16 h = (__bf16)v[j];
+    hi[j] = h;
+    lo[j] = (__bf16)(v[j] - (float)h);
   }
-  hi = __builtin_bit_cast(bf16x8_t, h);
-  lo = __builtin_bit_cast(bf16x8_t, l);
 }
 
 // Index-grid coordinate of component `comp` at `row` (comp != xg_axis).
@@ -545,7 +537,7 @@ __device__ __forceinline__ f32x4_t mfma3(const uint4 &ah, const uint4 &al, const
 // Forward rows of one parity (PAR 0: even k on sums, 1: odd k on
 // differences), prior filter, and the split of the filtered coefficients
 // into the inverse's B operand.
-template <int NTH, int KSH, int PAR>
+template <int NTH, int KSH, int PAR, bool EXACT>
 __device__ __forceinline__ void fused_forward(const FusedArgs &a, const uint4 *mh,
                                               const uint4 *ml, const bf16x8_t *bh,
                                               const bf16x8_t *bl, const float (*dv)[4],
@@ -577,24 +569,24 @@ __device__ __forceinline__ void fused_forward(const FusedArgs &a, const uint4 *m
               a.k0, a.k1);
           normals4(r, eps);
         }
+        // D p^2 summed over the 4 rows in FP32, then carried in FP64
+        float q4 = 0.f;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int m = m0 + r;
-          float p = 0.f;
-          if (colok && m < H) {
-            // 1 / den and sigma / sqrt(den) from the hardware reciprocal and
-            // reciprocal square root (1 ulp; the contraction itself carries
-            // about 2^-16 relative error)
-            const float Dv = dv[t][r];
-            const float den = fmaf(gs2, Dv, 1.0f);
-            p = acc[r] * __builtin_amdgcn_rcpf(den);
-            if (a.noise) p = fmaf(eps[r] * sigma, __builtin_amdgcn_rsqf(den), p);
-            const double pd = (double)p;
-            cost += 0.5 * gm * (double)Dv * pd * pd;
-            if (a.raw) raw += a.etarho[comp] * (double)Dv * pd * pd;
-          }
+          const bool ok = EXACT || (colok && m0 + r < H);
+          // 1 / den and sigma / sqrt(den) from the hardware reciprocal and
+          // reciprocal square root (1 ulp; the contraction itself carries
+          // about 2^-16 relative error).  Rows and columns out of range
+          // have F = D = 0 and no noise.
+          const float Dv = dv[t][r];
+          const float den = fmaf(gs2, Dv, 1.0f);
+          float p = acc[r] * __builtin_amdgcn_rcpf(den);
+          if (a.noise) p = fmaf(ok ? eps[r] * sigma : 0.f, __builtin_amdgcn_rsqf(den), p);
+          q4 = fmaf(Dv * p, p, q4);
           pv[4 * h + r] = p;
         }
+        cost += (0.5 * gm) * (double)q4;
+        if (a.raw) raw += a.etarho[comp] * (double)q4;
       } else {
 #pragma unroll
         for (int r = 0; r < 4; ++r) pv[4 * h + r] = 0.f;
@@ -606,7 +598,9 @@ __device__ __forceinline__ void fused_forward(const FusedArgs &a, const uint4 *m
 
 constexpr int kFusedThreads = 512;  // 2 wavefronts per SIMD: 256 VGPRs each
 
-template <int NTH>
+// EXACT: H and R are multiples of 16, so every row and column a lane owns
+// is in range.
+template <int NTH, bool EXACT>
 __global__ __launch_bounds__(kFusedThreads) void k_dct_x_fused(FusedArgs a, int comps) {
   constexpr int KSH = (NTH + 1) / 2;
   constexpr int NF = NTH * KSH * 64;
@@ -630,14 +624,15 @@ __global__ __launch_bounds__(kFusedThreads) void k_dct_x_fused(FusedArgs a, int 
        tile += (long long)gridDim.x * waves_per_block) {
     const int comp = (int)(tile / tiles_per_comp);
     const uint32_t c = (uint32_t)((tile % tiles_per_comp) * 16) + (uint32_t)(lane & 15);
-    const bool colok = c < (uint32_t)a.R;
+    const bool colok = EXACT || c < (uint32_t)a.R;
     const float *in = a.in + comp * a.cin;
     float *out = a.out + comp * a.cout;
     const double gm = a.gamma[comp];
     const float gs2 = (float)gm * a.sigma2;
     // per-iteration (opaque) copies of the loop invariants: the row offsets
-    // x * R are recomputed in 32 bits each tile instead of being hoisted
-    // out of the loop as dozens of live 64-bit values
+    // x * R are formed in 32 bits each tile instead of being hoisted out of
+    // the loop as dozens of live 64-bit values; within a tile they advance
+    // by additions of R
     const int z0 = opaque_zero();
     const uint4 *mh = msh + z0, *ml = msh + 4 * NF + z0;
     const uint32_t Rz = (uint32_t)a.R + (uint32_t)z0;
@@ -646,16 +641,19 @@ __global__ __launch_bounds__(kFusedThreads) void k_dct_x_fused(FusedArgs a, int 
 #pragma unroll
     for (int s = 0; s < KSH; ++s) {
       float sv[8], dv[8];
+      const int x0 = 32 * s + 8 * g;
+      uint32_t o1 = (uint32_t)x0 * Rz + c, o2 = (uint32_t)(N - 1 - x0) * Rz + c;
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const int x = 32 * s + 8 * g + j;
         float u1 = 0.f, u2 = 0.f;
-        if (colok && x < H) {
-          u1 = ldf(in, (uint32_t)x * Rz + c);
-          u2 = ldf(in, (uint32_t)(N - 1 - x) * Rz + c);
+        if (colok && x0 + j < H) {
+          u1 = ldf(in, o1);
+          u2 = ldf(in, o2);
         }
         sv[j] = u1 + u2;
         dv[j] = u1 - u2;
+        o1 += Rz;
+        o2 -= Rz;
       }
       split8(sv, sh[s], sl[s]);
       split8(dv, dh[s], dl[s]);
@@ -664,19 +662,21 @@ __global__ __launch_bounds__(kFusedThreads) void k_dct_x_fused(FusedArgs a, int 
     // data so that the filter does not wait on memory tile by tile
     float dve[NTH][4], dvo[NTH][4];
 #pragma unroll
-    for (int t = 0; t < NTH; ++t)
+    for (int t = 0; t < NTH; ++t) {
+      uint32_t od = (uint32_t)(2 * (16 * t + 4 * g)) * Rz + c;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = 16 * t + 4 * g + r;
-        const bool ok = colok && m < H;
-        dve[t][r] = ok ? ldf(a.D, (uint32_t)(2 * m) * Rz + c) : 0.f;
-        dvo[t][r] = ok ? ldf(a.D, (uint32_t)(2 * m + 1) * Rz + c) : 0.f;
+        const bool ok = EXACT || (colok && 16 * t + 4 * g + r < H);
+        dve[t][r] = ok ? ldf(a.D, od) : 0.f;
+        dvo[t][r] = ok ? ldf(a.D, od + Rz) : 0.f;
+        od += 2 * Rz;
       }
+    }
     bf16x8_t peh[KSH], pel[KSH], poh[KSH], pol[KSH];
-    fused_forward<NTH, KSH, 0>(a, mh, ml, sh, sl, dve, peh, pel, lane, comp, c, colok, H, gm,
-                               gs2, cost, raw);
-    fused_forward<NTH, KSH, 1>(a, mh, ml, dh, dl, dvo, poh, pol, lane, comp, c, colok, H, gm,
-                               gs2, cost, raw);
+    fused_forward<NTH, KSH, 0, EXACT>(a, mh, ml, sh, sl, dve, peh, pel, lane, comp, c, colok,
+                                      H, gm, gs2, cost, raw);
+    fused_forward<NTH, KSH, 1, EXACT>(a, mh, ml, dh, dl, dvo, poh, pol, lane, comp, c, colok,
+                                      H, gm, gs2, cost, raw);
     // inverse: Ve and Vo per 16-row x tile, then the mirrored pair of rows
 #pragma unroll
     for (int tp = 0; tp < NTH; ++tp) {
@@ -687,13 +687,16 @@ __global__ __launch_bounds__(kFusedThreads) void k_dct_x_fused(FusedArgs a, int 
         ae = mfma3(mh[fe + s * 64], ml[fe + s * 64], peh[s], pel[s], ae);
         ao = mfma3(mh[fo + s * 64], ml[fo + s * 64], poh[s], pol[s], ao);
       }
+      const int x0 = 16 * tp + 4 * g;
+      uint32_t o1 = (uint32_t)x0 * Rz + c, o2 = (uint32_t)(N - 1 - x0) * Rz + c;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int x = 16 * tp + 4 * g + r;
-        if (colok && x < H) {
-          stf(out, (uint32_t)x * Rz + c, ae[r] + ao[r]);
-          stf(out, (uint32_t)(N - 1 - x) * Rz + c, ae[r] - ao[r]);
+        if (EXACT || (colok && x0 + r < H)) {
+          stf(out, o1, ae[r] + ao[r]);
+          stf(out, o2, ae[r] - ao[r]);
         }
+        o1 += Rz;
+        o2 -= Rz;
       }
     }
   }
@@ -998,9 +1001,9 @@ long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in
 
 bool fused_x_supported(int n) { return n >= 2 && n % 2 == 0 && n <= 192; }
 
-template <int NTH>
+template <int NTH, bool EXACT>
 void launch_fused(int blocks, size_t lds, int comps, hipStream_t s, const FusedArgs &a) {
-  auto k = k_dct_x_fused<NTH>;
+  auto k = k_dct_x_fused<NTH, EXACT>;
   static bool attr_set = false;
   if (!attr_set) {
     HIP_OK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1043,14 +1046,23 @@ long long fused_x_pass(hipStream_t stream, const PassMatrix &m, const float *in,
   long long blocks = (tiles + kFusedThreads / 64 - 1) / (kFusedThreads / 64);
   if (blocks > persistent_blocks()) blocks = persistent_blocks();
   TimedLaunch tl(timer, name, stream);
+  const bool exact = H % 16 == 0 && R % 16 == 0;
+#define BINDER_FUSED(NT)                                                      \
+  case NT:                                                                    \
+    if (exact) launch_fused<NT, true>((int)blocks, lds, comps, stream, a);     \
+    else launch_fused<NT, false>((int)blocks, lds, comps, stream, a);          \
+    break;
   switch (nt) {
-    case 1: launch_fused<1>((int)blocks, lds, comps, stream, a); break;
-    case 2: launch_fused<2>((int)blocks, lds, comps, stream, a); break;
-    case 3: launch_fused<3>((int)blocks, lds, comps, stream, a); break;
-    case 4: launch_fused<4>((int)blocks, lds, comps, stream, a); break;
-    case 5: launch_fused<5>((int)blocks, lds, comps, stream, a); break;
-    default: launch_fused<6>((int)blocks, lds, comps, stream, a); break;
+    BINDER_FUSED(1)
+    BINDER_FUSED(2)
+    BINDER_FUSED(3)
+    BINDER_FUSED(4)
+    BINDER_FUSED(5)
+    default:
+      if (exact) launch_fused<6, true>((int)blocks, lds, comps, stream, a);
+      else launch_fused<6, false>((int)blocks, lds, comps, stream, a);
   }
+#undef BINDER_FUSED
   HIP_LAUNCH_OK();
   return blocks;
 }
