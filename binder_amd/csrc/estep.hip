@@ -766,34 +766,63 @@ __global__ __launch_bounds__(1024) void k_theta(ThetaArgs a) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_welford(WelfordBatch b, double *mean,
-                                                 double *m2, long long nv,
-                                                 int N) {
+// One sample's Welford step (Registration.cpp:593-623): d1 = x - mu_old,
+// mu += d1 / n, d2 = x - mu_new, M2 += d1 d2^T.
+template <int NC>
+__device__ __forceinline__ void welford_step(const float *xv, double inv_n, double *mu,
+                                             double *M) {
+  double d1[NC], d2[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const double val = (double)xv[c];
+    d1[c] = val - mu[c];
+    mu[c] = mu[c] + d1[c] * inv_n;
+    d2[c] = val - mu[c];
+  }
+  int p = 0;
+#pragma unroll
+  for (int r = 0; r < NC; ++r)
+#pragma unroll
+    for (int c = r; c < NC; ++c, ++p) M[p] += d1[r] * d2[c];
+}
+
+// CNT > 0: exactly CNT samples, all read before the updates so the reads
+// are in flight together (32-bit offsets: the host keeps N * nv below
+// 2^30); CNT == 0: b.count samples, one at a time.
+template <int NC, int CNT>
+__global__ __launch_bounds__(256) void k_welford(WelfordBatch b, double *mean, double *m2,
+                                                 long long nv) {
+  constexpr int NP = NC * (NC + 1) / 2;
   const long long stride = (long long)gridDim.x * blockDim.x;
-  const int np = N * (N + 1) / 2;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
        i += stride) {
-    double mu[3] = {0.0, 0.0, 0.0}, M[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int c = 0; c < N; ++c) mu[c] = mean[c * nv + i];
-    for (int p = 0; p < np; ++p) M[p] = m2[p * nv + i];
-    for (int k = 0; k < b.count; ++k) {
-      // Registration.cpp:593-623: d1 = x - mu_old, mu += d1 / n,
-      // d2 = x - mu_new, M2 += d1 d2^T
-      const double inv_n = 1.0 / ((double)(b.n0 + k) + 1.0);
-      const float *x = b.x[k];
-      double d1[3] = {0.0, 0.0, 0.0}, d2[3] = {0.0, 0.0, 0.0};
-      for (int c = 0; c < N; ++c) {
-        const double val = (double)x[c * nv + i];
-        d1[c] = val - mu[c];
-        mu[c] = mu[c] + d1[c] * inv_n;
-        d2[c] = val - mu[c];
+    double mu[NC], M[NP];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) mu[c] = mean[c * nv + i];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) M[p] = m2[p * nv + i];
+    if constexpr (CNT > 0) {
+      float xs[CNT][NC];
+      const uint32_t iu = (uint32_t)i, nv32 = (uint32_t)nv;
+#pragma unroll
+      for (int k = 0; k < CNT; ++k)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) xs[k][c] = ld_off32(b.x[k], c * nv32 + iu);
+#pragma unroll
+      for (int k = 0; k < CNT; ++k)
+        welford_step<NC>(xs[k], 1.0 / ((double)(b.n0 + k) + 1.0), mu, M);
+    } else {
+      for (int k = 0; k < b.count; ++k) {
+        float xv[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) xv[c] = b.x[k][c * nv + i];
+        welford_step<NC>(xv, 1.0 / ((double)(b.n0 + k) + 1.0), mu, M);
       }
-      int p = 0;
-      for (int r = 0; r < N; ++r)
-        for (int c = r; c < N; ++c, ++p) M[p] += d1[r] * d2[c];
     }
-    for (int c = 0; c < N; ++c) mean[c * nv + i] = mu[c];
-    for (int p = 0; p < np; ++p) m2[p * nv + i] = M[p];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) mean[c * nv + i] = mu[c];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) m2[p * nv + i] = M[p];
   }
 }
 
@@ -860,8 +889,15 @@ void launch_welford(const WelfordBatch &b, double *mean, double *m2,
   if (b.count <= 0) return;
   long long blocks = (nv + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(k_welford, dim3((unsigned)blocks), dim3(256), 0, s, b,
-                     mean, m2, nv, N);
+  const dim3 g((unsigned)blocks), t(256);
+  const bool full = b.count == kWelfordBatch;
+  if (N == 3) {
+    if (full) hipLaunchKernelGGL((k_welford<3, kWelfordBatch>), g, t, 0, s, b, mean, m2, nv);
+    else hipLaunchKernelGGL((k_welford<3, 0>), g, t, 0, s, b, mean, m2, nv);
+  } else {
+    if (full) hipLaunchKernelGGL((k_welford<2, kWelfordBatch>), g, t, 0, s, b, mean, m2, nv);
+    else hipLaunchKernelGGL((k_welford<2, 0>), g, t, 0, s, b, mean, m2, nv);
+  }
   HIP_LAUNCH_OK();
 }
 
