@@ -85,6 +85,8 @@ _topo3 = _sig("binder_topology_correction_3d", _c_int, _vp, _c_double, _c_double
 
 
 _philox = _sig("binder_philox4x32_10", _c_int, _vp, _vp, _vp)
+_reg_set_dct = _sig("binder_registration_set_field_from_dct", _c_int, _vp, _vp, _c_int, _c_int,
+                    _c_int, _c_int)
 
 
 _dct_prior = _sig("binder_dct_prior_eigenvalues", _c_int, _c_int, _c_int, _c_int, _c_int,
@@ -243,6 +245,19 @@ class Registration:
         cov = np.empty((Nx, Ny, Nz, N, N), np.float64)
         _check(_reg_sampler_result(self._h, _ptr(mean), _ptr(cov)))
         return mean, cov
+
+    def set_field_from_dct(self, coef, full_shape, resolution):
+        """Sets the current field to the warm start of a pyramid level at
+        `resolution`: the displacement whose 8x8x8 lowest orthonormal DCT-II
+        coefficients per component (on the full-resolution grid
+        `full_shape`) are sqrt(prod(full_shape)) * coef[3,8,8,8], sampled at
+        the full-resolution voxels resolution * p and divided by the
+        resolution.  No reference counterpart (see binder_amd.warmstart)."""
+        c = _arr(coef, np.float32)
+        if c.size != 3 * 512:
+            raise ValueError("coef must hold 3 x 8 x 8 x 8 values")
+        fx, fy, fz = (int(n) for n in full_shape)
+        _check(_reg_set_dct(self._h, _ptr(c), fx, fy, fz, int(resolution)))
 
     def get_final_locations_as_numpy_array(self):
         out = np.empty(self._shape, np.float32)

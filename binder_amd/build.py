@@ -28,6 +28,7 @@ SOURCES = {
     "topology.hip": ["-ffp-contract=off"],
     # FP64 resampling: same operation order as the host reference restatement
     "resample.hip": ["-ffp-contract=off"],
+    "warmstart.hip": [],
 }
 
 

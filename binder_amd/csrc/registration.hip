@@ -18,6 +18,7 @@
 #include "smoothing.hpp"
 #include "topology.hpp"
 #include "resample.hpp"
+#include "warmstart.hpp"
 
 namespace binder {
 
@@ -742,6 +743,22 @@ int binder_registration_correct_topology(binder_registration *r, double e1,
     BINDER_REQUIRE(r && voxel_sizes, "null argument");
     topology_on_device(r->dcur(), r->N, r->nxv, r->nyv, r->nzv, voxel_sizes, e1, e2,
                        max_outer_it, max_it, r->s(), stats);
+  });
+}
+
+int binder_registration_set_field_from_dct(binder_registration *r, const float *coef, int fx,
+                                           int fy, int fz, int res) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && coef, "null argument");
+    BINDER_REQUIRE(!r->is2d, "the DCT warm start is defined for 3D registrations");
+    BINDER_REQUIRE(res >= 1 && fx >= 1 && fy >= 1 && fz >= 1, "bad grid or resolution");
+    BINDER_REQUIRE((fx + res - 1) / res == r->nxv && (fy + res - 1) / res == r->nyv &&
+                       (fz + res - 1) / res == r->nzv,
+                   "the handle's grid is not ceil(full grid / res)");
+    DBuf<float> c;
+    c.upload(coef, (size_t)3 * kWarmCoef * kWarmCoef * kWarmCoef, r->s());
+    launch_dct_field(c.get(), fx, fy, fz, res, r->dcur(), r->nxv, r->nyv, r->nzv, r->s());
+    HIP_OK(hipStreamSynchronize(r->s()));
   });
 }
 

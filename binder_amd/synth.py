@@ -27,27 +27,50 @@ TISSUE = (0.1, 0.35, 0.6, 0.85)
 def phantom(shape, n_shapes, rng, semi=(10.0, 60.0), sigma=1.0, noise=0.02):
     shape = tuple(int(s) for s in shape)
     img = np.zeros(shape, np.float64)
-    grids = np.meshgrid(*[np.arange(s, dtype=np.float64) for s in shape], indexing="ij")
     lo, hi = semi
     for _ in range(n_shapes):
         c = [rng.uniform(0.15 * s, 0.85 * s) for s in shape]
         r = [min(rng.uniform(lo, hi), 0.45 * s) for s in shape]
-        m = sum(((g - ci) / ri) ** 2 for g, ci, ri in zip(grids, c, r)) <= 1.0
-        img[m] = TISSUE[rng.integers(0, len(TISSUE))]
+        # paint inside the bounding box only: outside it some axis term
+        # ((g - c) / r)^2 alone exceeds 1
+        box = tuple(slice(max(0, int(np.floor(ci - ri))), min(s, int(np.ceil(ci + ri)) + 1))
+                    for ci, ri, s in zip(c, r, shape))
+        axes = np.ix_(*[np.arange(b.start, b.stop, dtype=np.float64) for b in box])
+        m = sum(((g - ci) / ri) ** 2 for g, ci, ri in zip(axes, c, r)) <= 1.0
+        img[box][m] = TISSUE[rng.integers(0, len(TISSUE))]
     img = ndimage.gaussian_filter(img, sigma)
     return img + rng.normal(0.0, noise, shape)
 
 
-def dct_displacement(shape, rms, rng, ncoef=8):
-    """[*shape, D] smooth displacement, lowest ncoef^D DCT coefficients N(0,1)."""
+def _dct_rows(n, k):
+    """Rows a < k of the orthonormal DCT-II matrix of length n, [k][n]."""
+    a = np.arange(k)[:, None]
+    i = np.arange(n)[None, :]
+    C = np.sqrt(2.0 / n) * np.cos(np.pi * (2 * i + 1) * a / (2 * n))
+    C[0] = np.sqrt(1.0 / n)
+    return C
+
+
+def dct_displacement(shape, rms, rng, ncoef=8, separable=False):
+    """[*shape, D] smooth displacement, lowest ncoef^D DCT coefficients N(0,1).
+    separable=True synthesises the field from the ncoef basis rows per axis
+    (the same field up to rounding, several times faster at 160x192x160)."""
     shape = tuple(int(s) for s in shape)
     D = len(shape)
     out = np.empty(shape + (D,), np.float64)
     for n in range(D):
-        c = np.zeros(shape)
         sl = tuple(slice(0, min(ncoef, s)) for s in shape)
-        c[sl] = rng.standard_normal(c[sl].shape)
-        out[..., n] = idctn(c, norm="ortho")
+        coef = rng.standard_normal(tuple(x.stop for x in sl))
+        if separable:
+            f = coef
+            for ax, s in enumerate(shape):
+                C = _dct_rows(s, coef.shape[ax])  # [k][s]
+                f = np.moveaxis(np.tensordot(C.T, np.moveaxis(f, ax, 0), axes=1), 0, ax)
+            out[..., n] = f
+        else:
+            c = np.zeros(shape)
+            c[sl] = coef
+            out[..., n] = idctn(c, norm="ortho")
     cur = np.sqrt(np.mean(np.sum(out ** 2, axis=-1)))
     return out * (rms / max(cur, 1e-12))
 
@@ -61,10 +84,11 @@ class Pair:
     binned_moving: np.ndarray = field(default=None)
 
 
-def make_pair(shape, n_shapes, rms, seed, semi=(10.0, 60.0), bins=32, gamma_map=0.7):
+def make_pair(shape, n_shapes, rms, seed, semi=(10.0, 60.0), bins=32, gamma_map=0.7,
+              separable=False):
     rng = np.random.default_rng(seed)
     fixed = phantom(shape, n_shapes, rng, semi=semi)
-    u = dct_displacement(shape, rms, rng)
+    u = dct_displacement(shape, rms, rng, separable=separable)
     grids = np.meshgrid(*[np.arange(s, dtype=np.float64) for s in shape], indexing="ij")
     coords = [g + u[..., a] for a, g in enumerate(grids)]
     warped = ndimage.map_coordinates(fixed, coords, order=1, mode="nearest")

@@ -1,0 +1,236 @@
+"""Learned warm start of the 3D MI registration.
+
+A 3D convolutional encoder-decoder (binder_amd/csrc/warmstart.hip; channels
+16/32/48/64, 3x3x3 convolutions, BF16 weights and activations with FP32
+accumulation) reads the binned fixed and moving image at the half-resolution
+pyramid level and predicts the 8x8x8 lowest orthonormal DCT-II coefficients
+of each component of the full-resolution displacement.  It is trained on the
+GPU with its own forward and backward kernels on seeded synthetic pairs,
+each labelled by the field the plain (identity-start) pyramid converges to.
+
+The prediction only chooses where the plain EM starts: FastRegistration
+runs the same EM to convergence from it (``FastRegistration.run_warm``), and
+falls back to the plain cold pyramid whenever the warm start's objective is
+worse than the identity start's.  The reference has no counterpart; its
+pyramid always starts from the identity (BINDER/FastRegistration.py:206-225).
+
+Coefficient convention (``field_coefficients``): for the displacement
+u = F - x of a field F on an Nx x Ny x Nz grid,
+coef[n, a, b, c] = sum_ijk C_Nx[a, i] C_Ny[b, j] C_Nz[c, k] u_n[i, j, k] / sqrt(Nx Ny Nz),
+with C_N the orthonormal DCT-II matrix, a, b, c < 8.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _core
+
+NCOEF = 8
+_lib = _core._lib
+_vp = ctypes.c_void_p
+_c_int = ctypes.c_int
+_P = ctypes.POINTER
+_sig = _core._sig
+_check = _core._check
+_ptr = _core._ptr
+
+_ws_create = _sig("binder_warmstart_create", _c_int, _P(_vp), _c_int, _c_int, _c_int, _c_int,
+                  ctypes.c_ulonglong)
+_ws_destroy = _sig("binder_warmstart_destroy", None, _vp)
+_ws_nparam = _sig("binder_warmstart_num_params", _c_int, _vp, _P(ctypes.c_longlong))
+_ws_get = _sig("binder_warmstart_get_params", _c_int, _vp, _vp)
+_ws_set = _sig("binder_warmstart_set_params", _c_int, _vp, _vp)
+_ws_data = _sig("binder_warmstart_set_dataset", _c_int, _vp, _c_int, _vp, _vp, _vp, _c_int,
+                _c_int)
+_ws_train = _sig("binder_warmstart_train", _c_int, _vp, _vp, _c_int, _c_int, ctypes.c_double,
+                 _vp)
+_ws_grad = _sig("binder_warmstart_gradients", _c_int, _vp, _vp, _c_int, _P(ctypes.c_double), _vp)
+_ws_pred_data = _sig("binder_warmstart_predict_dataset", _c_int, _vp, _vp, _c_int, _vp)
+_ws_pred = _sig("binder_warmstart_predict", _c_int, _vp, _c_int, _vp, _vp, _c_int, _c_int, _vp)
+
+# (cin, cout) of the 3x3x3 convolutions, in parameter-vector order
+CONV_LAYERS = [(8, 16), (16, 16), (16, 32), (32, 32), (32, 48), (48, 48), (48, 64), (64, 64),
+               (64, 64), (64, 64), (112, 64), (64, 64)]
+HEAD_IN = 64
+
+
+def num_params():
+    n = sum(co * 27 * ci + co for ci, co in CONV_LAYERS)
+    return n + 3 * HEAD_IN + 3
+
+
+def network_grid(full_shape):
+    """Input grid of the network for a full-resolution shape: the
+    half-resolution pyramid level, ceil(N / 2) per axis, padded up to a
+    multiple of 8 (at least 32)."""
+    out = []
+    for n in full_shape:
+        half = -(-int(n) // 2)
+        out.append(max(32, -(-half // 8) * 8))
+    return tuple(out)
+
+
+def pair_inputs(full_binned_voxels, full_binned_nodes, grid):
+    """Network inputs of one pair: the half-resolution pyramid level's bins
+    (full[::2, ::2, ::2], as FastRegistration.move_resolution(2) takes them)
+    zero-padded into `grid`, uint8."""
+    out = []
+    for b in (full_binned_voxels, full_binned_nodes):
+        sub = np.asarray(b)[::2, ::2, ::2]
+        if np.any(sub < 0) or np.any(sub > 255):
+            raise ValueError("bins must be in [0, 255]")
+        a = np.zeros(grid, np.uint8)
+        a[:sub.shape[0], :sub.shape[1], :sub.shape[2]] = sub
+        out.append(a)
+    return out[0], out[1]
+
+
+def dct_rows(n, k=NCOEF):
+    """Rows a < k of the orthonormal DCT-II matrix of length n, [k][n]
+    (rows past n are zero)."""
+    C = np.zeros((k, n))
+    a = np.arange(min(k, n))[:, None]
+    i = np.arange(n)[None, :]
+    C[:min(k, n)] = np.sqrt(2.0 / n) * np.cos(np.pi * (2 * i + 1) * a / (2 * n))
+    C[0] = np.sqrt(1.0 / n)
+    return C
+
+
+def field_coefficients(F, positions=None):
+    """Normalised low-frequency coefficients [3, 8, 8, 8] of the displacement
+    F - positions of a field F [Nx, Ny, Nz, 3] (positions default to the
+    grid indices)."""
+    F = np.asarray(F, np.float64)
+    shape = F.shape[:3]
+    if positions is None:
+        positions = np.stack(np.indices(shape, dtype=np.float64), axis=-1)
+    u = F - positions
+    Cx, Cy, Cz = (dct_rows(n) for n in shape)
+    out = np.empty((3, NCOEF, NCOEF, NCOEF))
+    for n in range(3):
+        t = u[..., n] @ Cz.T                       # [X, Y, c]
+        t = np.einsum("bj,xjc->xbc", Cy, t)        # [X, b, c]
+        out[n] = np.einsum("ai,ibc->abc", Cx, t)   # [a, b, c]
+    return out / np.sqrt(float(np.prod(shape)))
+
+
+def coefficients_field(coef, shape, resolution=1):
+    """Host evaluation of the warm-start field of a pyramid level
+    (Registration.set_field_from_dct): [ceil(N/res)..., 3] positions."""
+    coef = np.asarray(coef, np.float64).reshape(3, NCOEF, NCOEF, NCOEF)
+    shape = tuple(int(n) for n in shape)
+    r = int(resolution)
+    lvl = tuple(-(-n // r) for n in shape)
+    Cs = [dct_rows(n)[:, ::r] for n in shape]  # basis at full indices r * p
+    amp = np.sqrt(float(np.prod(shape)))
+    out = np.empty(lvl + (3,))
+    grid = np.indices(lvl, dtype=np.float64)
+    for n in range(3):
+        u = np.einsum("abc,ai,bj,ck->ijk", coef[n], Cs[0], Cs[1], Cs[2], optimize=True)
+        out[..., n] = grid[n] + amp * u / r
+    return out
+
+
+class WarmStartNet:
+    """The warm-start network on the GPU (handle over the C ABI)."""
+
+    def __init__(self, grid, max_batch=8, seed=0):
+        self.grid = tuple(int(n) for n in grid)
+        self.max_batch = int(max_batch)
+        h = _vp()
+        _check(_ws_create(ctypes.byref(h), *self.grid, self.max_batch, int(seed)))
+        self._h = h
+        n = ctypes.c_longlong()
+        _check(_ws_nparam(self._h, ctypes.byref(n)))
+        self.nparam = int(n.value)
+        self._n = 0
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _ws_destroy(self._h)
+            self._h = None
+
+    def get_params(self):
+        out = np.empty(self.nparam, np.float32)
+        _check(_ws_get(self._h, _ptr(out)))
+        return out
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != self.nparam:
+            raise ValueError(f"expected {self.nparam} parameters")
+        _check(_ws_set(self._h, _ptr(p)))
+
+    def set_dataset(self, fixed_bins, moving_bins, targets=None, L=32, K=32):
+        """fixed_bins / moving_bins: [n, *grid] uint8; targets [n, 3, 8, 8, 8]."""
+        f = np.ascontiguousarray(fixed_bins, np.uint8)
+        m = np.ascontiguousarray(moving_bins, np.uint8)
+        n = f.shape[0]
+        if f.shape != (n,) + self.grid or m.shape != f.shape:
+            raise ValueError(f"bins must be [n, {self.grid}]")
+        t = None
+        if targets is not None:
+            t = np.ascontiguousarray(targets, np.float32)
+            if t.size != n * 3 * NCOEF ** 3:
+                raise ValueError("targets must be [n, 3, 8, 8, 8]")
+        _check(_ws_data(self._h, n, _ptr(f), _ptr(m), None if t is None else _ptr(t), int(L),
+                        int(K)))
+        self._n = n
+
+    def train(self, idx, lr=1e-3):
+        """Adam steps over idx [steps, batch] (sample indices); returns the
+        loss of each step."""
+        ix = np.ascontiguousarray(idx, np.int32)
+        steps, batch = ix.shape
+        losses = np.empty(steps, np.float64)
+        _check(_ws_train(self._h, _ptr(ix), steps, batch, float(lr), _ptr(losses)))
+        return losses
+
+    def gradients(self, idx):
+        ix = np.ascontiguousarray(idx, np.int32).ravel()
+        g = np.empty(self.nparam, np.float32)
+        loss = ctypes.c_double()
+        _check(_ws_grad(self._h, _ptr(ix), ix.size, ctypes.byref(loss), _ptr(g)))
+        return float(loss.value), g
+
+    def predict_dataset(self, idx):
+        ix = np.ascontiguousarray(idx, np.int32).ravel()
+        out = np.empty((ix.size, 3, NCOEF, NCOEF, NCOEF), np.float32)
+        for s in range(0, ix.size, self.max_batch):
+            part = np.ascontiguousarray(ix[s:s + self.max_batch])
+            _check(_ws_pred_data(self._h, _ptr(part), part.size, _ptr(out[s:s + part.size])))
+        return out
+
+    def predict(self, fixed_bins, moving_bins, L=32, K=32):
+        """Coefficients [B, 3, 8, 8, 8] for bins [B, *grid] (or one pair [*grid])."""
+        f = np.ascontiguousarray(fixed_bins, np.uint8)
+        m = np.ascontiguousarray(moving_bins, np.uint8)
+        single = f.ndim == 3
+        if single:
+            f, m = f[None], m[None]
+        B = f.shape[0]
+        if f.shape[1:] != self.grid or m.shape != f.shape or B > self.max_batch:
+            raise ValueError(f"bins must be [<= {self.max_batch}, {self.grid}]")
+        out = np.empty((B, 3, NCOEF, NCOEF, NCOEF), np.float32)
+        _check(_ws_pred(self._h, B, _ptr(f), _ptr(m), int(L), int(K), _ptr(out)))
+        return out[0] if single else out
+
+    def predict_pair(self, full_binned_voxels, full_binned_nodes, L=32, K=32):
+        f, m = pair_inputs(full_binned_voxels, full_binned_nodes, self.grid)
+        return self.predict(f, m, L, K)
+
+    def save(self, path, **meta):
+        np.savez(path, params=self.get_params(), grid=np.array(self.grid),
+                 **{k: np.asarray(v) for k, v in meta.items()})
+
+    @classmethod
+    def load(cls, path, max_batch=1):
+        with np.load(path, allow_pickle=False) as z:
+            net = cls(tuple(int(n) for n in z["grid"]), max_batch=max_batch)
+            net.set_params(z["params"])
+        return net
+
+
+WEIGHTS_C2 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",
+                          "warmstart_160x192x160.npz")
