@@ -73,8 +73,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-REG_KW = dict(num_voxel_intesity_levels=32, num_node_intensity_levels=32, gamma=1e-6,
-              topology_correction=True, increase_gamma_at_DS_resolution=True, alpha=2.0)
+from binder_amd.synth import REGISTRATION_KW as REG_KW  # noqa: E402
 TOLERANCE_VOX_RMS = 0.1
 
 
@@ -82,38 +81,65 @@ def _rms(a, b):
     return float(np.sqrt(np.mean(np.sum((np.asarray(a, np.float64) - b) ** 2, -1))))
 
 
-def registration(args, pair):
-    """Plain (cold-start) pyramid registration as run_BINDER runs it, then
-    its time to tolerance: the wall-clock of the same pipeline stopped at
-    the first last-level EM iteration whose field is within 0.1 voxel RMS of
-    the converged field (the pipeline is deterministic, so the truncated run
+def registration(args, pair, net=None):
+    """Pyramid registration as run_BINDER runs it -- from the identity (the
+    plain cold start) or, with `net`, from the learned warm start
+    (FastRegistration.run_warm) -- then its time to tolerance: the
+    wall-clock of the same pipeline stopped at the first last-level EM
+    iteration whose field is within 0.1 voxel RMS of the pipeline's
+    converged field (the pipeline is deterministic, so the truncated run
     retraces the converged one)."""
     from binder_amd.FastRegistration import FastRegistration
 
-    levels = [2, 1] if args.quick else list(args.resolutions)
+    warm = net is not None
+    if warm:
+        levels = list(args.warm_levels)
+    else:
+        levels = [2, 1] if args.quick else list(args.resolutions)
     iters = 3 if args.quick else args.em_iters
     kw = dict(REG_KW, convergence_th=args.convergence_th)
     if args.quick:
         kw["max_outer_it"] = 0
+
+    def new():
+        return FastRegistration(voxels=pair.fixed, nodes=pair.moving, **kw)
+
+    def run(fr, level_iterations=None):
+        if warm:
+            return fr.run_warm(net, levels, iters, level_iterations=level_iterations)
+        fr.run(levels, iters, level_iterations=level_iterations)
+        return True
+
     # A: to convergence (the reference's "Time for EM")
-    A = FastRegistration(voxels=pair.fixed, nodes=pair.moving, **kw)
+    A = new()
     t0 = time.perf_counter()
-    A.run(levels, iters)
+    used = run(A)
     conv_s = time.perf_counter() - t0
     F = np.asarray(A.deformedVoxelPositions, np.float64)
+    ran = [int(lv[0]) for lv in A.level_log]
     counts = [int(lv[1]) for lv in A.level_log]
-    log(f"pyramid {levels}: iterations {counts} in {conv_s:.3f}s")
-    out = {"resolutions": levels, "converged_s": round(conv_s, 4),
+    log(f"{'warm' if warm else 'cold'} pyramid {ran}: iterations {counts} in {conv_s:.3f}s")
+    out = {"resolutions": ran, "converged_s": round(conv_s, 4),
            "converged_iterations": counts,
            "level_seconds": [round(lv[2], 4) for lv in A.level_log]}
+    if warm:
+        out["warm_start_used"] = bool(used)
+        out["start_objectives_warm_identity"] = [round(v, 2) for v in A.warm_objectives]
     if args.quick:
         return A, F, out, F
     # B: retrace the last level in chunks to find the first iteration within
     # tolerance (untimed; the field is read back after every chunk)
-    B = FastRegistration(voxels=pair.fixed, nodes=pair.moving, **kw)
-    B.run(levels[:-1], iters, level_iterations=counts[:-1])
-    B.move_resolution(levels[-1])
-    reg = B._new_registration(np.ascontiguousarray(B.deformedVoxelPositions))
+    B = new()
+    if warm and used and len(ran) == 1:
+        B.move_resolution(ran[0], resize_field=False)
+        reg = B._warm_registration(A.warm_coef, ran[0])
+    else:
+        if warm and used:
+            B.run_warm(net, ran[:-1], iters, level_iterations=counts[:-1], compare_iterations=0)
+        else:
+            B.run(ran[:-1], iters, level_iterations=counts[:-1])
+        B.move_resolution(ran[-1])
+        reg = B._new_registration(np.ascontiguousarray(B.deformedVoxelPositions))
     reg.setMILikelihood(B.alpha, np.ascontiguousarray(B.theta).ravel(),
                         B.num_node_intensity_levels, B.num_voxel_intensity_levels, 1,
                         np.ascontiguousarray(B.binned_nodes).ravel(),
@@ -130,9 +156,9 @@ def registration(args, pair):
             k_tol = k
             break
     # C: the timed pipeline, stopped there (topology correction included)
-    C = FastRegistration(voxels=pair.fixed, nodes=pair.moving, **kw)
+    C = new()
     t0 = time.perf_counter()
-    C.run(levels, iters, level_iterations=counts[:-1] + [k_tol])
+    run(C, level_iterations=counts[:-1] + [k_tol])
     tt_s = time.perf_counter() - t0
     rms_c = _rms(C.deformedVoxelPositions, F)
     log(f"time to tolerance: {tt_s:.3f}s ({k_tol} of {counts[-1]} last-level iterations, "
@@ -141,8 +167,171 @@ def registration(args, pair):
                 "reference_field": "this pipeline's converged field (its FP64-oracle "
                                    "agreement is a GPU test on a reduced grid)",
                 "gpu_s": round(tt_s, 4), "iterations": counts[:-1] + [k_tol],
-                "rms_vox_at_stop": round(rms_c, 4)})
+                "rms_vox_at_stop": round(rms_c, 4),
+                "final_objective": round(float(A.registratorObj.get_objective_trace()[-1, 3]),
+                                         2)})
     return A, F, out, np.asarray(C.deformedVoxelPositions, np.float64)
+
+
+def inverse_warp_rms(pair, F, step=4, iterations=30):
+    """RMS distance (voxels) between a field F and the exact answer of the
+    synthetic pair, on every `step`-th voxel per axis: the moving image is
+    the fixed image sampled at y + u(y), so fixed voxel x corresponds to the
+    moving position y with y + u(y) = x (fixed-point iteration y = x - u(y),
+    trilinear u)."""
+    from scipy import ndimage
+    u = pair.displacement
+    shape = u.shape[:-1]
+    x = np.stack(np.meshgrid(*[np.arange(0, n, step, dtype=np.float64) for n in shape],
+                             indexing="ij"), -1)
+    y = x.copy()
+    for _ in range(iterations):
+        uy = np.stack([ndimage.map_coordinates(u[..., a], np.moveaxis(y, -1, 0), order=1,
+                                               mode="nearest") for a in range(len(shape))], -1)
+        y = x - uy
+    Fs = np.asarray(F, np.float64)[tuple(slice(0, n, step) for n in shape)]
+    return _rms(Fs, y)
+
+
+def load_warm_net(args):
+    from binder_amd import warmstart
+    path = args.warm_weights or warmstart.WEIGHTS_C2
+    if not os.path.exists(path):
+        return None, path, {}
+    net = warmstart.WarmStartNet.load(path, max_batch=1)
+    with np.load(path, allow_pickle=False) as z:
+        meta = {"weights": os.path.relpath(path, ROOT), "params": int(z["params"].size),
+                "train_pairs": int(z["train_seeds"].size) if "train_seeds" in z else None,
+                "holdout_pairs": int(z["holdout_seeds"].size) if "holdout_seeds" in z else None}
+        if "history" in z and z["history"].size:
+            h = z["history"]
+            meta.update({"epochs": int(h[-1, 0]) + 1, "train_loss": round(float(h[-1, 1]), 6),
+                         "holdout_loss": round(float(h[-1, 2]), 6)})
+        if "identity_loss" in z:
+            meta["holdout_loss_of_identity"] = round(float(z["identity_loss"]), 6)
+        meta["holdout_seeds"] = z["holdout_seeds"].tolist() if "holdout_seeds" in z else []
+    return net, path, meta
+
+
+def warm_start_comparison(args, pair, F_cold, cold_out):
+    """The learned warm start on the same pair: its time to tolerance (to
+    within 0.1 voxel RMS of its own converged field), against the cold
+    start's, and its converged field judged against the cold start's."""
+    net, path, meta = load_warm_net(args)
+    if net is None:
+        log(f"no trained warm-start network at {path}")
+        return {"status": f"no trained network at {os.path.relpath(path, ROOT)}"}
+    meta.pop("holdout_seeds", None)
+    # inference time of the network alone (host bins -> coefficients)
+    from binder_amd import warmstart
+    f, m = warmstart.pair_inputs(pair.binned_fixed, pair.binned_moving, net.grid)
+    net.predict(f, m)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        net.predict(f, m)
+    infer_ms = 1e3 * (time.perf_counter() - t0) / 10
+    A, Fw, out, _ = registration(args, pair, net=net)
+    out["network"] = meta
+    out["inference_ms"] = round(infer_ms, 3)
+    out["rms_vs_cold_converged_vox"] = round(_rms(Fw, F_cold), 4)
+    out["rms_vs_exact_vox"] = round(inverse_warp_rms(pair, Fw), 4)
+    out["start_rms_vs_cold_converged_vox"] = round(
+        _rms(warmstart.coefficients_field(A.warm_coef, F_cold.shape[:3]), F_cold), 4)
+    if "gpu_s" in out and "gpu_s" in cold_out:
+        out["speedup_vs_cold"] = round(cold_out["gpu_s"] / out["gpu_s"], 3)
+    log(f"warm start: {out.get('gpu_s')}s to tolerance vs cold {cold_out.get('gpu_s')}s; "
+        f"converged field {out['rms_vs_cold_converged_vox']} vox RMS from the cold one")
+    return out
+
+
+def warmstart_report(args):
+    """Warm against cold start over held-out pairs of the training
+    generator (seeds the network never saw)."""
+    from concurrent.futures import ProcessPoolExecutor
+    import multiprocessing as mp
+    from binder_amd import warmstart_train
+
+    net, path, meta = load_warm_net(args)
+    if net is None:
+        return {"report": "warmstart", "status": f"no trained network at {path}"}
+    seeds = meta.pop("holdout_seeds")[:args.warmstart_eval]
+    rows = []
+    with ProcessPoolExecutor(max_workers=4, mp_context=mp.get_context("spawn")) as ex:
+        for seed, pair in zip(seeds, ex.map(warmstart_train.make_pair, seeds)):
+            _, Fc, cold, _ = registration(args, pair)
+            _, Fw, warm, _ = registration(args, pair, net=net)
+            row = {"seed": int(seed), "cold_s": cold["gpu_s"], "warm_s": warm["gpu_s"],
+                   "cold_iterations": cold["iterations"], "warm_iterations": warm["iterations"],
+                   "warm_used": warm["warm_start_used"],
+                   "rms_warm_vs_cold_vox": round(_rms(Fw, Fc), 4),
+                   "cold_vs_exact_vox": round(inverse_warp_rms(pair, Fc), 4),
+                   "warm_vs_exact_vox": round(inverse_warp_rms(pair, Fw), 4),
+                   "cold_objective": cold["final_objective"],
+                   "warm_objective": warm["final_objective"]}
+            rows.append(row)
+            log(f"held-out seed {seed}: cold {row['cold_s']:.3f}s, warm {row['warm_s']:.3f}s "
+                f"(used {row['warm_used']}), fields {row['rms_warm_vs_cold_vox']} vox apart")
+    cs = np.array([r["cold_s"] for r in rows])
+    ws = np.array([r["warm_s"] for r in rows])
+    summary = {"pairs": len(rows), "cold_s_median": float(np.median(cs)),
+               "warm_s_median": float(np.median(ws)),
+               "speedup_of_medians": float(np.median(cs) / np.median(ws)),
+               "speedup_total": float(cs.sum() / ws.sum()),
+               "speedup_per_pair_median": float(np.median(cs / ws)),
+               "warm_used_fraction": float(np.mean([r["warm_used"] for r in rows])),
+               "rms_warm_vs_cold_median_vox": float(np.median(
+                   [r["rms_warm_vs_cold_vox"] for r in rows])),
+               "cold_vs_exact_median_vox": float(np.median(
+                   [r["cold_vs_exact_vox"] for r in rows])),
+               "warm_vs_exact_median_vox": float(np.median(
+                   [r["warm_vs_exact_vox"] for r in rows])),
+               "warm_objective_lower_fraction": float(np.mean(
+                   [r["warm_objective"] <= r["cold_objective"] for r in rows]))}
+    return {"report": "warmstart", "network": meta, "warm_levels": list(args.warm_levels),
+            "tolerance_vox_rms": TOLERANCE_VOX_RMS, "summary": summary, "pairs": rows}
+
+
+def batched_report(args):
+    """Many pairs and many chains: `--batched N` pairs of uneven shapes
+    (axes uniform on {96..192}, warp RMS U(2,6)) registered with several
+    pairs in flight, then 16 independent chains per pair of 1e6 site
+    updates each from the registered state (graph-captured sweeps).
+    Reports pairs/s and chain voxel-updates/s, and the same work one pair
+    at a time for comparison."""
+    from concurrent.futures import ProcessPoolExecutor
+    import multiprocessing as mp
+    from binder_amd import batch
+
+    specs = batch.batch_specs(args.batched, seed=args.seed)
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=16, mp_context=mp.get_context("spawn")) as ex:
+        pairs = list(ex.map(batch.make_pair, specs))
+    log(f"{len(pairs)} pairs generated in {time.perf_counter() - t0:.1f}s")
+    nvox = [int(np.prod(sp["shape"])) for sp in specs]
+    out = {"report": "batched", "pairs": len(pairs), "workers": args.batch_workers,
+           "shapes": [list(sp["shape"]) for sp in specs],
+           "voxels_total": int(sum(nvox))}
+    # warm-up: one pair (library set-up, first kernel loads)
+    batch.register_pairs(pairs[:1], workers=1)
+    regs, dt = batch.register_pairs(pairs, workers=args.batch_workers)
+    out["registration"] = {"seconds": round(dt, 3), "pairs_per_s": round(len(pairs) / dt, 2),
+                           "iterations": [[int(lv[1]) for lv in r.level_log] for r in regs]}
+    log(f"registration: {len(pairs)} pairs in {dt:.2f}s ({len(pairs) / dt:.1f} pairs/s)")
+    k = min(len(pairs), args.batch_serial)
+    _, dts = batch.register_pairs(pairs[:k], workers=1)
+    out["registration"]["one_at_a_time_pairs_per_s"] = round(k / dts, 2)
+    chains, upd = 16, 1e6
+    for use_graph in (True, False):
+        n, dtc = batch.chains_for_pairs(regs, chains, upd, workers=args.batch_workers,
+                                        use_graph=use_graph)
+        key = "chains_graph" if use_graph else "chains_eager"
+        out[key] = {"chains_per_pair": chains, "site_updates_per_chain": upd,
+                    "voxel_updates": n, "seconds": round(dtc, 3),
+                    "voxel_updates_per_s": round(n / dtc, 1)}
+        log(f"{key}: {n:.3e} voxel updates in {dtc:.2f}s ({n / dtc:.3e}/s)")
+    n1, dt1 = batch.chains_for_pairs(regs[:k], chains, upd, workers=1, use_graph=True)
+    out["chains_graph"]["one_pair_at_a_time_voxel_updates_per_s"] = round(n1 / dt1, 1)
+    return out
 
 
 def cpu_em_estimate(pair, args, counts, levels):
@@ -199,6 +388,11 @@ def run(args):
     reg_out["fold_detect_s"] = round(fold_s, 5)
     disp_rms = float(np.sqrt(np.mean(np.sum((F - prob.voxel_pos) ** 2, -1))))
     reg_out["displacement_rms_vox"] = round(disp_rms, 4)
+
+    reg_out["rms_vs_exact_vox"] = round(inverse_warp_rms(pair, F), 4)
+    warm_out = None
+    if not args.no_warm and not args.quick:
+        warm_out = warm_start_comparison(args, pair, F, reg_out)
 
     def full_res_handle():
         h = A._new_registration(np.ascontiguousarray(F))
@@ -278,6 +472,7 @@ def run(args):
                    "alpha": prob.alpha, "seed": args.seed},
         "roofline": roof,
         "registration": reg_out,
+        "warm_start": warm_out,
         "em_full_res": {"ms_per_iteration": round(em_it_ms, 4), "kernel_avg_us": em_k},
         "kernels": kernels,
         "input_gen_s": round(gen_s, 2),
@@ -487,12 +682,30 @@ def main():
     ap.add_argument("--c1", action="store_true",
                     help="only the 2D 192x160 registration report (GPU vs FP64 oracle)")
     ap.add_argument("--precision-sweeps", type=int, default=200)
+    ap.add_argument("--warm-levels", type=int, nargs="+", default=[1],
+                    help="pyramid levels run from the learned warm start")
+    ap.add_argument("--warm-weights", default=None,
+                    help="trained warm-start network (default: the packaged weights)")
+    ap.add_argument("--no-warm", action="store_true", help="skip the warm-start comparison")
+    ap.add_argument("--warmstart-eval", type=int, default=0, metavar="N",
+                    help="only the warm-vs-cold report over N held-out pairs")
+    ap.add_argument("--batched", type=int, default=0, metavar="N",
+                    help="only the many-pairs / many-chains report over N pairs")
+    ap.add_argument("--batch-workers", type=int, default=8)
+    ap.add_argument("--batch-serial", type=int, default=8,
+                    help="pairs of the one-at-a-time comparison")
     args = ap.parse_args()
     if args.precision:
         print(json.dumps(precision_report(args)), flush=True)
         return
     if args.c1:
         print(json.dumps(c1_report(args)), flush=True)
+        return
+    if args.warmstart_eval:
+        print(json.dumps(warmstart_report(args)), flush=True)
+        return
+    if args.batched:
+        print(json.dumps(batched_report(args)), flush=True)
         return
     if args.copy_calibration:
         copy_calibration()

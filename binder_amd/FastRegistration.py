@@ -170,8 +170,7 @@ class FastRegistration:
                 self.full_voxels_pos = self.full_voxels_pos + (com_nodes - com_voxels)
 
         self.current_resolution = 1
-        self.voxels_pos = self.full_voxels_pos
-        self.nodes_pos = self.full_nodes_pos
+        self._level_pos = {}
         self.binned_voxels = self.full_binned_voxels
         self.binned_nodes = self.full_binned_nodes
         self.nodes_shape = self.orig_nodes_shape
@@ -183,6 +182,28 @@ class FastRegistration:
         self._set_gammas(self.voxels_shape)
         # per-level record: (resolution, EM iterations, seconds)
         self.level_log = []
+
+    # positions of the current level (full[::res] / res), made on first use:
+    # the GPU pyramid generates the index grid on the device and never needs
+    # them, and the strided copy costs tens of milliseconds at 160x192x160
+    def _level_positions(self, which):
+        res = self.current_resolution
+        full = self.full_voxels_pos if which == "v" else self.full_nodes_pos
+        if res == 1:
+            return full
+        key = (which, res)
+        if key not in self._level_pos:
+            self._level_pos = {k: v for k, v in self._level_pos.items() if k[1] == res}
+            self._level_pos[key] = full[::res, ::res, ::res, :] / res
+        return self._level_pos[key]
+
+    @property
+    def voxels_pos(self):
+        return self._level_positions("v")
+
+    @property
+    def nodes_pos(self):
+        return self._level_positions("n")
 
     def _set_gammas(self, shape_for_scaling):
         # gamma per voxel at 1 mm, scaled by voxel volume, voxel count and the
@@ -206,15 +227,20 @@ class FastRegistration:
         chain = hasattr(cls, "resampled_from") and \
             (not self.top_cor_obj or hasattr(cls, "correct_topology"))
         prev = None  # the previous level's registration, whose field is on the GPU
+        warm, self._warm = getattr(self, "_warm", None), None
         for li, resolution in enumerate(resolutions):
             res, cur = int(resolution), self.current_resolution
             # ... unless the frequency-domain up-sampling is asked for
             # (FastRegistration.py:566-580)
             on_gpu = chain and prev is not None and \
                 not (self.upsample_in_frequency_domain and cur - res >= 0 and res != cur)
-            self.move_resolution(resolution, resize_field=not on_gpu)
+            start_warm = warm is not None and li == 0
+            self.move_resolution(resolution, resize_field=not on_gpu and not start_warm)
             t0 = time.perf_counter()
-            if on_gpu:
+            if start_warm:
+                # the first level starts from the predicted field (run_warm)
+                self.registratorObj = self._warm_registration(warm, res)
+            elif on_gpu:
                 vp = np.ascontiguousarray(self.voxels_pos)
                 # full[::res] / res of the index grid is the index grid again
                 self.registratorObj = Registration.Registration.resampled_from(
@@ -261,6 +287,75 @@ class FastRegistration:
             iters = len(self.registratorObj.get_objective_trace())
             prev = self.registratorObj
             self.level_log.append((resolution, iters, time.perf_counter() - t0))
+
+    def _warm_registration(self, coef, res):
+        """Registration at the current level whose field is the warm start
+        predicted for this pair: the voxel positions plus the predicted
+        low-frequency displacement (Registration.from_dct, evaluated on the
+        GPU; a host evaluation for backends without it)."""
+        cls = Registration.Registration
+        sh = self.voxels_shape
+        if hasattr(cls, "from_dct"):
+            vp = None if self._index_grid else np.ascontiguousarray(self.voxels_pos).ravel()
+            r = cls.from_dct(coef, self.orig_voxels_shape, res, vp, sh[0], sh[1], sh[2],
+                             self.nodes_shape[0], self.nodes_shape[1], self.nodes_shape[2],
+                             self.spline_order, self.non_zero_voxels)
+            self._set_nonlinear(r)
+            return r
+        from .warmstart import coefficients_field
+        start = coefficients_field(coef, self.orig_voxels_shape, res)
+        start = start - np.stack(np.indices(sh, dtype=np.float64), -1) + self.voxels_pos
+        return self._new_registration(start)
+
+    def _start_objective(self, coef, res, iterations):
+        """Objective (mlp) after `iterations` EM iterations at level `res`
+        from the warm start (coef) or, for coef None, from the identity."""
+        r = self._warm_registration(coef, res) if coef is not None else \
+            self._new_registration(np.ascontiguousarray(self.voxels_pos))
+        r.setMILikelihood(self.alpha, np.ascontiguousarray(self.theta).ravel(),
+                          self.num_node_intensity_levels, self.num_voxel_intensity_levels,
+                          self.threads, np.ascontiguousarray(self.binned_nodes).ravel(),
+                          np.ascontiguousarray(self.binned_voxels).ravel())
+        r.EM(int(iterations), -1e30, update_likelihood_parameters=1, debug=0)
+        return float(r.get_objective_trace()[-1, 3])
+
+    def run_warm(self, net, resolutions=(1,), max_number_of_iterations=1000,
+                 level_iterations=None, fallback_resolutions=(8, 4, 2, 1),
+                 compare_iterations=2, compare_resolution=8):
+        """Registration from a learned warm start (binder_amd.warmstart; not
+        in the reference).  The network predicts the low-frequency field from
+        this pair's half-resolution bins; the first level of `resolutions`
+        starts from it instead of the identity, and the plain EM then runs
+        to convergence at every level exactly as run() does.
+
+        Fallback: the objective (mlp) after `compare_iterations` EM
+        iterations at level `compare_resolution` is computed from the warm
+        start and from the identity; if the warm start's is worse (higher),
+        the plain cold pyramid `fallback_resolutions` runs instead
+        (level_iterations then applies to it).  Returns True when the warm
+        start was used."""
+        if self.number_of_dimensions != 3:
+            raise ValueError("the learned warm start is defined for 3D registrations")
+        coef = net.predict_pair(self.full_binned_voxels, self.full_binned_nodes,
+                                self.num_voxel_intensity_levels, self.num_node_intensity_levels)
+        self.warm_coef = coef
+        res0 = int(resolutions[0])
+        used = True
+        if compare_iterations:
+            cr = int(compare_resolution)
+            self.move_resolution(cr, resize_field=False)
+            self.warm_objectives = (self._start_objective(coef, cr, compare_iterations),
+                                    self._start_objective(None, cr, compare_iterations))
+            used = self.warm_objectives[0] <= self.warm_objectives[1]
+        self.warm_start_used = used
+        if not used:
+            self.move_resolution(1, resize_field=False)
+            self.run(fallback_resolutions, max_number_of_iterations,
+                     level_iterations=level_iterations)
+            return False
+        self._warm = coef
+        self.run(resolutions, max_number_of_iterations, level_iterations=level_iterations)
+        return True
 
     def correct_topology(self, positions):
         """Topology correction of a field at the current resolution, as run()
@@ -310,13 +405,10 @@ class FastRegistration:
         self.current_nodes_voxel_size = self.nodes_voxel_size * res
         self.voxels_shape = tuple(int(np.ceil(x / res)) for x in self.orig_voxels_shape)
         self.nodes_shape = np.array([int(np.ceil(x / res)) for x in self.orig_nodes_shape])
-        if res == 1:  # x / 1 == x: no copy of the full-resolution grids
-            self.voxels_pos, self.nodes_pos = self.full_voxels_pos, self.full_nodes_pos
-        else:
-            self.voxels_pos = self.full_voxels_pos[::res, ::res, ::res, :] / res
-            self.nodes_pos = self.full_nodes_pos[::res, ::res, ::res, :] / res
-
         cur = self.current_resolution
+        # voxels_pos / nodes_pos now refer to level `res` (made on first use)
+        self.current_resolution = res
+        self._level_pos = {}
         if not resize_field:
             pass
         elif not self.upsample_in_frequency_domain or cur - res < 0:
@@ -346,7 +438,6 @@ class FastRegistration:
         self.non_zero_voxels = np.count_nonzero(self.binned_voxels)
         self._set_gammas(self.orig_voxels_shape if self.increase_gamma_at_DS_resolution
                          else self.voxels_shape)
-        self.current_resolution = res
 
     def sample_param(self, burnin, samples, sample_gamma=0, sample_gamma_every=1,
                      sample_posteriors=1, sample_likelihood_parameters=1,

@@ -52,6 +52,8 @@ _reg_em = _sig("binder_registration_em", _c_int, _vp, _c_int, _c_double, _c_int,
 _reg_sampler = _sig("binder_registration_sampler", _c_int, _vp, *([_c_int] * 8), _vp, _vp)
 _reg_sampler_run = _sig("binder_registration_sampler_run", _c_int, _vp, *([_c_int] * 8))
 _reg_sampler_result = _sig("binder_registration_sampler_result", _c_int, _vp, _vp, _vp)
+_reg_sampler_chains = _sig("binder_registration_sampler_chains", _c_int, _vp, _c_int, _vp,
+                           *([_c_int] * 6))
 _reg_get_final = _sig("binder_registration_get_final_locations", _c_int, _vp, _vp)
 _reg_get_dream = _sig("binder_registration_get_dream_locations", _c_int, _vp, _vp)
 _reg_set_dream = _sig("binder_registration_set_dream_locations", _c_int, _vp, _vp)
@@ -87,6 +89,9 @@ _topo3 = _sig("binder_topology_correction_3d", _c_int, _vp, _c_double, _c_double
 _philox = _sig("binder_philox4x32_10", _c_int, _vp, _vp, _vp)
 _reg_set_dct = _sig("binder_registration_set_field_from_dct", _c_int, _vp, _vp, _c_int, _c_int,
                     _c_int, _c_int)
+_reg_create_dct = _sig("binder_registration_create_from_dct", _c_int, _P(_vp), _vp, _c_int,
+                       _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _c_int, _c_int,
+                       _c_int, _c_int, _c_double)
 
 
 _dct_prior = _sig("binder_dct_prior_eigenvalues", _c_int, _c_int, _c_int, _c_int, _c_int,
@@ -187,6 +192,35 @@ class Registration:
         self._KL = None
         return self
 
+    @classmethod
+    def from_dct(cls, coef, full_shape, resolution, voxel_pos, Nx_v, Ny_v, Nz_v, Nx_n, Ny_n,
+                 Nz_n, spline_order, non_zero_voxels):
+        """A Registration whose field starts at the learned warm start of a
+        pyramid level (see set_field_from_dct), made on the GPU without a
+        field upload.  voxel_pos=None stands for the grid indices.  No
+        reference counterpart."""
+        self = cls.__new__(cls)
+        self._h = None
+        c = _arr(coef, np.float32)
+        if c.size != 3 * 512:
+            raise ValueError("coef must hold 3 x 8 x 8 x 8 values")
+        vp = None
+        if voxel_pos is not None:
+            vp = _arr(voxel_pos, np.float32)
+            if vp.size != int(Nx_v) * int(Ny_v) * int(Nz_v) * 3:
+                raise ValueError("voxel_pos has the wrong size")
+        fx, fy, fz = (int(n) for n in full_shape)
+        h = _vp()
+        _check(_reg_create_dct(ctypes.byref(h), _ptr(c), fx, fy, fz, int(resolution),
+                               None if vp is None else _ptr(vp), int(Nx_v), int(Ny_v),
+                               int(Nz_v), int(Nx_n), int(Ny_n), int(Nz_n), int(spline_order),
+                               float(non_zero_voxels)))
+        self._h = h
+        self._shape = (int(Nx_v), int(Ny_v), int(Nz_v), 3)
+        self._nn = int(Nx_n) * int(Ny_n) * int(Nz_n)
+        self._KL = None
+        return self
+
     def __del__(self):
         if getattr(self, "_h", None):
             _reg_destroy(self._h)
@@ -238,6 +272,18 @@ class Registration:
                                 int(sample_gamma_every), int(sample_posteriors),
                                 int(sample_likelihood_parameters),
                                 int(sample_transformation_parameters)))
+
+    def sampler_chains(self, seeds, N_b, N_s, sample_posteriors=1,
+                       sample_likelihood_parameters=1, sample_transformation_parameters=1,
+                       use_graph=True):
+        """Independent chains (one per seed) from the current state, pooled
+        into one mean/M2 kept on the device (see sampler_result); with
+        use_graph the sweeps of chains 1.. replay one captured HIP graph.
+        No reference counterpart (the reference's sampler() is one chain)."""
+        sd = np.ascontiguousarray(seeds, np.int32)
+        _check(_reg_sampler_chains(self._h, int(sd.size), _ptr(sd), int(N_b), int(N_s),
+                                   int(sample_posteriors), int(sample_likelihood_parameters),
+                                   int(sample_transformation_parameters), int(bool(use_graph))))
 
     def sampler_result(self):
         Nx, Ny, Nz, N = self._shape

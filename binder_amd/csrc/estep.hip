@@ -32,6 +32,8 @@
 // k_estep, which merges runs of equal bins in registers before each LDS
 // atomic.  The sampling E-step counts its one drawn
 // node per voxel in a 32-bit LDS count array.
+#include <atomic>
+
 #include "estep.hpp"
 
 namespace binder {
@@ -172,6 +174,10 @@ __device__ __forceinline__ void slice_columns(const uint32_t q[16], int xa,
 // x-slice (16 or 4 nodes) at a time, so only a slice's packed bins are live.
 template <bool IS2D, bool SAMPLE>
 __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
+  if (a.keyx) {  // per-chain key mask (captured sweeps)
+    a.k0 ^= a.keyx[0];
+    a.k1 ^= a.keyx[1];
+  }
   extern __shared__ __align__(16) unsigned char smem[];
   const int KL = a.K * a.L;
   unsigned long long *hist = reinterpret_cast<unsigned long long *>(smem);
@@ -421,6 +427,10 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
 // selected column.  LDS: theta rows (padded stride) | 32-bit counts [K*L].
 template <bool IS2D>
 __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
+  if (a.keyx) {  // per-chain key mask (captured sweeps)
+    a.k0 ^= a.keyx[0];
+    a.k1 ^= a.keyx[1];
+  }
   extern __shared__ __align__(16) unsigned char smem[];
   const int KL = a.K * a.L;
   const int Kp = theta_row_stride(a.K, a.L);
@@ -705,6 +715,10 @@ __global__ __launch_bounds__(256) void k_hist_reduce2(
 // Dirichlet(H + alpha) draw, or no change; then the Dirichlet prior cost and
 // the transposed FP32 table.
 __global__ __launch_bounds__(1024) void k_theta(ThetaArgs a) {
+  if (a.keyx) {  // per-chain key mask (captured sweeps)
+    a.k0 ^= a.keyx[0];
+    a.k1 ^= a.keyx[1];
+  }
   const int KL = a.K * a.L;
   const int tid = threadIdx.x;
   __shared__ double red[1024];
@@ -846,7 +860,7 @@ void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
     const size_t lds32 = (size_t)a.K * a.L * sizeof(unsigned long long) +
                          (size_t)((Kp * a.L + 3) & ~3) * sizeof(float) +
                          (size_t)(kEm32Threads / 64) * kEm32MaxK * 64 * sizeof(float);
-    static bool attr_set[2] = {false, false};
+    static std::atomic<bool> attr_set[2] = {false, false};
     auto k = is2d ? k_estep_em32<true> : k_estep_em32<false>;
     if (!attr_set[is2d]) {
       HIP_OK(hipFuncSetAttribute((const void *)k,

@@ -37,6 +37,9 @@ struct EStepArgs {
   unsigned long long *hpart;  // [blocks][K*L] out
   double *mllpart;            // [blocks] out
   uint32_t k0, k1, sweep;     // sampling keys
+  // optional device-resident key mask: the key used is (k0, k1) ^ keyx[0..1]
+  // (lets one captured sweep graph serve chains with different seeds)
+  const uint32_t *keyx;
 };
 
 void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
@@ -60,6 +63,7 @@ struct ThetaArgs {
   int mode;
   double *stats;        // stats[0] = mll, stats[1] = prior_theta
   uint32_t k0, k1, sweep;
+  const uint32_t *keyx;  // optional key mask, as in EStepArgs
 };
 
 void launch_theta(const ThetaArgs &a, hipStream_t s);

@@ -82,6 +82,37 @@ __global__ __launch_bounds__(256) void k_check_index_grid(const float *__restric
   if (!ok) *flag = 0;
 }
 
+// x[i] = 0 (a kernel rather than hipMemsetAsync, so that it can sit in a
+// captured graph)
+__global__ __launch_bounds__(256) void k_zero_f64(double *__restrict__ x, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    x[i] = 0.0;
+}
+
+// Pooled mean / M2 of two sample sets (Chan, Golub & LeVeque 1979):
+// n = na + nb, delta = mean_b - mean_a, mean = mean_a + delta nb / n,
+// M2 = M2_a + M2_b + delta delta^T na nb / n.  Upper-triangular M2 planes
+// in the order (0,0), (0,1), .., (1,1), ..
+__global__ __launch_bounds__(256) void k_chan_merge(double *__restrict__ mean,
+                                                    double *__restrict__ m2,
+                                                    const double *__restrict__ mb,
+                                                    const double *__restrict__ m2b,
+                                                    double na, double nb, long long nv, int N) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const double n = na + nb;
+  double d[3];
+  for (int c = 0; c < N; ++c) {
+    d[c] = mb[c * nv + i] - mean[c * nv + i];
+    mean[c * nv + i] += d[c] * (nb / n);
+  }
+  int p = 0;
+  for (int a = 0; a < N; ++a)
+    for (int b = a; b < N; ++b, ++p)
+      m2[p * nv + i] += m2b[p * nv + i] + d[a] * d[b] * (na * nb / n);
+}
+
 int grid_blocks(long long n) {
   const long long b = (n + 255) / 256;
   return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
@@ -214,8 +245,10 @@ struct binder_registration {
     return a;
   }
 
-  void estep_only(bool sample, uint32_t k0, uint32_t k1, uint32_t sweep) {
+  void estep_only(bool sample, uint32_t k0, uint32_t k1, uint32_t sweep,
+                  const uint32_t *keyx = nullptr) {
     EStepArgs a = estep_args(sample, k0, k1, sweep);
+    a.keyx = keyx;
     TimedLaunch t(tm(), sample ? "estep_sample" : "estep", s());
     launch_estep(a, is2d, sample, estep_blocks, s());
   }
@@ -234,7 +267,7 @@ struct binder_registration {
   }
 
   void run_theta(int mode, uint32_t k0, uint32_t k1, uint32_t sweep,
-                 hipStream_t st = nullptr) {
+                 hipStream_t st = nullptr, const uint32_t *keyx = nullptr) {
     if (!st) st = s();
     ThetaArgs t{};
     t.Hd = Hd.get();
@@ -256,9 +289,70 @@ struct binder_registration {
     t.stats = stats.get();
     t.k0 = k0;
     t.k1 = k1;
+    t.keyx = keyx;
     t.sweep = sweep;
     TimedLaunch tl(tm(), mode == kThetaDirichlet ? "dirichlet" : "theta", st);
     launch_theta(t, st);
+  }
+
+  // Enqueue N_b + N_s sampling sweeps (E-draw, Dirichlet draw, deformation
+  // draw) with Philox key (k0, k1) [^ keyx], folding the N_s samples into the
+  // FP64 accumulators (mean_p, m2_p) with Welford (they must hold zeros).
+  // Nothing here synchronizes with the host or uses hipMemset, so the whole
+  // sequence can be captured into a graph.
+  void enqueue_sweeps(int N_b, int N_s, int sample_gamma, int every, int sp, int sl, int st,
+                      uint32_t k0, uint32_t k1, const uint32_t *keyx, double *mean_p,
+                      double *m2_p) {
+    WelfordBatch wb{};
+    long long folded = 0;  // samples already in the accumulators
+    const bool overlap = sp && sl && st;
+    for (int sweep = 0; sweep < N_b + N_s; ++sweep) {
+      if (overlap) {
+        // E-draw; then the histogram reduction and the Dirichlet draw on the
+        // side stream while the main stream draws the deformation from the
+        // E-draw's dream locations; the next E-draw waits for theta
+        if (sweep > 0) HIP_OK(hipStreamWaitEvent(s(), theta_done.e, 0));
+        estep_only(true, k0, k1, (uint32_t)sweep, keyx);
+        HIP_OK(hipEventRecord(estep_done.e, s()));
+        HIP_OK(hipStreamWaitEvent(side_stream.s, estep_done.e, 0));
+        hist_reduce_on(side_stream.s);
+        run_theta(kThetaDirichlet, k0, k1, (uint32_t)sweep, side_stream.s, keyx);
+        HIP_OK(hipEventRecord(theta_done.e, side_stream.s));
+      } else {
+        if (sp) {
+          estep_only(true, k0, k1, (uint32_t)sweep, keyx);
+          hist_reduce_on(s());
+        }
+        if (sl) run_theta(kThetaDirichlet, k0, k1, (uint32_t)sweep, nullptr, keyx);
+      }
+      if (st) {
+        SmoothOptions o;
+        o.sample = true;
+        o.sample_gamma = sample_gamma && every > 0 && (sweep % every == 0);
+        o.key0 = k0;
+        o.key1 = k1;
+        o.keyx = keyx;
+        o.sweep = (uint32_t)sweep;
+        o.stream = kStreamDeform;
+        o.want_prior = false;  // the sampler reports no objective
+        const int next = (dslot + 1) % kWelfordBatch;
+        sm->run(dream.get(), x.get(), slot(next), o, tm());
+        dslot = next;
+      }
+      if (sweep >= N_b) {
+        // the sample stays in its ring slot until the batch is folded in,
+        // which happens before the ring wraps onto it
+        wb.x[wb.count++] = dcur();
+        if (wb.count == kWelfordBatch || sweep == N_b + N_s - 1) {
+          wb.n0 = folded;
+          TimedLaunch t(tm(), "welford", s());
+          launch_welford(wb, mean_p, m2_p, nv, N, s());
+          folded += wb.count;
+          wb.count = 0;
+        }
+      }
+    }
+    if (overlap) HIP_OK(hipStreamWaitEvent(s(), theta_done.e, 0));
   }
 };
 
@@ -355,6 +449,12 @@ std::unique_ptr<binder_registration> new_registration(const float *voxel_pos, in
   r->mllpart.alloc(blocks);
   HIP_OK(hipStreamSynchronize(r->s()));
   return r;
+}
+
+// Philox key of a sampler seed.
+void sampler_key(int seed, uint32_t &k0, uint32_t &k1) {
+  k0 = (uint32_t)seed;
+  k1 = 0x85EBCA6Bu ^ (uint32_t)((long long)seed >> 32);
 }
 
 }  // namespace
@@ -558,56 +658,117 @@ int binder_registration_sampler_run(binder_registration *r, int seed, int N_b,
     r->m2.alloc((size_t)npairs * r->nv);
     r->m2.zero(r->s());
     r->samples = N_s;
-    const uint32_t k0 = (uint32_t)seed;
-    const uint32_t k1 = 0x85EBCA6Bu ^ (uint32_t)((long long)seed >> 32);
+    uint32_t k0, k1;
+    sampler_key(seed, k0, k1);
     if (st && !r->dextra.get()) r->dextra.alloc((size_t)(kWelfordBatch - 1) * N * r->nv);
-    WelfordBatch wb{};
-    long long folded = 0;  // samples already in mean / m2
-    const bool overlap = sp && sl && st;
-    for (int sweep = 0; sweep < N_b + N_s; ++sweep) {
-      if (overlap) {
-        // E-draw; then the histogram reduction and the Dirichlet draw on the
-        // side stream while the main stream draws the deformation from the
-        // E-draw's dream locations; the next E-draw waits for theta
-        if (sweep > 0) HIP_OK(hipStreamWaitEvent(r->s(), r->theta_done.e, 0));
-        r->estep_only(true, k0, k1, (uint32_t)sweep);
-        HIP_OK(hipEventRecord(r->estep_done.e, r->s()));
-        HIP_OK(hipStreamWaitEvent(r->side_stream.s, r->estep_done.e, 0));
-        r->hist_reduce_on(r->side_stream.s);
-        r->run_theta(kThetaDirichlet, k0, k1, (uint32_t)sweep, r->side_stream.s);
-        HIP_OK(hipEventRecord(r->theta_done.e, r->side_stream.s));
-      } else {
-        if (sp) r->e_and_theta(true, -1, k0, k1, (uint32_t)sweep);
-        if (sl) r->run_theta(kThetaDirichlet, k0, k1, (uint32_t)sweep);
-      }
-      if (st) {
-        SmoothOptions o;
-        o.sample = true;
-        o.sample_gamma = sample_gamma && every > 0 && (sweep % every == 0);
-        o.key0 = k0;
-        o.key1 = k1;
-        o.sweep = (uint32_t)sweep;
-        o.stream = kStreamDeform;
-        o.want_prior = false;  // the sampler reports no objective
-        const int next = (r->dslot + 1) % kWelfordBatch;
-        r->sm->run(r->dream.get(), r->x.get(), r->slot(next), o, r->tm());
-        r->dslot = next;
-      }
-      if (sweep >= N_b) {
-        // the sample stays in its ring slot until the batch is folded in,
-        // which happens before the ring wraps onto it
-        wb.x[wb.count++] = r->dcur();
-        if (wb.count == kWelfordBatch || sweep == N_b + N_s - 1) {
-          wb.n0 = folded;
-          TimedLaunch t(r->tm(), "welford", r->s());
-          launch_welford(wb, r->mean.get(), r->m2.get(), r->nv, N, r->s());
-          folded += wb.count;
-          wb.count = 0;
+    r->enqueue_sweeps(N_b, N_s, sample_gamma, every, sp, sl, st, k0, k1, nullptr,
+                      r->mean.get(), r->m2.get());
+    HIP_OK(hipStreamSynchronize(r->s()));
+  });
+}
+
+int binder_registration_sampler_chains(binder_registration *r, int nchains, const int *seeds,
+                                       int N_b, int N_s, int sp, int sl, int st,
+                                       int use_graph) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && seeds, "null argument");
+    BINDER_REQUIRE(nchains >= 1 && N_b >= 0 && N_s >= 1,
+                   "need nchains >= 1, N_b >= 0 and N_s >= 1");
+    r->require_ready();
+    BINDER_REQUIRE(!(use_graph && r->timer.enabled()),
+                   "kernel timing cannot be recorded inside a captured graph");
+    const int N = r->N;
+    const int npairs = N * (N + 1) / 2;
+    const size_t nf = (size_t)N * r->nv, KL = (size_t)r->K * r->L;
+    r->mean.alloc(nf);
+    r->m2.alloc((size_t)npairs * r->nv);
+    DBuf<double> cmean, cm2, th0;
+    cmean.alloc(nf);
+    cm2.alloc((size_t)npairs * r->nv);
+    DBuf<float> d0, thT0;
+    d0.alloc(nf);
+    th0.alloc(KL);
+    thT0.alloc(KL);
+    if (st && !r->dextra.get()) r->dextra.alloc((size_t)(kWelfordBatch - 1) * nf);
+    // the start state every chain begins from
+    HIP_OK(hipMemcpyAsync(d0.get(), r->dcur(), nf * sizeof(float), hipMemcpyDeviceToDevice,
+                          r->s()));
+    HIP_OK(hipMemcpyAsync(th0.get(), r->theta.get(), KL * sizeof(double),
+                          hipMemcpyDeviceToDevice, r->s()));
+    HIP_OK(hipMemcpyAsync(thT0.get(), r->thT.get(), KL * sizeof(float),
+                          hipMemcpyDeviceToDevice, r->s()));
+    // chain c's key is sampler_key(seeds[c]): the same draws as
+    // sampler_run(seeds[c], ...).  The sweeps use key (0, 0) ^ kslot, and
+    // kslot is set per chain from the table, so one graph serves all chains.
+    std::vector<uint32_t> keys((size_t)2 * nchains);
+    for (int c = 0; c < nchains; ++c) sampler_key(seeds[c], keys[2 * c], keys[2 * c + 1]);
+    DBuf<uint32_t> ktab, kslot;
+    ktab.upload(keys.data(), keys.size(), r->s());
+    kslot.alloc(2);
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto release = [&] {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      exec = nullptr;
+      graph = nullptr;
+    };
+    auto restore = [&] {
+      HIP_OK(hipMemcpyAsync(r->slot(0), d0.get(), nf * sizeof(float), hipMemcpyDeviceToDevice,
+                            r->s()));
+      HIP_OK(hipMemcpyAsync(r->theta.get(), th0.get(), KL * sizeof(double),
+                            hipMemcpyDeviceToDevice, r->s()));
+      HIP_OK(hipMemcpyAsync(r->thT.get(), thT0.get(), KL * sizeof(float),
+                            hipMemcpyDeviceToDevice, r->s()));
+      r->dslot = 0;
+    };
+    try {
+      for (int c = 0; c < nchains; ++c) {
+        restore();
+        HIP_OK(hipMemcpyAsync(kslot.get(), ktab.get() + 2 * c, 2 * sizeof(uint32_t),
+                              hipMemcpyDeviceToDevice, r->s()));
+        double *acc_m = c == 0 ? r->mean.get() : cmean.get();
+        double *acc_2 = c == 0 ? r->m2.get() : cm2.get();
+        hipLaunchKernelGGL(k_zero_f64, dim3(1024), dim3(256), 0, r->s(), acc_m, (long long)nf);
+        hipLaunchKernelGGL(k_zero_f64, dim3(1024), dim3(256), 0, r->s(), acc_2,
+                           (long long)npairs * r->nv);
+        HIP_LAUNCH_OK();
+        if (!use_graph || c == 0) {
+          // chain 0 runs eagerly: it also builds every lazily made buffer and
+          // pass matrix, so that the capture below records kernels only
+          r->enqueue_sweeps(N_b, N_s, 0, 1, sp, sl, st, 0u, 0u, kslot.get(), acc_m, acc_2);
+        } else {
+          if (!exec) {
+            HIP_OK(hipStreamBeginCapture(r->s(), hipStreamCaptureModeThreadLocal));
+            r->enqueue_sweeps(N_b, N_s, 0, 1, sp, sl, st, 0u, 0u, kslot.get(), cmean.get(),
+                              cm2.get());
+            HIP_OK(hipStreamEndCapture(r->s(), &graph));
+            HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+          }
+          HIP_OK(hipGraphLaunch(exec, r->s()));
+        }
+        if (c > 0) {
+          hipLaunchKernelGGL(k_chan_merge, dim3(grid_blocks(r->nv)), dim3(256), 0, r->s(),
+                             r->mean.get(), r->m2.get(), cmean.get(), cm2.get(),
+                             (double)c * N_s, (double)N_s, r->nv, N);
+          HIP_LAUNCH_OK();
         }
       }
+      restore();
+      HIP_OK(hipStreamSynchronize(r->s()));
+    } catch (...) {
+      hipGraph_t dummy = nullptr;
+      hipStreamCaptureStatus st_cap = hipStreamCaptureStatusNone;
+      if (hipStreamIsCapturing(r->s(), &st_cap) == hipSuccess &&
+          st_cap == hipStreamCaptureStatusActive)
+        (void)hipStreamEndCapture(r->s(), &dummy);
+      if (dummy) (void)hipGraphDestroy(dummy);
+      (void)hipStreamSynchronize(r->s());
+      release();
+      throw;
     }
-    if (overlap) HIP_OK(hipStreamWaitEvent(r->s(), r->theta_done.e, 0));
-    HIP_OK(hipStreamSynchronize(r->s()));
+    release();
+    r->samples = nchains * N_s;
   });
 }
 
@@ -757,8 +918,26 @@ int binder_registration_set_field_from_dct(binder_registration *r, const float *
                    "the handle's grid is not ceil(full grid / res)");
     DBuf<float> c;
     c.upload(coef, (size_t)3 * kWarmCoef * kWarmCoef * kWarmCoef, r->s());
-    launch_dct_field(c.get(), fx, fy, fz, res, r->dcur(), r->nxv, r->nyv, r->nzv, r->s());
+    launch_dct_field(c.get(), fx, fy, fz, res, r->x.get(), r->dcur(), r->nxv, r->nyv, r->nzv,
+                     r->s());
     HIP_OK(hipStreamSynchronize(r->s()));
+  });
+}
+
+int binder_registration_create_from_dct(binder_registration **out, const float *coef, int fx,
+                                        int fy, int fz, int res, const float *voxel_pos,
+                                        int Nx_v, int Ny_v, int Nz_v, int Nx_n, int Ny_n,
+                                        int Nz_n, int spline_order, double non_zero_voxels) {
+  return guarded([&] {
+    BINDER_REQUIRE(out != nullptr, "out handle pointer is null");
+    *out = nullptr;
+    BINDER_REQUIRE(coef != nullptr, "null coefficients");
+    auto r = new_registration(voxel_pos, Nx_v, Ny_v, Nz_v, Nx_n, Ny_n, Nz_n, spline_order,
+                              non_zero_voxels);
+    binder_registration *h = r.get();
+    const int rc = binder_registration_set_field_from_dct(h, coef, fx, fy, fz, res);
+    if (rc != BINDER_OK) throw Error(rc, binder_last_error());
+    *out = r.release();
   });
 }
 

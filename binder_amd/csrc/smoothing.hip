@@ -18,6 +18,7 @@
 // Reference semantics: BINDER/src/cpu/NonLinearTransformation.cpp:58-126,
 // :128-223, :344-364, :366-430, :433-451; ortho convention
 // BINDER/src/cpu/DCT.cpp:83-175.
+#include <atomic>
 #include <cmath>
 #include <cstring>
 
@@ -63,6 +64,7 @@ struct PassArgs {
   double etarho[3];
   int noise, raw;
   uint32_t k0, k1, sweep, stream;
+  const uint32_t *keyx;
   double *cost_part, *raw_part;
 };
 
@@ -114,6 +116,10 @@ __device__ __forceinline__ void normals4(u32x4 r, float n[4]) {
 // straight-line code and the B loads of a whole output tile issue together).
 template <int KS, bool VEC, int PRO, int EPI>
 __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
+  if (a.keyx) {  // per-chain key mask (captured sweeps)
+    a.k0 ^= a.keyx[0];
+    a.k1 ^= a.keyx[1];
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int comp = blockIdx.y;
   const long long row0 = (long long)blockIdx.x * kRowsPerBlock + wave * 16;
@@ -302,6 +308,10 @@ constexpr size_t kPersistMaxLds = 144 * 1024;
 template <int KS, bool VEC, int PRO, int EPI>
 __global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
                                                                  int comps) {
+  if (a.keyx) {  // per-chain key mask (captured sweeps)
+    a.k0 ^= a.keyx[0];
+    a.k1 ^= a.keyx[1];
+  }
   extern __shared__ __align__(16) uint4 bsh[];  // [nt][KS][64] hi, then lo
   const int nt_count = (a.n_out + 15) >> 4;
   const int nfrag = nt_count * KS * 64;
@@ -502,6 +512,7 @@ struct FusedArgs {
   double etarho[3];
   int noise, raw;
   uint32_t k0, k1, sweep, stream;
+  const uint32_t *keyx;
   double *cost_part, *raw_part;
 };
 
@@ -602,6 +613,10 @@ constexpr int kFusedThreads = 512;  // 2 wavefronts per SIMD: 256 VGPRs each
 // is in range.
 template <int NTH, bool EXACT>
 __global__ __launch_bounds__(kFusedThreads) void k_dct_x_fused(FusedArgs a, int comps) {
+  if (a.keyx) {  // per-chain key mask (captured sweeps)
+    a.k0 ^= a.keyx[0];
+    a.k1 ^= a.keyx[1];
+  }
   constexpr int KSH = (NTH + 1) / 2;
   constexpr int NF = NTH * KSH * 64;
   extern __shared__ __align__(16) uint4 msh[];
@@ -742,8 +757,12 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *part,
 // (NonLinearTransformation.cpp:116-124, :140-148).
 __global__ void k_draw_nu(const double *raw, double *gamma, int N, double nv,
                           double rho_prod, double e0, double e1, double e2,
-                          uint32_t k0, uint32_t k1, uint32_t sweep) {
+                          uint32_t k0, uint32_t k1, const uint32_t *keyx, uint32_t sweep) {
   if (threadIdx.x != 0) return;
+  if (keyx) {
+    k0 ^= keyx[0];
+    k1 ^= keyx[1];
+  }
   const double a_post = 1.0 + 0.5 * (double)N * nv;
   const double b_post = 10000.0 + 0.5 * nv * (*raw);
   const double gm = gamma_mt(a_post, u32x4{0u, sweep, 0u, kStreamNu}, k0, k1);
@@ -761,7 +780,7 @@ void launch_ks(dim3 grid, hipStream_t s, const PassArgs &a) {
 template <int KS, bool VEC, int PRO, int EPI>
 void launch_persist(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs &a) {
   auto k = k_contract_lds<KS, VEC, PRO, EPI>;
-  static bool attr_set = false;
+  static std::atomic<bool> attr_set{false};
   if (!attr_set) {
     HIP_OK(hipFuncSetAttribute((const void *)k,
                                hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -945,6 +964,7 @@ long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in
     a.raw = f->raw;
     a.k0 = f->k0;
     a.k1 = f->k1;
+    a.keyx = f->keyx;
     a.sweep = f->sweep;
     a.stream = f->stream;
     a.cost_part = f->cost_part;
@@ -1004,7 +1024,7 @@ bool fused_x_supported(int n) { return n >= 2 && n % 2 == 0 && n <= 192; }
 template <int NTH, bool EXACT>
 void launch_fused(int blocks, size_t lds, int comps, hipStream_t s, const FusedArgs &a) {
   auto k = k_dct_x_fused<NTH, EXACT>;
-  static bool attr_set = false;
+  static std::atomic<bool> attr_set{false};
   if (!attr_set) {
     HIP_OK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)kPersistMaxLds));
@@ -1035,6 +1055,7 @@ long long fused_x_pass(hipStream_t stream, const PassMatrix &m, const float *in,
   a.raw = f.raw;
   a.k0 = f.k0;
   a.k1 = f.k1;
+  a.keyx = f.keyx;
   a.sweep = f.sweep;
   a.stream = f.stream;
   a.cost_part = f.cost_part;
@@ -1141,6 +1162,7 @@ void Smoother::run(const float *dream, const float *x, float *out,
   f.raw = opt.sample_gamma ? 1 : 0;
   f.k0 = opt.key0;
   f.k1 = opt.key1;
+  f.keyx = opt.keyx;
   f.sweep = opt.sweep;
   f.stream = opt.stream;
   f.cost_part = part_cost_.get();
@@ -1248,7 +1270,7 @@ void Smoother::run(const float *dream, const float *x, float *out,
     hipLaunchKernelGGL(k_draw_nu, dim3(1), dim3(64), 0, stream_,
                        scal_.get() + 1, gamma_.get(), N_, (double)nv_,
                        rho_[0] * rho_[1] * rho_[2], eta_[0], eta_[1], eta_[2],
-                       opt.key0, opt.key1, opt.sweep);
+                       opt.key0, opt.key1, opt.keyx, opt.sweep);
     HIP_LAUNCH_OK();
   }
 }

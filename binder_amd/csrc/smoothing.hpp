@@ -39,6 +39,7 @@ struct FilterArgs {
   double etarho[3];
   int noise, raw;
   uint32_t k0, k1, sweep, stream;
+  const uint32_t *keyx;  // optional device key mask: key = (k0, k1) ^ keyx
   double *cost_part, *raw_part;
 };
 
@@ -81,6 +82,7 @@ struct SmoothOptions {
   bool sample = false;         // add DCT-domain Gaussian noise
   bool sample_gamma = false;   // accumulate the raw regulariser and draw nu
   uint32_t key0 = 0, key1 = 0; // Philox key
+  const uint32_t *keyx = nullptr;  // optional device key mask (key ^ keyx)
   uint32_t sweep = 0;          // Philox counter word
   uint32_t stream = 0;         // Philox stream word
   double *prior_out = nullptr; // device double: receives sum_n 0.5 g_n D p^2

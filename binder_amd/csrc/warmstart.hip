@@ -629,8 +629,8 @@ __global__ void k_pack(const float *w, int cin, int cout, int kpad, int kpadT, u
 
 // Low-frequency field of a pyramid level from normalised coefficients.
 __global__ __launch_bounds__(256) void k_dct_field(const float *coef, int fx, int fy, int fz,
-                                                   int res, float *d, int nx, int ny, int nz,
-                                                   float amp) {
+                                                   int res, const float *x, float *d, int nx,
+                                                   int ny, int nz, float amp) {
   __shared__ float c[3 * kNCoef];
   for (int i = threadIdx.x; i < 3 * kNCoef; i += 256) c[i] = coef[i];
   __syncthreads();
@@ -651,9 +651,10 @@ __global__ __launch_bounds__(256) void k_dct_field(const float *coef, int fx, in
     by[a] = a ? sy * cospif((float)((2 * J + 1) * a) / (float)(2 * fy)) : sy0;
     bz[a] = a ? sz * cospif((float)((2 * K + 1) * a) / (float)(2 * fz)) : sz0;
   }
-  const float pos[3] = {(float)i, (float)j, (float)k};
+  const float grid[3] = {(float)i, (float)j, (float)k};
 #pragma unroll
   for (int n = 0; n < 3; ++n) {
+    const float pos = x ? x[n * nv + v] : grid[n];
     float u = 0.f;
     for (int a = 0; a < 8; ++a) {
       float ua = 0.f;
@@ -667,7 +668,7 @@ __global__ __launch_bounds__(256) void k_dct_field(const float *coef, int fx, in
       }
       u += ua * bx[a];
     }
-    d[n * nv + v] = pos[n] + amp * u / (float)res;
+    d[n * nv + v] = pos + amp * u / (float)res;
   }
 }
 
@@ -724,12 +725,12 @@ void launch_wgrad(int cin, int cout, const WgradArgs &a, int P, hipStream_t s) {
 
 }  // namespace
 
-void launch_dct_field(const float *coef_dev, int fx, int fy, int fz, int res, float *d, int nx,
-                      int ny, int nz, hipStream_t s) {
+void launch_dct_field(const float *coef_dev, int fx, int fy, int fz, int res, const float *x,
+                      float *d, int nx, int ny, int nz, hipStream_t s) {
   const long long nv = (long long)nx * ny * nz;
   const float amp = (float)std::sqrt((double)fx * fy * fz);
   hipLaunchKernelGGL(k_dct_field, dim3(blocks_for(nv)), dim3(256), 0, s, coef_dev, fx, fy, fz,
-                     res, d, nx, ny, nz, amp);
+                     res, x, d, nx, ny, nz, amp);
   HIP_LAUNCH_OK();
 }
 

@@ -15,9 +15,10 @@ constexpr int kWarmCoef = 8;
 // (grid fx, fy, fz):
 //   u_c(i, j, k) = sqrt(fx fy fz) sum_{a,b,k'} coef[c][a][b][k'] C_fx[a][i] C_fy[b][j] C_fz[k'][k]
 // with C_n the orthonormal DCT-II basis of length n, and
-//   d_c(p) = p_c + u_c(res * p) / res
-// for level-grid voxel p (its full-resolution index is res * p).
-void launch_dct_field(const float *coef_dev, int fx, int fy, int fz, int res, float *d,
-                      int nx, int ny, int nz, hipStream_t s);
+//   d_c(p) = x_c(p) + u_c(res * p) / res
+// for level-grid voxel p (its full-resolution index is res * p), where x
+// are the voxel positions [c][nv] (x == nullptr: the grid indices p).
+void launch_dct_field(const float *coef_dev, int fx, int fy, int fz, int res, const float *x,
+                      float *d, int nx, int ny, int nz, hipStream_t s);
 
 }  // namespace binder

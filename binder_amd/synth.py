@@ -99,6 +99,16 @@ def make_pair(shape, n_shapes, rms, seed, semi=(10.0, 60.0), bins=32, gamma_map=
     return p
 
 
+# FastRegistration keywords of the 3D workloads (run_BINDER's defaults for
+# this path: 32 bins, gamma 1e-6 per voxel, gamma raised at coarse levels,
+# topology correction, convergence threshold 2.5e-5), pyramid [8, 4, 2, 1]
+# with a 1000 x resolution iteration cap per level
+REGISTRATION_KW = dict(num_voxel_intesity_levels=32, num_node_intensity_levels=32, gamma=1e-6,
+                       topology_correction=True, increase_gamma_at_DS_resolution=True,
+                       alpha=2.0, convergence_th=2.5e-5)
+PYRAMID = (8, 4, 2, 1)
+EM_ITERATIONS = 1000
+
 CONFIGS = {
     # 2D MI registration, 192x160, 6 ellipses, 3 px RMS warp
     "c1": dict(shape=(192, 160), n_shapes=6, rms=3.0),

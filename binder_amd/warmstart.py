@@ -24,30 +24,35 @@ import os
 
 import numpy as np
 
-from . import _core
-
 NCOEF = 8
-_lib = _core._lib
-_vp = ctypes.c_void_p
-_c_int = ctypes.c_int
-_P = ctypes.POINTER
-_sig = _core._sig
-_check = _core._check
-_ptr = _core._ptr
 
-_ws_create = _sig("binder_warmstart_create", _c_int, _P(_vp), _c_int, _c_int, _c_int, _c_int,
-                  ctypes.c_ulonglong)
-_ws_destroy = _sig("binder_warmstart_destroy", None, _vp)
-_ws_nparam = _sig("binder_warmstart_num_params", _c_int, _vp, _P(ctypes.c_longlong))
-_ws_get = _sig("binder_warmstart_get_params", _c_int, _vp, _vp)
-_ws_set = _sig("binder_warmstart_set_params", _c_int, _vp, _vp)
-_ws_data = _sig("binder_warmstart_set_dataset", _c_int, _vp, _c_int, _vp, _vp, _vp, _c_int,
-                _c_int)
-_ws_train = _sig("binder_warmstart_train", _c_int, _vp, _vp, _c_int, _c_int, ctypes.c_double,
-                 _vp)
-_ws_grad = _sig("binder_warmstart_gradients", _c_int, _vp, _vp, _c_int, _P(ctypes.c_double), _vp)
-_ws_pred_data = _sig("binder_warmstart_predict_dataset", _c_int, _vp, _vp, _c_int, _vp)
-_ws_pred = _sig("binder_warmstart_predict", _c_int, _vp, _c_int, _vp, _vp, _c_int, _c_int, _vp)
+
+class _Api:
+    """ctypes entry points of the network, bound on first use so that the
+    host-only helpers below (and worker processes that only build inputs)
+    never load the HIP library."""
+    _fns = None
+
+    @classmethod
+    def get(cls):
+        if cls._fns is None:
+            from . import _core
+            vp, ci, P, sig = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER, _core._sig
+            cls._fns = dict(
+                create=sig("binder_warmstart_create", ci, P(vp), ci, ci, ci, ci,
+                           ctypes.c_ulonglong),
+                destroy=sig("binder_warmstart_destroy", None, vp),
+                nparam=sig("binder_warmstart_num_params", ci, vp, P(ctypes.c_longlong)),
+                get=sig("binder_warmstart_get_params", ci, vp, vp),
+                set=sig("binder_warmstart_set_params", ci, vp, vp),
+                data=sig("binder_warmstart_set_dataset", ci, vp, ci, vp, vp, vp, ci, ci),
+                train=sig("binder_warmstart_train", ci, vp, vp, ci, ci, ctypes.c_double, vp),
+                grad=sig("binder_warmstart_gradients", ci, vp, vp, ci, P(ctypes.c_double), vp),
+                pred_data=sig("binder_warmstart_predict_dataset", ci, vp, vp, ci, vp),
+                pred=sig("binder_warmstart_predict", ci, vp, ci, vp, vp, ci, ci, vp),
+                check=_core._check, ptr=_core._ptr)
+        return cls._fns
+
 
 # (cin, cout) of the 3x3x3 convolutions, in parameter-vector order
 CONV_LAYERS = [(8, 16), (16, 16), (16, 32), (32, 32), (32, 48), (48, 48), (48, 64), (64, 64),
@@ -136,31 +141,38 @@ class WarmStartNet:
     """The warm-start network on the GPU (handle over the C ABI)."""
 
     def __init__(self, grid, max_batch=8, seed=0):
+        self._api = api = _Api.get()
+        self._h = None
         self.grid = tuple(int(n) for n in grid)
         self.max_batch = int(max_batch)
-        h = _vp()
-        _check(_ws_create(ctypes.byref(h), *self.grid, self.max_batch, int(seed)))
+        h = ctypes.c_void_p()
+        api["check"](api["create"](ctypes.byref(h), *self.grid, self.max_batch, int(seed)))
         self._h = h
         n = ctypes.c_longlong()
-        _check(_ws_nparam(self._h, ctypes.byref(n)))
+        api["check"](api["nparam"](self._h, ctypes.byref(n)))
         self.nparam = int(n.value)
-        self._n = 0
 
     def __del__(self):
         if getattr(self, "_h", None):
-            _ws_destroy(self._h)
+            self._api["destroy"](self._h)
             self._h = None
+
+    def _call(self, name, *args):
+        self._api["check"](self._api[name](self._h, *args))
+
+    def _p(self, a):
+        return self._api["ptr"](a)
 
     def get_params(self):
         out = np.empty(self.nparam, np.float32)
-        _check(_ws_get(self._h, _ptr(out)))
+        self._call("get", self._p(out))
         return out
 
     def set_params(self, params):
         p = np.ascontiguousarray(params, np.float32)
         if p.size != self.nparam:
             raise ValueError(f"expected {self.nparam} parameters")
-        _check(_ws_set(self._h, _ptr(p)))
+        self._call("set", self._p(p))
 
     def set_dataset(self, fixed_bins, moving_bins, targets=None, L=32, K=32):
         """fixed_bins / moving_bins: [n, *grid] uint8; targets [n, 3, 8, 8, 8]."""
@@ -174,24 +186,23 @@ class WarmStartNet:
             t = np.ascontiguousarray(targets, np.float32)
             if t.size != n * 3 * NCOEF ** 3:
                 raise ValueError("targets must be [n, 3, 8, 8, 8]")
-        _check(_ws_data(self._h, n, _ptr(f), _ptr(m), None if t is None else _ptr(t), int(L),
-                        int(K)))
-        self._n = n
+        self._call("data", n, self._p(f), self._p(m), None if t is None else self._p(t),
+                   int(L), int(K))
 
     def train(self, idx, lr=1e-3):
         """Adam steps over idx [steps, batch] (sample indices); returns the
-        loss of each step."""
+        loss of each step (before its update)."""
         ix = np.ascontiguousarray(idx, np.int32)
         steps, batch = ix.shape
         losses = np.empty(steps, np.float64)
-        _check(_ws_train(self._h, _ptr(ix), steps, batch, float(lr), _ptr(losses)))
+        self._call("train", self._p(ix), steps, batch, float(lr), self._p(losses))
         return losses
 
     def gradients(self, idx):
         ix = np.ascontiguousarray(idx, np.int32).ravel()
         g = np.empty(self.nparam, np.float32)
         loss = ctypes.c_double()
-        _check(_ws_grad(self._h, _ptr(ix), ix.size, ctypes.byref(loss), _ptr(g)))
+        self._call("grad", self._p(ix), ix.size, ctypes.byref(loss), self._p(g))
         return float(loss.value), g
 
     def predict_dataset(self, idx):
@@ -199,7 +210,7 @@ class WarmStartNet:
         out = np.empty((ix.size, 3, NCOEF, NCOEF, NCOEF), np.float32)
         for s in range(0, ix.size, self.max_batch):
             part = np.ascontiguousarray(ix[s:s + self.max_batch])
-            _check(_ws_pred_data(self._h, _ptr(part), part.size, _ptr(out[s:s + part.size])))
+            self._call("pred_data", self._p(part), part.size, self._p(out[s:s + part.size]))
         return out
 
     def predict(self, fixed_bins, moving_bins, L=32, K=32):
@@ -213,7 +224,7 @@ class WarmStartNet:
         if f.shape[1:] != self.grid or m.shape != f.shape or B > self.max_batch:
             raise ValueError(f"bins must be [<= {self.max_batch}, {self.grid}]")
         out = np.empty((B, 3, NCOEF, NCOEF, NCOEF), np.float32)
-        _check(_ws_pred(self._h, B, _ptr(f), _ptr(m), int(L), int(K), _ptr(out)))
+        self._call("pred", B, self._p(f), self._p(m), int(L), int(K), self._p(out))
         return out[0] if single else out
 
     def predict_pair(self, full_binned_voxels, full_binned_nodes, L=32, K=32):

@@ -141,6 +141,26 @@ int binder_registration_sampler_run(binder_registration *reg, int seed, int N_b,
 int binder_registration_sampler_result(binder_registration *reg,
                                        double *mean_out, double *cov_out);
 
+/* Independent Gibbs chains from the handle's current state: chain c runs
+ * N_b burn-in and N_s sample sweeps exactly as
+ * binder_registration_sampler_run(seeds[c], N_b, N_s, 0, 1, sp, sl, st)
+ * would (same draws), each from the same start field and theta; the mean
+ * and M2 accumulators pool all chains' samples (Welford within a chain,
+ * Chan et al.'s pairwise merge across chains), read with
+ * binder_registration_sampler_result (covariance = M2 / (nchains N_s - 1)).
+ * The handle's field and theta are restored afterwards.  use_graph != 0
+ * captures one chain's sweeps into a HIP graph and replays it for chains
+ * 1..nchains-1 (the Philox key is read from device memory, so the graph is
+ * seed-independent).  No nu sampling (sample_gamma = 0).  No reference
+ * counterpart: the reference runs one chain per sampler() call
+ * (BINDER/src/cpu/Registration.cpp:515-642). */
+int binder_registration_sampler_chains(binder_registration *reg, int nchains,
+                                       const int *seeds, int N_b, int N_s,
+                                       int sample_posteriors,
+                                       int sample_likelihood_parameters,
+                                       int sample_transformation_parameters,
+                                       int use_graph);
+
 /* get_final_locations_as_numpy_array() -- BINDER/src/gpu/bindings.cpp:142-157. */
 int binder_registration_get_final_locations(binder_registration *reg,
                                             float *out);
@@ -352,7 +372,8 @@ int binder_warmstart_predict(binder_warmstart *ws, int batch,
                              float *coef_out);
 
 /* Sets the handle's current field to the warm-start field of a pyramid
- * level: d(p) = p + u(res * p) / res, with u the full-resolution
+ * level: d(p) = x(p) + u(res * p) / res, with x the handle's voxel
+ * positions (the grid indices p for an index grid), u the full-resolution
  * displacement (grid fx x fy x fz) whose 8x8x8 lowest orthonormal DCT
  * coefficients per component are sqrt(fx fy fz) * coef[3][8][8][8] and all
  * others zero.  The handle's grid must be ceil(f / res) per axis; 3D only.
@@ -361,6 +382,18 @@ int binder_warmstart_predict(binder_warmstart *ws, int batch,
 int binder_registration_set_field_from_dct(binder_registration *reg,
                                            const float *coef, int fx, int fy,
                                            int fz, int res);
+
+/* binder_registration_create (voxel_pos NULL: the grid indices, generated on
+ * the device) followed by binder_registration_set_field_from_dct: a
+ * registration whose field starts at the warm start, with no field upload.
+ * No reference counterpart (replaces the identity-start constructor call of
+ * BINDER/FastRegistration.py:443-450 for the first level run). */
+int binder_registration_create_from_dct(binder_registration **out,
+                                        const float *coef, int fx, int fy,
+                                        int fz, int res, const float *voxel_pos,
+                                        int Nx_v, int Ny_v, int Nz_v, int Nx_n,
+                                        int Ny_n, int Nz_n, int spline_order,
+                                        double non_zero_voxels);
 
 #ifdef __cplusplus
 }

@@ -150,3 +150,80 @@ def test_field_from_dct_matches_host_evaluation(core, ws, rng, res):
     assert np.max(np.abs(got - ref)) < 2e-4
     with pytest.raises(ValueError):
         r.set_field_from_dct(coef, (shape[0] + 2 * res, shape[1], shape[2]), res)
+
+
+class _FixedNet:
+    """Stands in for the trained network: returns given coefficients (the
+    network itself is checked above)."""
+
+    def __init__(self, coef):
+        self.coef = np.asarray(coef, np.float32)
+
+    def predict_pair(self, *args, **kwargs):
+        return self.coef
+
+
+def _warm_pyramid(backend_module, monkeypatch, pair, net, levels):
+    import binder_amd.FastRegistration as fr
+    monkeypatch.setattr(fr, "Registration", backend_module)
+    reg = fr.FastRegistration(voxels=pair.fixed, nodes=pair.moving,
+                              num_voxel_intesity_levels=32, num_node_intensity_levels=32,
+                              gamma=1e-6, convergence_th=2.5e-5, topology_correction=True,
+                              increase_gamma_at_DS_resolution=True)
+    used = reg.run_warm(net, levels, 10, compare_resolution=4)
+    return reg, used
+
+
+@pytest.mark.parametrize("levels", [(1,), (2, 1)])
+def test_warm_pyramid_matches_oracle(core, orc, monkeypatch, ws, levels):
+    """The warm-started pipeline (start field evaluated on the GPU) against
+    the FP64 oracle running the same pipeline from the host-evaluated start:
+    same fallback decision and iteration counts, field within 0.025 voxel
+    RMS (the bound of the cold pyramid test)."""
+    from binder_amd import synth
+    print(f"warm pyramid 48x40x44 levels {levels} (HIP vs oracle)", flush=True)
+    pair = synth.make_pair((48, 40, 44), 5, 2.0, seed=3, semi=(3.0, 12.0))
+    # start from the low-frequency part of the true inverse warp's estimate
+    # by the cold pipeline (what the network is trained to predict)
+    import binder_amd.FastRegistration as fr
+    monkeypatch.setattr(fr, "Registration", core)
+    cold = fr.FastRegistration(voxels=pair.fixed, nodes=pair.moving,
+                               num_voxel_intesity_levels=32, num_node_intensity_levels=32,
+                               gamma=1e-6, convergence_th=2.5e-5, topology_correction=True,
+                               increase_gamma_at_DS_resolution=True)
+    cold.run([4, 2, 1], 10)
+    net = _FixedNet(ws.field_coefficients(cold.deformedVoxelPositions))
+    g, gu = _warm_pyramid(core, monkeypatch, pair, net, levels)
+    o, ou = _warm_pyramid(orc, monkeypatch, pair, net, levels)
+    assert gu and ou
+    assert np.allclose(g.warm_objectives, o.warm_objectives, rtol=1e-5)
+    gi = [lv[1] for lv in g.level_log]
+    oi = [lv[1] for lv in o.level_log]
+    diff = np.asarray(g.deformedVoxelPositions, np.float64) - o.deformedVoxelPositions
+    rms = float(np.sqrt(np.mean(np.sum(diff ** 2, -1))))
+    print(f"iterations HIP {gi} oracle {oi}; field RMS diff {rms:.2e}", flush=True)
+    assert gi == oi
+    assert rms < 0.025
+
+
+def test_warm_start_falls_back_to_the_cold_pyramid(core, monkeypatch, ws):
+    """A warm start far from the answer (a large random displacement) has a
+    worse start objective than the identity: run_warm then runs the plain
+    cold pyramid and returns exactly its result."""
+    from binder_amd import synth
+    import binder_amd.FastRegistration as fr
+    print("warm start fallback 48x40x44", flush=True)
+    pair = synth.make_pair((48, 40, 44), 5, 2.0, seed=3, semi=(3.0, 12.0))
+    bad = np.zeros((3, 8, 8, 8), np.float32)
+    bad[:, 1, 1, 1] = 1.5  # about 1.5 voxel-RMS displacement per component
+    bad[:, 2, 0, 3] = -1.5
+    w, used = _warm_pyramid(core, monkeypatch, pair, _FixedNet(bad), (1,))
+    assert not used
+    assert w.warm_objectives[0] > w.warm_objectives[1]
+    c = fr.FastRegistration(voxels=pair.fixed, nodes=pair.moving,
+                            num_voxel_intesity_levels=32, num_node_intensity_levels=32,
+                            gamma=1e-6, convergence_th=2.5e-5, topology_correction=True,
+                            increase_gamma_at_DS_resolution=True)
+    c.run([8, 4, 2, 1], 10)
+    assert [lv[1] for lv in w.level_log] == [lv[1] for lv in c.level_log]
+    np.testing.assert_array_equal(w.deformedVoxelPositions, c.deformedVoxelPositions)
