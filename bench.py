@@ -173,12 +173,13 @@ def registration(args, pair, net=None):
     return A, F, out, np.asarray(C.deformedVoxelPositions, np.float64)
 
 
-def inverse_warp_rms(pair, F, step=4, iterations=30):
+def inverse_warp_rms(pair, F, step=4, iterations=30, tissue=0.05):
     """RMS distance (voxels) between a field F and the exact answer of the
-    synthetic pair, on every `step`-th voxel per axis: the moving image is
-    the fixed image sampled at y + u(y), so fixed voxel x corresponds to the
-    moving position y with y + u(y) = x (fixed-point iteration y = x - u(y),
-    trilinear u)."""
+    synthetic pair, over the fixed image's tissue voxels (intensity above
+    `tissue`; the background carries no information for any start) on every
+    `step`-th voxel per axis: the moving image is the fixed image sampled at
+    y + u(y), so fixed voxel x corresponds to the moving position y with
+    y + u(y) = x (fixed-point iteration y = x - u(y), trilinear u)."""
     from scipy import ndimage
     u = pair.displacement
     shape = u.shape[:-1]
@@ -189,8 +190,10 @@ def inverse_warp_rms(pair, F, step=4, iterations=30):
         uy = np.stack([ndimage.map_coordinates(u[..., a], np.moveaxis(y, -1, 0), order=1,
                                                mode="nearest") for a in range(len(shape))], -1)
         y = x - uy
-    Fs = np.asarray(F, np.float64)[tuple(slice(0, n, step) for n in shape)]
-    return _rms(Fs, y)
+    sl = tuple(slice(0, n, step) for n in shape)
+    Fs = np.asarray(F, np.float64)[sl]
+    m = np.asarray(pair.fixed)[sl] > tissue
+    return _rms(Fs[m], y[m])
 
 
 def load_warm_net(args):

@@ -309,15 +309,19 @@ class FastRegistration:
 
     def _start_objective(self, coef, res, iterations):
         """Objective (mlp) after `iterations` EM iterations at level `res`
-        from the warm start (coef) or, for coef None, from the identity."""
+        from the warm start (coef) or, for coef None, from the identity, and
+        the likelihood parameters theta fitted there."""
         r = self._warm_registration(coef, res) if coef is not None else \
             self._new_registration(np.ascontiguousarray(self.voxels_pos))
         r.setMILikelihood(self.alpha, np.ascontiguousarray(self.theta).ravel(),
                           self.num_node_intensity_levels, self.num_voxel_intensity_levels,
                           self.threads, np.ascontiguousarray(self.binned_nodes).ravel(),
                           np.ascontiguousarray(self.binned_voxels).ravel())
-        r.EM(int(iterations), -1e30, update_likelihood_parameters=1, debug=0)
-        return float(r.get_objective_trace()[-1, 3])
+        r.EM(int(iterations), -1e30, update_likelihood_parameters=self.update_likelihood_parameters,
+             debug=0)
+        theta = np.array(r.get_likelihood_parameters()).reshape(
+            self.num_node_intensity_levels, self.num_voxel_intensity_levels)
+        return float(r.get_objective_trace()[-1, 3]), theta
 
     def run_warm(self, net, resolutions=(1,), max_number_of_iterations=1000,
                  level_iterations=None, fallback_resolutions=(8, 4, 2, 1),
@@ -332,8 +336,10 @@ class FastRegistration:
         iterations at level `compare_resolution` is computed from the warm
         start and from the identity; if the warm start's is worse (higher),
         the plain cold pyramid `fallback_resolutions` runs instead
-        (level_iterations then applies to it).  Returns True when the warm
-        start was used."""
+        (level_iterations then applies to it).  Otherwise the likelihood
+        parameters theta fitted from the warm start in that check start the
+        first level, as a coarser level's theta starts the next one in the
+        plain pyramid.  Returns True when the warm start was used."""
         if self.number_of_dimensions != 3:
             raise ValueError("the learned warm start is defined for 3D registrations")
         coef = net.predict_pair(self.full_binned_voxels, self.full_binned_nodes,
@@ -344,9 +350,14 @@ class FastRegistration:
         if compare_iterations:
             cr = int(compare_resolution)
             self.move_resolution(cr, resize_field=False)
-            self.warm_objectives = (self._start_objective(coef, cr, compare_iterations),
-                                    self._start_objective(None, cr, compare_iterations))
-            used = self.warm_objectives[0] <= self.warm_objectives[1]
+            (mw, theta_w), (mc, _) = (self._start_objective(coef, cr, compare_iterations),
+                                      self._start_objective(None, cr, compare_iterations))
+            self.warm_objectives = (mw, mc)
+            used = mw <= mc
+            if used:
+                # like a coarser pyramid level, the check hands its fitted
+                # likelihood parameters on to the first level run
+                self.theta = np.ascontiguousarray(theta_w)
         self.warm_start_used = used
         if not used:
             self.move_resolution(1, resize_field=False)
@@ -386,6 +397,13 @@ class FastRegistration:
         return r
 
     def _set_nonlinear(self, r):
+        if hasattr(r, "setNonLinearTransformationDCTPrior") and \
+                self.finite_difference_order in (1, 2):
+            # the same D values (dct_prior_eigenvalues), made on the GPU
+            r.setNonLinearTransformationDCTPrior(
+                self.gamma_x, self.gamma_y, self.gamma_z, *self.current_voxels_voxel_size[:3],
+                *self.current_nodes_voxel_size[:3], self.finite_difference_order)
+            return
         self.D = np.ascontiguousarray(self.D)
         r.setNonLinearTransformation(self.gamma_x, self.gamma_y, self.gamma_z,
                                      self.current_voxels_voxel_size[0],

@@ -49,6 +49,8 @@ _reg_set_mi = _sig("binder_registration_set_mi_likelihood", _c_int, _vp, _c_doub
 _reg_set_nlt = _sig("binder_registration_set_nonlinear_transformation", _c_int, _vp,
                     *([_c_double] * 9), _vp)
 _reg_em = _sig("binder_registration_em", _c_int, _vp, _c_int, _c_double, _c_int, _c_int)
+_reg_set_nlt_prior = _sig("binder_registration_set_nonlinear_transformation_dct_prior", _c_int,
+                          _vp, *([_c_double] * 9), _c_int)
 _reg_sampler = _sig("binder_registration_sampler", _c_int, _vp, *([_c_int] * 8), _vp, _vp)
 _reg_sampler_run = _sig("binder_registration_sampler_run", _c_int, _vp, *([_c_int] * 8))
 _reg_sampler_result = _sig("binder_registration_sampler_result", _c_int, _vp, _vp, _vp)
@@ -248,6 +250,19 @@ class Registration:
                             float(voxels_size_x), float(voxels_size_y), float(voxels_size_z),
                             float(nodes_size_x), float(nodes_size_y), float(nodes_size_z),
                             _ptr(Dv)))
+
+    def setNonLinearTransformationDCTPrior(self, gamma_x, gamma_y, gamma_z, voxels_size_x,
+                                           voxels_size_y, voxels_size_z, nodes_size_x,
+                                           nodes_size_y, nodes_size_z,
+                                           finite_difference_order=2):
+        """setNonLinearTransformation with D = dct_prior_eigenvalues(grid,
+        order, voxel sizes) made on the GPU (the same float32 values; no D
+        upload).  No reference counterpart."""
+        _check(_reg_set_nlt_prior(self._h, float(gamma_x), float(gamma_y), float(gamma_z),
+                                  float(voxels_size_x), float(voxels_size_y),
+                                  float(voxels_size_z), float(nodes_size_x),
+                                  float(nodes_size_y), float(nodes_size_z),
+                                  int(finite_difference_order)))
 
     def EM(self, max_EM_iterations, convergence_th, update_likelihood_parameters, debug):
         _check(_reg_em(self._h, int(max_EM_iterations), float(convergence_th),

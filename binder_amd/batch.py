@@ -55,13 +55,13 @@ def register_pairs(pairs, workers=8, **kw):
 
 
 def full_resolution_handle(fr):
-    """A registration handle at the pair's registered field and theta."""
-    h = fr._new_registration(np.ascontiguousarray(fr.deformedVoxelPositions))
-    h.setMILikelihood(fr.alpha, np.ascontiguousarray(fr.theta).ravel(),
-                      fr.num_node_intensity_levels, fr.num_voxel_intensity_levels, 1,
-                      np.ascontiguousarray(fr.binned_nodes).ravel(),
-                      np.ascontiguousarray(fr.binned_voxels).ravel())
-    return h
+    """The registration handle holding the pair's registered state: the
+    pyramid's last (full-resolution) level, whose device field is the
+    topology-corrected result and whose theta is the fitted one -- the state
+    FastRegistration.sample_param starts from, without re-uploading it."""
+    if fr.current_resolution != 1:
+        raise ValueError("the pyramid did not end at full resolution")
+    return fr.registratorObj
 
 
 def sweeps_for(site_updates, nv):

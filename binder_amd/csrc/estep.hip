@@ -808,6 +808,7 @@ __global__ __launch_bounds__(256) void k_welford(WelfordBatch b, double *mean, d
                                                  long long nv) {
   constexpr int NP = NC * (NC + 1) / 2;
   const long long stride = (long long)gridDim.x * blockDim.x;
+  if (b.n0_base) b.n0 += *b.n0_base;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
        i += stride) {
     double mu[NC], M[NP];

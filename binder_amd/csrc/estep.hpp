@@ -78,6 +78,9 @@ struct WelfordBatch {
   const float *x[kWelfordBatch];
   int count;
   long long n0;
+  // optional device-resident count added to n0 (samples folded by earlier
+  // chains; lets one captured graph serve every chain)
+  const long long *n0_base;
 };
 void launch_welford(const WelfordBatch &b, double *mean, double *m2,
                     long long nv, int N, hipStream_t s);

@@ -9,10 +9,14 @@
 //   bins2d           uint8 [(Nx_n+2)(Ny_n+2)]          padded moving bins (2D)
 //   theta (f64), thT (f32, transposed), Hd (f64) K*L tables
 //   mean (f64 [N][nv]), m2 (f64 [N(N+1)/2][nv]) sampler accumulators
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
+#include <utility>
 
 #include "estep.hpp"
 #include "smoothing.hpp"
@@ -88,29 +92,6 @@ __global__ __launch_bounds__(256) void k_zero_f64(double *__restrict__ x, long l
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     x[i] = 0.0;
-}
-
-// Pooled mean / M2 of two sample sets (Chan, Golub & LeVeque 1979):
-// n = na + nb, delta = mean_b - mean_a, mean = mean_a + delta nb / n,
-// M2 = M2_a + M2_b + delta delta^T na nb / n.  Upper-triangular M2 planes
-// in the order (0,0), (0,1), .., (1,1), ..
-__global__ __launch_bounds__(256) void k_chan_merge(double *__restrict__ mean,
-                                                    double *__restrict__ m2,
-                                                    const double *__restrict__ mb,
-                                                    const double *__restrict__ m2b,
-                                                    double na, double nb, long long nv, int N) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  const double n = na + nb;
-  double d[3];
-  for (int c = 0; c < N; ++c) {
-    d[c] = mb[c * nv + i] - mean[c * nv + i];
-    mean[c * nv + i] += d[c] * (nb / n);
-  }
-  int p = 0;
-  for (int a = 0; a < N; ++a)
-    for (int b = a; b < N; ++b, ++p)
-      m2[p * nv + i] += m2b[p * nv + i] + d[a] * d[b] * (na * nb / n);
 }
 
 int grid_blocks(long long n) {
@@ -189,6 +170,7 @@ struct binder_registration {
   DBuf<float> stage;  // [nv][N] staging for host transfers
   bool x_index_grid = false;  // x holds the grid indices exactly
   int dslot = 0;
+  int ring = kWelfordBatch;   // slots in use (a call with fewer sweeps needs fewer)
   float *slot(int k) {
     return k == 0 ? d.get() : dextra.get() + (size_t)(k - 1) * N * nv;
   }
@@ -202,6 +184,13 @@ struct binder_registration {
   DBuf<double> theta, Hd, rowbuf, mllpart, stats;
   DBuf<float> thT;
   DBuf<unsigned long long> hpart, hpart2;
+  // second buffers of the EM loop's state (field, theta, dream, objective
+  // terms): iteration k reads parity k & 1 and writes parity (k + 1) & 1, so
+  // iteration k + 1 can be queued before the host has read iteration k's
+  // objective
+  DBuf<float> em_d, em_dream, em_thT;
+  DBuf<double> em_theta, em_stats;
+  Event em_done[2];
   int estep_blocks = 0;
   // transformation
   std::unique_ptr<Smoother> sm;
@@ -267,12 +256,14 @@ struct binder_registration {
   }
 
   void run_theta(int mode, uint32_t k0, uint32_t k1, uint32_t sweep,
-                 hipStream_t st = nullptr, const uint32_t *keyx = nullptr) {
+                 hipStream_t st = nullptr, const uint32_t *keyx = nullptr,
+                 double *theta_out = nullptr, float *thT_out = nullptr,
+                 double *stats_out = nullptr) {
     if (!st) st = s();
     ThetaArgs t{};
     t.Hd = Hd.get();
-    t.theta = theta.get();
-    t.thT = thT.get();
+    t.theta = theta_out ? theta_out : theta.get();
+    t.thT = thT_out ? thT_out : thT.get();
     t.rowbuf = rowbuf.get();
     t.mllpart = mllpart.get();
     t.nmll = estep_blocks;
@@ -286,13 +277,30 @@ struct binder_registration {
     t.lg_sum_alpha = std::lgamma(sa);
     t.lg_alpha = std::lgamma(alpha);
     t.mode = mode;
-    t.stats = stats.get();
+    t.stats = stats_out ? stats_out : stats.get();
     t.k0 = k0;
     t.k1 = k1;
     t.keyx = keyx;
     t.sweep = sweep;
     TimedLaunch tl(tm(), mode == kThetaDirichlet ? "dirichlet" : "theta", st);
     launch_theta(t, st);
+  }
+
+  // Makes the ring ready for a call of `sweeps` sweeps: the current field
+  // moves to slot 0 and the ring gets min(kWelfordBatch, sweeps + 1) slots
+  // (no sample is overwritten before it is folded in either way).
+  void prepare_ring(int sweeps) {
+    const size_t nf = (size_t)N * nv;
+    if (dslot != 0) {
+      HIP_OK(hipMemcpyAsync(d.get(), dcur(), nf * sizeof(float), hipMemcpyDeviceToDevice, s()));
+      dslot = 0;
+    }
+    ring = std::min(kWelfordBatch, sweeps + 1);
+    const size_t need = (size_t)(ring - 1) * nf;
+    if (need && dextra.n < need) {
+      HIP_OK(hipStreamSynchronize(s()));
+      dextra.alloc(need);
+    }
   }
 
   // Enqueue N_b + N_s sampling sweeps (E-draw, Dirichlet draw, deformation
@@ -302,8 +310,9 @@ struct binder_registration {
   // sequence can be captured into a graph.
   void enqueue_sweeps(int N_b, int N_s, int sample_gamma, int every, int sp, int sl, int st,
                       uint32_t k0, uint32_t k1, const uint32_t *keyx, double *mean_p,
-                      double *m2_p) {
+                      double *m2_p, const long long *n0_base = nullptr) {
     WelfordBatch wb{};
+    wb.n0_base = n0_base;
     long long folded = 0;  // samples already in the accumulators
     const bool overlap = sp && sl && st;
     for (int sweep = 0; sweep < N_b + N_s; ++sweep) {
@@ -335,7 +344,7 @@ struct binder_registration {
         o.sweep = (uint32_t)sweep;
         o.stream = kStreamDeform;
         o.want_prior = false;  // the sampler reports no objective
-        const int next = (dslot + 1) % kWelfordBatch;
+        const int next = (dslot + 1) % ring;
         sm->run(dream.get(), x.get(), slot(next), o, tm());
         dslot = next;
       }
@@ -451,6 +460,30 @@ std::unique_ptr<binder_registration> new_registration(const float *voxel_pos, in
   return r;
 }
 
+template <class T>
+void swap_buffers(DBuf<T> &a, DBuf<T> &b) {
+  std::swap(a.p, b.p);
+  std::swap(a.n, b.n);
+}
+
+// fn(begin, end) over [0, n) split across up to 16 host threads (one thread
+// when n < min_n).
+template <class F>
+void parallel_ranges(long long n, F &&fn, long long min_n = 1ll << 15) {
+  unsigned hw = std::thread::hardware_concurrency();
+  long long T = hw ? (long long)std::min(hw, 16u) : 1;
+  if (n < min_n) T = 1;
+  if (T > n) T = n > 0 ? n : 1;
+  if (T == 1) {
+    fn(0ll, n);
+    return;
+  }
+  std::vector<std::thread> ts;
+  for (long long t = 0; t < T; ++t)
+    ts.emplace_back([&, t] { fn(n * t / T, n * (t + 1) / T); });
+  for (auto &t : ts) t.join();
+}
+
 // Philox key of a sampler seed.
 void sampler_key(int seed, uint32_t &k0, uint32_t &k1) {
   k0 = (uint32_t)seed;
@@ -518,11 +551,15 @@ int binder_registration_set_mi_likelihood(binder_registration *r, double alpha,
     BINDER_REQUIRE(K >= 1 && K <= 256 && L >= 1 && L <= 256 && K * L <= 4096,
                    "K and L must be in [1,256] with K*L <= 4096");
     std::vector<uint8_t> v8(r->nv);
-    for (long long i = 0; i < r->nv; ++i) {
-      const int b = binned_voxels[i];
-      BINDER_REQUIRE(b >= 0 && b < L, "binned_voxels value outside [0, L)");
-      v8[i] = (uint8_t)b;
-    }
+    std::atomic<bool> bad{false};
+    parallel_ranges(r->nv, [&](long long b0, long long b1) {
+      for (long long i = b0; i < b1; ++i) {
+        const int b = binned_voxels[i];
+        if (b < 0 || b >= L) bad = true;
+        v8[i] = (uint8_t)b;
+      }
+    });
+    BINDER_REQUIRE(!bad, "binned_voxels value outside [0, L)");
     r->K = K;
     r->L = L;
     r->alpha = alpha;
@@ -550,18 +587,25 @@ int binder_registration_set_mi_likelihood(binder_registration *r, double alpha,
       // column's bins behind four zeros
       const int Zq = r->nzn + 5;
       std::vector<uint32_t> q((size_t)Xp * Yp * Zq, 0u);
-      std::vector<uint8_t> pad((size_t)r->nzn + 8, 0);
-      for (int x = 0; x < r->nxn; ++x)
-        for (int y = 0; y < r->nyn; ++y) {
-          const int *row = binned_nodes + ((long long)x * r->nyn + y) * r->nzn;
-          for (int z = 0; z < r->nzn; ++z) pad[z + 4] = node_bin(row[z]);
-          uint32_t *qr = &q[((size_t)(x + 1) * Yp + (y + 1)) * Zq];
-          for (int zq = 0; zq < Zq; ++zq) {
-            const uint8_t *b = &pad[zq];
-            qr[zq] = (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) |
-                     ((uint32_t)b[3] << 24);
+      parallel_ranges(r->nxn, [&](long long x0, long long x1) {
+        std::vector<uint8_t> pad((size_t)r->nzn + 8, 0);
+        for (long long x = x0; x < x1; ++x)
+          for (int y = 0; y < r->nyn; ++y) {
+            const int *row = binned_nodes + (x * r->nyn + y) * r->nzn;
+            for (int z = 0; z < r->nzn; ++z) {
+              const int b = row[z];
+              if (b < 0 || b >= K) bad = true;
+              pad[z + 4] = (uint8_t)b;
+            }
+            uint32_t *qr = &q[((size_t)(x + 1) * Yp + (y + 1)) * Zq];
+            for (int zq = 0; zq < Zq; ++zq) {
+              const uint8_t *b = &pad[zq];
+              qr[zq] = (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) |
+                       ((uint32_t)b[3] << 24);
+            }
           }
-        }
+      }, 32);
+      BINDER_REQUIRE(!bad, "binned_nodes value outside [0, K)");
       r->quads.upload(q.data(), q.size(), r->s());
       HIP_OK(hipStreamSynchronize(r->s()));
     }
@@ -596,35 +640,113 @@ int binder_registration_set_nonlinear_transformation(
   });
 }
 
+int binder_registration_set_nonlinear_transformation_dct_prior(
+    binder_registration *r, double gamma_x, double gamma_y, double gamma_z,
+    double voxels_size_x, double voxels_size_y, double voxels_size_z,
+    double nodes_size_x, double nodes_size_y, double nodes_size_z, int order) {
+  return guarded([&] {
+    BINDER_REQUIRE(r, "null handle");
+    BINDER_REQUIRE(order == 1 || order == 2, "finite-difference order must be 1 or 2");
+    const double g[3] = {gamma_x, gamma_y, gamma_z};
+    const double rho[3] = {voxels_size_x, voxels_size_y, voxels_size_z};
+    const double eta[3] = {nodes_size_x, nodes_size_y, nodes_size_z};
+    // the per-axis terms of binder_dct_prior_eigenvalues, same expressions
+    const int dims[3] = {r->nxv, r->nyv, r->nzv};
+    const int nd = r->is2d ? 2 : 3;
+    std::vector<double> t((size_t)dims[0] + dims[1] + dims[2], 0.0);
+    size_t o = 0;
+    for (int a = 0; a < 3; ++a) {
+      const double v2 = rho[a] * rho[a];
+      for (int k = 0; k < dims[a]; ++k, ++o)
+        if (a < nd) t[o] = (2.0 - 2.0 * std::cos(M_PI * (double)k / (double)dims[a])) / v2;
+    }
+    r->sm = std::make_unique<Smoother>(r->nxv, r->nyv, r->nzv, g, rho, eta, t.data(), order,
+                                       std::sqrt(9.0 / (8.0 * M_PI)), r->s());
+    r->sm->set_index_grid(r->x_index_grid);
+  });
+}
+
 int binder_registration_em(binder_registration *r, int max_it, double th,
                            int update, int debug) {
   return guarded([&] {
     BINDER_REQUIRE(r, "null handle");
     r->require_ready();
     r->trace.clear();
+    if (max_it <= 0) return;
+    const size_t nf = (size_t)r->N * r->nv, KL = (size_t)r->K * r->L;
+    if (r->dslot != 0) {  // the current field goes back to slot 0
+      HIP_OK(hipMemcpyAsync(r->d.get(), r->dcur(), nf * sizeof(float), hipMemcpyDeviceToDevice,
+                            r->s()));
+      r->dslot = 0;
+    }
+    r->em_d.alloc(nf);
+    r->em_dream.alloc(nf);
+    r->em_stats.alloc(32);
+    if (update) {
+      r->em_theta.alloc(KL);
+      r->em_thT.alloc(KL);
+    }
+    float *F[2] = {r->d.get(), r->em_d.get()};
+    float *DR[2] = {r->dream.get(), r->em_dream.get()};
+    // theta only changes with update_likelihood_parameters (MAP step)
+    double *TH[2] = {r->theta.get(), update ? r->em_theta.get() : r->theta.get()};
+    float *TT[2] = {r->thT.get(), update ? r->em_thT.get() : r->thT.get()};
+    double *ST[2] = {r->em_stats.get(), r->em_stats.get() + 16};
+    double *host = r->host_stats.p;  // pinned: parity p at host + 8 p
+    auto enqueue = [&](int k) {
+      const int p = k & 1, q = p ^ 1;
+      EStepArgs a = r->estep_args(false, 0, 0, 0);
+      a.d = F[p];
+      a.thT = TT[p];
+      a.dream = DR[p];
+      {
+        TimedLaunch t(r->tm(), "estep", r->s());
+        launch_estep(a, r->is2d, false, r->estep_blocks, r->s());
+      }
+      r->hist_reduce_on(r->s());
+      r->run_theta(update ? kThetaMap : kThetaKeep, 0, 0, 0, nullptr, nullptr,
+                   TH[update ? q : p], TT[update ? q : p], ST[p]);
+      SmoothOptions opt;
+      opt.prior_out = ST[p] + 2;
+      r->sm->run(DR[p], r->x.get(), F[q], opt, r->tm());
+      HIP_OK(hipMemcpyAsync(host + 8 * p, ST[p], 3 * sizeof(double), hipMemcpyDeviceToHost,
+                            r->s()));
+      HIP_OK(hipEventRecord(r->em_done[p].e, r->s()));
+    };
     double mlp_old = 1e100;
-    int it = 0;
-    bool converged = false;
-    SmoothOptions opt;
-    opt.prior_out = r->stats.get() + 2;
-    while (it < max_it && !converged) {
-      r->e_and_theta(false, update ? kThetaMap : kThetaKeep, 0, 0, 0);
-      r->sm->run(r->dream.get(), r->x.get(), r->dcur(), opt, r->tm());
-      HIP_OK(hipMemcpyAsync(r->host_stats.p, r->stats.get(), 3 * sizeof(double),
-                            hipMemcpyDeviceToHost, r->s()));
-      HIP_OK(hipStreamSynchronize(r->s()));
-      const double mll = r->host_stats.p[0], pt = r->host_stats.p[1],
-                   pd = r->host_stats.p[2];
+    int k = 0;
+    bool stop = false;
+    enqueue(0);
+    while (!stop) {
+      // queue the next iteration before reading this one's objective; if
+      // this one turns out to be the last, the next one's outputs (in the
+      // other parity) are simply not used
+      if (k + 1 < max_it) enqueue(k + 1);
+      HIP_OK(hipEventSynchronize(r->em_done[k & 1].e));
+      const double *hs = host + 8 * (k & 1);
+      const double mll = hs[0], pt = hs[1], pd = hs[2];
       const double mlp = mll + pd + pt;
       r->trace.insert(r->trace.end(), {mll, pt, pd, mlp});
       if (debug)
         std::printf("EM %d: mll %.8f prior_theta %.8f prior_d %.8f mlp %.8f "
                     "per nonzero voxel %.8f\n",
-                    it + 1, mll, pt, pd, mlp, mlp / r->nnz);
-      if ((mlp_old - mlp) / r->nnz < th) converged = true;
+                    k + 1, mll, pt, pd, mlp, mlp / r->nnz);
+      if ((mlp_old - mlp) / r->nnz < th) stop = true;
       mlp_old = mlp;
-      ++it;
+      ++k;
+      if (k >= max_it) stop = true;
     }
+    HIP_OK(hipStreamSynchronize(r->s()));
+    // k iterations counted: the state after them is parity k & 1, the dream
+    // locations are those of iteration k - 1 (parity (k - 1) & 1)
+    if (k & 1) {
+      swap_buffers(r->d, r->em_d);
+      if (update) {
+        swap_buffers(r->theta, r->em_theta);
+        swap_buffers(r->thT, r->em_thT);
+      }
+    }
+    if ((k - 1) & 1) swap_buffers(r->dream, r->em_dream);
   });
 }
 
@@ -660,7 +782,7 @@ int binder_registration_sampler_run(binder_registration *r, int seed, int N_b,
     r->samples = N_s;
     uint32_t k0, k1;
     sampler_key(seed, k0, k1);
-    if (st && !r->dextra.get()) r->dextra.alloc((size_t)(kWelfordBatch - 1) * N * r->nv);
+    if (st) r->prepare_ring(N_b + N_s);
     r->enqueue_sweeps(N_b, N_s, sample_gamma, every, sp, sl, st, k0, k1, nullptr,
                       r->mean.get(), r->m2.get());
     HIP_OK(hipStreamSynchronize(r->s()));
@@ -682,29 +804,46 @@ int binder_registration_sampler_chains(binder_registration *r, int nchains, cons
     const size_t nf = (size_t)N * r->nv, KL = (size_t)r->K * r->L;
     r->mean.alloc(nf);
     r->m2.alloc((size_t)npairs * r->nv);
-    DBuf<double> cmean, cm2, th0;
-    cmean.alloc(nf);
-    cm2.alloc((size_t)npairs * r->nv);
-    DBuf<float> d0, thT0;
-    d0.alloc(nf);
+    if (st) r->prepare_ring(N_b + N_s);  // the current field is now slot 0
+    // slot 0 keeps the start field unless the ring wraps onto it (more than
+    // kWelfordBatch - 1 sweeps): only then does a chain restore it
+    const bool wraps = st && N_b + N_s + 1 > kWelfordBatch;
+    DBuf<float> d0;
+    DBuf<double> th0;
+    DBuf<float> thT0;
     th0.alloc(KL);
     thT0.alloc(KL);
-    if (st && !r->dextra.get()) r->dextra.alloc((size_t)(kWelfordBatch - 1) * nf);
-    // the start state every chain begins from
-    HIP_OK(hipMemcpyAsync(d0.get(), r->dcur(), nf * sizeof(float), hipMemcpyDeviceToDevice,
-                          r->s()));
     HIP_OK(hipMemcpyAsync(th0.get(), r->theta.get(), KL * sizeof(double),
                           hipMemcpyDeviceToDevice, r->s()));
     HIP_OK(hipMemcpyAsync(thT0.get(), r->thT.get(), KL * sizeof(float),
                           hipMemcpyDeviceToDevice, r->s()));
+    if (wraps) {
+      d0.alloc(nf);
+      HIP_OK(hipMemcpyAsync(d0.get(), r->d.get(), nf * sizeof(float), hipMemcpyDeviceToDevice,
+                            r->s()));
+    }
     // chain c's key is sampler_key(seeds[c]): the same draws as
-    // sampler_run(seeds[c], ...).  The sweeps use key (0, 0) ^ kslot, and
-    // kslot is set per chain from the table, so one graph serves all chains.
-    std::vector<uint32_t> keys((size_t)2 * nchains);
-    for (int c = 0; c < nchains; ++c) sampler_key(seeds[c], keys[2 * c], keys[2 * c + 1]);
+    // sampler_run(seeds[c], ...).  The sweeps use key (0, 0) ^ slot[0..1]
+    // and fold samples with Welford counts offset by slot[2..3] (c * N_s), so
+    // the accumulators see all chains' samples in order and one graph serves
+    // every chain.
+    std::vector<uint32_t> tab((size_t)4 * nchains);
+    for (int c = 0; c < nchains; ++c) {
+      sampler_key(seeds[c], tab[4 * c], tab[4 * c + 1]);
+      const unsigned long long base = (unsigned long long)c * (unsigned long long)N_s;
+      tab[4 * c + 2] = (uint32_t)base;
+      tab[4 * c + 3] = (uint32_t)(base >> 32);
+    }
     DBuf<uint32_t> ktab, kslot;
-    ktab.upload(keys.data(), keys.size(), r->s());
-    kslot.alloc(2);
+    ktab.upload(tab.data(), tab.size(), r->s());
+    kslot.alloc(4);
+    const uint32_t *keyx = kslot.get();
+    const long long *n0b = reinterpret_cast<const long long *>(kslot.get() + 2);
+    hipLaunchKernelGGL(k_zero_f64, dim3(1024), dim3(256), 0, r->s(), r->mean.get(),
+                       (long long)nf);
+    hipLaunchKernelGGL(k_zero_f64, dim3(1024), dim3(256), 0, r->s(), r->m2.get(),
+                       (long long)npairs * r->nv);
+    HIP_LAUNCH_OK();
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     auto release = [&] {
@@ -714,8 +853,9 @@ int binder_registration_sampler_chains(binder_registration *r, int nchains, cons
       graph = nullptr;
     };
     auto restore = [&] {
-      HIP_OK(hipMemcpyAsync(r->slot(0), d0.get(), nf * sizeof(float), hipMemcpyDeviceToDevice,
-                            r->s()));
+      if (wraps)
+        HIP_OK(hipMemcpyAsync(r->d.get(), d0.get(), nf * sizeof(float),
+                              hipMemcpyDeviceToDevice, r->s()));
       HIP_OK(hipMemcpyAsync(r->theta.get(), th0.get(), KL * sizeof(double),
                             hipMemcpyDeviceToDevice, r->s()));
       HIP_OK(hipMemcpyAsync(r->thT.get(), thT0.get(), KL * sizeof(float),
@@ -724,34 +864,23 @@ int binder_registration_sampler_chains(binder_registration *r, int nchains, cons
     };
     try {
       for (int c = 0; c < nchains; ++c) {
-        restore();
-        HIP_OK(hipMemcpyAsync(kslot.get(), ktab.get() + 2 * c, 2 * sizeof(uint32_t),
+        if (c > 0) restore();
+        HIP_OK(hipMemcpyAsync(kslot.get(), ktab.get() + 4 * c, 4 * sizeof(uint32_t),
                               hipMemcpyDeviceToDevice, r->s()));
-        double *acc_m = c == 0 ? r->mean.get() : cmean.get();
-        double *acc_2 = c == 0 ? r->m2.get() : cm2.get();
-        hipLaunchKernelGGL(k_zero_f64, dim3(1024), dim3(256), 0, r->s(), acc_m, (long long)nf);
-        hipLaunchKernelGGL(k_zero_f64, dim3(1024), dim3(256), 0, r->s(), acc_2,
-                           (long long)npairs * r->nv);
-        HIP_LAUNCH_OK();
         if (!use_graph || c == 0) {
           // chain 0 runs eagerly: it also builds every lazily made buffer and
           // pass matrix, so that the capture below records kernels only
-          r->enqueue_sweeps(N_b, N_s, 0, 1, sp, sl, st, 0u, 0u, kslot.get(), acc_m, acc_2);
+          r->enqueue_sweeps(N_b, N_s, 0, 1, sp, sl, st, 0u, 0u, keyx, r->mean.get(),
+                            r->m2.get(), n0b);
         } else {
           if (!exec) {
             HIP_OK(hipStreamBeginCapture(r->s(), hipStreamCaptureModeThreadLocal));
-            r->enqueue_sweeps(N_b, N_s, 0, 1, sp, sl, st, 0u, 0u, kslot.get(), cmean.get(),
-                              cm2.get());
+            r->enqueue_sweeps(N_b, N_s, 0, 1, sp, sl, st, 0u, 0u, keyx, r->mean.get(),
+                              r->m2.get(), n0b);
             HIP_OK(hipStreamEndCapture(r->s(), &graph));
             HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
           }
           HIP_OK(hipGraphLaunch(exec, r->s()));
-        }
-        if (c > 0) {
-          hipLaunchKernelGGL(k_chan_merge, dim3(grid_blocks(r->nv)), dim3(256), 0, r->s(),
-                             r->mean.get(), r->m2.get(), cmean.get(), cm2.get(),
-                             (double)c * N_s, (double)N_s, r->nv, N);
-          HIP_LAUNCH_OK();
         }
       }
       restore();

@@ -1104,6 +1104,22 @@ long long contract_blocks(long long R, int comps) {
   return nonpersist > p ? nonpersist : p;
 }
 
+// D from its separable FP64 terms, in the operation order of the host loop
+// (binder_dct_prior_eigenvalues): ((tx + ty) + tz), squared by one multiply.
+__global__ __launch_bounds__(256) void k_dct_prior(float *__restrict__ D,
+                                                   const double *__restrict__ t, int nx, int ny,
+                                                   int nz, int order) {
+  const long long nv = (long long)nx * ny * nz;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int z = (int)(i % nz);
+  const int y = (int)((i / nz) % ny);
+  const int x = (int)(i / ((long long)ny * nz));
+  const double sxy = t[x] + t[nx + y];
+  const double s = nz > 1 ? sxy + t[nx + ny + z] : sxy;
+  D[i] = (float)(order == 1 ? s : s * s);
+}
+
 Smoother::Smoother(int nx, int ny, int nz, const double gamma[3],
                    const double rho[3], const double eta[3],
                    const float *D_host, double sigma, hipStream_t stream)
@@ -1113,12 +1129,36 @@ Smoother::Smoother(int nx, int ny, int nz, const double gamma[3],
                  "deformation grid must be at least 2 x 2 (x 1)");
   BINDER_REQUIRE(nx <= 256 && ny <= 256 && nz <= 256,
                  "each grid axis must be at most 256 voxels");
+  D_.upload(D_host, nv_, stream_);
+  init_buffers(gamma, rho, eta);
+}
+
+Smoother::Smoother(int nx, int ny, int nz, const double gamma[3], const double rho[3],
+                   const double eta[3], const double *axis_terms, int order, double sigma,
+                   hipStream_t stream)
+    : nx_(nx), ny_(ny), nz_(nz), N_(nz == 1 ? 2 : 3),
+      nv_((long long)nx * ny * nz), sigma_(sigma), stream_(stream) {
+  BINDER_REQUIRE(nx >= 2 && ny >= 2 && nz >= 1,
+                 "deformation grid must be at least 2 x 2 (x 1)");
+  BINDER_REQUIRE(nx <= 256 && ny <= 256 && nz <= 256,
+                 "each grid axis must be at most 256 voxels");
+  BINDER_REQUIRE(order == 1 || order == 2, "device-made D supports order 1 or 2");
+  DBuf<double> t;
+  t.upload(axis_terms, (size_t)nx + ny + nz, stream_);
+  D_.alloc(nv_);
+  hipLaunchKernelGGL(k_dct_prior, dim3((unsigned)((nv_ + 255) / 256)), dim3(256), 0, stream_,
+                     D_.get(), t.get(), nx, ny, nz, order);
+  HIP_LAUNCH_OK();
+  init_buffers(gamma, rho, eta);  // synchronizes before t is freed
+}
+
+void Smoother::init_buffers(const double gamma[3], const double rho[3], const double eta[3]) {
+  const int ny = ny_, nz = nz_;
   for (int i = 0; i < 3; ++i) {
     rho_[i] = rho[i];
     eta_[i] = eta[i];
   }
   nu_ = gamma[0] / (rho[0] * rho[1] * rho[2] * eta[0] * eta[0] * (double)nv_);
-  D_.upload(D_host, nv_, stream_);
   gamma_.upload(gamma, 3, stream_);
   bufA_.alloc((size_t)N_ * nv_);
   bufB_.alloc((size_t)N_ * nv_);

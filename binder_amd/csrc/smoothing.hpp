@@ -95,6 +95,12 @@ class Smoother {
   Smoother(int nx, int ny, int nz, const double gamma[3], const double rho[3],
            const double eta[3], const float *D_host, double sigma,
            hipStream_t stream);
+  // The same with D made on the device from its separable terms:
+  // D[x][y][z] = (float)(t[x] + t[nx + y] (+ t[nx + ny + z]))^order (FP64,
+  // additions in axis order; order 1 or 2), terms t of length nx + ny + nz.
+  Smoother(int nx, int ny, int nz, const double gamma[3], const double rho[3],
+           const double eta[3], const double *axis_terms, int order, double sigma,
+           hipStream_t stream);
 
   // dream, x: [N][nv] SoA on the device (x may be null: zero).  out: [N][nv].
   // Computes out = x + IDCT(filter(DCT(dream - x)) [+ noise]).
@@ -119,6 +125,7 @@ class Smoother {
   double sigma_, rho_[3], eta_[3], nu_;
   bool index_grid_ = false;
   hipStream_t stream_;
+  void init_buffers(const double gamma[3], const double rho[3], const double eta[3]);
   DBuf<float> D_, bufA_, bufB_;
   DBuf<double> gamma_, part_cost_, part_raw_, scal_;
   MatrixCache mats_;

@@ -76,17 +76,22 @@ def network_grid(full_shape):
     return tuple(out)
 
 
-def pair_inputs(full_binned_voxels, full_binned_nodes, grid):
+def pair_inputs(full_binned_voxels, full_binned_nodes, grid, check=True):
     """Network inputs of one pair: the half-resolution pyramid level's bins
     (full[::2, ::2, ::2], as FastRegistration.move_resolution(2) takes them)
-    zero-padded into `grid`, uint8."""
+    zero-padded into `grid`, uint8.  check=False skips the [0, 255] range
+    test (for callers whose bin counts are at most 256)."""
     out = []
     for b in (full_binned_voxels, full_binned_nodes):
         sub = np.asarray(b)[::2, ::2, ::2]
-        if np.any(sub < 0) or np.any(sub > 255):
+        if check and (sub.min() < 0 or sub.max() > 255):
             raise ValueError("bins must be in [0, 255]")
-        a = np.zeros(grid, np.uint8)
-        a[:sub.shape[0], :sub.shape[1], :sub.shape[2]] = sub
+        a8 = sub.astype(np.uint8)
+        if sub.shape == tuple(grid):
+            a = a8
+        else:
+            a = np.zeros(grid, np.uint8)
+            a[:sub.shape[0], :sub.shape[1], :sub.shape[2]] = a8
         out.append(a)
     return out[0], out[1]
 
@@ -228,7 +233,11 @@ class WarmStartNet:
         return out[0] if single else out
 
     def predict_pair(self, full_binned_voxels, full_binned_nodes, L=32, K=32):
-        f, m = pair_inputs(full_binned_voxels, full_binned_nodes, self.grid)
+        """Coefficients for one pair from its full-resolution bins (in
+        [0, L) and [0, K); L, K <= 256)."""
+        if not (2 <= L <= 256 and 2 <= K <= 256):
+            raise ValueError("bin counts must be in [2, 256]")
+        f, m = pair_inputs(full_binned_voxels, full_binned_nodes, self.grid, check=False)
         return self.predict(f, m, L, K)
 
     def save(self, path, **meta):

@@ -105,6 +105,17 @@ int binder_registration_set_nonlinear_transformation(
     double nodes_size_x, double nodes_size_y, double nodes_size_z,
     const float *D);
 
+/* setNonLinearTransformation with the prior spectrum D of
+ * binder_dct_prior_eigenvalues(Nx, Ny, Nz, N, order, voxel sizes) made on the
+ * device from its separable terms (bit-identical float32 values, order 1 or
+ * 2), so no D array crosses PCIe.  No reference counterpart: the reference
+ * takes D from BINDER/utils.py:331-349 through setNonLinearTransformation
+ * (BINDER/src/gpu/bindings.cpp:72-95). */
+int binder_registration_set_nonlinear_transformation_dct_prior(
+    binder_registration *reg, double gamma_x, double gamma_y, double gamma_z,
+    double voxels_size_x, double voxels_size_y, double voxels_size_z,
+    double nodes_size_x, double nodes_size_y, double nodes_size_z, int order);
+
 /* EM(max_EM_iterations, convergence_th, update_likelihood_parameters, debug)
  * -- BINDER/src/gpu/bindings.cpp:99-103, BINDER/src/cpu/Registration.cpp:122-217.
  * Iterates E-step -> theta M-step -> deformation M-step until
@@ -144,16 +155,15 @@ int binder_registration_sampler_result(binder_registration *reg,
 /* Independent Gibbs chains from the handle's current state: chain c runs
  * N_b burn-in and N_s sample sweeps exactly as
  * binder_registration_sampler_run(seeds[c], N_b, N_s, 0, 1, sp, sl, st)
- * would (same draws), each from the same start field and theta; the mean
- * and M2 accumulators pool all chains' samples (Welford within a chain,
- * Chan et al.'s pairwise merge across chains), read with
- * binder_registration_sampler_result (covariance = M2 / (nchains N_s - 1)).
- * The handle's field and theta are restored afterwards.  use_graph != 0
- * captures one chain's sweeps into a HIP graph and replays it for chains
- * 1..nchains-1 (the Philox key is read from device memory, so the graph is
- * seed-independent).  No nu sampling (sample_gamma = 0).  No reference
- * counterpart: the reference runs one chain per sampler() call
- * (BINDER/src/cpu/Registration.cpp:515-642). */
+ * would (same draws), each from the same start field and theta; the FP64
+ * Welford accumulators fold all chains' samples, chain after chain, read
+ * with binder_registration_sampler_result (covariance = M2 / (nchains N_s
+ * - 1)).  The handle's field and theta are restored afterwards.
+ * use_graph != 0 captures one chain's sweeps into a HIP graph and replays
+ * it for chains 1..nchains-1 (the Philox key and the Welford sample offset
+ * are read from device memory, so the graph is chain-independent).  No nu
+ * sampling (sample_gamma = 0).  No reference counterpart: the reference
+ * runs one chain per sampler() call (BINDER/src/cpu/Registration.cpp:515-642). */
 int binder_registration_sampler_chains(binder_registration *reg, int nchains,
                                        const int *seeds, int N_b, int N_s,
                                        int sample_posteriors,

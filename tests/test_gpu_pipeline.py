@@ -181,3 +181,27 @@ def test_resampled_registration_index_grid(core):
                           b.get_final_locations_as_numpy_array())
     assert np.array_equal(src.get_final_locations_as_numpy_array(),
                           prob.final_locations.astype(np.float32))
+
+
+@pytest.mark.parametrize("shape", [(24, 20, 18), (40, 36, 1)])
+def test_device_prior_spectrum_matches_uploaded(core, shape):
+    """setNonLinearTransformationDCTPrior (D made on the GPU) gives exactly
+    the EM trajectory of setNonLinearTransformation with the host D."""
+    from binder_amd import synth
+    print(f"device-made D, {shape}", flush=True)
+    nd = 2 if shape[2] == 1 else 3
+    pair = synth.make_pair(shape[:nd], 4, 1.5, seed=2, semi=(3.0, 8.0))
+    prob = synth.problem_from_pair(pair, gamma=1e-4)
+    vs = (1.0, 1.0, 1.0)
+    D = core.dct_prior_eigenvalues(prob.shape[:nd], 2, nd, voxel_size=vs)
+    fields = []
+    for device in (False, True):
+        r = synth.build_registration(core, prob)
+        if device:
+            r.setNonLinearTransformationDCTPrior(*prob.gammas, *vs, *vs, 2)
+        else:
+            r.setNonLinearTransformation(*prob.gammas, *vs, *vs, D.ravel())
+        r.EM(6, -1e30, 1, 0)
+        fields.append((r.get_final_locations_as_numpy_array(), r.get_objective_trace()))
+    np.testing.assert_array_equal(fields[0][0], fields[1][0])
+    np.testing.assert_array_equal(fields[0][1], fields[1][1])
