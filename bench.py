@@ -337,6 +337,56 @@ def batched_report(args):
     return out
 
 
+def cnn_report(args):
+    """The warm-start network alone at the 160x192x160 workload's input grid
+    (80x96x80, two input channels): inference latency of one pair and the
+    time of one training step (batch 8: forward, MSE, backward, Adam), with
+    the convolutions' algorithmic flop rate (2 * 27 * cin * cout per output
+    voxel per layer, x3 for a training step: forward, input and weight
+    gradients; the first layer's input gradient is not computed)."""
+    from binder_amd import warmstart
+
+    grid = warmstart.network_grid((160, 192, 160))
+    rng = np.random.default_rng(args.seed)
+    n = 16
+    fb = rng.integers(0, 32, (n,) + grid).astype(np.uint8)
+    mb = rng.integers(0, 32, (n,) + grid).astype(np.uint8)
+    tg = (0.1 * rng.standard_normal((n, 3, 8, 8, 8))).astype(np.float32)
+    net = warmstart.WarmStartNet(grid, max_batch=8, seed=args.seed)
+    if os.path.exists(warmstart.WEIGHTS_C2):
+        with np.load(warmstart.WEIGHTS_C2, allow_pickle=False) as z:
+            net.set_params(z["params"])
+    net.set_dataset(fb, mb, tg)
+    nv = [int(np.prod([g >> lvl for g in grid])) for lvl in range(4)]
+    levels = [0, 0, 1, 1, 2, 2, 3, 3, 3, 3, 2, 2]
+    fwd = sum(2.0 * 27 * ci * co * nv[lv]
+              for (ci, co), lv in zip(warmstart.CONV_LAYERS, levels))
+    bwd_data = sum(2.0 * 27 * ci * co * nv[lv]
+                   for i, ((ci, co), lv) in enumerate(zip(warmstart.CONV_LAYERS, levels)) if i)
+    for _ in range(3):
+        net.predict(fb[0], mb[0])
+    reps = max(args.steps // 4, 10)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.predict(fb[0], mb[0])
+    infer_s = (time.perf_counter() - t0) / reps
+    idx = np.stack([rng.choice(n, 8, replace=False) for _ in range(reps + 3)]).astype(np.int32)
+    net.train(idx[:3])
+    t0 = time.perf_counter()
+    net.train(idx[3:])
+    step_s = (time.perf_counter() - t0) / reps
+    train_flops = 8 * (2 * fwd + bwd_data)
+    return {"report": "cnn", "grid": list(grid), "params": net.nparam,
+            "inference": {"batch": 1, "ms": round(1e3 * infer_s, 3),
+                          "conv_gflop": round(fwd / 1e9, 2),
+                          "conv_tflops": round(fwd / infer_s / 1e12, 2),
+                          "note": "wall-clock incl. host bins upload and coefficient download"},
+            "training_step": {"batch": 8, "ms": round(1e3 * step_s, 3),
+                              "conv_gflop": round(train_flops / 1e9, 2),
+                              "conv_tflops": round(train_flops / step_s / 1e12, 2)},
+            "bf16_dense_peak_tflops": 2500.0}
+
+
 def cpu_em_estimate(pair, args, counts, levels):
     """FP64 oracle: seconds per EM iteration at each pyramid level (all host
     cores), times the GPU run's iteration counts."""
@@ -692,6 +742,8 @@ def main():
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start comparison")
     ap.add_argument("--warmstart-eval", type=int, default=0, metavar="N",
                     help="only the warm-vs-cold report over N held-out pairs")
+    ap.add_argument("--cnn", action="store_true",
+                    help="only the warm-start network's inference / training-step report")
     ap.add_argument("--batched", type=int, default=0, metavar="N",
                     help="only the many-pairs / many-chains report over N pairs")
     ap.add_argument("--batch-workers", type=int, default=8)
@@ -709,6 +761,9 @@ def main():
         return
     if args.batched:
         print(json.dumps(batched_report(args)), flush=True)
+        return
+    if args.cnn:
+        print(json.dumps(cnn_report(args)), flush=True)
         return
     if args.copy_calibration:
         copy_calibration()

@@ -78,7 +78,7 @@ class FastRegistration:
         self.increase_gamma_at_DS_resolution = increase_gamma_at_DS_resolution
         self.threads = threads
         self.seed = seed
-        self.D = None
+        self._D = None
 
         voxels = np.asarray(voxels, dtype=np.float64)
         nodes = np.asarray(nodes, dtype=np.float64)
@@ -204,6 +204,26 @@ class FastRegistration:
     @property
     def nodes_pos(self):
         return self._level_positions("n")
+
+    @property
+    def D(self):
+        """The current level's DCT prior spectrum (utils.py:331-349); the GPU
+        path makes it on the device instead and never reads this."""
+        if self._D is None:
+            if hasattr(Registration, "dct_prior_eigenvalues"):
+                # the float32 values this backend keeps (utils.py:331-349 rounded)
+                self._D = Registration.dct_prior_eigenvalues(
+                    self.voxels_shape, self.finite_difference_order,
+                    self.number_of_dimensions, voxel_size=self.current_voxels_voxel_size)
+            else:
+                self._D = utils.initialize_fourier_domain_smoothing(
+                    self.voxels_shape, True, self.finite_difference_order,
+                    self.number_of_dimensions, voxel_size=self.current_voxels_voxel_size)
+        return self._D
+
+    @D.setter
+    def D(self, value):
+        self._D = value
 
     def _set_gammas(self, shape_for_scaling):
         # gamma per voxel at 1 mm, scaled by voxel volume, voxel count and the
@@ -444,15 +464,7 @@ class FastRegistration:
 
         self.binned_voxels = self.full_binned_voxels[::res, ::res, ::res]
         self.binned_nodes = self.full_binned_nodes[::res, ::res, ::res]
-        if hasattr(Registration, "dct_prior_eigenvalues"):
-            # the float32 values this backend keeps (utils.py:331-349 rounded)
-            self.D = Registration.dct_prior_eigenvalues(
-                self.voxels_shape, self.finite_difference_order, self.number_of_dimensions,
-                voxel_size=self.current_voxels_voxel_size)
-        else:
-            self.D = utils.initialize_fourier_domain_smoothing(
-                self.voxels_shape, True, self.finite_difference_order,
-                self.number_of_dimensions, voxel_size=self.current_voxels_voxel_size)
+        self._D = None  # the level's D, made on first use (D below)
         self.non_zero_voxels = np.count_nonzero(self.binned_voxels)
         self._set_gammas(self.orig_voxels_shape if self.increase_gamma_at_DS_resolution
                          else self.voxels_shape)

@@ -174,10 +174,9 @@ __device__ __forceinline__ void slice_columns(const uint32_t q[16], int xa,
 // x-slice (16 or 4 nodes) at a time, so only a slice's packed bins are live.
 template <bool IS2D, bool SAMPLE>
 __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
-  if (a.keyx) {  // per-chain key mask (captured sweeps)
-    a.k0 ^= a.keyx[0];
-    a.k1 ^= a.keyx[1];
-  }
+  // Philox key, with the per-chain device mask when given (captured sweeps)
+  const uint32_t key0 = a.keyx ? a.k0 ^ a.keyx[0] : a.k0;
+  const uint32_t key1 = a.keyx ? a.k1 ^ a.keyx[1] : a.k1;
   extern __shared__ __align__(16) unsigned char smem[];
   const int KL = a.K * a.L;
   unsigned long long *hist = reinterpret_cast<unsigned long long *>(smem);
@@ -317,8 +316,8 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
       // the first node whose running sum reaches r*Z.  The (x,y) column is
       // found from the column sums, then the node within the column.
       const u32x4 rr = philox4x32_10(
-          u32x4{(uint32_t)ii, (uint32_t)(ii >> 32), a.sweep, kStreamEStep}, a.k0,
-          a.k1);
+          u32x4{(uint32_t)ii, (uint32_t)(ii >> 32), a.sweep, kStreamEStep}, key0,
+          key1);
       const float target = u01f(rr.x) * Z;
       int scol = -1, lastcol = -1;
       float before = 0.f, cum = 0.f, before_last = 0.f;
@@ -427,10 +426,9 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
 // selected column.  LDS: theta rows (padded stride) | 32-bit counts [K*L].
 template <bool IS2D>
 __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
-  if (a.keyx) {  // per-chain key mask (captured sweeps)
-    a.k0 ^= a.keyx[0];
-    a.k1 ^= a.keyx[1];
-  }
+  // Philox key, with the per-chain device mask when given (captured sweeps)
+  const uint32_t key0 = a.keyx ? a.k0 ^ a.keyx[0] : a.k0;
+  const uint32_t key1 = a.keyx ? a.k1 ^ a.keyx[1] : a.k1;
   extern __shared__ __align__(16) unsigned char smem[];
   const int KL = a.K * a.L;
   const int Kp = theta_row_stride(a.K, a.L);
@@ -463,7 +461,7 @@ __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
       if (qv == 0 || row == rbeg) {  // wavefront-uniform
         const long long g = (row >> 2) * 64 + lane;
         rr = philox4x32_10(u32x4{(uint32_t)g, (uint32_t)(g >> 32), a.sweep, kStreamEStep},
-                           a.k0, a.k1);
+                           key0, key1);
         for (int k = 0; k < qv; ++k) rr = u32x4{rr.y, rr.z, rr.w, rr.x};
       }
       const long long i = row * 64 + lane;
@@ -715,10 +713,9 @@ __global__ __launch_bounds__(256) void k_hist_reduce2(
 // Dirichlet(H + alpha) draw, or no change; then the Dirichlet prior cost and
 // the transposed FP32 table.
 __global__ __launch_bounds__(1024) void k_theta(ThetaArgs a) {
-  if (a.keyx) {  // per-chain key mask (captured sweeps)
-    a.k0 ^= a.keyx[0];
-    a.k1 ^= a.keyx[1];
-  }
+  // Philox key, with the per-chain device mask when given (captured sweeps)
+  const uint32_t key0 = a.keyx ? a.k0 ^ a.keyx[0] : a.k0;
+  const uint32_t key1 = a.keyx ? a.k1 ^ a.keyx[1] : a.k1;
   const int KL = a.K * a.L;
   const int tid = threadIdx.x;
   __shared__ double red[1024];
@@ -731,7 +728,7 @@ __global__ __launch_bounds__(1024) void k_theta(ThetaArgs a) {
     const double h = a.Hd[kl];
     if (a.mode == kThetaDirichlet)
       sh[kl] = gamma_mt(h + a.alpha, u32x4{(uint32_t)kl, a.sweep, 0u, kStreamTheta},
-                        a.k0, a.k1);
+                        key0, key1);
     else if (a.mode == kThetaMap)
       sh[kl] = h + a.alpha - 1.0;
     else

@@ -116,10 +116,6 @@ __device__ __forceinline__ void normals4(u32x4 r, float n[4]) {
 // straight-line code and the B loads of a whole output tile issue together).
 template <int KS, bool VEC, int PRO, int EPI>
 __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
-  if (a.keyx) {  // per-chain key mask (captured sweeps)
-    a.k0 ^= a.keyx[0];
-    a.k1 ^= a.keyx[1];
-  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int comp = blockIdx.y;
   const long long row0 = (long long)blockIdx.x * kRowsPerBlock + wave * 16;
@@ -243,7 +239,7 @@ __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
         const u32x4 r = philox4x32_10(
             u32x4{(uint32_t)orow, (uint32_t)kc, a.sweep,
                   ((uint32_t)comp << 16) | a.stream},
-            a.k0, a.k1);
+            a.keyx ? a.k0 ^ a.keyx[0] : a.k0, a.keyx ? a.k1 ^ a.keyx[1] : a.k1);
         normals4(r, eps);
       }
 #pragma unroll
@@ -308,10 +304,6 @@ constexpr size_t kPersistMaxLds = 144 * 1024;
 template <int KS, bool VEC, int PRO, int EPI>
 __global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
                                                                  int comps) {
-  if (a.keyx) {  // per-chain key mask (captured sweeps)
-    a.k0 ^= a.keyx[0];
-    a.k1 ^= a.keyx[1];
-  }
   extern __shared__ __align__(16) uint4 bsh[];  // [nt][KS][64] hi, then lo
   const int nt_count = (a.n_out + 15) >> 4;
   const int nfrag = nt_count * KS * 64;
@@ -424,7 +416,7 @@ __global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
           const u32x4 r = philox4x32_10(
               u32x4{(uint32_t)orow, (uint32_t)kc, a.sweep,
                     ((uint32_t)comp << 16) | a.stream},
-              a.k0, a.k1);
+              a.keyx ? a.k0 ^ a.keyx[0] : a.k0, a.keyx ? a.k1 ^ a.keyx[1] : a.k1);
           normals4(r, eps);
         }
 #pragma unroll
@@ -577,7 +569,7 @@ __device__ __forceinline__ void fused_forward(const FusedArgs &a, const uint4 *m
           const u32x4 r = philox4x32_10(
               u32x4{c, ((uint32_t)m0 << 1) | (uint32_t)PAR, a.sweep,
                     ((uint32_t)comp << 16) | a.stream},
-              a.k0, a.k1);
+              a.keyx ? a.k0 ^ a.keyx[0] : a.k0, a.keyx ? a.k1 ^ a.keyx[1] : a.k1);
           normals4(r, eps);
         }
         // D p^2 summed over the 4 rows in FP32, then carried in FP64
@@ -613,10 +605,6 @@ constexpr int kFusedThreads = 512;  // 2 wavefronts per SIMD: 256 VGPRs each
 // is in range.
 template <int NTH, bool EXACT>
 __global__ __launch_bounds__(kFusedThreads) void k_dct_x_fused(FusedArgs a, int comps) {
-  if (a.keyx) {  // per-chain key mask (captured sweeps)
-    a.k0 ^= a.keyx[0];
-    a.k1 ^= a.keyx[1];
-  }
   constexpr int KSH = (NTH + 1) / 2;
   constexpr int NF = NTH * KSH * 64;
   extern __shared__ __align__(16) uint4 msh[];

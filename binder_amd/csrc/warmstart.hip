@@ -171,7 +171,8 @@ struct WgradArgs {
   int ld_x, off_x;
   const uint16_t *g;
   int ld_g, off_g;
-  float *part;  // [P][COUT][27*CIN]
+  float *part;   // [P][COUT][27*CIN]
+  float *bpart;  // [P][COUT] bias-gradient partials (written by the centre tap)
   int X, Y, Z, nvox, per_block;
 };
 
@@ -194,6 +195,9 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
   const int v_end = min(a.nvox, v_begin + a.per_block);
   if (CP > CIN)
     for (int i = threadIdx.x; i < (CP - CIN) * PITCH; i += 256) xT[CIN * PITCH + i] = 0;
+  // the centre tap's blocks also sum g per channel: the bias gradient
+  const bool bias_block = tap == 13 && a.bpart != nullptr;
+  float bsum = 0.f;
   for (int v0 = v_begin; v0 < v_end; v0 += 64) {
     __syncthreads();
     constexpr int GG = COUT / 8;
@@ -229,6 +233,11 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
       for (int j = 0; j < 8; ++j) xT[(8 * cg + j) * PITCH + vox] = e[j];
     }
     __syncthreads();
+    if (bias_block && threadIdx.x < COUT) {
+      const uint16_t *row = &gT[threadIdx.x * PITCH];
+#pragma unroll 8
+      for (int v = 0; v < 64; ++v) bsum += bf2f(row[v]);
+    }
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
 #pragma unroll
@@ -245,6 +254,7 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
       }
     }
   }
+  if (bias_block && threadIdx.x < COUT) a.bpart[(size_t)p * COUT + threadIdx.x] = bsum;
   // lane holds dW[co = 16 mt + 4g + j][ci = 16 nt + r]
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {
@@ -261,6 +271,103 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
   }
 }
 
+// The same for narrow layers (cin, cout <= 32), where one tile per 64
+// voxels leaves most of a block idle: every wavefront stages and contracts
+// its own 64-voxel chunks in a private LDS region (no block barrier in the
+// loop) and writes its own partial row p * 4 + wave.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_wgrad_wave(WgradArgs a) {
+  constexpr int CP = CIN < 16 ? 16 : CIN;
+  constexpr int MT = COUT / 16, NTI = CP / 16, TILES = MT * NTI;
+  constexpr int PITCH = 72;
+  __shared__ __align__(16) uint16_t gT_all[4][COUT * PITCH];
+  __shared__ __align__(16) uint16_t xT_all[4][CP * PITCH];
+  const int tap = blockIdx.y, p = blockIdx.x;
+  const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int YZ = a.Y * a.Z;
+  uint16_t *gT = gT_all[wave];
+  uint16_t *xT = xT_all[wave];
+  f32x4_t acc[TILES];
+#pragma unroll
+  for (int t = 0; t < TILES; ++t) acc[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  const int v_begin = p * a.per_block;
+  const int v_end = min(a.nvox, v_begin + a.per_block);
+  if (CP > CIN)
+    for (int i = lane; i < (CP - CIN) * PITCH; i += 64) xT[CIN * PITCH + i] = 0;
+  const bool bias_block = tap == 13 && a.bpart != nullptr;
+  float bsum = 0.f;
+  for (int v0 = v_begin + 64 * wave; v0 < v_end; v0 += 256) {
+    // the previous chunk's LDS reads are issued before these writes (a
+    // wavefront's LDS operations complete in order)
+    __builtin_amdgcn_wave_barrier();
+    constexpr int GG = COUT / 8;
+    for (int it = lane; it < 64 * GG; it += 64) {
+      const int vox = it / GG, cg = it - vox * GG;
+      const int v = v0 + vox;
+      uint4 q = make_uint4(0u, 0u, 0u, 0u);
+      if (v < v_end)
+        q = *reinterpret_cast<const uint4 *>(a.g + (size_t)v * a.ld_g + a.off_g + 8 * cg);
+      const uint16_t *e = reinterpret_cast<const uint16_t *>(&q);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) gT[(8 * cg + j) * PITCH + vox] = e[j];
+    }
+    constexpr int XG = CIN / 8;
+    for (int it = lane; it < 64 * XG; it += 64) {
+      const int vox = it / XG, cg = it - vox * XG;
+      const int v = v0 + vox;
+      uint4 q = make_uint4(0u, 0u, 0u, 0u);
+      if (v < v_end) {
+        const int t = v / a.Z;
+        const int z = v - t * a.Z;
+        const int t2 = t / a.Y;
+        const int y = t - t2 * a.Y;
+        const int x = t2 % a.X;
+        const int px = x + dx, py = y + dy, pz = z + dz;
+        if ((unsigned)px < (unsigned)a.X && (unsigned)py < (unsigned)a.Y &&
+            (unsigned)pz < (unsigned)a.Z)
+          q = *reinterpret_cast<const uint4 *>(
+              a.x + (size_t)(v + dx * YZ + dy * a.Z + dz) * a.ld_x + a.off_x + 8 * cg);
+      }
+      const uint16_t *e = reinterpret_cast<const uint16_t *>(&q);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) xT[(8 * cg + j) * PITCH + vox] = e[j];
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (bias_block && lane < COUT) {
+      const uint16_t *row = &gT[lane * PITCH];
+#pragma unroll 8
+      for (int v = 0; v < 64; ++v) bsum += bf2f(row[v]);
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+#pragma unroll
+      for (int t = 0; t < TILES; ++t) {
+        const int mt = t / NTI, nt = t - mt * NTI;
+        const bf16x8_t A = __builtin_bit_cast(
+            bf16x8_t, *reinterpret_cast<const uint4 *>(&gT[(16 * mt + r) * PITCH + 32 * s + 8 * g]));
+        const bf16x8_t B = __builtin_bit_cast(
+            bf16x8_t, *reinterpret_cast<const uint4 *>(&xT[(16 * nt + r) * PITCH + 32 * s + 8 * g]));
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  const size_t q = (size_t)p * 4 + wave;
+  if (bias_block && lane < COUT) a.bpart[q * COUT + lane] = bsum;
+#pragma unroll
+  for (int t = 0; t < TILES; ++t) {
+    const int mt = t / NTI, nt = t - mt * NTI;
+    const int ci = 16 * nt + r;
+    if (ci >= CIN) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int co = 16 * mt + 4 * g + j;
+      a.part[(q * COUT + co) * (27 * CIN) + tap * CIN + ci] = acc[t][j];
+    }
+  }
+}
+
 // dst[i] = sum_p part[p * n + i], p ascending.
 __global__ void k_reduce_rows(const float *part, int P, long long n, float *dst) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -268,26 +375,6 @@ __global__ void k_reduce_rows(const float *part, int P, long long n, float *dst)
   float s = 0.f;
   for (int p = 0; p < P; ++p) s += part[(size_t)p * n + i];
   dst[i] = s;
-}
-
-// Per-block channel sums of a gradient tensor (bias gradients): part[p][c].
-__global__ __launch_bounds__(256) void k_chan_sum(const uint16_t *g, int ld, int off, int C,
-                                                  int nvox, int per_block, float *part) {
-  __shared__ float red[256];
-  const int lanes = 256 / C;
-  const int c = threadIdx.x % C, l = threadIdx.x / C;
-  const int v_begin = blockIdx.x * per_block;
-  const int v_end = min(nvox, v_begin + per_block);
-  float s = 0.f;
-  if (l < lanes)
-    for (int v = v_begin + l; v < v_end; v += lanes) s += bf2f(g[(size_t)v * ld + off + c]);
-  red[threadIdx.x] = l < lanes ? s : 0.f;
-  __syncthreads();
-  if (threadIdx.x < C) {
-    float t = 0.f;
-    for (int k = 0; k < lanes; ++k) t += red[k * C + threadIdx.x];
-    part[(size_t)blockIdx.x * C + threadIdx.x] = t;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -702,8 +789,25 @@ void launch_conv(int cin, int cout, const ConvArgs &a, hipStream_t s) {
   throw Error(BINDER_EINVAL, "no convolution kernel for this channel pair");
 }
 
+// Narrow layers use the per-wavefront kernel, which writes 4 partial rows
+// per block; returns the number of partial rows written.
+bool wgrad_per_wave(int cin, int cout) { return cin <= 32 && cout <= 32; }
+
 void launch_wgrad(int cin, int cout, const WgradArgs &a, int P, hipStream_t s) {
   const dim3 grid(P, 27), block(256);
+  if (wgrad_per_wave(cin, cout)) {
+#define BINDER_WGRAD_WAVE_CASE(CI, CO)                                     \
+    if (cin == CI && cout == CO) {                                         \
+      hipLaunchKernelGGL((k_wgrad_wave<CI, CO>), grid, block, 0, s, a);    \
+      HIP_LAUNCH_OK();                                                     \
+      return;                                                              \
+    }
+    BINDER_WGRAD_WAVE_CASE(8, 16)
+    BINDER_WGRAD_WAVE_CASE(16, 16)
+    BINDER_WGRAD_WAVE_CASE(16, 32)
+    BINDER_WGRAD_WAVE_CASE(32, 32)
+#undef BINDER_WGRAD_WAVE_CASE
+  }
 #define BINDER_WGRAD_CASE(CI, CO)                                          \
   if (cin == CI && cout == CO) {                                           \
     hipLaunchKernelGGL((k_wgrad<CI, CO>), grid, block, 0, s, a);           \
@@ -842,6 +946,11 @@ struct binder_warmstart {
     long long P = nvox / 4096;
     return (int)std::max(1ll, std::min(64ll, P));
   }
+  // the head's weight gradient: one thread per weight, so many short blocks
+  static constexpr int kHeadMaxParts = 1024;
+  int head_partitions(long long nvb) const {
+    return (int)std::max(1ll, std::min((long long)kHeadMaxParts, nvb / 64));
+  }
 
   void conv_wgrad(ConvLayer &L, Tensor x, Tensor g, int B) {
     const int *d = dims[L.level];
@@ -856,23 +965,20 @@ struct binder_warmstart {
     a.ld_g = g.ld;
     a.off_g = g.off;
     a.part = part.get();
+    a.bpart = cpart.get();
     a.X = d[0];
     a.Y = d[1];
     a.Z = d[2];
     a.nvox = (int)nvox;
     a.per_block = per;
     launch_wgrad(L.cin, L.cout, a, P, stream);
+    const int rows = wgrad_per_wave(L.cin, L.cout) ? 4 * P : P;
     const long long nw = (long long)L.cout * 27 * L.cin;
-    hipLaunchKernelGGL(k_reduce_rows, dim3(blocks_for(nw)), dim3(256), 0, stream, part.get(), P,
-                       nw, grad.get() + L.woff);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(blocks_for(nw)), dim3(256), 0, stream, part.get(),
+                       rows, nw, grad.get() + L.woff);
     HIP_LAUNCH_OK();
-    // bias: channel sums of g
-    const int Pb = partitions(nvox);
-    const int perb = (int)((nvox + Pb - 1) / Pb);
-    hipLaunchKernelGGL(k_chan_sum, dim3(Pb), dim3(256), 0, stream, g.p, g.ld, g.off, L.cout,
-                       (int)nvox, perb, cpart.get());
-    HIP_LAUNCH_OK();
-    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, stream, cpart.get(), Pb,
+    // bias: the centre tap's blocks summed g per channel
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, stream, cpart.get(), rows,
                        (long long)L.cout, grad.get() + L.boff);
     HIP_LAUNCH_OK();
   }
@@ -946,7 +1052,7 @@ struct binder_warmstart {
     HIP_LAUNCH_OK();
     {
       const long long nvb = (long long)B * n2;
-      const int P = partitions(nvb);
+      const int P = head_partitions(nvb);
       const int per = (int)((nvb + P - 1) / P);
       hipLaunchKernelGGL(k_head_wgrad, dim3(P), dim3(256), 0, stream, dU.get(), s5.get(), B, n2,
                          per, hpart.get());
@@ -1175,11 +1281,12 @@ int binder_warmstart_create(binder_warmstart **out, int nx, int ny, int nz, int 
     ws->dcoef.alloc((size_t)(B * 3 * kNCoef));
     long long maxpart = 0;
     for (auto &L : ws->conv)
-      maxpart = std::max(maxpart, (long long)ws->partitions(B * ws->nv[L.level]) * L.cout * 27 *
+      maxpart = std::max(maxpart, (long long)ws->partitions(B * ws->nv[L.level]) *
+                                      (wgrad_per_wave(L.cin, L.cout) ? 4 : 1) * L.cout * 27 *
                                       L.cin);
     ws->part.alloc((size_t)maxpart);
-    ws->cpart.alloc((size_t)64 * 64);
-    ws->hpart.alloc((size_t)64 * kHeadParams);
+    ws->cpart.alloc((size_t)4 * 64 * 64);
+    ws->hpart.alloc((size_t)binder_warmstart::kHeadMaxParts * kHeadParams);
     std::vector<float> t;
     dct_table(ws->dims[2][0], t);
     ws->cx.upload(t.data(), t.size(), ws->stream);
