@@ -130,6 +130,9 @@ def registration(args, pair, net=None):
     # B: retrace the last level in chunks to find the first iteration within
     # tolerance (untimed; the field is read back after every chunk)
     B = new()
+    if warm and used:
+        # the first level starts from the theta the fallback check fitted
+        B.theta = A.warm_theta.copy()
     if warm and used and len(ran) == 1:
         B.move_resolution(ran[0], resize_field=False)
         reg = B._warm_registration(A.warm_coef, ran[0])

@@ -272,11 +272,11 @@ class FastRegistration:
             else:
                 self.deformedVoxelPositions = np.ascontiguousarray(self.deformedVoxelPositions)
                 self.registratorObj = self._new_registration(self.deformedVoxelPositions)
-            self.registratorObj.setMILikelihood(self.alpha, np.ascontiguousarray(self.theta).ravel(),
-                                                self.num_node_intensity_levels,
-                                                self.num_voxel_intensity_levels, self.threads,
-                                                np.ascontiguousarray(self.binned_nodes).ravel(),
-                                                np.ascontiguousarray(self.binned_voxels).ravel())
+            self.registratorObj.setMILikelihood(
+                self.alpha, np.ascontiguousarray(self.theta).ravel(),
+                self.num_node_intensity_levels, self.num_voxel_intensity_levels,
+                self.threads, np.ascontiguousarray(self.binned_nodes).ravel(),
+                np.ascontiguousarray(self.binned_voxels).ravel())
             cap = max_number_of_iterations * self.current_resolution
             if level_iterations is not None:
                 cap = int(level_iterations[li])
@@ -378,6 +378,7 @@ class FastRegistration:
                 # like a coarser pyramid level, the check hands its fitted
                 # likelihood parameters on to the first level run
                 self.theta = np.ascontiguousarray(theta_w)
+        self.warm_theta = np.ascontiguousarray(self.theta).copy()
         self.warm_start_used = used
         if not used:
             self.move_resolution(1, resize_field=False)

@@ -94,6 +94,31 @@ __global__ __launch_bounds__(256) void k_zero_f64(double *__restrict__ x, long l
     x[i] = 0.0;
 }
 
+// Padded, packed moving bins of the 3D E-step: word (x, y, zq) of the
+// [nx+2][ny+2][nz+5] array holds the bins of z = zq - 4 .. zq - 1 of node
+// (x - 1, y - 1) in bytes 0..3, zero outside the grid.
+__global__ __launch_bounds__(256) void k_pack_quads(const uint8_t *__restrict__ bins, int nx,
+                                                    int ny, int nz, uint32_t *__restrict__ q) {
+  const int Yp = ny + 2, Zq = nz + 5;
+  const long long n = (long long)(nx + 2) * Yp * Zq;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int zq = (int)(i % Zq);
+    const int yp = (int)((i / Zq) % Yp);
+    const int xp = (int)(i / ((long long)Zq * Yp));
+    uint32_t w = 0u;
+    if (xp >= 1 && xp <= nx && yp >= 1 && yp <= ny) {
+      const uint8_t *col = bins + ((long long)(xp - 1) * ny + (yp - 1)) * nz;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int z = zq - 4 + c;
+        if (z >= 0 && z < nz) w |= (uint32_t)col[z] << (8 * c);
+      }
+    }
+    q[i] = w;
+  }
+}
+
 int grid_blocks(long long n) {
   const long long b = (n + 255) / 256;
   return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
@@ -583,30 +608,25 @@ int binder_registration_set_mi_likelihood(binder_registration *r, double alpha,
       HIP_OK(hipStreamSynchronize(r->s()));
     } else {
       // word zq of a column packs the bins of z = zq - 4 .. zq - 1 (byte c =
-      // z - zq + 4), zero outside the grid: four consecutive bytes of the
-      // column's bins behind four zeros
+      // z - zq + 4), zero outside the grid: the bins go up as bytes and
+      // k_pack_quads lays the words out on the device
       const int Zq = r->nzn + 5;
-      std::vector<uint32_t> q((size_t)Xp * Yp * Zq, 0u);
-      parallel_ranges(r->nxn, [&](long long x0, long long x1) {
-        std::vector<uint8_t> pad((size_t)r->nzn + 8, 0);
-        for (long long x = x0; x < x1; ++x)
-          for (int y = 0; y < r->nyn; ++y) {
-            const int *row = binned_nodes + (x * r->nyn + y) * r->nzn;
-            for (int z = 0; z < r->nzn; ++z) {
-              const int b = row[z];
-              if (b < 0 || b >= K) bad = true;
-              pad[z + 4] = (uint8_t)b;
-            }
-            uint32_t *qr = &q[((size_t)(x + 1) * Yp + (y + 1)) * Zq];
-            for (int zq = 0; zq < Zq; ++zq) {
-              const uint8_t *b = &pad[zq];
-              qr[zq] = (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) |
-                       ((uint32_t)b[3] << 24);
-            }
-          }
-      }, 32);
+      std::vector<uint8_t> n8(r->nn);
+      parallel_ranges(r->nn, [&](long long b0, long long b1) {
+        for (long long i = b0; i < b1; ++i) {
+          const int b = binned_nodes[i];
+          if (b < 0 || b >= K) bad = true;
+          n8[i] = (uint8_t)b;
+        }
+      });
       BINDER_REQUIRE(!bad, "binned_nodes value outside [0, K)");
-      r->quads.upload(q.data(), q.size(), r->s());
+      DBuf<uint8_t> nb;
+      nb.upload(n8.data(), n8.size(), r->s());
+      const long long nq = (long long)Xp * Yp * Zq;
+      r->quads.alloc((size_t)nq);
+      hipLaunchKernelGGL(k_pack_quads, dim3(grid_blocks(nq)), dim3(256), 0, r->s(), nb.get(),
+                         r->nxn, r->nyn, r->nzn, r->quads.get());
+      HIP_LAUNCH_OK();
       HIP_OK(hipStreamSynchronize(r->s()));
     }
     std::vector<float> tt((size_t)K * L);

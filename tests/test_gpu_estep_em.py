@@ -36,6 +36,9 @@ def _random_problem(shape, K, L, seed, jitter=1.5, oob=False):
     ((17, 13, 11), 32, 32, True),
     ((31, 23, 1), 16, 8, False),
     ((40, 33, 1), 32, 32, True),
+    # more packed moving-bin words (134 x 130 x 135) than one grid-stride
+    # pass of the packing kernel covers
+    ((132, 128, 130), 32, 32, False),
 ])
 def test_estep_matches_oracle(core, orc, shape, K, L, oob):
     print(f"estep parity shape={shape} K={K} L={L} oob={oob}", flush=True)
