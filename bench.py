@@ -32,7 +32,7 @@ FP32_PEAK_TFLOPS = 157.3
 # per-kernel table: profiles/gibbs_sweep_160x192x160_hbm_traffic.md.
 MEASURED_TRAFFIC = {
     (160, 192, 160): {
-        "estep_sample": (46533.7 * 2.0 + 74002.8) * 1024,
+        "estep_sample": (46608.6 * 2.0 + 74002.5) * 1024,
     },
 }
 
