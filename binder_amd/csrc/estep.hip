@@ -24,7 +24,7 @@
 // EM histogram, K <= 32 (k_estep_em32): the posterior mass of moving bin v
 // in voxel i is theta[v, u_i] * S_v / Z_i, where S_v is the spline weight of
 // the neighbours whose bin is v.  Each lane adds its nodes' spline weights,
-// as 32-bit fixed point (scale 2^31, S_v <= 1), into a private per-bin LDS
+// as 32-bit fixed point (scale 2^31, truncated, S_v <= 1), into a private per-bin LDS
 // column laid out bin-major / lane-minor (bank = lane: the integer adds never
 // conflict), marks the bins it touched in a 32-bit mask, and after the voxel
 // flushes only those bins (about 6 of 32 on noisy images, where runs of equal
@@ -565,9 +565,12 @@ __global__ __launch_bounds__(kEm32Threads) void k_estep_em32(EStepArgs a) {
   uint32_t *priv = reinterpret_cast<uint32_t *>(th + ((a.L * Kp + 3) & ~3));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint32_t *mine = priv + wave * (kEm32MaxK * 64) + lane;  // bin v at mine[64 * v]
+  // theta is kept scaled by 2^-31 and the z weights by 2^31, so one product
+  // gives both the fixed-point spline weight and (exactly, the scalings
+  // being powers of two) the node's posterior weight
   for (int i = threadIdx.x; i < KL; i += blockDim.x) {
     hist[i] = 0ull;
-    th[(i / a.K) * Kp + (i % a.K)] = a.thT[i];
+    th[(i / a.K) * Kp + (i % a.K)] = a.thT[i] * (1.0f / kSplineScale);
   }
   for (int v = 0; v < kEm32MaxK; ++v) mine[64 * v] = 0u;
   __syncthreads();
@@ -596,6 +599,9 @@ __global__ __launch_bounds__(kEm32Threads) void k_estep_em32(EStepArgs a) {
     spline3(tx, wx);
     spline3(ty, wy);
     if (!IS2D) spline3(tz, wz);
+    float wzs[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * kSplineScale;
     uint32_t qall[16];
     load_columns<IS2D>(a, ix, iy, iz, qall);
 
@@ -616,9 +622,9 @@ __global__ __launch_bounds__(kEm32Threads) void k_estep_em32(EStepArgs a) {
 #pragma unroll
         for (int zc = 0; zc < NZ; ++zc) {
           const int v = (qq >> (8 * zc)) & 0xFF;
-          const float sw = wxy * wz[zc];
-          const float wj = sw * thu[v];
-          w[yb * NZ + zc] = __float2uint_rn(sw * kSplineScale);
+          const float sws = wxy * wzs[zc];  // spline weight x 2^31
+          const float wj = sws * thu[v];    // = spline weight x theta
+          w[yb * NZ + zc] = (uint32_t)sws;  // truncated: one conversion
           vv[yb * NZ + zc] = v;
           zy += wj;
           if (!IS2D) sz += wj * (float)(zc - 1);
@@ -651,7 +657,7 @@ __global__ __launch_bounds__(kEm32Threads) void k_estep_em32(EStepArgs a) {
       const uint32_t sv = mine[64 * v];
       mine[64 * v] = 0u;
       if (add) {
-        const float mass = thu[v] * ((float)sv * (1.0f / kSplineScale)) * invZ;
+        const float mass = thu[v] * (float)sv * invZ;  // theta S_v / Z
         const unsigned long long f = to_fixed(mass);
         if (f) atomicAdd(hist + v * a.L + u, f);
       }
