@@ -260,7 +260,8 @@ def warmstart_report(args):
     net, path, meta = load_warm_net(args)
     if net is None:
         return {"report": "warmstart", "status": f"no trained network at {path}"}
-    seeds = meta.pop("holdout_seeds")[:args.warmstart_eval]
+    first = args.warmstart_first
+    seeds = meta.pop("holdout_seeds")[first:first + args.warmstart_eval]
     rows = []
     with ProcessPoolExecutor(max_workers=4, mp_context=mp.get_context("spawn")) as ex:
         for seed, pair in zip(seeds, ex.map(warmstart_train.make_pair, seeds)):
@@ -747,6 +748,8 @@ def main():
                     help="only the warm-vs-cold report over N held-out pairs")
     ap.add_argument("--cnn", action="store_true",
                     help="only the warm-start network's inference / training-step report")
+    ap.add_argument("--warmstart-first", type=int, default=0, metavar="I",
+                    help="first held-out pair of --warmstart-eval (0-based)")
     ap.add_argument("--batched", type=int, default=0, metavar="N",
                     help="only the many-pairs / many-chains report over N pairs")
     ap.add_argument("--batch-workers", type=int, default=8)
