@@ -176,6 +176,7 @@ class FastRegistration:
         self.nodes_shape = self.orig_nodes_shape
         self.voxels_shape = self.orig_voxels_shape
         self.non_zero_voxels = np.count_nonzero(self.binned_voxels)
+        self._non_zero = {1: self.non_zero_voxels}  # per resolution (bins are fixed)
         self.deformedVoxelPositions = self.full_voxels_pos
         self.dreamLocations = np.zeros_like(self.deformedVoxelPositions)
         self.spline_order = spline_order
@@ -452,8 +453,17 @@ class FastRegistration:
             pass
         elif not self.upsample_in_frequency_domain or cur - res < 0:
             # skimage resize(order=1) semantics, on the GPU
-            self.deformedVoxelPositions = Registration.resize_field(
-                self.deformedVoxelPositions, self.voxels_shape, cur / res)
+            if self._index_grid and cur == 1 and \
+                    self.deformedVoxelPositions is self.full_voxels_pos and \
+                    hasattr(Registration, "resize_index_grid"):
+                # the untouched initial field is the index grid: the GPU makes
+                # it instead of receiving 24 bytes per voxel from the host
+                self.deformedVoxelPositions = Registration.resize_index_grid(
+                    self.orig_voxels_shape, self.number_of_dimensions, self.voxels_shape,
+                    cur / res)
+            else:
+                self.deformedVoxelPositions = Registration.resize_field(
+                    self.deformedVoxelPositions, self.voxels_shape, cur / res)
         elif res != cur:
             full = self.full_voxels_pos[::cur, ::cur, ::cur, :] / cur
             delta = np.ascontiguousarray(self.deformedVoxelPositions - full)
@@ -466,7 +476,9 @@ class FastRegistration:
         self.binned_voxels = self.full_binned_voxels[::res, ::res, ::res]
         self.binned_nodes = self.full_binned_nodes[::res, ::res, ::res]
         self._D = None  # the level's D, made on first use (D below)
-        self.non_zero_voxels = np.count_nonzero(self.binned_voxels)
+        if res not in self._non_zero:
+            self._non_zero[res] = np.count_nonzero(self.binned_voxels)
+        self.non_zero_voxels = self._non_zero[res]
         self._set_gammas(self.orig_voxels_shape if self.increase_gamma_at_DS_resolution
                          else self.voxels_shape)
 

@@ -578,10 +578,28 @@ def resize_field(field, output_shape, scale=1.0):
     return out
 
 
+_resize_grid = _sig("binder_resize_index_grid", _c_int, _c_int, _c_int, _c_int, _c_int,
+                    _c_int, _c_int, _c_int, _c_double, _vp)
+
+
+def resize_index_grid(shape, comps, output_shape, scale=1.0):
+    """resize_field of the index grid of `shape` (3 ints; component c of a
+    voxel is its index along axis c, comps 2 or 3), made on the GPU instead
+    of uploaded: [*output_shape, comps] float64."""
+    shp = tuple(int(v) for v in shape)
+    oshp = tuple(int(v) for v in output_shape)
+    if len(shp) != 3 or len(oshp) != 3:
+        raise ValueError("shape and output_shape must be 3 ints")
+    out = np.empty(oshp + (int(comps),), np.float64)
+    _check(_resize_grid(*shp, int(comps), *oshp, float(scale), _ptr(out)))
+    return out
+
+
 def exported_symbols():
     """Names of the C entry points this module binds (for ABI checks)."""
     return [
         "binder_last_error", "binder_backend", "binder_philox4x32_10", "binder_resize_field",
+        "binder_resize_index_grid",
         "binder_topology_correction_2d", "binder_spline_weights",
         "binder_registration_correct_topology", "binder_registration_create_resampled",
         "binder_dct_prior_eigenvalues",

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <utility>
@@ -135,14 +136,60 @@ void upload_planes(const float *host, DBuf<float> &dst, DBuf<float> &tmp, long l
 }
 
 // Device planes -> host [nv][N] floats (staged through `tmp`); synchronises.
+// fn(begin, end) over [0, n) split across up to 16 host threads (one thread
+// when n < min_n).
+template <class F>
+void parallel_ranges(long long n, F &&fn, long long min_n = 1ll << 15) {
+  unsigned hw = std::thread::hardware_concurrency();
+  long long T = hw ? (long long)std::min(hw, 16u) : 1;
+  if (n < min_n) T = 1;
+  if (T > n) T = n > 0 ? n : 1;
+  if (T == 1) {
+    fn(0ll, n);
+    return;
+  }
+  std::vector<std::thread> ts;
+  for (long long t = 0; t < T; ++t)
+    ts.emplace_back([&, t] { fn(n * t / T, n * (t + 1) / T); });
+  for (auto &t : ts) t.join();
+}
+
+// Page-locked staging for large downloads, shared by every handle of the
+// process (one download at a time) and held until the process ends.
+std::mutex g_pinned_mu;
+void *g_pinned = nullptr;
+size_t g_pinned_bytes = 0;
+
 void download_planes(const float *src, float *host, DBuf<float> &tmp, long long nv, int N,
                      hipStream_t s) {
   if (!tmp.get() || tmp.n < (size_t)(nv * N)) tmp.alloc((size_t)(nv * N));
   hipLaunchKernelGGL(k_planes_to_interleaved, dim3(grid_blocks(nv)), dim3(256), 0, s, src,
                      tmp.get(), nv, N);
   HIP_LAUNCH_OK();
-  HIP_OK(hipMemcpyAsync(host, tmp.get(), sizeof(float) * nv * N, hipMemcpyDeviceToHost, s));
+  const size_t bytes = sizeof(float) * (size_t)(nv * N);
+  if (bytes < ((size_t)4 << 20)) {
+    HIP_OK(hipMemcpyAsync(host, tmp.get(), bytes, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    return;
+  }
+  // a full-size field: one DMA into page-locked memory, then a threaded copy
+  // into the caller's (pageable) buffer, instead of the driver's staged
+  // pageable transfer
+  std::lock_guard<std::mutex> lock(g_pinned_mu);
+  if (g_pinned_bytes < bytes) {
+    if (g_pinned) HIP_OK(hipHostFree(g_pinned));
+    g_pinned = nullptr;
+    g_pinned_bytes = 0;
+    HIP_OK(hipHostMalloc(&g_pinned, bytes, hipHostMallocDefault));
+    g_pinned_bytes = bytes;
+  }
+  HIP_OK(hipMemcpyAsync(g_pinned, tmp.get(), bytes, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
+  const char *from = static_cast<const char *>(g_pinned);
+  char *to = reinterpret_cast<char *>(host);
+  parallel_ranges((long long)bytes, [&](long long b0, long long b1) {
+    std::memcpy(to + b0, from + b0, (size_t)(b1 - b0));
+  }, (long long)4 << 20);
 }
 
 struct Stream {
@@ -491,23 +538,6 @@ void swap_buffers(DBuf<T> &a, DBuf<T> &b) {
   std::swap(a.n, b.n);
 }
 
-// fn(begin, end) over [0, n) split across up to 16 host threads (one thread
-// when n < min_n).
-template <class F>
-void parallel_ranges(long long n, F &&fn, long long min_n = 1ll << 15) {
-  unsigned hw = std::thread::hardware_concurrency();
-  long long T = hw ? (long long)std::min(hw, 16u) : 1;
-  if (n < min_n) T = 1;
-  if (T > n) T = n > 0 ? n : 1;
-  if (T == 1) {
-    fn(0ll, n);
-    return;
-  }
-  std::vector<std::thread> ts;
-  for (long long t = 0; t < T; ++t)
-    ts.emplace_back([&, t] { fn(n * t / T, n * (t + 1) / T); });
-  for (auto &t : ts) t.join();
-}
 
 // Philox key of a sampler seed.
 void sampler_key(int seed, uint32_t &k0, uint32_t &k1) {

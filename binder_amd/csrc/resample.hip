@@ -205,6 +205,22 @@ std::vector<double> gauss_weights(int n_in, int n_out, int &radius) {
   return wh;
 }
 
+// The index grid as a [nx][ny][nz][comps] field: component c of voxel
+// (x, y, z) is its index along axis c.
+__global__ __launch_bounds__(256) void k_index_grid(double *__restrict__ out, int nx, int ny,
+                                                    int nz, int comps) {
+  const long long n = (long long)nx * ny * nz * comps;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long v = e / comps;
+    const int c = (int)(e - v * comps);
+    const int z = (int)(v % nz);
+    const int y = (int)((v / nz) % ny);
+    const int x = (int)(v / ((long long)nz * ny));
+    out[e] = (double)(c == 0 ? x : c == 1 ? y : z);
+  }
+}
+
 // Gaussian passes on `a` (layout L, scratch b), then the zoom into `out`.
 // `range` is a device pointer to {min, max}.  Synchronises the stream.
 template <typename OUT>
@@ -274,11 +290,41 @@ extern "C" int binder_resize_field(const double *in, int nx, int ny, int nz,
                    "sizes must be positive");
     const long long nin = (long long)nx * ny * nz * comps;
     const long long nout = (long long)nxo * nyo * nzo * comps;
-    double lohi[2] = {in[0], in[0]};
-    for (long long i = 1; i < nin; ++i) {
-      lohi[0] = std::min(lohi[0], in[i]);
-      lohi[1] = std::max(lohi[1], in[i]);
-    }
+    hipStream_t s;
+    HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{s};
+    DBuf<double> a, b, o, part, range;
+    a.upload(in, (size_t)nin, s);
+    // the clipping range, min and max over every value, reduced on the device
+    const int nparts = std::min(grid_for(nin), 1024);
+    part.alloc((size_t)2 * nparts);
+    range.alloc(2);
+    hipLaunchKernelGGL(k_minmax_part, dim3(nparts), dim3(256), 0, s, a.get(), nin, part.get());
+    HIP_LAUNCH_OK();
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(256), 0, s, part.get(), nparts,
+                       range.get());
+    HIP_LAUNCH_OK();
+    o.alloc((size_t)nout);
+    resize_device<double>(a, b, Layout{1, comps}, nx, ny, nz, comps, o.get(),
+                          Layout{1, comps}, nxo, nyo, nzo, range.get(), scale, s);
+    o.download(out, (size_t)nout, s);
+    HIP_OK(hipStreamSynchronize(s));
+  });
+}
+
+extern "C" int binder_resize_index_grid(int nx, int ny, int nz, int comps, int nxo, int nyo,
+                                        int nzo, double scale, double *out) {
+  return guarded([&] {
+    BINDER_REQUIRE(out, "null argument");
+    BINDER_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nxo > 0 && nyo > 0 && nzo > 0,
+                   "sizes must be positive");
+    BINDER_REQUIRE(comps == 2 || comps == 3, "comps must be 2 or 3");
+    BINDER_REQUIRE(comps == 3 || nz == 1, "a 2-component grid has nz == 1");
+    const long long nin = (long long)nx * ny * nz * comps;
+    const long long nout = (long long)nxo * nyo * nzo * comps;
     hipStream_t s;
     HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     struct StreamGuard {
@@ -286,7 +332,14 @@ extern "C" int binder_resize_field(const double *in, int nx, int ny, int nz,
       ~StreamGuard() { (void)hipStreamDestroy(s); }
     } sg{s};
     DBuf<double> a, b, o, range;
-    a.upload(in, (size_t)nin, s);
+    a.alloc((size_t)nin);
+    hipLaunchKernelGGL(k_index_grid, dim3(grid_for(nin)), dim3(256), 0, s, a.get(), nx, ny, nz,
+                       comps);
+    HIP_LAUNCH_OK();
+    // min and max over every value of the grid: 0 and the largest index
+    int hi = std::max(nx, ny);
+    if (comps == 3) hi = std::max(hi, nz);
+    const double lohi[2] = {0.0, (double)(hi - 1)};
     range.upload(lohi, 2, s);
     o.alloc((size_t)nout);
     resize_device<double>(a, b, Layout{1, comps}, nx, ny, nz, comps, o.get(),

@@ -332,6 +332,14 @@ int binder_topology_correction_2d(const double *h_init, double e1, double e2,
 int binder_resize_field(const double *in, int nx, int ny, int nz, int comps,
                         int nxo, int nyo, int nzo, double scale, double *out);
 
+/* binder_resize_field of the index grid ([nx,ny,nz,comps], component c of a
+ * voxel = its index along axis c; comps 2 needs nz == 1), with the grid made
+ * on the device instead of uploaded: the first level of a pyramid that
+ * starts from the identity (BINDER/FastRegistration.py:566-568 applied to
+ * the initial positions).  out: [nxo,nyo,nzo,comps], FP64, host buffer. */
+int binder_resize_index_grid(int nx, int ny, int nz, int comps, int nxo, int nyo, int nzo,
+                             double scale, double *out);
+
 /* ------------------------------------------------------------------ */
 /* Learned warm start (no counterpart in the reference, whose pyramid  */
 /* always starts from the identity, BINDER/FastRegistration.py:206-225) */

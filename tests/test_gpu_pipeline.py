@@ -89,6 +89,25 @@ def test_resize_field_matches_host(core, shape_in, shape_out, comps):
     assert np.max(np.abs(got - exp)) <= 1e-10 * np.abs(exp).max()
 
 
+@pytest.mark.parametrize("shape_in,shape_out,comps", [
+    ((160, 192, 160), (20, 24, 20), 3),   # first level of the cold pyramid
+    ((17, 9, 13), (9, 18, 5), 3),         # mixed shrink / grow, odd sizes
+    ((96, 80, 1), (12, 10, 1), 2)])       # 2D
+def test_resize_index_grid(core, shape_in, shape_out, comps):
+    """The device-made index grid resampled equals the uploaded index grid
+    resampled (the same kernels: bit for bit) and the host restatement of
+    skimage.transform.resize(order=1)."""
+    from binder_amd import resample
+    print(f"resize index grid {shape_in} -> {shape_out} x{comps}", flush=True)
+    grid = np.stack(np.indices(shape_in, dtype=np.float64), -1)[..., :comps]
+    scale = shape_out[0] / shape_in[0]
+    got = core.resize_index_grid(shape_in, comps, shape_out, scale)
+    assert got.shape == shape_out + (comps,)
+    assert np.array_equal(got, core.resize_field(grid, shape_out, scale))
+    exp = resample.resize(grid, shape_out, order=1) * scale
+    assert np.max(np.abs(got - exp)) <= 1e-10 * np.abs(exp).max()
+
+
 def test_pyramid_2d_matches_oracle(core, orc, monkeypatch):
     """2D pyramid (96x80 synthetic pair, 6 ellipses, 3 px warp) with 2D
     topology correction: same per-level iteration counts as the FP64 oracle
