@@ -355,11 +355,12 @@ def cnn_report(args):
     n = 16
     fb = rng.integers(0, 32, (n,) + grid).astype(np.uint8)
     mb = rng.integers(0, 32, (n,) + grid).astype(np.uint8)
-    tg = (0.1 * rng.standard_normal((n, 3, 8, 8, 8))).astype(np.float32)
-    net = warmstart.WarmStartNet(grid, max_batch=8, seed=args.seed)
     if os.path.exists(warmstart.WEIGHTS_C2):
-        with np.load(warmstart.WEIGHTS_C2, allow_pickle=False) as z:
-            net.set_params(z["params"])
+        net = warmstart.WarmStartNet.load(warmstart.WEIGHTS_C2, max_batch=8)
+    else:
+        net = warmstart.WarmStartNet(grid, max_batch=8, seed=args.seed)
+    C = net.ncoef
+    tg = (0.1 * rng.standard_normal((n, 3, C, C, C))).astype(np.float32)
     net.set_dataset(fb, mb, tg)
     nv = [int(np.prod([g >> lvl for g in grid])) for lvl in range(4)]
     levels = [0, 0, 1, 1, 2, 2, 3, 3, 3, 3, 2, 2]

@@ -90,10 +90,19 @@ _topo3 = _sig("binder_topology_correction_3d", _c_int, _vp, _c_double, _c_double
 
 _philox = _sig("binder_philox4x32_10", _c_int, _vp, _vp, _vp)
 _reg_set_dct = _sig("binder_registration_set_field_from_dct", _c_int, _vp, _vp, _c_int, _c_int,
-                    _c_int, _c_int)
+                    _c_int, _c_int, _c_int)
 _reg_create_dct = _sig("binder_registration_create_from_dct", _c_int, _P(_vp), _vp, _c_int,
-                       _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _c_int, _c_int,
-                       _c_int, _c_int, _c_double)
+                       _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _c_int,
+                       _c_int, _c_int, _c_int, _c_double)
+
+
+def _dct_coefficients(coef):
+    """coef as float32 [3, C, C, C] (C in 1..16) and C."""
+    c = _arr(coef, np.float32)
+    C = int(round((c.size / 3) ** (1.0 / 3.0)))
+    if C < 1 or C > 16 or c.size != 3 * C ** 3:
+        raise ValueError("coef must hold 3 x C x C x C values, C in 1..16")
+    return c, C
 
 
 _dct_prior = _sig("binder_dct_prior_eigenvalues", _c_int, _c_int, _c_int, _c_int, _c_int,
@@ -203,9 +212,7 @@ class Registration:
         reference counterpart."""
         self = cls.__new__(cls)
         self._h = None
-        c = _arr(coef, np.float32)
-        if c.size != 3 * 512:
-            raise ValueError("coef must hold 3 x 8 x 8 x 8 values")
+        c, C = _dct_coefficients(coef)
         vp = None
         if voxel_pos is not None:
             vp = _arr(voxel_pos, np.float32)
@@ -213,7 +220,7 @@ class Registration:
                 raise ValueError("voxel_pos has the wrong size")
         fx, fy, fz = (int(n) for n in full_shape)
         h = _vp()
-        _check(_reg_create_dct(ctypes.byref(h), _ptr(c), fx, fy, fz, int(resolution),
+        _check(_reg_create_dct(ctypes.byref(h), _ptr(c), C, fx, fy, fz, int(resolution),
                                None if vp is None else _ptr(vp), int(Nx_v), int(Ny_v),
                                int(Nz_v), int(Nx_n), int(Ny_n), int(Nz_n), int(spline_order),
                                float(non_zero_voxels)))
@@ -309,16 +316,14 @@ class Registration:
 
     def set_field_from_dct(self, coef, full_shape, resolution):
         """Sets the current field to the warm start of a pyramid level at
-        `resolution`: the displacement whose 8x8x8 lowest orthonormal DCT-II
-        coefficients per component (on the full-resolution grid
-        `full_shape`) are sqrt(prod(full_shape)) * coef[3,8,8,8], sampled at
+        `resolution`: the displacement whose C x C x C lowest orthonormal
+        DCT-II coefficients per component (on the full-resolution grid
+        `full_shape`) are sqrt(prod(full_shape)) * coef[3,C,C,C], sampled at
         the full-resolution voxels resolution * p and divided by the
         resolution.  No reference counterpart (see binder_amd.warmstart)."""
-        c = _arr(coef, np.float32)
-        if c.size != 3 * 512:
-            raise ValueError("coef must hold 3 x 8 x 8 x 8 values")
+        c, C = _dct_coefficients(coef)
         fx, fy, fz = (int(n) for n in full_shape)
-        _check(_reg_set_dct(self._h, _ptr(c), fx, fy, fz, int(resolution)))
+        _check(_reg_set_dct(self._h, _ptr(c), C, fx, fy, fz, int(resolution)))
 
     def get_final_locations_as_numpy_array(self):
         out = np.empty(self._shape, np.float32)

@@ -1086,8 +1086,8 @@ int binder_registration_correct_topology(binder_registration *r, double e1,
   });
 }
 
-int binder_registration_set_field_from_dct(binder_registration *r, const float *coef, int fx,
-                                           int fy, int fz, int res) {
+int binder_registration_set_field_from_dct(binder_registration *r, const float *coef,
+                                           int ncoef, int fx, int fy, int fz, int res) {
   return guarded([&] {
     BINDER_REQUIRE(r && coef, "null argument");
     BINDER_REQUIRE(!r->is2d, "the DCT warm start is defined for 3D registrations");
@@ -1095,16 +1095,18 @@ int binder_registration_set_field_from_dct(binder_registration *r, const float *
     BINDER_REQUIRE((fx + res - 1) / res == r->nxv && (fy + res - 1) / res == r->nyv &&
                        (fz + res - 1) / res == r->nzv,
                    "the handle's grid is not ceil(full grid / res)");
+    BINDER_REQUIRE(ncoef >= 1 && ncoef <= kMaxWarmCoef, "ncoef must be in [1, 16]");
     DBuf<float> c;
-    c.upload(coef, (size_t)3 * kWarmCoef * kWarmCoef * kWarmCoef, r->s());
-    launch_dct_field(c.get(), fx, fy, fz, res, r->x.get(), r->dcur(), r->nxv, r->nyv, r->nzv,
-                     r->s());
+    c.upload(coef, (size_t)3 * ncoef * ncoef * ncoef, r->s());
+    launch_dct_field(c.get(), ncoef, fx, fy, fz, res, r->x.get(), r->dcur(), r->nxv, r->nyv,
+                     r->nzv, r->s());
     HIP_OK(hipStreamSynchronize(r->s()));
   });
 }
 
-int binder_registration_create_from_dct(binder_registration **out, const float *coef, int fx,
-                                        int fy, int fz, int res, const float *voxel_pos,
+int binder_registration_create_from_dct(binder_registration **out, const float *coef,
+                                        int ncoef, int fx, int fy, int fz, int res,
+                                        const float *voxel_pos,
                                         int Nx_v, int Ny_v, int Nz_v, int Nx_n, int Ny_n,
                                         int Nz_n, int spline_order, double non_zero_voxels) {
   return guarded([&] {
@@ -1114,7 +1116,7 @@ int binder_registration_create_from_dct(binder_registration **out, const float *
     auto r = new_registration(voxel_pos, Nx_v, Ny_v, Nz_v, Nx_n, Ny_n, Nz_n, spline_order,
                               non_zero_voxels);
     binder_registration *h = r.get();
-    const int rc = binder_registration_set_field_from_dct(h, coef, fx, fy, fz, res);
+    const int rc = binder_registration_set_field_from_dct(h, coef, ncoef, fx, fy, fz, res);
     if (rc != BINDER_OK) throw Error(rc, binder_last_error());
     *out = r.release();
   });

@@ -1,7 +1,8 @@
 // Learned warm start of the registration: a 3D convolutional encoder-decoder
 // that maps a pair of binned images (the fixed and moving image at the
-// half-resolution pyramid level) to the 8x8x8 lowest orthonormal-DCT
-// coefficients of each displacement component, trained on the GPU with its
+// half-resolution pyramid level) to the C x C x C lowest orthonormal-DCT
+// coefficients of each displacement component (C = ncoef <= 16, fixed per
+// network), trained on the GPU with its
 // own forward and backward kernels.  No counterpart in the reference, which
 // always starts the pyramid from the identity
 // (BINDER/FastRegistration.py:206-225); the registration driven from the
@@ -19,7 +20,8 @@
 //                    conv 112->64, conv 64->64 -> S5
 //   head:            1x1x1 conv 64->3 (FP32) -> U on the G/4 grid; the
 //                    coefficients are the orthonormal DCT-II of U over that
-//                    grid, the 8x8x8 lowest, divided by sqrt(|G/4|).
+//                    grid, the C x C x C lowest (zero at frequencies the
+//                    grid does not have), divided by sqrt(|G/4|).
 // Every conv is 3x3x3, stride 1, zero padding, bias, ReLU.
 //
 // Arithmetic: activations and the compute copies of the weights are BF16;
@@ -56,7 +58,6 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 
 constexpr int kHeadC = 64;         // channels entering the head
-constexpr int kNCoef = kWarmCoef * kWarmCoef * kWarmCoef;  // 512 per component
 constexpr int kInC = 8;            // stored input channels (2 used)
 
 // ---------------------------------------------------------------------------
@@ -551,57 +552,60 @@ __global__ void k_head_u(const uint16_t *s5, const float *wh, float *U, int B, i
   for (int n = 0; n < 3; ++n) U[((size_t)b * 3 + n) * nc + v] = acc[n];
 }
 
-// One block per (b, component): coef[b][n][a][bb][k].
+// One block per (b, component): coef[b][n][a][bb][k] (a, bb, k < C), with
+// the tables c*[a][x] zero for a at or above the grid side.
 __global__ __launch_bounds__(256) void k_head_proj(const float *U, const float *cx,
                                                    const float *cy, const float *cz, int hx,
-                                                   int hy, int hz, float scale, float *coef) {
+                                                   int hy, int hz, int C, float scale,
+                                                   float *coef) {
   extern __shared__ float sm[];
-  float *T1 = sm;                 // [hx][hy][8]
-  float *T2 = sm + hx * hy * 8;   // [hx][8][8]
+  float *T1 = sm;                 // [hx][hy][C]
+  float *T2 = sm + hx * hy * C;   // [hx][C][C]
   const float *u = U + (size_t)blockIdx.x * hx * hy * hz;
-  const int t = threadIdx.x;
-  for (int i = t; i < hx * hy * 8; i += 256) {
-    const int k = i & 7, xy = i >> 3;
+  const int t = threadIdx.x, C3 = C * C * C;
+  for (int i = t; i < hx * hy * C; i += 256) {
+    const int k = i % C, xy = i / C;
     float s = 0.f;
     for (int z = 0; z < hz; ++z) s += cz[k * hz + z] * u[(size_t)xy * hz + z];
     T1[i] = s;
   }
   __syncthreads();
-  for (int i = t; i < hx * 64; i += 256) {
-    const int k = i & 7, bb = (i >> 3) & 7, x = i >> 6;
+  for (int i = t; i < hx * C * C; i += 256) {
+    const int k = i % C, bb = (i / C) % C, x = i / (C * C);
     float s = 0.f;
-    for (int y = 0; y < hy; ++y) s += cy[bb * hy + y] * T1[(x * hy + y) * 8 + k];
+    for (int y = 0; y < hy; ++y) s += cy[bb * hy + y] * T1[(x * hy + y) * C + k];
     T2[i] = s;
   }
   __syncthreads();
-  for (int i = t; i < kNCoef; i += 256) {
-    const int k = i & 7, bb = (i >> 3) & 7, aa = i >> 6;
+  for (int i = t; i < C3; i += 256) {
+    const int k = i % C, bb = (i / C) % C, aa = i / (C * C);
     float s = 0.f;
-    for (int x = 0; x < hx; ++x) s += cx[aa * hx + x] * T2[(x * 8 + bb) * 8 + k];
-    coef[(size_t)blockIdx.x * kNCoef + i] = scale * s;
+    for (int x = 0; x < hx; ++x) s += cx[aa * hx + x] * T2[(x * C + bb) * C + k];
+    coef[(size_t)blockIdx.x * C3 + i] = scale * s;
   }
 }
 
 // Adjoint of k_head_proj: dU[b][n] from dcoef[b][n].
 __global__ __launch_bounds__(256) void k_head_expand(const float *dcoef, const float *cx,
                                                      const float *cy, const float *cz, int hx,
-                                                     int hy, int hz, float scale, float *dU) {
+                                                     int hy, int hz, int C, float scale,
+                                                     float *dU) {
   extern __shared__ float sm[];
-  float *E1 = sm;             // [8][8][hz]   (a, bb, z)
-  float *E2 = sm + 64 * hz;   // [8][hy][hz]  (a, y, z)
-  const float *dc = dcoef + (size_t)blockIdx.x * kNCoef;
+  float *E1 = sm;                // [C][C][hz]   (a, bb, z)
+  float *E2 = sm + C * C * hz;   // [C][hy][hz]  (a, y, z)
+  const float *dc = dcoef + (size_t)blockIdx.x * C * C * C;
   const int t = threadIdx.x;
-  for (int i = t; i < 64 * hz; i += 256) {
+  for (int i = t; i < C * C * hz; i += 256) {
     const int z = i % hz, ab = i / hz;
     float s = 0.f;
-    for (int k = 0; k < 8; ++k) s += cz[k * hz + z] * dc[ab * 8 + k];
+    for (int k = 0; k < C; ++k) s += cz[k * hz + z] * dc[ab * C + k];
     E1[i] = s;
   }
   __syncthreads();
-  for (int i = t; i < 8 * hy * hz; i += 256) {
+  for (int i = t; i < C * hy * hz; i += 256) {
     const int z = i % hz, y = (i / hz) % hy, aa = i / (hy * hz);
     float s = 0.f;
-    for (int bb = 0; bb < 8; ++bb) s += cy[bb * hy + y] * E1[(aa * 8 + bb) * hz + z];
+    for (int bb = 0; bb < C; ++bb) s += cy[bb * hy + y] * E1[(aa * C + bb) * hz + z];
     E2[i] = s;
   }
   __syncthreads();
@@ -609,7 +613,7 @@ __global__ __launch_bounds__(256) void k_head_expand(const float *dcoef, const f
   for (int i = t; i < hx * hy * hz; i += 256) {
     const int yz = i % (hy * hz), x = i / (hy * hz);
     float s = 0.f;
-    for (int aa = 0; aa < 8; ++aa) s += cx[aa * hx + x] * E2[aa * hy * hz + yz];
+    for (int aa = 0; aa < C; ++aa) s += cx[aa * hx + x] * E2[aa * hy * hz + yz];
     out[i] = scale * s;
   }
 }
@@ -657,12 +661,12 @@ __global__ __launch_bounds__(256) void k_head_wgrad(const float *dU, const uint1
   part[(size_t)blockIdx.x * kHeadParams + t] = s;
 }
 
-// Mean squared error over batch x 1536 coefficients and its gradient.
+// Mean squared error over batch x 3 C^3 coefficients (per = 3 C^3) and its
+// gradient.
 __global__ __launch_bounds__(256) void k_mse(const float *coef, const float *targets,
-                                             const int *idx, int B, float *dcoef,
+                                             const int *idx, int B, int per, float *dcoef,
                                              double *loss) {
   __shared__ double red[256];
-  const int per = 3 * kNCoef;
   const int n = B * per;
   double s = 0.0;
   for (int i = threadIdx.x; i < n; i += 256) {
@@ -714,48 +718,55 @@ __global__ void k_pack(const float *w, int cin, int cout, int kpad, int kpadT, u
   }
 }
 
-// Low-frequency field of a pyramid level from normalised coefficients.
-__global__ __launch_bounds__(256) void k_dct_field(const float *coef, int fx, int fy, int fz,
-                                                   int res, const float *x, float *d, int nx,
-                                                   int ny, int nz, float amp) {
-  __shared__ float c[3 * kNCoef];
-  for (int i = threadIdx.x; i < 3 * kNCoef; i += 256) c[i] = coef[i];
+// Low-frequency field of a pyramid level from normalised coefficients: one
+// block per (i, j) row of the level grid.  The row's x and y basis values
+// fold the C^3 coefficients of each component into C weights of the z basis
+// (P[n][q] = sum_{a,b} Cx[a](i) Cy[b](j) coef[n][a][b][q]), so each voxel
+// costs C basis values and 3 C products.
+__global__ __launch_bounds__(64) void k_dct_field(const float *coef, int C, int fx, int fy,
+                                                  int fz, int res, const float *x, float *d,
+                                                  int nx, int ny, int nz, float amp) {
+  __shared__ float bxy[2 * kMaxWarmCoef];
+  __shared__ float P[3 * kMaxWarmCoef];
+  const int i = (int)(blockIdx.x / ny), j = (int)(blockIdx.x % ny);
+  const int t = threadIdx.x;
+  if (t < 2 * C) {
+    const int a = t % C;
+    const int f = t < C ? fx : fy, I = res * (t < C ? i : j);
+    bxy[t] = a ? sqrtf(2.f / f) * cospif((float)((2 * I + 1) * a) / (float)(2 * f))
+               : sqrtf(1.f / f);
+  }
+  __syncthreads();
+  if (t < 3 * C) {
+    const int n = t / C, q = t % C;
+    const float *cn = coef + (size_t)n * C * C * C + q;
+    float s = 0.f;
+    for (int a = 0; a < C; ++a) {
+      float sa = 0.f;
+      for (int b = 0; b < C; ++b) sa += bxy[C + b] * cn[(a * C + b) * C];
+      s += bxy[a] * sa;
+    }
+    P[t] = s;
+  }
   __syncthreads();
   const long long nv = (long long)nx * ny * nz;
-  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nv) return;
-  const int k = (int)(v % nz);
-  const int j = (int)((v / nz) % ny);
-  const int i = (int)(v / ((long long)ny * nz));
-  float bx[8], by[8], bz[8];
-  const float sx0 = sqrtf(1.f / fx), sx = sqrtf(2.f / fx);
-  const float sy0 = sqrtf(1.f / fy), sy = sqrtf(2.f / fy);
   const float sz0 = sqrtf(1.f / fz), sz = sqrtf(2.f / fz);
-  const int I = res * i, J = res * j, K = res * k;
-#pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    bx[a] = a ? sx * cospif((float)((2 * I + 1) * a) / (float)(2 * fx)) : sx0;
-    by[a] = a ? sy * cospif((float)((2 * J + 1) * a) / (float)(2 * fy)) : sy0;
-    bz[a] = a ? sz * cospif((float)((2 * K + 1) * a) / (float)(2 * fz)) : sz0;
-  }
-  const float grid[3] = {(float)i, (float)j, (float)k};
-#pragma unroll
-  for (int n = 0; n < 3; ++n) {
-    const float pos = x ? x[n * nv + v] : grid[n];
-    float u = 0.f;
-    for (int a = 0; a < 8; ++a) {
-      float ua = 0.f;
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        const float *cc = c + n * kNCoef + a * 64 + b * 8;
-        float ub = 0.f;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) ub += cc[q] * bz[q];
-        ua += ub * by[b];
-      }
-      u += ua * bx[a];
+  for (int k = t; k < nz; k += blockDim.x) {
+    const long long v = (long long)blockIdx.x * nz + k;
+    const int K = res * k;
+    float u[3] = {0.f, 0.f, 0.f};
+    for (int q = 0; q < C; ++q) {
+      const float bz = q ? sz * cospif((float)((2 * K + 1) * q) / (float)(2 * fz)) : sz0;
+      u[0] += P[q] * bz;
+      u[1] += P[C + q] * bz;
+      u[2] += P[2 * C + q] * bz;
     }
-    d[n * nv + v] = pos + amp * u / (float)res;
+    const float grid[3] = {(float)i, (float)j, (float)k};
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const float pos = x ? x[n * nv + v] : grid[n];
+      d[n * nv + v] = pos + amp * u[n] / (float)res;
+    }
   }
 }
 
@@ -829,12 +840,12 @@ void launch_wgrad(int cin, int cout, const WgradArgs &a, int P, hipStream_t s) {
 
 }  // namespace
 
-void launch_dct_field(const float *coef_dev, int fx, int fy, int fz, int res, const float *x,
-                      float *d, int nx, int ny, int nz, hipStream_t s) {
-  const long long nv = (long long)nx * ny * nz;
+void launch_dct_field(const float *coef_dev, int ncoef, int fx, int fy, int fz, int res,
+                      const float *x, float *d, int nx, int ny, int nz, hipStream_t s) {
+  BINDER_REQUIRE(ncoef >= 1 && ncoef <= kMaxWarmCoef, "ncoef must be in [1, 16]");
   const float amp = (float)std::sqrt((double)fx * fy * fz);
-  hipLaunchKernelGGL(k_dct_field, dim3(blocks_for(nv)), dim3(256), 0, s, coef_dev, fx, fy, fz,
-                     res, x, d, nx, ny, nz, amp);
+  hipLaunchKernelGGL(k_dct_field, dim3((unsigned)((long long)nx * ny)), dim3(64), 0, s,
+                     coef_dev, ncoef, fx, fy, fz, res, x, d, nx, ny, nz, amp);
   HIP_LAUNCH_OK();
 }
 
@@ -868,6 +879,7 @@ struct binder_warmstart {
   int dims[4][3];     // grid per level
   long long nv[4];    // voxels per sample per level
   int max_batch = 0;
+  int ncoef = 8;      // C: coefficients per axis and component
   ConvLayer conv[NCONV];
   long long head_off = 0, nparam = 0;
   DBuf<float> w, grad, m, v;        // FP32 master parameters and Adam state
@@ -877,7 +889,7 @@ struct binder_warmstart {
   DBuf<uint16_t> dt1a, ds1, dp1, dt2a, ds2, dp2, dt3a, ds3, dcat, dp3, dt4a, dt4b, dt4c, ds4,
       dt5a, ds5;
   DBuf<float> U, dU, coef, dcoef, part, cpart, hpart;
-  DBuf<float> cx, cy, cz;           // head DCT tables [8][h]
+  DBuf<float> cx, cy, cz;           // head DCT tables [C][h]
   DBuf<double> losses;
   // training set on the device
   DBuf<uint8_t> data, stage;
@@ -894,6 +906,7 @@ struct binder_warmstart {
   }
 
   Tensor T(DBuf<uint16_t> &b, int ld, int off = 0) { return Tensor{b.get(), ld, off}; }
+  int coefs_per_sample() const { return 3 * ncoef * ncoef * ncoef; }
 
   void conv_fwd(ConvLayer &L, Tensor in, Tensor out, int B) {
     const int *d = dims[L.level];
@@ -1031,10 +1044,11 @@ struct binder_warmstart {
                        s5.get(), wh, U.get(), B, n2);
     HIP_LAUNCH_OK();
     const int *h = dims[2];
-    const size_t shm = (size_t)(h[0] * h[1] * 8 + h[0] * 64) * sizeof(float);
+    const int C = ncoef;
+    const size_t shm = (size_t)(h[0] * h[1] * C + h[0] * C * C) * sizeof(float);
     hipLaunchKernelGGL(k_head_proj, dim3(3 * B), dim3(256), shm, stream, U.get(), cx.get(),
-                       cy.get(), cz.get(), h[0], h[1], h[2], (float)(1.0 / std::sqrt((double)n2)),
-                       coef.get());
+                       cy.get(), cz.get(), h[0], h[1], h[2], C,
+                       (float)(1.0 / std::sqrt((double)n2)), coef.get());
     HIP_LAUNCH_OK();
   }
 
@@ -1042,9 +1056,10 @@ struct binder_warmstart {
     const int n2 = (int)nv[2];
     const int *h = dims[2];
     const float scale = (float)(1.0 / std::sqrt((double)n2));
-    const size_t shm = (size_t)(64 * h[2] + 8 * h[1] * h[2]) * sizeof(float);
+    const int C = ncoef;
+    const size_t shm = (size_t)(C * C * h[2] + C * h[1] * h[2]) * sizeof(float);
     hipLaunchKernelGGL(k_head_expand, dim3(3 * B), dim3(256), shm, stream, dcoef.get(),
-                       cx.get(), cy.get(), cz.get(), h[0], h[1], h[2], scale, dU.get());
+                       cx.get(), cy.get(), cz.get(), h[0], h[1], h[2], C, scale, dU.get());
     HIP_LAUNCH_OK();
     const float *wh = w.get() + head_off;
     hipLaunchKernelGGL(k_head_bwd_data, dim3(blocks_for((long long)B * n2 * 8)), dim3(256), 0,
@@ -1143,7 +1158,7 @@ struct binder_warmstart {
 
   void mse(const int *didx, int B, double *loss_dev) {
     hipLaunchKernelGGL(k_mse, dim3(1), dim3(256), 0, stream, coef.get(), targets.get(), didx, B,
-                       dcoef.get(), loss_dev);
+                       coefs_per_sample(), dcoef.get(), loss_dev);
     HIP_LAUNCH_OK();
   }
 
@@ -1169,10 +1184,11 @@ namespace {
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
-void dct_table(int h, std::vector<float> &t) {
-  t.assign((size_t)8 * h, 0.f);
+// Orthonormal DCT-II rows a < C of length h, zero for a >= h.
+void dct_table(int h, int C, std::vector<float> &t) {
+  t.assign((size_t)C * h, 0.f);
   const double pi = 3.14159265358979323846;
-  for (int a = 0; a < 8; ++a)
+  for (int a = 0; a < std::min(C, h); ++a)
     for (int x = 0; x < h; ++x)
       t[(size_t)a * h + x] =
           (float)(std::sqrt((a ? 2.0 : 1.0) / h) * std::cos(pi * (2 * x + 1) * a / (2.0 * h)));
@@ -1183,15 +1199,17 @@ void dct_table(int h, std::vector<float> &t) {
 extern "C" {
 
 int binder_warmstart_create(binder_warmstart **out, int nx, int ny, int nz, int max_batch,
-                            unsigned long long seed) {
+                            int ncoef, unsigned long long seed) {
   return guarded([&] {
     BINDER_REQUIRE(out != nullptr, "out handle pointer is null");
     *out = nullptr;
+    BINDER_REQUIRE(ncoef >= 1 && ncoef <= kMaxWarmCoef, "ncoef must be in [1, 16]");
     for (int d : {nx, ny, nz})
       BINDER_REQUIRE(d % 8 == 0 && d >= 32 && d <= 128,
                      "network grid sides must be multiples of 8 in [32, 128]");
     BINDER_REQUIRE(max_batch >= 1 && max_batch <= 64, "max_batch must be in [1, 64]");
     auto ws = std::make_unique<binder_warmstart>();
+    ws->ncoef = ncoef;
     HIP_OK(hipStreamCreate(&ws->stream));
     for (int l = 0; l < 4; ++l) {
       ws->dims[l][0] = nx >> l;
@@ -1277,8 +1295,8 @@ int binder_warmstart_create(binder_warmstart **out, int nx, int ny, int nz, int 
       act(*b, 3, 64);
     ws->U.alloc((size_t)(B * 3 * ws->nv[2]));
     ws->dU.alloc((size_t)(B * 3 * ws->nv[2]));
-    ws->coef.alloc((size_t)(B * 3 * kNCoef));
-    ws->dcoef.alloc((size_t)(B * 3 * kNCoef));
+    ws->coef.alloc((size_t)B * ws->coefs_per_sample());
+    ws->dcoef.alloc((size_t)B * ws->coefs_per_sample());
     long long maxpart = 0;
     for (auto &L : ws->conv)
       maxpart = std::max(maxpart, (long long)ws->partitions(B * ws->nv[L.level]) *
@@ -1288,12 +1306,29 @@ int binder_warmstart_create(binder_warmstart **out, int nx, int ny, int nz, int 
     ws->cpart.alloc((size_t)4 * 64 * 64);
     ws->hpart.alloc((size_t)binder_warmstart::kHeadMaxParts * kHeadParams);
     std::vector<float> t;
-    dct_table(ws->dims[2][0], t);
+    dct_table(ws->dims[2][0], ws->ncoef, t);
     ws->cx.upload(t.data(), t.size(), ws->stream);
-    dct_table(ws->dims[2][1], t);
+    dct_table(ws->dims[2][1], ws->ncoef, t);
     ws->cy.upload(t.data(), t.size(), ws->stream);
-    dct_table(ws->dims[2][2], t);
+    dct_table(ws->dims[2][2], ws->ncoef, t);
     ws->cz.upload(t.data(), t.size(), ws->stream);
+    {
+      // the head's projection and its adjoint keep two partial transforms in
+      // LDS: above 64 KB (large grids with many coefficients) they need the
+      // larger per-block limit
+      const int *h = ws->dims[2];
+      const int C = ws->ncoef;
+      const size_t proj = (size_t)(h[0] * h[1] * C + h[0] * C * C) * sizeof(float);
+      const size_t expd = (size_t)(C * C * h[2] + C * h[1] * h[2]) * sizeof(float);
+      BINDER_REQUIRE(std::max(proj, expd) <= (size_t)160 * 1024,
+                     "the head's transforms do not fit in LDS for this grid and ncoef");
+      if (std::max(proj, expd) > (size_t)64 * 1024) {
+        HIP_OK(hipFuncSetAttribute((const void *)k_head_proj,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        HIP_OK(hipFuncSetAttribute((const void *)k_head_expand,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+      }
+    }
     ws->losses.alloc(1);
     ws->pack_all();
     HIP_OK(hipStreamSynchronize(ws->stream));
@@ -1347,7 +1382,7 @@ int binder_warmstart_set_dataset(binder_warmstart *ws, int n, const unsigned cha
                             moving_bins + (size_t)i * nv, nv, hipMemcpyHostToDevice, ws->stream));
     }
     if (targets)
-      ws->targets.upload(targets, (size_t)n * 3 * kNCoef, ws->stream);
+      ws->targets.upload(targets, (size_t)n * ws->coefs_per_sample(), ws->stream);
     else
       ws->targets.release();
     ws->ndata = n;
@@ -1408,7 +1443,7 @@ int binder_warmstart_predict_dataset(binder_warmstart *ws, const int *idx, int b
     ws->idx.upload(idx, (size_t)batch, ws->stream);
     ws->build_input(ws->data.get(), ws->idx.get(), batch);
     ws->forward(batch);
-    ws->coef.download(coef_out, (size_t)batch * 3 * kNCoef, ws->stream);
+    ws->coef.download(coef_out, (size_t)batch * ws->coefs_per_sample(), ws->stream);
     HIP_OK(hipStreamSynchronize(ws->stream));
   });
 }
@@ -1435,7 +1470,7 @@ int binder_warmstart_predict(binder_warmstart *ws, int batch, const unsigned cha
     ws->inv_l = il;
     ws->inv_k = ik;
     ws->forward(batch);
-    ws->coef.download(coef_out, (size_t)batch * 3 * kNCoef, ws->stream);
+    ws->coef.download(coef_out, (size_t)batch * ws->coefs_per_sample(), ws->stream);
     HIP_OK(hipStreamSynchronize(ws->stream));
   });
 }

@@ -3,8 +3,9 @@
 A 3D convolutional encoder-decoder (binder_amd/csrc/warmstart.hip; channels
 16/32/48/64, 3x3x3 convolutions, BF16 weights and activations with FP32
 accumulation) reads the binned fixed and moving image at the half-resolution
-pyramid level and predicts the 8x8x8 lowest orthonormal DCT-II coefficients
-of each component of the full-resolution displacement.  It is trained on the
+pyramid level and predicts the C x C x C lowest orthonormal DCT-II
+coefficients (C = ncoef, at most 16; 16 for the shipped network) of each
+component of the full-resolution displacement.  It is trained on the
 GPU with its own forward and backward kernels on seeded synthetic pairs,
 each labelled by the field the plain (identity-start) pyramid converges to.
 
@@ -17,14 +18,14 @@ pyramid always starts from the identity (BINDER/FastRegistration.py:206-225).
 Coefficient convention (``field_coefficients``): for the displacement
 u = F - x of a field F on an Nx x Ny x Nz grid,
 coef[n, a, b, c] = sum_ijk C_Nx[a, i] C_Ny[b, j] C_Nz[c, k] u_n[i, j, k] / sqrt(Nx Ny Nz),
-with C_N the orthonormal DCT-II matrix, a, b, c < 8.
+with C_N the orthonormal DCT-II matrix, a, b, c < C.
 """
 import ctypes
 import os
 
 import numpy as np
 
-NCOEF = 8
+NCOEF = 16  # coefficients per axis of new networks and labels
 
 
 class _Api:
@@ -39,7 +40,7 @@ class _Api:
             from . import _core
             vp, ci, P, sig = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER, _core._sig
             cls._fns = dict(
-                create=sig("binder_warmstart_create", ci, P(vp), ci, ci, ci, ci,
+                create=sig("binder_warmstart_create", ci, P(vp), ci, ci, ci, ci, ci,
                            ctypes.c_ulonglong),
                 destroy=sig("binder_warmstart_destroy", None, vp),
                 nparam=sig("binder_warmstart_num_params", ci, vp, P(ctypes.c_longlong)),
@@ -107,17 +108,17 @@ def dct_rows(n, k=NCOEF):
     return C
 
 
-def field_coefficients(F, positions=None):
-    """Normalised low-frequency coefficients [3, 8, 8, 8] of the displacement
-    F - positions of a field F [Nx, Ny, Nz, 3] (positions default to the
-    grid indices)."""
+def field_coefficients(F, positions=None, ncoef=NCOEF):
+    """Normalised low-frequency coefficients [3, C, C, C] (C = ncoef) of the
+    displacement F - positions of a field F [Nx, Ny, Nz, 3] (positions
+    default to the grid indices)."""
     F = np.asarray(F, np.float64)
     shape = F.shape[:3]
     if positions is None:
         positions = np.stack(np.indices(shape, dtype=np.float64), axis=-1)
     u = F - positions
-    Cx, Cy, Cz = (dct_rows(n) for n in shape)
-    out = np.empty((3, NCOEF, NCOEF, NCOEF))
+    Cx, Cy, Cz = (dct_rows(n, ncoef) for n in shape)
+    out = np.empty((3, ncoef, ncoef, ncoef))
     for n in range(3):
         t = u[..., n] @ Cz.T                       # [X, Y, c]
         t = np.einsum("bj,xjc->xbc", Cy, t)        # [X, b, c]
@@ -128,11 +129,13 @@ def field_coefficients(F, positions=None):
 def coefficients_field(coef, shape, resolution=1):
     """Host evaluation of the warm-start field of a pyramid level
     (Registration.set_field_from_dct): [ceil(N/res)..., 3] positions."""
-    coef = np.asarray(coef, np.float64).reshape(3, NCOEF, NCOEF, NCOEF)
+    coef = np.asarray(coef, np.float64)
+    C = int(round((coef.size / 3) ** (1.0 / 3.0)))
+    coef = coef.reshape(3, C, C, C)
     shape = tuple(int(n) for n in shape)
     r = int(resolution)
     lvl = tuple(-(-n // r) for n in shape)
-    Cs = [dct_rows(n)[:, ::r] for n in shape]  # basis at full indices r * p
+    Cs = [dct_rows(n, C)[:, ::r] for n in shape]  # basis at full indices r * p
     amp = np.sqrt(float(np.prod(shape)))
     out = np.empty(lvl + (3,))
     grid = np.indices(lvl, dtype=np.float64)
@@ -145,13 +148,15 @@ def coefficients_field(coef, shape, resolution=1):
 class WarmStartNet:
     """The warm-start network on the GPU (handle over the C ABI)."""
 
-    def __init__(self, grid, max_batch=8, seed=0):
+    def __init__(self, grid, max_batch=8, seed=0, ncoef=NCOEF):
         self._api = api = _Api.get()
         self._h = None
         self.grid = tuple(int(n) for n in grid)
         self.max_batch = int(max_batch)
+        self.ncoef = int(ncoef)
         h = ctypes.c_void_p()
-        api["check"](api["create"](ctypes.byref(h), *self.grid, self.max_batch, int(seed)))
+        api["check"](api["create"](ctypes.byref(h), *self.grid, self.max_batch, self.ncoef,
+                                   int(seed)))
         self._h = h
         n = ctypes.c_longlong()
         api["check"](api["nparam"](self._h, ctypes.byref(n)))
@@ -180,7 +185,7 @@ class WarmStartNet:
         self._call("set", self._p(p))
 
     def set_dataset(self, fixed_bins, moving_bins, targets=None, L=32, K=32):
-        """fixed_bins / moving_bins: [n, *grid] uint8; targets [n, 3, 8, 8, 8]."""
+        """fixed_bins / moving_bins: [n, *grid] uint8; targets [n, 3, C, C, C]."""
         f = np.ascontiguousarray(fixed_bins, np.uint8)
         m = np.ascontiguousarray(moving_bins, np.uint8)
         n = f.shape[0]
@@ -189,8 +194,9 @@ class WarmStartNet:
         t = None
         if targets is not None:
             t = np.ascontiguousarray(targets, np.float32)
-            if t.size != n * 3 * NCOEF ** 3:
-                raise ValueError("targets must be [n, 3, 8, 8, 8]")
+            if t.size != n * 3 * self.ncoef ** 3:
+                raise ValueError(f"targets must be [n, 3, {self.ncoef}, {self.ncoef}, "
+                                 f"{self.ncoef}]")
         self._call("data", n, self._p(f), self._p(m), None if t is None else self._p(t),
                    int(L), int(K))
 
@@ -212,14 +218,15 @@ class WarmStartNet:
 
     def predict_dataset(self, idx):
         ix = np.ascontiguousarray(idx, np.int32).ravel()
-        out = np.empty((ix.size, 3, NCOEF, NCOEF, NCOEF), np.float32)
+        C = self.ncoef
+        out = np.empty((ix.size, 3, C, C, C), np.float32)
         for s in range(0, ix.size, self.max_batch):
             part = np.ascontiguousarray(ix[s:s + self.max_batch])
             self._call("pred_data", self._p(part), part.size, self._p(out[s:s + part.size]))
         return out
 
     def predict(self, fixed_bins, moving_bins, L=32, K=32):
-        """Coefficients [B, 3, 8, 8, 8] for bins [B, *grid] (or one pair [*grid])."""
+        """Coefficients [B, 3, C, C, C] for bins [B, *grid] (or one pair [*grid])."""
         f = np.ascontiguousarray(fixed_bins, np.uint8)
         m = np.ascontiguousarray(moving_bins, np.uint8)
         single = f.ndim == 3
@@ -228,7 +235,8 @@ class WarmStartNet:
         B = f.shape[0]
         if f.shape[1:] != self.grid or m.shape != f.shape or B > self.max_batch:
             raise ValueError(f"bins must be [<= {self.max_batch}, {self.grid}]")
-        out = np.empty((B, 3, NCOEF, NCOEF, NCOEF), np.float32)
+        C = self.ncoef
+        out = np.empty((B, 3, C, C, C), np.float32)
         self._call("pred", B, self._p(f), self._p(m), int(L), int(K), self._p(out))
         return out[0] if single else out
 
@@ -242,12 +250,14 @@ class WarmStartNet:
 
     def save(self, path, **meta):
         np.savez(path, params=self.get_params(), grid=np.array(self.grid),
-                 **{k: np.asarray(v) for k, v in meta.items()})
+                 ncoef=np.array(self.ncoef), **{k: np.asarray(v) for k, v in meta.items()})
 
     @classmethod
     def load(cls, path, max_batch=1):
         with np.load(path, allow_pickle=False) as z:
-            net = cls(tuple(int(n) for n in z["grid"]), max_batch=max_batch)
+            # files written before the count was stored hold 8 x 8 x 8 networks
+            ncoef = int(z["ncoef"]) if "ncoef" in z.files else 8
+            net = cls(tuple(int(n) for n in z["grid"]), max_batch=max_batch, ncoef=ncoef)
             net.set_params(z["params"])
         return net
 

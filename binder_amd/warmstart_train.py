@@ -40,7 +40,8 @@ def make_pair(seed):
 
 
 def label_one(seed):
-    """(seed, coefficients [3,8,8,8] f32, per-level iterations, seconds)."""
+    """(seed, coefficients [3,C,C,C] f32 (C = NCOEF), per-level iterations,
+    seconds)."""
     from .FastRegistration import FastRegistration
     pair = make_pair(seed)
     fr = FastRegistration(voxels=pair.fixed, nodes=pair.moving, **synth.REGISTRATION_KW)
@@ -118,7 +119,7 @@ def train(label_paths, out, holdout=200, epochs=60, batch=8, lr=1e-3, seed=0,
     _say(f"{n} labelled pairs: {ntrain} train (seeds {seeds[0]}..{seeds[ntrain - 1]}), "
          f"{holdout} held out")
     fb, mb = regenerate_inputs(seeds, workers)
-    net = WarmStartNet(network_grid(SHAPE), max_batch=batch, seed=seed)
+    net = WarmStartNet(network_grid(SHAPE), max_batch=batch, seed=seed, ncoef=coefs.shape[-1])
     net.set_dataset(fb, mb, coefs)
     del fb, mb
     rng = np.random.default_rng(seed)

@@ -347,18 +347,20 @@ int binder_resize_index_grid(int nx, int ny, int nz, int comps, int nxo, int nyo
 /* A 3D convolutional encoder-decoder (channels 16/32/48/64, 3x3x3 convs,
  * BF16 weights and activations, FP32 accumulation, FP32 master weights and
  * Adam) mapping a binned image pair on an nx x ny x nz grid (each side a
- * multiple of 8 in [32, 128]) to coef[3][8][8][8]: the 8 lowest orthonormal
- * DCT-II coefficients per axis of each displacement component of the
- * full-resolution field, divided by sqrt(full voxel count).  Parameters are
+ * multiple of 8 in [32, 128]) to coef[3][C][C][C]: the C = ncoef (<= 16)
+ * lowest orthonormal DCT-II coefficients per axis of each displacement
+ * component of the full-resolution field, divided by sqrt(full voxel
+ * count).  Parameters are
  * one FP32 vector: per conv layer W[cout][27][cin] then bias[cout] (layers
  * 8->16, 16->16, 16->32, 32->32, 32->48, 48->48, 48->64, 3 x 64->64,
  * 112->64, 64->64), then the head W[3][64], bias[3]. */
 typedef struct binder_warmstart binder_warmstart;
 
-/* New network with He-normal weights drawn from `seed`; buffers for up to
- * max_batch (<= 64) pairs. */
+/* New network predicting ncoef (1..16) coefficients per axis, with
+ * He-normal weights drawn from `seed`; buffers for up to max_batch (<= 64)
+ * pairs.  The parameter vector does not depend on ncoef. */
 int binder_warmstart_create(binder_warmstart **out, int nx, int ny, int nz,
-                            int max_batch, unsigned long long seed);
+                            int max_batch, int ncoef, unsigned long long seed);
 void binder_warmstart_destroy(binder_warmstart *ws);
 int binder_warmstart_num_params(binder_warmstart *ws, long long *n);
 int binder_warmstart_get_params(binder_warmstart *ws, float *out);
@@ -367,7 +369,7 @@ int binder_warmstart_set_params(binder_warmstart *ws, const float *in);
 
 /* Device-resident training set of n pairs: fixed_bins / moving_bins are
  * [n][nx][ny][nz] bins in [0, L) / [0, K) (network inputs bin/(L-1),
- * bin/(K-1)); targets [n][3][8][8][8] or NULL (prediction only). */
+ * bin/(K-1)); targets [n][3][C][C][C] or NULL (prediction only). */
 int binder_warmstart_set_dataset(binder_warmstart *ws, int n,
                                  const unsigned char *fixed_bins,
                                  const unsigned char *moving_bins,
@@ -380,7 +382,7 @@ int binder_warmstart_train(binder_warmstart *ws, const int *idx, int steps,
 /* Loss and parameter gradient (num_params floats) of one batch; no update. */
 int binder_warmstart_gradients(binder_warmstart *ws, const int *idx, int batch,
                                double *loss, float *grad_out);
-/* coef_out[batch][3][8][8][8] for dataset samples idx[0..batch). */
+/* coef_out[batch][3][C][C][C] for dataset samples idx[0..batch). */
 int binder_warmstart_predict_dataset(binder_warmstart *ws, const int *idx,
                                      int batch, float *coef_out);
 /* coef_out for pairs given as host bins ([batch][nx][ny][nz] each). */
@@ -392,14 +394,14 @@ int binder_warmstart_predict(binder_warmstart *ws, int batch,
 /* Sets the handle's current field to the warm-start field of a pyramid
  * level: d(p) = x(p) + u(res * p) / res, with x the handle's voxel
  * positions (the grid indices p for an index grid), u the full-resolution
- * displacement (grid fx x fy x fz) whose 8x8x8 lowest orthonormal DCT
- * coefficients per component are sqrt(fx fy fz) * coef[3][8][8][8] and all
- * others zero.  The handle's grid must be ceil(f / res) per axis; 3D only.
+ * displacement (grid fx x fy x fz) whose C x C x C lowest orthonormal DCT
+ * coefficients per component (C = ncoef, 1..16) are
+ * sqrt(fx fy fz) * coef[3][C][C][C] and all others zero.  The handle's grid must be ceil(f / res) per axis; 3D only.
  * No reference counterpart (it replaces the identity start of
  * BINDER/FastRegistration.py:206-225 for the first level run). */
 int binder_registration_set_field_from_dct(binder_registration *reg,
-                                           const float *coef, int fx, int fy,
-                                           int fz, int res);
+                                           const float *coef, int ncoef, int fx,
+                                           int fy, int fz, int res);
 
 /* binder_registration_create (voxel_pos NULL: the grid indices, generated on
  * the device) followed by binder_registration_set_field_from_dct: a
@@ -407,8 +409,9 @@ int binder_registration_set_field_from_dct(binder_registration *reg,
  * No reference counterpart (replaces the identity-start constructor call of
  * BINDER/FastRegistration.py:443-450 for the first level run). */
 int binder_registration_create_from_dct(binder_registration **out,
-                                        const float *coef, int fx, int fy,
-                                        int fz, int res, const float *voxel_pos,
+                                        const float *coef, int ncoef, int fx,
+                                        int fy, int fz, int res,
+                                        const float *voxel_pos,
                                         int Nx_v, int Ny_v, int Nz_v, int Nx_n,
                                         int Ny_n, int Nz_n, int spline_order,
                                         double non_zero_voxels);

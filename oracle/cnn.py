@@ -8,7 +8,8 @@ identity, BINDER/FastRegistration.py:206-225); what pins this restatement:
 
 * conv3d: scipy.ndimage.correlate of each (output, input) channel pair;
 * pooling / up-sampling backward: the adjoint (dot-product) identity;
-* the head: scipy.fft.dctn(norm="ortho") restricted to 8x8x8;
+* the head: scipy.fft.dctn(norm="ortho") restricted to the lowest C x C x C
+  (zero at frequencies past the grid);
 * the whole backward pass: central finite differences of the FP64 loss.
 
 ``bf16=True`` rounds, like the HIP path, every stored activation and
@@ -24,7 +25,7 @@ import numpy as np
 CONV_LAYERS = [(8, 16), (16, 16), (16, 32), (32, 32), (32, 48), (48, 48), (48, 64), (64, 64),
                (64, 64), (64, 64), (112, 64), (64, 64)]
 A1, B1, A2, B2, A3, B3, A4, B4, C4, D4, A5, B5 = range(12)
-NCOEF = 8
+NCOEF = 8  # default coefficient count per axis (a network's own is ncoef)
 
 
 def round_bf16(x):
@@ -104,24 +105,28 @@ def up_backward(g):
 
 
 def dct_rows(n, k=NCOEF):
-    a = np.arange(k)[:, None]
+    """Rows a < k of the orthonormal DCT-II matrix of length n; rows at or
+    past n (frequencies the grid does not have) are zero."""
+    C = np.zeros((k, n))
+    a = np.arange(min(k, n))[:, None]
     i = np.arange(n)[None, :]
-    C = np.sqrt(2.0 / n) * np.cos(np.pi * (2 * i + 1) * a / (2 * n))
+    C[:min(k, n)] = np.sqrt(2.0 / n) * np.cos(np.pi * (2 * i + 1) * a / (2 * n))
     C[0] = np.sqrt(1.0 / n)
     return C
 
 
-def head_project(U):
-    """U [B,3,hx,hy,hz] -> coef [B,3,8,8,8] = DCT-II_ortho(U)[:8,:8,:8] / sqrt(h)."""
+def head_project(U, ncoef=NCOEF):
+    """U [B,3,hx,hy,hz] -> coef [B,3,C,C,C] = DCT-II_ortho(U)[:C,:C,:C] / sqrt(h)."""
     hx, hy, hz = U.shape[2:]
-    Cx, Cy, Cz = dct_rows(hx), dct_rows(hy), dct_rows(hz)
+    Cx, Cy, Cz = (dct_rows(h, ncoef) for h in (hx, hy, hz))
     c = np.einsum("ai,bj,ck,snijk->snabc", Cx, Cy, Cz, U, optimize=True)
     return c / np.sqrt(hx * hy * hz)
 
 
 def head_expand(dc, shape):
     hx, hy, hz = shape
-    Cx, Cy, Cz = dct_rows(hx), dct_rows(hy), dct_rows(hz)
+    ncoef = dc.shape[-1]
+    Cx, Cy, Cz = (dct_rows(h, ncoef) for h in (hx, hy, hz))
     dU = np.einsum("ai,bj,ck,snabc->snijk", Cx, Cy, Cz, dc, optimize=True)
     return dU / np.sqrt(hx * hy * hz)
 
@@ -138,8 +143,9 @@ def make_input(fixed_bins, moving_bins, L=32, K=32, bf16=True):
 
 
 class Net:
-    def __init__(self, params, bf16=True):
+    def __init__(self, params, bf16=True, ncoef=NCOEF):
         self.bf16 = bf16
+        self.ncoef = ncoef
         self.conv, (self.Wh, self.bh) = split_params(params)
         if bf16:
             self.conv = [(round_bf16(W), b) for W, b in self.conv]
@@ -170,7 +176,7 @@ class Net:
         c["s5"] = layer(B5, c["t5a"])
         U = np.einsum("bxyzc,nc->bnxyz", c["s5"], self.Wh) + self.bh[None, :, None, None, None]
         c["U"] = U
-        coef = head_project(U)
+        coef = head_project(U, self.ncoef)
         self.cache = c
         return coef
 

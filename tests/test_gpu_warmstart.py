@@ -23,7 +23,7 @@ def ws():
     return warmstart
 
 
-def _data(rng, n, grid=GRID):
+def _data(rng, n, grid=GRID, C=8):
     # smooth blobs rather than white noise, so ReLUs are not all on a knife edge
     from scipy import ndimage
     fb = np.empty((n,) + grid, np.uint8)
@@ -33,7 +33,7 @@ def _data(rng, n, grid=GRID):
             f = ndimage.gaussian_filter(rng.standard_normal(grid), 2.0)
             f = (f - f.min()) / (f.max() - f.min())
             a[i] = np.minimum((f * 32).astype(np.int64), 31)
-    t = (0.2 * rng.standard_normal((n, 3, 8, 8, 8))).astype(np.float32)
+    t = (0.2 * rng.standard_normal((n, 3, C, C, C))).astype(np.float32)
     return fb, mb, t
 
 
@@ -53,38 +53,42 @@ def _layer_slices():
     return out
 
 
-def test_forward_matches_oracle(ws, rng):
-    print(f"warm-start forward, grid {GRID}, batch 2", flush=True)
-    net = ws.WarmStartNet(GRID, max_batch=2, seed=3)
+@pytest.mark.parametrize("C", [8, 16])
+def test_forward_matches_oracle(ws, rng, C):
+    """C = 16 on this grid (head grid 8 x 10 x 8) also checks the zero
+    coefficients at frequencies the head grid does not have."""
+    print(f"warm-start forward, grid {GRID}, batch 2, {C} coefficients", flush=True)
+    net = ws.WarmStartNet(GRID, max_batch=2, seed=3, ncoef=C)
     p = net.get_params()
     # a head large enough for the coefficients to carry signal
     p[-195:] = (0.05 * rng.standard_normal(195)).astype(np.float32)
     net.set_params(p)
     fb, mb, _ = _data(rng, 2)
     got = net.predict(fb, mb)
-    ref = cnn.Net(p, bf16=True).forward(cnn.make_input(fb, mb))
-    assert got.shape == (2, 3, 8, 8, 8)
+    ref = cnn.Net(p, bf16=True, ncoef=C).forward(cnn.make_input(fb, mb))
+    assert got.shape == (2, 3, C, C, C)
     assert _rel(got, ref) < 1e-2, _rel(got, ref)
 
 
-def test_gradients_match_oracle(ws, rng):
-    print(f"warm-start gradients, grid {GRID}, batch 2", flush=True)
-    net = ws.WarmStartNet(GRID, max_batch=2, seed=5)
+@pytest.mark.parametrize("C", [8, 16])
+def test_gradients_match_oracle(ws, rng, C):
+    print(f"warm-start gradients, grid {GRID}, batch 2, {C} coefficients", flush=True)
+    net = ws.WarmStartNet(GRID, max_batch=2, seed=5, ncoef=C)
     p = net.get_params()
     p[-195:] = (0.05 * rng.standard_normal(195)).astype(np.float32)
     net.set_params(p)
-    fb, mb, t = _data(rng, 3)
+    fb, mb, t = _data(rng, 3, C=C)
     net.set_dataset(fb, mb, t)
     idx = np.array([2, 0], np.int32)
     loss, g = net.gradients(idx)
-    o = cnn.Net(p, bf16=True)
+    o = cnn.Net(p, bf16=True, ncoef=C)
     c = o.forward(cnn.make_input(fb[idx], mb[idx]))
     lo, dc = cnn.mse(c, t[idx])
     go = o.backward(dc)
     assert abs(loss - lo) <= 1e-2 * lo
     # the same gradient without any BF16 rounding: the scale of the
     # differences that BF16 storage itself causes
-    o64 = cnn.Net(p, bf16=False)
+    o64 = cnn.Net(p, bf16=False, ncoef=C)
     c64 = o64.forward(cnn.make_input(fb[idx], mb[idx], bf16=False))
     g64 = o64.backward(cnn.mse(c64, t[idx])[1])
     worst = []
@@ -100,14 +104,21 @@ def test_gradients_match_oracle(ws, rng):
         worst.append((_rel(a, b), cos, _rel(b, e), name))
     # the HIP gradient sits closer to the BF16 oracle than BF16 storage
     # itself moves the gradient (a wrong tap, mask or transpose gives a
-    # relative error near 1 and a cosine near 0)
+    # relative error near 1 and a cosine near 0).  With C = 16 on this grid
+    # most targets are unreachable coefficients, which the HIP and the
+    # oracle's BF16 roundings then weight differently in the convolution
+    # layers: those are held to the direction (the same kernels as C = 8),
+    # the head, where the count enters, to the full bound.
     for r, cos, rb, name in worst:
-        assert r < max(3e-2, 1.5 * rb) and cos > 0.98, (name, r, cos, rb)
+        if C == 8 or name.startswith("head"):
+            assert r < max(3e-2, 1.5 * rb) and cos > 0.98, (name, r, cos, rb)
+        else:
+            assert cos > 0.99, (name, r, cos, rb)
 
 
 def test_predict_paths_agree_adam_step_and_training(ws, rng):
     print(f"warm-start training, grid {GRID}, 6 samples", flush=True)
-    net = ws.WarmStartNet(GRID, max_batch=3, seed=7)
+    net = ws.WarmStartNet(GRID, max_batch=3, seed=7, ncoef=8)
     fb, mb, t = _data(rng, 6)
     net.set_dataset(fb, mb, t)
     a = net.predict_dataset(np.array([4, 1, 5]))
@@ -130,20 +141,21 @@ def test_predict_paths_agree_adam_step_and_training(ws, rng):
     assert losses[-5:].mean() < losses[:5].mean(), (losses[:5], losses[-5:])
     # parameters round-trip and reproduce the predictions
     p = net.get_params()
-    net2 = ws.WarmStartNet(GRID, max_batch=3, seed=99)
+    net2 = ws.WarmStartNet(GRID, max_batch=3, seed=99, ncoef=8)
     net2.set_params(p)
     np.testing.assert_array_equal(net2.predict(fb[:3], mb[:3]), net.predict(fb[:3], mb[:3]))
 
 
-@pytest.mark.parametrize("res", [1, 2])
-def test_field_from_dct_matches_host_evaluation(core, ws, rng, res):
+@pytest.mark.parametrize("res,C", [(1, 8), (2, 8), (1, 16), (2, 16)])
+def test_field_from_dct_matches_host_evaluation(core, ws, rng, res, C):
     shape = (40, 36, 30)
-    print(f"set_field_from_dct, full grid {shape}, resolution {res}", flush=True)
+    print(f"set_field_from_dct, full grid {shape}, resolution {res}, {C} coefficients",
+          flush=True)
     lvl = tuple(-(-n // res) for n in shape)
     pos = np.stack(np.indices(lvl, dtype=np.float64), -1)
     r = core.Registration(pos.ravel(), 1, pos.ravel(), pos.ravel(), *lvl, *lvl, 3,
                           int(np.prod(lvl)))
-    coef = (0.05 * rng.standard_normal((3, 8, 8, 8))).astype(np.float32)
+    coef = (0.05 * rng.standard_normal((3, C, C, C))).astype(np.float32)
     r.set_field_from_dct(coef, shape, res)
     got = r.get_final_locations_as_numpy_array()
     ref = ws.coefficients_field(coef, shape, res)

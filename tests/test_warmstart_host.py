@@ -49,12 +49,19 @@ def test_pool_and_up_adjoints(rng):
     np.testing.assert_allclose(cnn.pool(cnn.up(y)), y)
 
 
-def test_head_projection_is_the_truncated_orthonormal_dct(rng):
+@pytest.mark.parametrize("C", [8, 16])
+def test_head_projection_is_the_truncated_orthonormal_dct(rng, C):
+    """C = 16 exceeds the grid (10 x 12 x 9): the coefficients it does not
+    have are zero."""
     U = rng.standard_normal((2, 3, 10, 12, 9))
-    c = cnn.head_project(U)
+    c = cnn.head_project(U, C)
+    assert c.shape == (2, 3, C, C, C)
     for b in range(2):
         for n in range(3):
-            ref = dctn(U[b, n], norm="ortho")[:8, :8, :8] / np.sqrt(10 * 12 * 9)
+            ref = np.zeros((C, C, C))
+            full = dctn(U[b, n], norm="ortho")[:C, :C, :C]
+            ref[:full.shape[0], :full.shape[1], :full.shape[2]] = full
+            ref /= np.sqrt(10 * 12 * 9)
             np.testing.assert_allclose(c[b, n], ref, rtol=1e-10, atol=1e-12)
     # the expansion is its adjoint
     dc = rng.standard_normal(c.shape)
@@ -114,16 +121,17 @@ def test_backward_matches_finite_differences(rng):
         assert abs(fd - g[i]) <= 1e-5 * max(abs(fd), 1e-3), (i, fd, g[i])
 
 
-def test_field_coefficients_round_trip(rng):
+@pytest.mark.parametrize("C", [8, 16])
+def test_field_coefficients_round_trip(rng, C):
     from binder_amd.warmstart import coefficients_field, field_coefficients
     shape = (20, 24, 18)
-    coef = rng.standard_normal((3, 8, 8, 8)) * 0.1
+    coef = rng.standard_normal((3, C, C, C)) * 0.1
     F = coefficients_field(coef, shape)
-    np.testing.assert_allclose(field_coefficients(F), coef, atol=1e-12)
+    np.testing.assert_allclose(field_coefficients(F, ncoef=C), coef, atol=1e-12)
     # the same field through scipy: zero-padded coefficients, idctn
     for n in range(3):
         c = np.zeros(shape)
-        c[:8, :8, :8] = coef[n] * np.sqrt(np.prod(shape))
+        c[:C, :C, :C] = coef[n] * np.sqrt(np.prod(shape))
         np.testing.assert_allclose(F[..., n] - np.indices(shape)[n], idctn(c, norm="ortho"),
                                    atol=1e-12)
     # a pyramid level samples the full-resolution displacement at 2p, / 2
