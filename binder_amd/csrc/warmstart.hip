@@ -662,14 +662,17 @@ __global__ __launch_bounds__(256) void k_head_wgrad(const float *dU, const uint1
 }
 
 // Mean squared error over batch x 3 C^3 coefficients (per = 3 C^3) and its
-// gradient.
-__global__ __launch_bounds__(256) void k_mse(const float *coef, const float *targets,
-                                             const int *idx, int B, int per, float *dcoef,
-                                             double *loss) {
+// gradient, in two steps: each block sums its fixed grid-stride share of
+// the squared errors into lpart[block], then one block adds the partials in
+// index order (the loss does not depend on scheduling).
+constexpr int kMseBlocks = 64;
+__global__ __launch_bounds__(256) void k_mse_part(const float *coef, const float *targets,
+                                                  const int *idx, int B, int per,
+                                                  float *dcoef, double *lpart) {
   __shared__ double red[256];
   const int n = B * per;
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     const int b = i / per, k = i - b * per;
     const float e = coef[i] - targets[(size_t)idx[b] * per + k];
     s += (double)e * e;
@@ -681,7 +684,14 @@ __global__ __launch_bounds__(256) void k_mse(const float *coef, const float *tar
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *loss = red[0] / n;
+  if (threadIdx.x == 0) lpart[blockIdx.x] = red[0];
+}
+
+__global__ void k_mse_final(const double *lpart, int P, int n, double *loss) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int p = 0; p < P; ++p) s += lpart[p];
+  *loss = s / n;
 }
 
 // Adam (Kingma & Ba 2015) with bias correction folded into c1, c2.
@@ -890,7 +900,7 @@ struct binder_warmstart {
       dt5a, ds5;
   DBuf<float> U, dU, coef, dcoef, part, cpart, hpart;
   DBuf<float> cx, cy, cz;           // head DCT tables [C][h]
-  DBuf<double> losses;
+  DBuf<double> losses, lpart;
   // training set on the device
   DBuf<uint8_t> data, stage;
   DBuf<float> targets;
@@ -1157,8 +1167,11 @@ struct binder_warmstart {
   }
 
   void mse(const int *didx, int B, double *loss_dev) {
-    hipLaunchKernelGGL(k_mse, dim3(1), dim3(256), 0, stream, coef.get(), targets.get(), didx, B,
-                       coefs_per_sample(), dcoef.get(), loss_dev);
+    hipLaunchKernelGGL(k_mse_part, dim3(kMseBlocks), dim3(256), 0, stream, coef.get(),
+                       targets.get(), didx, B, coefs_per_sample(), dcoef.get(), lpart.get());
+    HIP_LAUNCH_OK();
+    hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(64), 0, stream, lpart.get(), kMseBlocks,
+                       B * coefs_per_sample(), loss_dev);
     HIP_LAUNCH_OK();
   }
 
@@ -1330,6 +1343,7 @@ int binder_warmstart_create(binder_warmstart **out, int nx, int ny, int nz, int 
       }
     }
     ws->losses.alloc(1);
+    ws->lpart.alloc(kMseBlocks);
     ws->pack_all();
     HIP_OK(hipStreamSynchronize(ws->stream));
     *out = ws.release();
