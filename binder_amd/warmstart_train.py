@@ -1,14 +1,17 @@
 """Training of the learned warm start on seeded synthetic 160x192x160 pairs.
 
     python -m binder_amd.warmstart_train label --first 1000 --count 1100 --out labels_a.npz
+    (and --first 2100, --first 3200 for labels_b.npz, labels_c.npz)
     python -m binder_amd.warmstart_train train --labels labels_a.npz labels_b.npz \\
-        --holdout 200 --out binder_amd/weights/warmstart_160x192x160.npz
+        labels_c.npz --holdout 200 --epochs 60 \\
+        --out binder_amd/weights/warmstart_160x192x160.npz
 
 label: for each seed, the 3D workload generator (12 ellipsoids, 6-voxel-RMS
 DCT warp, x^0.7 intensity map, 32 bins; binder_amd.synth) makes a pair, the
 plain pyramid (identity start, [8, 4, 2, 1], topology correction) registers
 it on the GPU, and the label is the converged field's normalised
-low-frequency DCT coefficients (binder_amd.warmstart.field_coefficients).
+low-frequency DCT coefficients, 16 x 16 x 16 per component
+(binder_amd.warmstart.field_coefficients).
 Several worker processes share the GPU; each pair is independent.
 
 train: regenerates every labelled pair's network input (the
