@@ -61,6 +61,7 @@ _reg_get_dream = _sig("binder_registration_get_dream_locations", _c_int, _vp, _v
 _reg_set_dream = _sig("binder_registration_set_dream_locations", _c_int, _vp, _vp)
 _reg_get_theta = _sig("binder_registration_get_likelihood_parameters", _c_int, _vp, _vp)
 _reg_get_shape = _sig("binder_registration_get_shape", _c_int, _vp, _vp)
+_reg_get_gamma = _sig("binder_registration_get_gamma", _c_int, _vp, _vp)
 _reg_create_resampled = _sig("binder_registration_create_resampled", _c_int, _P(_vp), _vp,
                              _c_double, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                              _c_int, _c_double)
@@ -352,6 +353,14 @@ class Registration:
         return list(self._shape)
 
     # -- additions beyond the reference surface ----------------------------
+    def get_gamma(self):
+        """Current deformation-prior weights [gamma_x, gamma_y, gamma_z]: as
+        set, or as left by the last nu draw of a sampler call with
+        sample_gamma (NonLinearTransformation.cpp:116-124, :140-148)."""
+        out = np.empty(3, np.float64)
+        _check(_reg_get_gamma(self._h, _ptr(out)))
+        return out.tolist()
+
     def get_objective_trace(self):
         """Rows of (mll, prior_theta, prior_d, mlp) of the last EM call."""
         rows = _c_int(0)
@@ -615,6 +624,7 @@ def exported_symbols():
         "binder_registration_sampler_result", "binder_registration_get_final_locations",
         "binder_registration_get_dream_locations", "binder_registration_set_dream_locations",
         "binder_registration_get_likelihood_parameters", "binder_registration_get_shape",
+        "binder_registration_get_gamma",
         "binder_registration_get_objective_trace", "binder_registration_e_step",
         "binder_registration_set_profiling", "binder_registration_get_kernel_times",
         "binder_nlt_create", "binder_nlt_destroy", "binder_nlt_smooth_deformation",

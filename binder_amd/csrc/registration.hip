@@ -370,8 +370,10 @@ struct binder_registration {
     ring = std::min(kWelfordBatch, sweeps + 1);
     const size_t need = (size_t)(ring - 1) * nf;
     if (need && dextra.n < need) {
+      // the full ring at once, so a later call with more sweeps does not
+      // reallocate (and synchronize) inside its sweeps
       HIP_OK(hipStreamSynchronize(s()));
-      dextra.alloc(need);
+      dextra.alloc((size_t)(kWelfordBatch - 1) * nf);
     }
   }
 
@@ -1154,6 +1156,14 @@ int binder_registration_get_likelihood_parameters(binder_registration *r,
     if (!r->has_mi) throw Error(BINDER_ESTATE, "setMILikelihood was not called");
     r->theta.download(theta_out, (size_t)r->K * r->L, r->s());
     HIP_OK(hipStreamSynchronize(r->s()));
+  });
+}
+
+int binder_registration_get_gamma(binder_registration *r, double *gamma3) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && gamma3, "null argument");
+    if (!r->sm) throw Error(BINDER_ESTATE, "setNonLinearTransformation was not called");
+    r->sm->get_gamma(gamma3);
   });
 }
 

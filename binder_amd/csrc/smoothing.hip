@@ -742,17 +742,19 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *part,
 
 // nu ~ Gamma(alpha0 + N nv / 2, scale 1 / (beta0 + nv raw / 2)), then
 // gamma_k = nu rho_x rho_y rho_z eta_k^2 nv
-// (NonLinearTransformation.cpp:116-124, :140-148).
+// (NonLinearTransformation.cpp:116-124, :140-148).  raw is the sum over
+// components of eta_n^2 rho D p^2 over the sampled (noisy) coefficients p.
 __global__ void k_draw_nu(const double *raw, double *gamma, int N, double nv,
                           double rho_prod, double e0, double e1, double e2,
+                          double alpha0, double beta0,
                           uint32_t k0, uint32_t k1, const uint32_t *keyx, uint32_t sweep) {
   if (threadIdx.x != 0) return;
   if (keyx) {
     k0 ^= keyx[0];
     k1 ^= keyx[1];
   }
-  const double a_post = 1.0 + 0.5 * (double)N * nv;
-  const double b_post = 10000.0 + 0.5 * nv * (*raw);
+  const double a_post = alpha0 + 0.5 * (double)N * nv;
+  const double b_post = beta0 + 0.5 * nv * (*raw);
   const double gm = gamma_mt(a_post, u32x4{0u, sweep, 0u, kStreamNu}, k0, k1);
   const double nu = gm / b_post;
   gamma[0] = nu * rho_prod * e0 * e0 * nv;
@@ -1298,7 +1300,7 @@ void Smoother::run(const float *dream, const float *x, float *out,
     hipLaunchKernelGGL(k_draw_nu, dim3(1), dim3(64), 0, stream_,
                        scal_.get() + 1, gamma_.get(), N_, (double)nv_,
                        rho_[0] * rho_[1] * rho_[2], eta_[0], eta_[1], eta_[2],
-                       opt.key0, opt.key1, opt.keyx, opt.sweep);
+                       kNuAlpha0, kNuBeta0, opt.key0, opt.key1, opt.keyx, opt.sweep);
     HIP_LAUNCH_OK();
   }
 }

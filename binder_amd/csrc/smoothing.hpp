@@ -78,6 +78,11 @@ void up_sample_field(const float *in_soa, int nx, int ny, int nz, int nxo,
                      int nyo, int nzo, float *out_soa, hipStream_t stream,
                      MatrixCache &mats);
 
+// Gamma(alpha0, rate beta0) prior of the regularisation weight nu
+// (NonLinearTransformation.cpp:41-42).
+constexpr double kNuAlpha0 = 1.0;
+constexpr double kNuBeta0 = 10000.0;
+
 struct SmoothOptions {
   bool sample = false;         // add DCT-domain Gaussian noise
   bool sample_gamma = false;   // accumulate the raw regulariser and draw nu

@@ -183,6 +183,13 @@ int binder_registration_set_dream_locations(binder_registration *reg,
 /* get_likelihood_parameters() -> theta[K*L] -- BINDER/src/gpu/bindings.cpp:97. */
 int binder_registration_get_likelihood_parameters(binder_registration *reg,
                                                   double *theta_out);
+/* The deformation prior's current weights gamma_x, gamma_y, gamma_z: the
+ * values set by setNonLinearTransformation, or, after a sampler call with
+ * sample_gamma != 0, those of the last nu draw, gamma_n = nu rho_x rho_y rho_z
+ * eta_n^2 Nv (BINDER/src/cpu/NonLinearTransformation.cpp:116-124, :140-148).
+ * The reference keeps gamma inside its NonLinearTransformation object and
+ * only prints it (:145-146); this getter has no binding there. */
+int binder_registration_get_gamma(binder_registration *r, double *gamma3);
 /* get_shape() -> {Nx, Ny, Nz, N} -- BINDER/src/cpu/bindings.cpp:294. */
 int binder_registration_get_shape(binder_registration *reg, int *shape4);
 

@@ -30,6 +30,7 @@ void orc_reg_get_dream(void *h, double *out);
 void orc_reg_set_dream(void *h, const double *in);
 void orc_reg_get_theta(void *h, double *out);
 int orc_reg_get_trace(void *h, double *out, int max_rows);
+void orc_reg_get_gamma(void *h, double *out3);
 
 void *orc_nlt_create(double gx, double gy, double gz, double rx, double ry,
                      double rz, double ex, double ey, double ez,

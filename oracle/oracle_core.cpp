@@ -565,6 +565,12 @@ void orc_reg_get_theta(void *h, double *out) {
   auto *r = (Registration *)h;
   std::memcpy(out, r->theta.data(), sizeof(double) * r->theta.size());
 }
+// gamma of the transformation model: as set, or as the last nu draw left it
+// (NonLinearTransformation.cpp:140-148)
+void orc_reg_get_gamma(void *h, double *out3) {
+  auto *r = (Registration *)h;
+  for (int k = 0; k < 3; ++k) out3[k] = r->sm->gamma[k];
+}
 int orc_reg_get_trace(void *h, double *out, int max_rows) {
   auto *r = (Registration *)h;
   const int rows = (int)(r->trace.size() / 4);

@@ -57,6 +57,7 @@ _get_dream = _sig("orc_reg_get_dream", None, _vp, _vp)
 _set_dream = _sig("orc_reg_set_dream", None, _vp, _vp)
 _get_theta = _sig("orc_reg_get_theta", None, _vp, _vp)
 _get_trace = _sig("orc_reg_get_trace", _i, _vp, _vp, _i)
+_get_gamma = _sig("orc_reg_get_gamma", None, _vp, _vp)
 _nlt_create = _sig("orc_nlt_create", _vp, *([_d] * 9), _vp, _i, _i, _i, _i, _d, _d)
 _nlt_destroy = _sig("orc_nlt_destroy", None, _vp)
 _nlt_smooth = _sig("orc_nlt_smooth", _d, _vp, _vp, _vp)
@@ -152,6 +153,11 @@ class Registration:
 
     def get_shape(self):
         return list(self._shape)
+
+    def get_gamma(self):
+        out = np.empty(3)
+        _get_gamma(self._h, _p(out))
+        return out.tolist()
 
     def get_objective_trace(self):
         rows = _get_trace(self._h, None, 0)
