@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -1160,11 +1161,89 @@ void topology_on_device(float *d_soa, int N, int Nx, int Ny, int Nz, const doubl
   HIP_OK(hipStreamSynchronize(s));
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free fold mask (binder_fold_mask_3d_noalloc).  Its only scratch
+// is k_jacobian3d's per-block partials (min, max, #(J < e1), #(J > e2)) for at
+// most kFoldBlocks blocks, held in a file-scope device array.  Host code
+// never passes the array's name to a kernel: it is a host-side stand-in, so
+// the entry point looks up each device's own address once with
+// hipGetSymbolAddress and uses that.
+// ---------------------------------------------------------------------------
+constexpr int kFoldBlocks = 4096;
+constexpr int kMaxScratchDevices = 64;
+constexpr long long kFoldNoallocMaxVoxels = 2147483647LL;  // 2^31 - 1
+
+}  // namespace binder
+
+static __device__ __align__(256) double scratch[binder::kFoldBlocks * 4];
+
+namespace binder {
+namespace {
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call made with that device current.
+double *fold_scratch() {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  BINDER_REQUIRE(dev >= 0 && dev < kMaxScratchDevices, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    BINDER_REQUIRE(p != nullptr, "hipGetSymbolAddress returned no address");
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+// Fixed-order sum of the blocks' fold counts (integers held in FP64: exact).
+__global__ __launch_bounds__(256) void k_fold_count(const double *part, int blocks,
+                                                    long long *count) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double s = 0.0;
+  for (int b = t; b < blocks; b += 256) s += part[b * 4 + 2];
+  red[t] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) *count = (long long)red[0];
+}
+
+}  // namespace
 }  // namespace binder
 
 using namespace binder;
 
 extern "C" {
+
+int binder_fold_mask_3d_noalloc(const double *field_planes, int Nx, int Ny, int Nz,
+                                double vx, double vy, double vz, double e1,
+                                unsigned char *mask, long long *count, void *stream) {
+  return guarded([&] {
+    BINDER_REQUIRE(field_planes && count, "null argument");
+    BINDER_REQUIRE(Nx > 0 && Ny > 0 && Nz > 0, "Dimensions must be positive");
+    const long long nv = (long long)Nx * Ny * Nz;
+    BINDER_REQUIRE(nv <= kFoldNoallocMaxVoxels,
+                   "binder_fold_mask_3d_noalloc: at most 2^31 - 1 voxels");
+    double *part = fold_scratch();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int blocks = (int)std::min<long long>((nv + 255) / 256, kFoldBlocks);
+    hipLaunchKernelGGL(k_jacobian3d, dim3(blocks), dim3(256), 0, s,
+                       Field{field_planes, field_planes + nv, field_planes + 2 * nv}, Nx,
+                       Ny, Nz, 1.0 / (vx * vy * vz), e1, INFINITY, (double *)nullptr,
+                       (uint8_t *)mask, part);
+    HIP_LAUNCH_OK();
+    hipLaunchKernelGGL(k_fold_count, dim3(1), dim3(256), 0, s, (const double *)part, blocks,
+                       count);
+    HIP_LAUNCH_OK();
+  });
+}
 
 int binder_jacobian_determinant_3d(const double *field, int Nx, int Ny, int Nz,
                                    double vx, double vy, double vz,

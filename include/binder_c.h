@@ -284,6 +284,27 @@ int binder_fold_mask_3d(const double *field, int Nx, int Ny, int Nz,
                         double voxel_size_z, double e1, unsigned char *mask,
                         long long *count);
 
+/* binder_fold_mask_3d on device memory, allocating nothing: no counterpart
+ * in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  field_planes: device pointer to the field as three FP64
+ * planes [3][Nx*Ny*Nz] (x, y, z components, each C order over (x, y, z));
+ * mask: device pointer to Nx*Ny*Nz bytes (mask[i] = J[i] < e1) or NULL;
+ * count: device pointer to one long long, written with the number of voxels
+ * with J < e1.  stream: a hipStream_t (NULL: the null stream); the call only
+ * enqueues work on it and returns without synchronizing, so the results are
+ * ready once the stream reaches them.  The Jacobian is the one of
+ * binder_jacobian_determinant_3d (bit-exact with the reference).
+ * Limit: Nx*Ny*Nz <= 2^31 - 1 voxels (larger -> BINDER_EINVAL).  The only
+ * scratch is a 128 KiB per-block partial array in a file-scope device array,
+ * held by each GPU the library runs on until the process ends; every call on
+ * one GPU shares it, so calls that overlap on one GPU (concurrent streams or
+ * threads) race. */
+int binder_fold_mask_3d_noalloc(const double *field_planes, int Nx, int Ny, int Nz,
+                                double voxel_size_x, double voxel_size_y,
+                                double voxel_size_z, double e1, unsigned char *mask,
+                                long long *count, void *stream);
+
 /* topology_correction_3D().correct_topology(h_init, e1, e2, max_outer_it,
  *   max_it, threads, voxel_sizes, Nx_v, Ny_v, Nz_v)
  * -- BINDER/src/gpu/bindings.cpp:257-287,
