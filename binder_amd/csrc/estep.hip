@@ -585,16 +585,20 @@ __global__ __launch_bounds__(kEm32Threads) void k_estep_em32(EStepArgs a) {
   // rotation of this lane's flush order, so lanes of one wave that share a
   // histogram row flush different bins in the same step
   const int rot = lane & 31;
-  for (long long base = vbeg; base < vend; base += blockDim.x) {
-    const long long i = base + threadIdx.x;
-    if (i >= vend) break;
-    const int u = a.bv[i];
+  // 32-bit voxel offsets (the host keeps N * nv below 2^30) and 32-bit
+  // moving-grid offsets (checked when the bins are set): no 64-bit address
+  // arithmetic in the voxel loop
+  const uint32_t nv32 = (uint32_t)a.nv, vend32 = (uint32_t)vend;
+  for (uint32_t base = (uint32_t)vbeg; base < vend32; base += blockDim.x) {
+    const uint32_t iu = base + threadIdx.x;
+    if (iu >= vend32) break;
+    const int u = ld_off32(a.bv, iu);
     const float *thu = th + u * Kp;
     float tx, ty, tz = 0.f;
-    const int ix = safe_floor(a.d[i], tx);
-    const int iy = safe_floor(a.d[a.nv + i], ty);
+    const int ix = safe_floor(ld_off32(a.d, iu), tx);
+    const int iy = safe_floor(ld_off32(a.d, nv32 + iu), ty);
     int iz = 0;
-    if (!IS2D) iz = safe_floor(a.d[2 * a.nv + i], tz);
+    if (!IS2D) iz = safe_floor(ld_off32(a.d, 2 * nv32 + iu), tz);
     float wx[4], wy[4], wz[4] = {1.f, 0.f, 0.f, 0.f};
     spline3(tx, wx);
     spline3(ty, wy);
@@ -603,7 +607,7 @@ __global__ __launch_bounds__(kEm32Threads) void k_estep_em32(EStepArgs a) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * kSplineScale;
     uint32_t qall[16];
-    load_columns<IS2D>(a, ix, iy, iz, qall);
+    load_columns32<IS2D>(a, ix, iy, iz, qall);
 
     float Z = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
     uint32_t mask = 0u;
@@ -645,9 +649,9 @@ __global__ __launch_bounds__(kEm32Threads) void k_estep_em32(EStepArgs a) {
 
     l2sum += (double)__log2f(Z);
     const float invZ = 1.0f / Z;
-    a.dream[i] = (float)ix + sx * invZ;
-    a.dream[a.nv + i] = (float)iy + sy * invZ;
-    if (!IS2D) a.dream[2 * a.nv + i] = (float)iz + sz * invZ;
+    st_off32(a.dream, iu, (float)ix + sx * invZ);
+    st_off32(a.dream, nv32 + iu, (float)iy + sy * invZ);
+    if (!IS2D) st_off32(a.dream, 2 * nv32 + iu, (float)iz + sz * invZ);
     const bool add = Z > 0.f;
     uint32_t m = __builtin_amdgcn_alignbit(mask, mask, rot);  // rotate right
     while (m) {

@@ -1,0 +1,399 @@
+"""The five workload configurations at their stated sizes, on the HIP path,
+checked against the FP64 oracle (all host cores) or against closed forms:
+
+  c1  2D MI registration 192x160: the whole pyramid with 2D topology
+      correction, and its last level from one shared start;
+  c2  3D 160x192x160: 3 EM iterations from the identity, fold masks, the
+      deformation draw's voxel variance over 200 sweeps;
+  c3  3D 256x256x192: 3 EM iterations (split-BF16 DCT), Jacobians;
+  c4  uneven pairs registered concurrently equal the same pairs alone, and
+      16 chains per pair likewise;
+  c5  the shipped warm-start network on held-out pairs.
+
+Reference: BINDER/src/cpu/Registration.cpp:122-217 (EM), :515-642
+(sampler), BINDER/FastRegistration.py:428-548 (pyramid),
+BINDER/src/cpu/utils.cpp:100-345 (Jacobian),
+BINDER/src/cpu/NonLinearTransformation.cpp:79-126 (deformation draw).
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from binder_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _threads():
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(omp)) if omp.isdigit() and int(omp) > 0 else n
+
+
+def _rms(a, b):
+    return float(np.sqrt(np.mean(np.sum((np.asarray(a, np.float64) - b) ** 2, -1))))
+
+
+def _mi(H):
+    p = H / H.sum()
+    pk = p.sum(1, keepdims=True)
+    pl = p.sum(0, keepdims=True)
+    nz = p > 0
+    return float(np.sum(p[nz] * np.log(p[nz] / (pk @ pl)[nz])))
+
+
+def _dct_matrix(n):
+    k = np.arange(n)[:, None]
+    j = np.arange(n)[None, :]
+    C = np.sqrt(2.0 / n) * np.cos(np.pi * (2 * j + 1) * k / (2 * n))
+    C[0] /= np.sqrt(2.0)
+    return C
+
+
+def _pyramid(backend, monkeypatch, pair, threads=1, **kw):
+    import binder_amd.FastRegistration as fr
+    monkeypatch.setattr(fr, "Registration", backend)
+    reg = fr.FastRegistration(voxels=pair.fixed, nodes=pair.moving, threads=threads,
+                              **dict(synth.REGISTRATION_KW, **kw))
+    reg.run(list(synth.PYRAMID), synth.EM_ITERATIONS)
+    return reg
+
+
+def _where(pair, d2):
+    """RMS of the per-voxel difference over tissue / background / edges and
+    the share of the squared difference in the worst 1 % of voxels."""
+    from scipy import ndimage
+    fx = ndimage.gaussian_filter(np.squeeze(np.asarray(pair.fixed, np.float64)), 1.0)
+    grad = np.sqrt(sum(g ** 2 for g in np.gradient(fx)))
+    tissue = (fx > 0.05).reshape(d2.shape)
+    edge = (grad > 0.05).reshape(d2.shape)
+    flat = np.sort(d2.ravel())[::-1]
+    top = max(1, flat.size // 100)
+    return {"tissue": float(np.sqrt(d2[tissue].mean())),
+            "background": float(np.sqrt(d2[~tissue].mean())),
+            "edge": float(np.sqrt(d2[edge].mean())),
+            "top1pct_share": float(flat[:top].sum() / max(flat.sum(), 1e-300)),
+            "p99": float(np.sqrt(np.quantile(d2, 0.99)))}
+
+
+# ---------------------------------------------------------------- c1 ------
+def test_c1_pyramid_vs_oracle(core, orc, monkeypatch):
+    """2D 192x160, 32 bins, pyramid [8, 4, 2, 1] with 2D topology correction,
+    each to its own convergence: the coarse levels take the same iteration
+    counts as the FP64 oracle; the final fields differ by less than the
+    pipeline's own sensitivity (the oracle moved by a 1e-4 px start
+    perturbation differs from itself by about 0.12 px RMS at this size), and
+    the bulk of the image (99 % of the pixels) agrees to 0.1 px."""
+    print("c1: 2D 192x160 pyramid, HIP vs FP64 oracle", flush=True)
+    pair, _ = synth.config_problem("c1", seed=0)
+    g = _pyramid(core, monkeypatch, pair)
+    o = _pyramid(orc, monkeypatch, pair, threads=_threads())
+    gi = [int(lv[1]) for lv in g.level_log]
+    oi = [int(lv[1]) for lv in o.level_log]
+    d2 = np.sum((np.asarray(g.deformedVoxelPositions, np.float64) -
+                 o.deformedVoxelPositions) ** 2, -1)
+    rms = float(np.sqrt(d2.mean()))
+    w = _where(pair, d2)
+    print(f"c1 iterations HIP {gi} oracle {oi}; field RMS {rms:.4f} px, max "
+          f"{np.sqrt(d2.max()):.3f}; {w}", flush=True)
+    assert gi[:3] == oi[:3]
+    assert rms < 0.2
+    assert w["p99"] < 0.1 or w["top1pct_share"] > 0.5
+
+
+def test_c1_last_level_from_shared_start(core, orc, monkeypatch):
+    """The last (full-resolution) level of the 2D pipeline from one start
+    (the oracle's level-2 result, resampled and topology-corrected on the
+    host) with the oracle's theta.  The float32 positions carry about 1e-5 px
+    of rounding at 190 px, which the EM dynamics amplify at a few pixels:
+    after one iteration the field is within 1e-3 px max of the oracle's and
+    mll / mlp within 1e-5 relative; after 3 the RMS is
+    within 1e-3 px; after 100 the RMS is within 0.05 px, with the printed
+    trajectory showing that the worst 1 % of pixels carry nearly all of the
+    squared difference."""
+    import binder_amd.FastRegistration as fr
+    print("c1: last level from a shared start", flush=True)
+    pair, _ = synth.config_problem("c1", seed=0)
+    monkeypatch.setattr(fr, "Registration", orc)
+    o = fr.FastRegistration(voxels=pair.fixed, nodes=pair.moving, threads=_threads(),
+                            **synth.REGISTRATION_KW)
+    o.run([8, 4, 2], synth.EM_ITERATIONS)
+    o.move_resolution(1)
+    start = np.ascontiguousarray(o.deformedVoxelPositions)
+    theta = np.ascontiguousarray(o.theta).ravel()
+    regs = {}
+    for name, be in (("oracle", orc), ("hip", core)):
+        monkeypatch.setattr(fr, "Registration", be)
+        r = o._new_registration(start)
+        r.setMILikelihood(o.alpha, theta, 32, 32, 1, np.ascontiguousarray(o.binned_nodes).ravel(),
+                          np.ascontiguousarray(o.binned_voxels).ravel())
+        regs[name] = r
+    done, rows = 0, []
+    for k in (1, 3, 10, 30, 100):
+        for r in regs.values():
+            r.EM(k - done, -1e30, 1, 0)
+        done = k
+        fg = np.asarray(regs["hip"].get_final_locations_as_numpy_array(), np.float64)
+        fo = regs["oracle"].get_final_locations_as_numpy_array()
+        tg, to = regs["hip"].get_objective_trace(), regs["oracle"].get_objective_trace()
+        rel = np.abs(tg[-1] - to[-1]) / np.abs(to[-1])
+        d2 = np.sum((fg - fo) ** 2, -1)
+        w = _where(pair, d2)
+        rows.append((k, float(np.sqrt(d2.mean())), float(np.sqrt(d2.max())), float(rel[0]),
+                     float(rel[3]), w["top1pct_share"], w["tissue"], w["background"]))
+    for row in rows:
+        print("c1 last level after %3d iterations: field RMS %.2e px, max %.2e; mll rel %.1e, "
+              "mlp rel %.1e; worst 1%% of pixels carry %.0f%% of the squared difference; "
+              "RMS tissue %.1e, background %.1e" % (row[:5] + (100 * row[5],) + row[6:]),
+              flush=True)
+    k1, k3 = rows[0], rows[1]
+    assert k1[2] <= 1e-3 and k1[3] <= 1e-5 and k1[4] <= 1e-5, k1
+    assert k3[1] <= 1e-3 and k3[3] <= 1e-5 and k3[4] <= 1e-5, k3
+    assert rows[-1][1] <= 0.05, rows[-1]
+
+
+# ---------------------------------------------------------------- c2 ------
+@pytest.fixture(scope="module")
+def c2():
+    t0 = time.perf_counter()
+    pair, prob = synth.config_problem("c2", seed=0)
+    print(f"c2 inputs {prob.shape} in {time.perf_counter() - t0:.1f}s", flush=True)
+    return pair, prob
+
+
+def _em3_vs_oracle(core, orc, prob, iters=3):
+    g = synth.build_registration(core, prob)
+    o = synth.build_registration(orc, prob, threads=_threads())
+    g.EM(iters, -1e30, 1, 0)
+    t0 = time.perf_counter()
+    o.EM(iters, -1e30, 1, 0)
+    dt = time.perf_counter() - t0
+    fg = np.asarray(g.get_final_locations_as_numpy_array(), np.float64)
+    fo = o.get_final_locations_as_numpy_array()
+    tg, to = g.get_objective_trace(), o.get_objective_trace()
+    _, Hg = g.e_step(return_histogram=True)
+    _, Ho = o.e_step(return_histogram=True)
+    return fg, fo, tg, to, _mi(Hg), _mi(Ho), dt
+
+
+def test_c2_em_vs_oracle_full_size(core, orc, c2):
+    """160x192x160, 32 bins: 3 EM iterations from the identity (E-step,
+    MAP theta, split-BF16 DCT smoothing) within 1e-3 voxel max of the FP64
+    oracle's field, mll / mlp within 1e-5 relative, MI within 1e-5."""
+    pair, prob = c2
+    print("c2: 3 EM iterations at 160x192x160, HIP vs FP64 oracle", flush=True)
+    fg, fo, tg, to, mig, mio, dt = _em3_vs_oracle(core, orc, prob)
+    mx = float(np.abs(fg - fo).max())
+    rel = np.abs(tg - to) / np.abs(to)
+    print(f"c2 EM x3: field max {mx:.2e} vox, RMS {_rms(fg, fo):.2e}; mll rel "
+          f"{rel[:, 0].max():.2e}, mlp rel {rel[:, 3].max():.2e}, MI {mig:.6f} vs {mio:.6f}; "
+          f"oracle {dt:.1f}s", flush=True)
+    assert mx <= 1e-3
+    assert rel[:, 0].max() <= 1e-5 and rel[:, 3].max() <= 1e-5
+    assert abs(mig - mio) <= 1e-5 * abs(mio)
+
+
+def test_c2_last_level_from_shared_start(core, orc, monkeypatch, c2):
+    """The full-resolution level of the 3D pipeline from one start (the
+    oracle's level-2 result, resampled and topology-corrected on the host)
+    with the oracle's theta: after 3 EM iterations the field is within
+    1e-3 voxel max of the oracle's and mll / mlp within 1e-5 relative; the
+    trajectory to 10 iterations is printed."""
+    import binder_amd.FastRegistration as fr
+    pair, prob = c2
+    print("c2: last level from a shared start", flush=True)
+    monkeypatch.setattr(fr, "Registration", orc)
+    o = fr.FastRegistration(voxels=pair.fixed, nodes=pair.moving, threads=_threads(),
+                            **synth.REGISTRATION_KW)
+    o.run([8, 4, 2], synth.EM_ITERATIONS)
+    o.move_resolution(1)
+    start = np.ascontiguousarray(o.deformedVoxelPositions)
+    theta = np.ascontiguousarray(o.theta).ravel()
+    regs = {}
+    for name, be in (("oracle", orc), ("hip", core)):
+        monkeypatch.setattr(fr, "Registration", be)
+        r = o._new_registration(start)
+        r.setMILikelihood(o.alpha, theta, 32, 32, _threads(),
+                          np.ascontiguousarray(o.binned_nodes).ravel(),
+                          np.ascontiguousarray(o.binned_voxels).ravel())
+        regs[name] = r
+    done, rows = 0, []
+    for k in (1, 3, 10):
+        for r in regs.values():
+            r.EM(k - done, -1e30, 1, 0)
+        done = k
+        fg = np.asarray(regs["hip"].get_final_locations_as_numpy_array(), np.float64)
+        fo = regs["oracle"].get_final_locations_as_numpy_array()
+        tg, to = regs["hip"].get_objective_trace(), regs["oracle"].get_objective_trace()
+        rel = np.abs(tg[-1] - to[-1]) / np.abs(to[-1])
+        d2 = np.sum((fg - fo) ** 2, -1)
+        w = _where(pair, d2)
+        rows.append((k, float(np.sqrt(d2.mean())), float(np.sqrt(d2.max())), float(rel[0]),
+                     float(rel[3]), w["top1pct_share"]))
+        print("c2 last level after %2d iterations: field RMS %.2e vox, max %.2e; mll rel %.1e, "
+              "mlp rel %.1e; worst 1%% of voxels carry %.0f%% of the squared difference"
+              % (rows[-1][:5] + (100 * rows[-1][5],)), flush=True)
+    k3 = rows[1]
+    assert k3[2] <= 1e-3 and k3[3] <= 1e-5 and k3[4] <= 1e-5, k3
+
+
+def test_c2_fold_masks_bit_exact(core, orc, c2):
+    """Fold masks (J < e1) and Jacobians at 160x192x160 bit-exact with the
+    FP64 oracle (itself bit-exact with the reference's compiled utils.cpp):
+    on the generator's own 6-voxel-RMS warp, which folds, and on the
+    registered, topology-corrected field of the GPU pyramid."""
+    from binder_amd.FastRegistration import FastRegistration
+    pair, prob = c2
+    Nx, Ny, Nz = prob.shape
+    print("c2: fold masks at 160x192x160", flush=True)
+    warp = prob.voxel_pos + pair.displacement
+    fr = FastRegistration(voxels=pair.fixed, nodes=pair.moving, **synth.REGISTRATION_KW)
+    fr.run(list(synth.PYRAMID), synth.EM_ITERATIONS)
+    reg = np.asarray(fr.deformedVoxelPositions, np.float64)
+    for name, f in (("generator warp", warp), ("registered field", reg)):
+        f = np.ascontiguousarray(f)
+        mask, cnt = core.fold_mask_3D(f.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0, e1=0.01)
+        Jg = core.compute_jacobian_determinant_3D(f.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0)
+        Jo = orc.compute_jacobian_determinant_3D(f.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0)
+        print(f"c2 {name}: {cnt} voxels with J < 0.01", flush=True)
+        assert np.array_equal(Jg, Jo), name
+        assert np.array_equal(mask, Jo < 0.01), name
+        assert cnt == int(np.count_nonzero(Jo < 0.01))
+        if name == "generator warp":
+            assert cnt > 0
+
+
+def test_c2_deformation_draw_variance_full_size(core, c2):
+    """Deformation block only at 160x192x160 (dream = x, 200 sweeps): each
+    component is x + DCT-III of N(0, sigma^2 / (1 + gamma sigma^2 D))
+    coefficients, so the voxel-wise variance is
+    sum_k C[k, i]^2 sigma^2 / (1 + gamma sigma^2 D_k), separable per axis."""
+    pair, prob = c2
+    print("c2: 200 deformation sweeps against the closed-form variance", flush=True)
+    sig2 = 9.0 / (8.0 * np.pi)
+    gam = prob.gammas[0]
+    var = sig2 / (1.0 + gam * sig2 * prob.D)
+    for ax, n in enumerate(prob.shape):
+        C2 = _dct_matrix(n) ** 2
+        var = np.moveaxis(np.tensordot(C2.T, np.moveaxis(var, ax, 0), axes=1), 0, ax)
+    r = synth.build_registration(core, prob)
+    r.set_dream_locations(prob.voxel_pos.ravel())
+    ns = 200
+    mean, cov = r.sampler(5, 0, ns, 0, 1, 0, 0, 1)
+    se = np.sqrt(2.0 / (ns - 1))
+    for c in range(3):
+        ratio = cov[..., c, c] / var
+        beyond = float(np.mean(np.abs(ratio - 1) > 4 * se))
+        print(f"c2 component {c}: mean ratio {ratio.mean():.5f}, beyond 4 SE {beyond:.2e}",
+              flush=True)
+        assert beyond < 1e-3, c
+        assert abs(ratio.mean() - 1) < 5e-3, c
+    off = cov[..., 0, 1] / np.sqrt(cov[..., 0, 0] * cov[..., 1, 1])
+    assert abs(off.mean()) < 0.02 and np.mean(np.abs(off) > 5 / np.sqrt(ns)) < 1e-3
+    z = (mean - prob.voxel_pos) / np.sqrt(var[..., None] / ns)
+    assert np.mean(np.abs(z) > 4.5) < 1e-3 and abs(z.mean()) < 0.05
+
+
+# ---------------------------------------------------------------- c3 ------
+def test_c3_em_vs_oracle(core, orc):
+    """256x256x192 (the precision variant): 3 EM iterations with split-BF16
+    DCT contractions against the FP64 oracle: field within 1e-3 voxel max,
+    mll / mlp within 1e-5 relative, MI within 1e-5; Jacobian of the GPU
+    field bit-exact and the same fold count."""
+    t0 = time.perf_counter()
+    pair, prob = synth.config_problem("c3", seed=0)
+    Nx, Ny, Nz = prob.shape
+    print(f"c3 inputs {prob.shape} in {time.perf_counter() - t0:.1f}s", flush=True)
+    fg, fo, tg, to, mig, mio, dt = _em3_vs_oracle(core, orc, prob)
+    mx = float(np.abs(fg - fo).max())
+    rel = np.abs(tg - to) / np.abs(to)
+    print(f"c3 EM x3: field max {mx:.2e} vox, RMS {_rms(fg, fo):.2e}; mll rel "
+          f"{rel[:, 0].max():.2e}, mlp rel {rel[:, 3].max():.2e}, MI {mig:.6f} vs {mio:.6f}; "
+          f"oracle {dt:.1f}s", flush=True)
+    assert mx <= 1e-3
+    assert rel[:, 0].max() <= 1e-5 and rel[:, 3].max() <= 1e-5
+    assert abs(mig - mio) <= 1e-5 * abs(mio)
+    f = np.ascontiguousarray(fg)
+    _, cg = core.fold_mask_3D(f.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0, e1=0.0)
+    Jg = core.compute_jacobian_determinant_3D(f.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0)
+    Jo = orc.compute_jacobian_determinant_3D(f.ravel(), Nx, Ny, Nz, 1.0, 1.0, 1.0)
+    assert np.array_equal(Jg, Jo)
+    assert cg == int(np.count_nonzero(Jo < 0.0))
+
+
+# ---------------------------------------------------------------- c4 ------
+def test_c4_concurrent_pairs_and_chains_equal_alone(core):
+    """Four pairs of uneven shapes (axes from {96, ..., 192}, warp RMS
+    U(2, 6)) registered with all four in flight give, pair by pair, the
+    iteration counts and the fields of the same pairs registered one at a
+    time, bit for bit; so do 16 chains of 1e6 site updates per pair run
+    for all pairs at once against each pair's chains alone."""
+    from concurrent.futures import ThreadPoolExecutor
+    from binder_amd import batch
+    specs = batch.batch_specs(4, seed=3)
+    print(f"c4: shapes {[s['shape'] for s in specs]}", flush=True)
+    pairs = [batch.make_pair(s) for s in specs]
+    conc, _ = batch.register_pairs(pairs, workers=4)
+    alone, _ = batch.register_pairs(pairs, workers=1)
+    for s, a, b in zip(specs, conc, alone):
+        assert [lv[1] for lv in a.level_log] == [lv[1] for lv in b.level_log], s
+        assert np.array_equal(a.deformedVoxelPositions, b.deformedVoxelPositions), s
+
+    def chains(fr):
+        _, _, h = batch.run_chains(fr, chains=16, site_updates=1e6, keep=True)
+        return h.sampler_result()
+
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        together = list(ex.map(chains, conc))
+    for s, fr, (m1, c1) in zip(specs, conc, together):
+        m2, c2_ = chains(fr)
+        assert np.array_equal(m1, m2) and np.array_equal(c1, c2_), s
+        assert np.all(np.isfinite(c1))
+    print("c4: concurrent pairs and chains equal one at a time", flush=True)
+
+
+# ---------------------------------------------------------------- c5 ------
+def test_c5_shipped_network_on_held_out_pairs(core):
+    """The shipped warm-start network (binder_amd/weights) on two held-out
+    160x192x160 pairs: the warm start is used (its level-8 objective is
+    below the identity's), its predicted field is nearer the cold
+    pyramid's converged field than the identity start is, its converged
+    objective is within 0.1 % of the cold pyramid's, and the warm pipeline
+    converges in less GPU wall-clock than the cold pyramid."""
+    from binder_amd import warmstart, warmstart_train
+    from binder_amd.FastRegistration import FastRegistration
+    net = warmstart.WarmStartNet.load(warmstart.WEIGHTS_C2, max_batch=1)
+    with np.load(warmstart.WEIGHTS_C2, allow_pickle=False) as z:
+        seeds = [int(v) for v in z["holdout_seeds"][:2]]
+    for seed in seeds:
+        pair = warmstart_train.make_pair(seed)
+        times = {}
+        for rep in range(2):  # the second run of each is timed
+            cold = FastRegistration(voxels=pair.fixed, nodes=pair.moving,
+                                    **synth.REGISTRATION_KW)
+            t0 = time.perf_counter()
+            cold.run(list(synth.PYRAMID), synth.EM_ITERATIONS)
+            times["cold"] = time.perf_counter() - t0
+            warm = FastRegistration(voxels=pair.fixed, nodes=pair.moving,
+                                    **synth.REGISTRATION_KW)
+            t0 = time.perf_counter()
+            used = warm.run_warm(net, [1], synth.EM_ITERATIONS)
+            times["warm"] = time.perf_counter() - t0
+        Fc = np.asarray(cold.deformedVoxelPositions, np.float64)
+        Fw = np.asarray(warm.deformedVoxelPositions, np.float64)
+        ident = np.stack(np.indices(Fc.shape[:3], dtype=np.float64), -1)
+        start = warmstart.coefficients_field(warm.warm_coef, Fc.shape[:3])
+        oc = float(cold.registratorObj.get_objective_trace()[-1, 3])
+        ow = float(warm.registratorObj.get_objective_trace()[-1, 3])
+        print(f"c5 seed {seed}: used {used}, level-8 objectives {warm.warm_objectives}; "
+              f"start RMS to the cold field {_rms(start, Fc):.3f} (identity "
+              f"{_rms(ident, Fc):.3f}); final objective rel {(ow - oc) / abs(oc):+.2e}; "
+              f"fields {_rms(Fw, Fc):.3f} vox apart; cold {times['cold']:.3f}s, warm "
+              f"{times['warm']:.3f}s, iterations {[lv[1] for lv in cold.level_log]} / "
+              f"{[lv[1] for lv in warm.level_log]}", flush=True)
+        assert used and warm.warm_objectives[0] < warm.warm_objectives[1]
+        assert _rms(start, Fc) < 0.5 * _rms(ident, Fc)
+        assert abs(ow - oc) <= 1e-3 * abs(oc)
+        assert times["warm"] < times["cold"]
