@@ -443,6 +443,18 @@ def batched_report(args):
     k = min(len(pairs), args.batch_serial)
     _, dts = batch.register_pairs(pairs[:k], workers=1)
     out["registration"]["one_at_a_time_pairs_per_s"] = round(k / dts, 2)
+    # level-synchronous lockstep pyramid with batched launches
+    batch.register_pairs_batched(pairs[:2], workers=2)  # warm-up
+    lregs, dtl = batch.register_pairs_batched(pairs, workers=args.batch_workers)
+    same = all([lv[1] for lv in a.level_log] == [lv[1] for lv in b.level_log] and
+               np.array_equal(a.deformedVoxelPositions, b.deformedVoxelPositions)
+               for a, b in zip(lregs, regs))
+    out["registration_lockstep"] = {"seconds": round(dtl, 3),
+                                    "pairs_per_s": round(len(pairs) / dtl, 2),
+                                    "identical_to_threaded": bool(same),
+                                    "scheduler": batch.register_pairs_batched.stats}
+    log(f"lockstep registration: {len(pairs)} pairs in {dtl:.2f}s "
+        f"({len(pairs) / dtl:.1f} pairs/s), identical: {same}")
     chains, upd = 16, 1e6
     for use_graph in (True, False):
         n, dtc = batch.chains_for_pairs(regs, chains, upd, workers=args.batch_workers,

@@ -114,16 +114,20 @@ class FastRegistration:
         if histogram_matching:
             nodes = utils.hist_norm(nodes, voxels)
 
+        # masks: None stands for "all ones" (made on first read, see the
+        # properties below); binning then skips the masking, which leaves
+        # every value unchanged
+        self._mask_shapes = (nodes.shape, voxels.shape)
         if mask_nodes is None:
-            self.full_mask_nodes = utils.maskImage(nodes, th_value=mask_threshold) if use_masks \
-                else np.ones(nodes.shape)
+            self._full_mask_nodes = utils.maskImage(nodes, th_value=mask_threshold) \
+                if use_masks else None
         else:
-            self.full_mask_nodes = np.asarray(mask_nodes).reshape(nodes.shape)
+            self._full_mask_nodes = np.asarray(mask_nodes).reshape(nodes.shape)
         if mask_voxels is None:
-            self.full_mask_voxels = utils.maskImage(voxels, th_value=mask_threshold) if use_masks \
-                else np.ones(voxels.shape)
+            self._full_mask_voxels = utils.maskImage(voxels, th_value=mask_threshold) \
+                if use_masks else None
         else:
-            self.full_mask_voxels = np.asarray(mask_voxels).reshape(voxels.shape)
+            self._full_mask_voxels = np.asarray(mask_voxels).reshape(voxels.shape)
 
         self.finite_difference_order = finite_difference_order
         self.orig_voxels_shape = voxels.shape
@@ -132,28 +136,30 @@ class FastRegistration:
         self.num_node_intensity_levels = num_node_intensity_levels
 
         def _bin(img, mask, levels):
+            masked = img if mask is None else img * mask
             if bin_robust:
-                b, _ = utils.bin_signal_robust(img * mask, levels)
+                b, _ = utils.bin_signal_robust(masked, levels)
             else:
-                b = utils.bin_signal(img * mask, levels)
-            b[np.logical_not(mask)] = 0
+                b = utils.bin_signal(masked, levels)
+            if mask is not None:
+                b[np.logical_not(mask)] = 0
             return b
 
-        self.full_binned_voxels = _bin(voxels, self.full_mask_voxels,
+        self.full_binned_voxels = _bin(voxels, self._full_mask_voxels,
                                        self.num_voxel_intensity_levels)
-        self.full_binned_nodes = _bin(nodes, self.full_mask_nodes,
+        self.full_binned_nodes = _bin(nodes, self._full_mask_nodes,
                                       self.num_node_intensity_levels)
         self.alpha = alpha
         # theta initialised to 1/L for all k, l (FastRegistration.py:224-225)
         self.theta = np.ones([self.num_node_intensity_levels, self.num_voxel_intensity_levels]) \
             / self.num_voxel_intensity_levels
 
-        grid_v = np.stack(np.indices(voxels.shape, dtype=np.float64), axis=-1)
-        grid_n = np.stack(np.indices(nodes.shape, dtype=np.float64), axis=-1)
-        if self.number_of_dimensions == 2:
-            grid_v, grid_n = grid_v[..., 0:2], grid_n[..., 0:2]
-        self.full_voxels_pos = grid_v
-        self.full_nodes_pos = grid_n
+        # full-resolution positions: the grid indices, made on first read
+        # (the GPU pyramid generates them on the device and never reads them)
+        self._full_vpos = None
+        self._full_npos = None
+        self._deformed = None  # None: the field is still the initial positions
+        self._dream = None
 
         # voxel positions expressed in node space (FastRegistration.py:375-395);
         # without a transformation they stay the grid indices at every level
@@ -177,25 +183,84 @@ class FastRegistration:
         self.voxels_shape = self.orig_voxels_shape
         self.non_zero_voxels = np.count_nonzero(self.binned_voxels)
         self._non_zero = {1: self.non_zero_voxels}  # per resolution (bins are fixed)
-        self.deformedVoxelPositions = self.full_voxels_pos
-        self.dreamLocations = np.zeros_like(self.deformedVoxelPositions)
         self.spline_order = spline_order
         self._set_gammas(self.voxels_shape)
         # per-level record: (resolution, EM iterations, seconds)
         self.level_log = []
+
+    def _grid(self, shape):
+        g = np.stack(np.indices(shape, dtype=np.float64), axis=-1)
+        return g[..., 0:2] if self.number_of_dimensions == 2 else g
+
+    @property
+    def full_voxels_pos(self):
+        if self._full_vpos is None:
+            self._full_vpos = self._grid(self.orig_voxels_shape)
+        return self._full_vpos
+
+    @full_voxels_pos.setter
+    def full_voxels_pos(self, value):
+        self._full_vpos = value
+
+    @property
+    def full_nodes_pos(self):
+        if self._full_npos is None:
+            self._full_npos = self._grid(self.orig_nodes_shape)
+        return self._full_npos
+
+    @full_nodes_pos.setter
+    def full_nodes_pos(self, value):
+        self._full_npos = value
+
+    @property
+    def deformedVoxelPositions(self):
+        return self.full_voxels_pos if self._deformed is None else self._deformed
+
+    @deformedVoxelPositions.setter
+    def deformedVoxelPositions(self, value):
+        self._deformed = value
+
+    @property
+    def dreamLocations(self):
+        if self._dream is None:
+            self._dream = np.zeros_like(self.deformedVoxelPositions)
+        return self._dream
+
+    @dreamLocations.setter
+    def dreamLocations(self, value):
+        self._dream = value
+
+    @property
+    def full_mask_voxels(self):
+        if self._full_mask_voxels is None:
+            self._full_mask_voxels = np.ones(self._mask_shapes[1])
+        return self._full_mask_voxels
+
+    @property
+    def full_mask_nodes(self):
+        if self._full_mask_nodes is None:
+            self._full_mask_nodes = np.ones(self._mask_shapes[0])
+        return self._full_mask_nodes
 
     # positions of the current level (full[::res] / res), made on first use:
     # the GPU pyramid generates the index grid on the device and never needs
     # them, and the strided copy costs tens of milliseconds at 160x192x160
     def _level_positions(self, which):
         res = self.current_resolution
-        full = self.full_voxels_pos if which == "v" else self.full_nodes_pos
+        lazy = self._full_vpos is None if which == "v" else self._full_npos is None
         if res == 1:
-            return full
+            return self.full_voxels_pos if which == "v" else self.full_nodes_pos
         key = (which, res)
         if key not in self._level_pos:
             self._level_pos = {k: v for k, v in self._level_pos.items() if k[1] == res}
-            self._level_pos[key] = full[::res, ::res, ::res, :] / res
+            if lazy and (which == "n" or self._index_grid):
+                # full[::res] / res of the index grid is the level's index
+                # grid exactly (k res / res == k in FP64): made at level size
+                shape = self.voxels_shape if which == "v" else tuple(self.nodes_shape)
+                self._level_pos[key] = self._grid(shape)
+            else:
+                full = self.full_voxels_pos if which == "v" else self.full_nodes_pos
+                self._level_pos[key] = full[::res, ::res, ::res, :] / res
         return self._level_pos[key]
 
     @property
@@ -240,6 +305,19 @@ class FastRegistration:
         correction (FastRegistration.py:443-546).  `level_iterations` (not in
         the reference) replaces the per-level cap
         max_number_of_iterations * resolution with explicit counts."""
+        for reg, cap in self.pyramid_levels(resolutions, max_number_of_iterations,
+                                            level_iterations):
+            reg.EM(cap, self.convergence_th,
+                   update_likelihood_parameters=self.update_likelihood_parameters,
+                   debug=self.debug)
+
+    def pyramid_levels(self, resolutions, max_number_of_iterations, level_iterations=None):
+        """The pyramid of run() as a generator: for each level it sets up the
+        level's registration and yields (registration, iteration cap); the
+        caller runs that level's EM (Registration.EM, or Batch.EM over several
+        pairs' levels at once), and resuming the generator finishes the level
+        (topology correction, theta hand-over) and sets up the next.  Not in
+        the reference."""
         import time
         cls = Registration.Registration
         # with this backend the field stays on the GPU from level to level:
@@ -262,11 +340,12 @@ class FastRegistration:
                 # the first level starts from the predicted field (run_warm)
                 self.registratorObj = self._warm_registration(warm, res)
             elif on_gpu:
-                vp = np.ascontiguousarray(self.voxels_pos)
-                # full[::res] / res of the index grid is the index grid again
+                # full[::res] / res of the index grid is the index grid again,
+                # which the device generates
+                vp = None if self._index_grid else np.ascontiguousarray(self.voxels_pos).ravel()
+                sh = self.voxels_shape
                 self.registratorObj = Registration.Registration.resampled_from(
-                    prev, cur / res, None if self._index_grid else vp.ravel(),
-                    vp.shape[0], vp.shape[1], vp.shape[2],
+                    prev, cur / res, vp, sh[0], sh[1], sh[2],
                     self.nodes_shape[0], self.nodes_shape[1], self.nodes_shape[2],
                     self.spline_order, self.non_zero_voxels)
                 self._set_nonlinear(self.registratorObj)
@@ -281,10 +360,7 @@ class FastRegistration:
             cap = max_number_of_iterations * self.current_resolution
             if level_iterations is not None:
                 cap = int(level_iterations[li])
-            self.registratorObj.EM(cap,
-                                   self.convergence_th,
-                                   update_likelihood_parameters=self.update_likelihood_parameters,
-                                   debug=self.debug)
+            yield self.registratorObj, cap
             if self.top_cor_obj and chain:
                 # the same correction as correct_topology(), on the field where it lives
                 n = 2 if self.number_of_dimensions == 2 else 3
@@ -453,9 +529,8 @@ class FastRegistration:
             pass
         elif not self.upsample_in_frequency_domain or cur - res < 0:
             # skimage resize(order=1) semantics, on the GPU
-            if self._index_grid and cur == 1 and \
-                    self.deformedVoxelPositions is self.full_voxels_pos and \
-                    hasattr(Registration, "resize_index_grid"):
+            if self._index_grid and cur == 1 and self._deformed is None and \
+                    self._full_vpos is None and hasattr(Registration, "resize_index_grid"):
                 # the untouched initial field is the index grid: the GPU makes
                 # it instead of receiving 24 bytes per voxel from the host
                 self.deformedVoxelPositions = Registration.resize_index_grid(

@@ -405,6 +405,78 @@ class Registration:
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(keys)}
 
 
+_batch_create = _sig("binder_batch_create", _c_int, _P(_vp), _c_int, _vp)
+_batch_destroy = _sig("binder_batch_destroy", None, _vp)
+_batch_em = _sig("binder_batch_em", _c_int, _vp, _vp, _c_double, _c_int, _vp)
+_batch_set = _sig("binder_batch_set", _c_int, _vp, _c_int, _vp, _c_int, _c_int)
+_batch_run = _sig("binder_batch_run", _c_int, _vp, _c_double, _c_int, _vp)
+
+
+class Batch:
+    """Slots of Registration objects whose EM calls run in lockstep: each
+    iteration runs every running slot's E-step in one launch, their
+    histogram reductions and theta steps in one launch each, their
+    smoothings as batched launches (small grids) or on a few streams (large
+    grids), with one host read of all objectives.  Each registration ends
+    exactly where its own EM(...) call would.  Registrations must share K
+    and L (K <= 32) and dimensionality, and outlive their slot.  No
+    reference counterpart (the reference registers one pair per object,
+    BINDER/FastRegistration.py:443-450).
+
+    Batch(registrations) fills one slot per registration; Batch(n) makes n
+    empty slots.  EM() runs one EM call per slot to the end; set() and run()
+    let slots finish and be refilled (with another pair, or the next
+    pyramid level of the same pair) while the others keep iterating."""
+
+    def __init__(self, registrations):
+        self._h = None
+        if isinstance(registrations, int):
+            self._regs = [None] * registrations
+        else:
+            self._regs = list(registrations)
+        arr = (_vp * len(self._regs))(*[r._h.value if r is not None else None
+                                        for r in self._regs])
+        h = _vp()
+        _check(_batch_create(ctypes.byref(h), len(self._regs), arr))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _batch_destroy(self._h)
+            self._h = None
+
+    def __len__(self):
+        return len(self._regs)
+
+    def EM(self, max_EM_iterations, convergence_th, update_likelihood_parameters=1):
+        """EM of every slot's registration (Registration.EM semantics per
+        registration; max_EM_iterations is one cap or one per slot).
+        Returns the iterations each ran."""
+        n = len(self._regs)
+        caps = np.ascontiguousarray(np.broadcast_to(np.asarray(max_EM_iterations), (n,)),
+                                    np.int32)
+        out = np.zeros(n, np.int32)
+        _check(_batch_em(self._h, _ptr(caps), float(convergence_th),
+                         int(bool(update_likelihood_parameters)), _ptr(out)))
+        return out
+
+    def set(self, slot, registration, max_EM_iterations, update_likelihood_parameters=1):
+        """Puts `registration` (None: nothing) into `slot` with an EM call of
+        max_EM_iterations ahead of it; it starts at the next run()."""
+        _check(_batch_set(self._h, int(slot), None if registration is None else
+                          registration._h, int(max_EM_iterations),
+                          int(bool(update_likelihood_parameters))))
+        self._regs[int(slot)] = registration
+
+    def run(self, convergence_th, min_finished=1):
+        """Iterates every running slot until at least `min_finished` of them
+        finish their EM call (or all have).  Returns, per slot, the
+        iterations of a call that finished during this run, else -1."""
+        out = np.full(len(self._regs), -1, np.int32)
+        _check(_batch_run(self._h, float(convergence_th), int(min_finished), _ptr(out)))
+        return out
+
+
 class NonLinearTransformation:
     """NonLinearTransformation(gamma_x, gamma_y, gamma_z, voxels_size_x..z,
     nodes_size_x..z, D, Nx_v, Ny_v, Nz_v, threads, sigma_spline, spline_offset)
@@ -609,6 +681,40 @@ def resize_index_grid(shape, comps, output_shape, scale=1.0):
     return out
 
 
+_minmax = _sig("binder_min_max", _c_int, _vp, ctypes.c_longlong, _P(_c_double), _P(_c_double))
+_hist_uniform = _sig("binder_histogram_uniform", _c_int, _vp, ctypes.c_longlong, _c_int, _vp,
+                     _vp)
+_bin_floor = _sig("binder_bin_floor", _c_int, _vp, ctypes.c_longlong, _c_double, _c_double,
+                  _c_double, _c_int, _vp)
+
+
+def histogram_uniform(a, bins):
+    """numpy.histogram(a, bins) for an integer bin count, counted over all
+    host cores: (counts int64, edges float64)."""
+    x = _arr(a, np.float64).ravel()
+    lo, hi = _c_double(0.0), _c_double(0.0)
+    _check(_minmax(_ptr(x), x.size, ctypes.byref(lo), ctypes.byref(hi)))
+    if not (np.isfinite(lo.value) and np.isfinite(hi.value)):
+        raise ValueError(f"autodetected range of [{lo.value}, {hi.value}] is not finite")
+    first, last = lo.value, hi.value
+    if first == last:
+        first, last = first - 0.5, last + 0.5
+    edges = np.linspace(first, last, int(bins) + 1, endpoint=True)
+    counts = np.zeros(int(bins), np.int64)
+    _check(_hist_uniform(_ptr(x), x.size, int(bins), _ptr(edges), _ptr(counts)))
+    return counts, edges
+
+
+def bin_floor(a, lo, hi, width, number_of_bins):
+    """floor((a - lo) / width) as int32, 0 below lo and number_of_bins - 1
+    above hi, over all host cores."""
+    x = _arr(a, np.float64)
+    out = np.empty(x.shape, np.int32)
+    _check(_bin_floor(_ptr(x), x.size, float(lo), float(hi), float(width), int(number_of_bins),
+                      _ptr(out)))
+    return out
+
+
 def exported_symbols():
     """Names of the C entry points this module binds (for ABI checks)."""
     return [
@@ -624,7 +730,9 @@ def exported_symbols():
         "binder_registration_sampler_result", "binder_registration_get_final_locations",
         "binder_registration_get_dream_locations", "binder_registration_set_dream_locations",
         "binder_registration_get_likelihood_parameters", "binder_registration_get_shape",
-        "binder_registration_get_gamma",
+        "binder_registration_get_gamma", "binder_batch_create", "binder_batch_destroy",
+        "binder_batch_em", "binder_batch_set", "binder_batch_run", "binder_min_max",
+        "binder_histogram_uniform", "binder_bin_floor",
         "binder_registration_get_objective_trace", "binder_registration_e_step",
         "binder_registration_set_profiling", "binder_registration_get_kernel_times",
         "binder_nlt_create", "binder_nlt_destroy", "binder_nlt_smooth_deformation",

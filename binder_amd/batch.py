@@ -54,6 +54,72 @@ def register_pairs(pairs, workers=8, **kw):
     return regs, time.perf_counter() - t0
 
 
+def register_pairs_batched(pairs, workers=8, **kw):
+    """Registers every pair with the plain pyramid, all pairs' EM iterations
+    sharing launches: one Batch slot per pair runs the pair's current level;
+    when a pair's level converges, a worker thread finishes that level
+    (topology correction on the GPU) and sets up the next one while the other
+    slots keep iterating, and the next level rejoins the batch.  Each
+    iteration runs the E-steps of all running pairs in one launch over
+    (pair, block) items, their theta steps in one launch, and the smoothing
+    passes of small grids as one launch per pass.  Every pair ends with
+    exactly the field and iteration counts of register_one.  Returns the
+    FastRegistration objects in order and the wall-clock."""
+    from concurrent.futures import FIRST_COMPLETED, wait
+    from .FastRegistration import FastRegistration, Registration
+    t0 = time.perf_counter()
+    n = len(pairs)
+    frs, gens = [None] * n, [None] * n
+    kw = dict(synth.REGISTRATION_KW, **kw)
+    th = kw.get("convergence_th", 5e-5)
+    upd = int(kw.get("update_likelihood_parameters", 1))
+    bt = Registration.Batch(n)
+
+    stats = {"runs": 0, "idle_waits": 0, "run_s": 0.0, "wait_s": 0.0, "setup_s": 0.0,
+             "slot_iterations": 0}
+
+    def timed(fn):
+        t = time.perf_counter()
+        out = fn()
+        stats["setup_s"] += time.perf_counter() - t
+        return out
+
+    def first(i):
+        frs[i] = FastRegistration(voxels=pairs[i].fixed, nodes=pairs[i].moving, **kw)
+        gens[i] = frs[i].pyramid_levels(list(synth.PYRAMID), synth.EM_ITERATIONS)
+        return next(gens[i], None)
+
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        pending = {ex.submit(timed, lambda i=i: first(i)): i for i in range(n)}
+        running = 0
+        while pending or running:
+            if running:
+                ready = [f for f in pending if f.done()]
+            else:
+                stats["idle_waits"] += 1
+                tw = time.perf_counter()
+                ready = list(wait(list(pending), return_when=FIRST_COMPLETED).done)
+                stats["wait_s"] += time.perf_counter() - tw
+            for f in ready:
+                i = pending.pop(f)
+                step = f.result()
+                if step is not None:
+                    bt.set(i, step[0], step[1], upd)
+                    running += 1
+            if running:
+                tr = time.perf_counter()
+                fin = bt.run(th, 1)
+                stats["run_s"] += time.perf_counter() - tr
+                stats["runs"] += 1
+                for i in np.nonzero(fin >= 0)[0]:
+                    running -= 1
+                    bt.set(int(i), None, 0, upd)
+                    pending[ex.submit(timed, lambda i=int(i): next(gens[i], None))] = int(i)
+    stats = {k: round(v, 3) if isinstance(v, float) else v for k, v in stats.items()}
+    register_pairs_batched.stats = stats
+    return frs, time.perf_counter() - t0
+
+
 def full_resolution_handle(fr):
     """The registration handle holding the pair's registered state: the
     pyramid's last (full-resolution) level, whose device field is the

@@ -555,8 +555,12 @@ constexpr int kEm32Threads = 512;
 constexpr int kEm32MaxK = 32;
 constexpr float kSplineScale = 2147483648.0f;  // 2^31
 
+// One logical block `lb` of `nb` of the EM E-step of registration `a`: the
+// voxel range, the histogram partial row and the mll partial of that block.
+// The result depends only on (a, lb, nb), never on which physical block runs
+// it, so one pair's blocks give the same partials alone or in a batch.
 template <bool IS2D>
-__global__ __launch_bounds__(kEm32Threads) void k_estep_em32(EStepArgs a) {
+__device__ __forceinline__ void estep_em32_block(const EStepArgs &a, int lb, int nb) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int KL = a.K * a.L;
   unsigned long long *hist = reinterpret_cast<unsigned long long *>(smem);
@@ -577,9 +581,8 @@ __global__ __launch_bounds__(kEm32Threads) void k_estep_em32(EStepArgs a) {
 
   constexpr int NZ = IS2D ? 1 : 4;
   double l2sum = 0.0;  // sum of log2 Z_i
-  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const long long per =
-      ((a.nv + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
+      ((a.nv + nb - 1) / nb + blockDim.x - 1) / blockDim.x * blockDim.x;
   const long long vbeg = (long long)lb * per;
   const long long vend = vbeg + per < a.nv ? vbeg + per : a.nv;
   // rotation of this lane's flush order, so lanes of one wave that share a
@@ -683,29 +686,54 @@ __global__ __launch_bounds__(kEm32Threads) void k_estep_em32(EStepArgs a) {
   if (threadIdx.x == 0) a.mllpart[lb] = -kLn2 * red[0];
 }
 
+template <bool IS2D>
+__global__ __launch_bounds__(kEm32Threads) void k_estep_em32(EStepArgs a) {
+  estep_em32_block<IS2D>(a, xcd_logical_block(blockIdx.x, gridDim.x), gridDim.x);
+}
+
+// The same E-step for several registrations in one launch: work item w
+// (pair << 16 | logical block) runs estep_em32_block of that pair, with the
+// pair's own block count.  Items are ordered pair-major, and consecutive
+// items share an XCD, as one pair's logical blocks do alone.
+template <bool IS2D>
+__global__ __launch_bounds__(kEm32Threads) void k_estep_em32_batch(const EStepArgs *args,
+                                                                 const uint32_t *work) {
+  const uint32_t w = work[xcd_logical_block(blockIdx.x, gridDim.x)];
+  const EStepArgs a = args[w >> 16];
+  estep_em32_block<IS2D>(a, (int)(w & 0xFFFFu), a.nblocks);
+}
+
 constexpr int kRowsPerReduce = 16;
 
-__global__ __launch_bounds__(256) void k_hist_reduce1(
-    const unsigned long long *__restrict__ hpart, int rows, int KL,
-    unsigned long long *__restrict__ hpart2) {
-  const int kl = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void hist_reduce1_block(const unsigned long long *__restrict__ hpart,
+                                                   int rows, int KL,
+                                                   unsigned long long *__restrict__ hpart2,
+                                                   int bx, int by) {
+  const int kl = bx * 256 + threadIdx.x;
   if (kl >= KL) return;
-  const int r0 = blockIdx.y * kRowsPerReduce;
+  const int r0 = by * kRowsPerReduce;
+  if (r0 >= rows) return;
   unsigned long long s = 0;
 #pragma unroll
   for (int r = 0; r < kRowsPerReduce; ++r)
     if (r0 + r < rows) s += hpart[(long long)(r0 + r) * KL + kl];
-  hpart2[(long long)blockIdx.y * KL + kl] = s;
+  hpart2[(long long)by * KL + kl] = s;
+}
+
+__global__ __launch_bounds__(256) void k_hist_reduce1(
+    const unsigned long long *__restrict__ hpart, int rows, int KL,
+    unsigned long long *__restrict__ hpart2) {
+  hist_reduce1_block(hpart, rows, KL, hpart2, blockIdx.x, blockIdx.y);
 }
 
 // 64 entries per block, four row subsets per entry summed in parallel and
 // combined in a fixed order (integer sums: exact).
-__global__ __launch_bounds__(256) void k_hist_reduce2(
-    const unsigned long long *__restrict__ hpart2, int rows2, int KL,
-    double *__restrict__ Hd) {
+__device__ __forceinline__ void hist_reduce2_block(const unsigned long long *__restrict__ hpart2,
+                                                   int rows2, int KL, double *__restrict__ Hd,
+                                                   int bx) {
   __shared__ unsigned long long part[4][64];
   const int e = threadIdx.x & 63, sub = threadIdx.x >> 6;
-  const int kl = blockIdx.x * 64 + e;
+  const int kl = bx * 64 + e;
   unsigned long long s = 0;
   if (kl < KL) {
 #pragma unroll 8
@@ -718,11 +746,28 @@ __global__ __launch_bounds__(256) void k_hist_reduce2(
              (1.0 / kHistScale);
 }
 
+__global__ __launch_bounds__(256) void k_hist_reduce2(
+    const unsigned long long *__restrict__ hpart2, int rows2, int KL,
+    double *__restrict__ Hd) {
+  hist_reduce2_block(hpart2, rows2, KL, Hd, blockIdx.x);
+}
+
+// Both reduction stages for several registrations (blockIdx.z = pair).
+__global__ __launch_bounds__(256) void k_hist_reduce1_batch(const HistReduceArgs *args) {
+  const HistReduceArgs a = args[blockIdx.z];
+  hist_reduce1_block(a.hpart, a.rows, a.KL, a.hpart2, blockIdx.x, blockIdx.y);
+}
+__global__ __launch_bounds__(256) void k_hist_reduce2_batch(const HistReduceArgs *args) {
+  const HistReduceArgs a = args[blockIdx.z];
+  hist_reduce2_block(a.hpart2, (a.rows + kRowsPerReduce - 1) / kRowsPerReduce, a.KL, a.Hd,
+                     blockIdx.x);
+}
+
 // Likelihood-parameter step in one block, working in LDS: sum of the mll
 // partials, then the MAP update (H + alpha - 1)/row sum, or a
 // Dirichlet(H + alpha) draw, or no change; then the Dirichlet prior cost and
 // the transposed FP32 table.
-__global__ __launch_bounds__(1024) void k_theta(ThetaArgs a) {
+__device__ __forceinline__ void theta_block(const ThetaArgs &a) {
   // Philox key, with the per-chain device mask when given (captured sweeps)
   const uint32_t key0 = a.keyx ? a.k0 ^ a.keyx[0] : a.k0;
   const uint32_t key1 = a.keyx ? a.k1 ^ a.keyx[1] : a.k1;
@@ -785,6 +830,14 @@ __global__ __launch_bounds__(1024) void k_theta(ThetaArgs a) {
     a.stats[0] = red[0];
     a.stats[1] = c;
   }
+}
+
+__global__ __launch_bounds__(1024) void k_theta(ThetaArgs a) { theta_block(a); }
+
+// The theta step of several registrations, one block each.
+__global__ __launch_bounds__(1024) void k_theta_batch(const ThetaArgs *args) {
+  const ThetaArgs a = args[blockIdx.x];
+  theta_block(a);
 }
 
 // One sample's Welford step (Registration.cpp:593-623): d1 = x - mu_old,
@@ -865,9 +918,7 @@ void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
     return;
   }
   if (a.K <= kEm32MaxK) {
-    const size_t lds32 = (size_t)a.K * a.L * sizeof(unsigned long long) +
-                         (size_t)((Kp * a.L + 3) & ~3) * sizeof(float) +
-                         (size_t)(kEm32Threads / 64) * kEm32MaxK * 64 * sizeof(float);
+    const size_t lds32 = estep_em32_lds(a.K, a.L);
     static std::atomic<bool> attr_set[2] = {false, false};
     auto k = is2d ? k_estep_em32<true> : k_estep_em32<false>;
     if (!attr_set[is2d]) {
@@ -887,6 +938,48 @@ void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
     if (sample) hipLaunchKernelGGL((k_estep<false, true>), g, b, lds, s, a);
     else hipLaunchKernelGGL((k_estep<false, false>), g, b, lds, s, a);
   }
+  HIP_LAUNCH_OK();
+}
+
+size_t estep_em32_lds(int K, int L) {
+  const int Kp = theta_row_stride(K, L);
+  return (size_t)K * L * sizeof(unsigned long long) + (size_t)((Kp * L + 3) & ~3) * sizeof(float) +
+         (size_t)(kEm32Threads / 64) * kEm32MaxK * 64 * sizeof(float);
+}
+
+bool estep_em32_supported(int K) { return K <= kEm32MaxK; }
+
+void launch_estep_em_batch(const EStepArgs *d_args, const uint32_t *d_work, int nwork,
+                           bool is2d, int K, int L, hipStream_t s) {
+  if (nwork <= 0) return;
+  BINDER_REQUIRE(K <= kEm32MaxK, "batched E-step needs K <= 32");
+  static std::atomic<bool> attr_set[2] = {false, false};
+  auto k = is2d ? k_estep_em32_batch<true> : k_estep_em32_batch<false>;
+  if (!attr_set[is2d]) {
+    HIP_OK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               160 * 1024));
+    attr_set[is2d] = true;
+  }
+  hipLaunchKernelGGL(k, dim3(nwork), dim3(kEm32Threads), estep_em32_lds(K, L), s, d_args,
+                     d_work);
+  HIP_LAUNCH_OK();
+}
+
+void launch_hist_reduce_batch(const HistReduceArgs *d_args, int npairs, int max_rows, int KL,
+                              hipStream_t s) {
+  if (npairs <= 0) return;
+  const int rows2 = (max_rows + kRowsPerReduce - 1) / kRowsPerReduce;
+  hipLaunchKernelGGL(k_hist_reduce1_batch, dim3((KL + 255) / 256, rows2, npairs), dim3(256), 0,
+                     s, d_args);
+  HIP_LAUNCH_OK();
+  hipLaunchKernelGGL(k_hist_reduce2_batch, dim3((KL + 63) / 64, 1, npairs), dim3(256), 0, s,
+                     d_args);
+  HIP_LAUNCH_OK();
+}
+
+void launch_theta_batch(const ThetaArgs *d_args, int npairs, hipStream_t s) {
+  if (npairs <= 0) return;
+  hipLaunchKernelGGL(k_theta_batch, dim3(npairs), dim3(1024), 0, s, d_args);
   HIP_LAUNCH_OK();
 }
 

@@ -40,6 +40,9 @@ struct EStepArgs {
   // optional device-resident key mask: the key used is (k0, k1) ^ keyx[0..1]
   // (lets one captured sweep graph serve chains with different seeds)
   const uint32_t *keyx;
+  // logical blocks of the EM E-step of this registration (batched launches;
+  // a single launch uses its grid size)
+  int nblocks;
 };
 
 void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
@@ -48,6 +51,25 @@ void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
 // Sum `rows` partial histograms into Hd (doubles, divided by the scale).
 void launch_hist_reduce(const unsigned long long *hpart, int rows, int KL,
                         unsigned long long *hpart2, double *Hd, hipStream_t s);
+
+// Several registrations' EM E-steps in one launch (K <= 32 only): d_work
+// holds nwork items (pair << 16 | logical block), pair-major; every pair's
+// EStepArgs (d_args[pair]) carries its own block count, so its partials are
+// those of its own single launch.
+bool estep_em32_supported(int K);
+size_t estep_em32_lds(int K, int L);
+void launch_estep_em_batch(const EStepArgs *d_args, const uint32_t *d_work, int nwork,
+                           bool is2d, int K, int L, hipStream_t s);
+
+struct HistReduceArgs {
+  const unsigned long long *hpart;  // [rows][KL]
+  unsigned long long *hpart2;       // [ceil(rows / 16)][KL]
+  double *Hd;                       // [KL]
+  int rows, KL;
+};
+// launch_hist_reduce for npairs registrations (max_rows: the largest rows).
+void launch_hist_reduce_batch(const HistReduceArgs *d_args, int npairs, int max_rows, int KL,
+                              hipStream_t s);
 
 enum ThetaMode { kThetaKeep = 0, kThetaMap = 1, kThetaDirichlet = 2 };
 
@@ -67,6 +89,8 @@ struct ThetaArgs {
 };
 
 void launch_theta(const ThetaArgs &a, hipStream_t s);
+// launch_theta for npairs registrations, one block each.
+void launch_theta_batch(const ThetaArgs *d_args, int npairs, hipStream_t s);
 
 // FP64 Welford update of mean [N][nv] and upper-triangular M2 [N(N+1)/2][nv]
 // with `count` samples x[0..count) (each [N][nv]) taken in that order, n0

@@ -332,6 +332,14 @@ struct binder_registration {
                  double *theta_out = nullptr, float *thT_out = nullptr,
                  double *stats_out = nullptr) {
     if (!st) st = s();
+    const ThetaArgs t = theta_args(mode, k0, k1, sweep, keyx, theta_out, thT_out, stats_out);
+    TimedLaunch tl(tm(), mode == kThetaDirichlet ? "dirichlet" : "theta", st);
+    launch_theta(t, st);
+  }
+
+  ThetaArgs theta_args(int mode, uint32_t k0, uint32_t k1, uint32_t sweep,
+                       const uint32_t *keyx, double *theta_out, float *thT_out,
+                       double *stats_out) {
     ThetaArgs t{};
     t.Hd = Hd.get();
     t.theta = theta_out ? theta_out : theta.get();
@@ -354,8 +362,7 @@ struct binder_registration {
     t.k1 = k1;
     t.keyx = keyx;
     t.sweep = sweep;
-    TimedLaunch tl(tm(), mode == kThetaDirichlet ? "dirichlet" : "theta", st);
-    launch_theta(t, st);
+    return t;
   }
 
   // Makes the ring ready for a call of `sweeps` sweeps: the current field
@@ -436,6 +443,61 @@ struct binder_registration {
       }
     }
     if (overlap) HIP_OK(hipStreamWaitEvent(s(), theta_done.e, 0));
+  }
+};
+
+// Several registrations iterated in lockstep (batched EM): every iteration
+// runs the E-step of all active registrations in one launch, then their
+// histogram reductions and theta steps in one launch each, then their
+// smoothings (batched launches for small grids, a few pool streams for large
+// ones), and one copy of all objective terms to the host.  A registration's
+// kernels compute exactly what its own EM loop computes (same logical
+// blocks, same kernels), so each keeps the result it would have alone.
+// Slots can be refilled between runs, so pairs at different pyramid levels
+// share the launches.
+constexpr int kBatchPool = 4;
+
+struct binder_batch {
+  Stream stream;
+  Stream pool[kBatchPool];
+  Event theta_done[2], iter_done[2];
+  Event pool_done[2][kBatchPool];
+  int K = -1, L = -1;
+  bool is2d = false;
+  // one slot per registration: its handle, its EM call's state (as the
+  // local variables of binder_registration_em keep it)
+  struct Slot {
+    binder_registration *r = nullptr;
+    int cap = 0, k = 0;
+    double mlp_old = 1e100;
+    bool done = true;
+    float *F[2] = {}, *DR[2] = {}, *TT[2] = {};
+    double *TH[2] = {};
+  };
+  std::vector<Slot> slots;
+  int update = 1;  // the update flag the slots' buffers were prepared for
+  DBuf<double> stats;      // [2][P][8] objective terms per parity and slot
+  double *host = nullptr;  // pinned copy of stats
+  // per iteration parity of a run: the device argument set of the (slot,
+  // buffer parity) list it was built for
+  struct ArgSet {
+    std::vector<long long> key;
+    DBuf<unsigned char> dev;  // EStepArgs[n] | HistReduceArgs[n] | ThetaArgs[n] | work
+    unsigned char *pin = nullptr;
+    size_t pin_cap = 0;
+    size_t off_h = 0, off_t = 0, off_w = 0;
+    int nwork = 0, max_rows = 0;
+    bool valid = false;
+    // the smoothing of registrations whose passes are all small runs as
+    // batched launches on the batch stream; the others' on the pool streams
+    SmoothBatch smooth;
+    bool any_small = false;
+    std::vector<int> big;  // positions in the list
+  } set[2];
+  ~binder_batch() {
+    if (host) (void)hipHostFree(host);
+    for (auto &a : set)
+      if (a.pin) (void)hipHostFree(a.pin);
   }
 };
 
@@ -799,6 +861,320 @@ int binder_registration_em(binder_registration *r, int max_it, double th,
       }
     }
     if ((k - 1) & 1) swap_buffers(r->dream, r->em_dream);
+  });
+}
+
+namespace {
+
+// Puts `r` (or nothing) into slot `p` of a batch with an EM call of `cap`
+// iterations ahead of it, preparing its double buffers as
+// binder_registration_em does.
+void batch_fill(binder_batch *b, int p, binder_registration *r, int cap, int update) {
+  binder_batch::Slot &q = b->slots[p];
+  q = binder_batch::Slot{};
+  for (auto &A : b->set) A.valid = false;
+  if (!r) return;
+  r->require_ready();
+  BINDER_REQUIRE(estep_em32_supported(r->K), "batched EM needs K <= 32 moving bins");
+  if (b->K < 0) {
+    b->K = r->K;
+    b->L = r->L;
+    b->is2d = r->is2d;
+  }
+  BINDER_REQUIRE(r->K == b->K && r->L == b->L,
+                 "all registrations of a batch need the same K and L");
+  BINDER_REQUIRE(r->is2d == b->is2d, "a batch is either all 2D or all 3D");
+  for (int o = 0; o < (int)b->slots.size(); ++o)
+    BINDER_REQUIRE(o == p || b->slots[o].r != r, "a registration appears twice in the batch");
+  const size_t KL = (size_t)r->K * r->L;
+  r->trace.clear();
+  const size_t nf = (size_t)r->N * r->nv;
+  if (r->dslot != 0) {
+    HIP_OK(hipMemcpyAsync(r->d.get(), r->dcur(), nf * sizeof(float), hipMemcpyDeviceToDevice,
+                          r->s()));
+    r->dslot = 0;
+  }
+  r->em_d.alloc(nf);
+  r->em_dream.alloc(nf);
+  if (update) {
+    r->em_theta.alloc(KL);
+    r->em_thT.alloc(KL);
+  }
+  // the handle's own stream has finished its set-up work before the batch
+  // stream reads the handle's buffers
+  HIP_OK(hipStreamSynchronize(r->s()));
+  q.r = r;
+  q.cap = cap;
+  q.done = cap <= 0;
+  q.F[0] = r->d.get();
+  q.F[1] = r->em_d.get();
+  q.DR[0] = r->dream.get();
+  q.DR[1] = r->em_dream.get();
+  q.TH[0] = r->theta.get();
+  q.TH[1] = update ? r->em_theta.get() : r->theta.get();
+  q.TT[0] = r->thT.get();
+  q.TT[1] = update ? r->em_thT.get() : r->thT.get();
+}
+
+// The end of a slot's EM call: the state after q.k iterations is parity
+// q.k & 1, the dream locations those of iteration q.k - 1.
+void batch_finish(binder_batch::Slot &q, int update) {
+  binder_registration *r = q.r;
+  const int kp = q.k;
+  if (kp == 0) return;
+  if (kp & 1) {
+    swap_buffers(r->d, r->em_d);
+    if (update) {
+      swap_buffers(r->theta, r->em_theta);
+      swap_buffers(r->thT, r->em_thT);
+    }
+  }
+  if ((kp - 1) & 1) swap_buffers(r->dream, r->em_dream);
+}
+
+// Runs the EM iterations of every unfinished slot in lockstep until at
+// least `min_finished` slots finish in this call (or all have); finished[p]
+// receives the iteration count of a slot that finished in this call, else
+// -1.  Iteration k + 1 is queued before iteration k's objectives are read,
+// and the last queued iteration is drained before returning, so a slot left
+// unfinished resumes exactly where it stands.
+void batch_run(binder_batch *b, double th, int min_finished, int *finished) {
+  const int P = (int)b->slots.size();
+  const int update = b->update;
+  const size_t KL = (size_t)std::max(b->K, 0) * std::max(b->L, 0);
+  for (int p = 0; p < P; ++p) finished[p] = -1;
+  double *dstats = b->stats.get();
+  // argument set of call-iteration parity `ip` for `list` (each slot at its
+  // own buffer parity q.k & 1); rebuilt only when the list changes, once the
+  // iteration that last used it has completed
+  auto build = [&](int ip, const std::vector<int> &list) {
+    binder_batch::ArgSet &A = b->set[ip];
+    std::vector<long long> key;
+    for (int p : list) key.push_back(((long long)p << 1) | (b->slots[p].k & 1));
+    if (A.valid && A.key == key) return;
+    const int n = (int)list.size();
+    int nwork = 0, max_rows = 0;
+    for (int p : list) {
+      nwork += b->slots[p].r->estep_blocks;
+      max_rows = std::max(max_rows, b->slots[p].r->estep_blocks);
+    }
+    auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    A.off_h = align(sizeof(EStepArgs) * n);
+    A.off_t = A.off_h + align(sizeof(HistReduceArgs) * n);
+    A.off_w = A.off_t + align(sizeof(ThetaArgs) * n);
+    const size_t bytes = A.off_w + sizeof(uint32_t) * (size_t)nwork;
+    if (A.pin_cap < bytes) {
+      if (A.pin) HIP_OK(hipHostFree(A.pin));
+      A.pin = nullptr;
+      HIP_OK(hipHostMalloc((void **)&A.pin, bytes, 0));
+      A.pin_cap = bytes;
+    }
+    if (A.dev.n < bytes) A.dev.alloc(bytes);
+    auto *ea = reinterpret_cast<EStepArgs *>(A.pin);
+    auto *ha = reinterpret_cast<HistReduceArgs *>(A.pin + A.off_h);
+    auto *ta = reinterpret_cast<ThetaArgs *>(A.pin + A.off_t);
+    auto *wk = reinterpret_cast<uint32_t *>(A.pin + A.off_w);
+    std::vector<SmoothBatch::Job> jobs;
+    A.big.clear();
+    int w = 0;
+    for (int j = 0; j < n; ++j) {
+      const int p = list[j];
+      const binder_batch::Slot &x = b->slots[p];
+      binder_registration *r = x.r;
+      const int par = x.k & 1, q = par ^ 1;
+      EStepArgs e = r->estep_args(false, 0, 0, 0);
+      e.d = x.F[par];
+      e.thT = x.TT[par];
+      e.dream = x.DR[par];
+      e.nblocks = r->estep_blocks;
+      ea[j] = e;
+      ha[j] = HistReduceArgs{r->hpart.get(), r->hpart2.get(), r->Hd.get(), r->estep_blocks,
+                             (int)KL};
+      ta[j] = r->theta_args(update ? kThetaMap : kThetaKeep, 0, 0, 0, nullptr,
+                            x.TH[update ? q : par], x.TT[update ? q : par],
+                            dstats + ((size_t)ip * P + p) * 8);
+      for (int lb = 0; lb < r->estep_blocks; ++lb) wk[w++] = ((uint32_t)j << 16) | (uint32_t)lb;
+      if (r->sm->batchable_em())
+        jobs.push_back(SmoothBatch::Job{r->sm.get(), x.DR[par], r->x.get(), x.F[q],
+                                        dstats + ((size_t)ip * P + p) * 8 + 2});
+      else
+        A.big.push_back(j);
+    }
+    HIP_OK(hipMemcpyAsync(A.dev.get(), A.pin, bytes, hipMemcpyHostToDevice, b->stream.s));
+    A.any_small = !jobs.empty();
+    if (A.any_small) A.smooth.build(jobs, b->stream.s);
+    A.key = key;
+    A.nwork = nwork;
+    A.max_rows = max_rows;
+    A.valid = true;
+  };
+  auto enqueue = [&](int ip, const std::vector<int> &list) {
+    build(ip, list);
+    binder_batch::ArgSet &A = b->set[ip];
+    const int n = (int)list.size();
+    hipStream_t bs = b->stream.s;
+    unsigned char *dev = A.dev.get();
+    launch_estep_em_batch(reinterpret_cast<const EStepArgs *>(dev),
+                          reinterpret_cast<const uint32_t *>(dev + A.off_w), A.nwork, b->is2d,
+                          b->K, b->L, bs);
+    launch_hist_reduce_batch(reinterpret_cast<const HistReduceArgs *>(dev + A.off_h), n,
+                             A.max_rows, (int)KL, bs);
+    launch_theta_batch(reinterpret_cast<const ThetaArgs *>(dev + A.off_t), n, bs);
+    const int nbig = (int)A.big.size();
+    const int np = std::min(nbig, kBatchPool);
+    if (np > 0) {
+      HIP_OK(hipEventRecord(b->theta_done[ip].e, bs));
+      for (int i = 0; i < np; ++i) HIP_OK(hipStreamWaitEvent(b->pool[i].s, b->theta_done[ip].e, 0));
+    }
+    if (A.any_small) A.smooth.launch(bs);
+    for (int i = 0; i < nbig; ++i) {
+      const int p = list[A.big[i]];
+      const binder_batch::Slot &x = b->slots[p];
+      const int par = x.k & 1;
+      SmoothOptions opt;
+      opt.prior_out = dstats + ((size_t)ip * P + p) * 8 + 2;
+      opt.exec_stream = b->pool[i % kBatchPool].s;
+      x.r->sm->run(x.DR[par], x.r->x.get(), x.F[par ^ 1], opt, nullptr);
+    }
+    for (int i = 0; i < np; ++i) {
+      HIP_OK(hipEventRecord(b->pool_done[ip][i].e, b->pool[i].s));
+      HIP_OK(hipStreamWaitEvent(bs, b->pool_done[ip][i].e, 0));
+    }
+    HIP_OK(hipMemcpyAsync(b->host + (size_t)ip * P * 8, dstats + (size_t)ip * P * 8,
+                          sizeof(double) * P * 8, hipMemcpyDeviceToHost, bs));
+    HIP_OK(hipEventRecord(b->iter_done[ip].e, bs));
+  };
+  // objective of list's iteration at call-iteration parity ip: each slot's
+  // stop test; returns the slots that go on
+  int nfin = 0;
+  auto read = [&](int ip, const std::vector<int> &list) {
+    std::vector<int> still;
+    for (int p : list) {
+      binder_batch::Slot &x = b->slots[p];
+      binder_registration *r = x.r;
+      const double *hs = b->host + ((size_t)ip * P + p) * 8;
+      const double mll = hs[0], pt = hs[1], pd = hs[2];
+      const double mlp = mll + pd + pt;
+      r->trace.insert(r->trace.end(), {mll, pt, pd, mlp});
+      bool stop = (x.mlp_old - mlp) / r->nnz < th;
+      x.mlp_old = mlp;
+      x.k += 1;
+      if (x.k >= x.cap) stop = true;
+      if (stop) {
+        x.done = true;
+        finished[p] = x.k;
+        ++nfin;
+        batch_finish(x, update);
+      } else {
+        still.push_back(p);
+      }
+    }
+    return still;
+  };
+  std::vector<int> cur;
+  for (int p = 0; p < P; ++p)
+    if (b->slots[p].r && !b->slots[p].done) cur.push_back(p);
+  if (cur.empty()) return;
+  if (min_finished > (int)cur.size()) min_finished = (int)cur.size();
+  int ip = 0;
+  enqueue(ip, cur);
+  while (true) {
+    // queue the next iteration (each slot's k + 1) before reading this one;
+    // a slot that stops now has that iteration computed into buffers that
+    // are not used (as in binder_registration_em), so its state was already
+    // fixed by batch_finish in read() -- the queued iteration writes only
+    // the other parity
+    std::vector<int> nxt;
+    for (int p : cur)
+      if (b->slots[p].k + 1 < b->slots[p].cap) nxt.push_back(p);
+    const bool more = nfin < min_finished;  // decided before this read
+    if (!nxt.empty() && more) {
+      // the queued iteration's slots are at buffer parity k + 1: advance
+      // their counters for the argument build, then restore
+      for (int p : nxt) b->slots[p].k += 1;
+      enqueue(ip ^ 1, nxt);
+      for (int p : nxt) b->slots[p].k -= 1;
+    }
+    HIP_OK(hipEventSynchronize(b->iter_done[ip].e));
+    std::vector<int> still = read(ip, cur);
+    if (!more || nxt.empty()) {
+      if (nxt.empty() || still.empty()) break;
+      // drain: the queued iteration was not enqueued (more == false), so
+      // every slot in `still` stands after its iteration k: return
+      break;
+    }
+    // slots still running whose next iteration is queued
+    std::vector<int> go;
+    for (int p : still)
+      if (std::find(nxt.begin(), nxt.end(), p) != nxt.end()) go.push_back(p);
+    cur = go;
+    ip ^= 1;
+    if (cur.empty()) {
+      HIP_OK(hipStreamSynchronize(b->stream.s));
+      break;
+    }
+  }
+  HIP_OK(hipStreamSynchronize(b->stream.s));
+  for (int i = 0; i < kBatchPool; ++i) HIP_OK(hipStreamSynchronize(b->pool[i].s));
+}
+
+}  // namespace
+
+int binder_batch_create(binder_batch **out, int npairs, binder_registration *const *regs) {
+  return guarded([&] {
+    BINDER_REQUIRE(out && npairs >= 1, "need at least one slot");
+    BINDER_REQUIRE(npairs < 65536, "at most 65535 slots per batch");
+    auto b = std::make_unique<binder_batch>();
+    b->slots.resize(npairs);
+    for (int p = 0; p < npairs; ++p)
+      if (regs && regs[p]) batch_fill(b.get(), p, regs[p], 0, 1);
+    b->stats.alloc((size_t)2 * npairs * 8);
+    HIP_OK(hipHostMalloc((void **)&b->host, sizeof(double) * 2 * npairs * 8, 0));
+    *out = b.release();
+  });
+}
+
+void binder_batch_destroy(binder_batch *b) {
+  if (!b) return;
+  (void)hipStreamSynchronize(b->stream.s);
+  delete b;
+}
+
+int binder_batch_set(binder_batch *b, int slot, binder_registration *reg, int max_iterations,
+                     int update) {
+  return guarded([&] {
+    BINDER_REQUIRE(b, "null batch");
+    BINDER_REQUIRE(slot >= 0 && slot < (int)b->slots.size(), "slot out of range");
+    bool others = false;
+    for (int p = 0; p < (int)b->slots.size(); ++p)
+      if (p != slot && b->slots[p].r && !b->slots[p].done) others = true;
+    BINDER_REQUIRE(!others || (update != 0) == (b->update != 0),
+                   "all running slots of a batch share update_likelihood_parameters");
+    b->update = update ? 1 : 0;
+    batch_fill(b, slot, reg, max_iterations, b->update);
+  });
+}
+
+int binder_batch_run(binder_batch *b, double th, int min_finished, int *finished) {
+  return guarded([&] {
+    BINDER_REQUIRE(b && finished, "null argument");
+    batch_run(b, th, min_finished, finished);
+  });
+}
+
+int binder_batch_em(binder_batch *b, const int *max_iterations, double th, int update,
+                    int *iterations_out) {
+  return guarded([&] {
+    BINDER_REQUIRE(b && max_iterations && iterations_out, "null argument");
+    const int P = (int)b->slots.size();
+    b->update = update ? 1 : 0;
+    for (int p = 0; p < P; ++p) {
+      BINDER_REQUIRE(b->slots[p].r, "binder_batch_em needs every slot filled");
+      batch_fill(b, p, b->slots[p].r, max_iterations[p], b->update);
+    }
+    std::vector<int> fin(P);
+    batch_run(b, th, P, fin.data());
+    for (int p = 0; p < P; ++p) iterations_out[p] = b->slots[p].k;
   });
 }
 

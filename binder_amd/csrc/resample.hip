@@ -348,3 +348,113 @@ extern "C" int binder_resize_index_grid(int nx, int ny, int nz, int comps, int n
     HIP_OK(hipStreamSynchronize(s));
   });
 }
+
+// ---------------------------------------------------------------------------
+// Host-side intensity binning (BINDER/utils.py:20-82, bin_signal_robust):
+// the 1000-bin histogram and the final floor binning over all host cores.
+// The arithmetic is numpy's, operation for operation, so the counts and bins
+// equal numpy.histogram's and the reference's numpy binning exactly.
+// ---------------------------------------------------------------------------
+#include <thread>
+#include <vector>
+
+namespace {
+
+template <class F>
+void host_parallel(long long n, F &&fn) {
+  unsigned nt = std::thread::hardware_concurrency();
+  if (nt == 0) nt = 1;
+  if (nt > 32) nt = 32;
+  if (n < (1ll << 16)) nt = 1;
+  std::vector<std::thread> th;
+  const long long per = (n + nt - 1) / nt;
+  for (unsigned t = 0; t < nt; ++t) {
+    const long long b = (long long)t * per, e = std::min<long long>(n, b + per);
+    if (b >= e) break;
+    th.emplace_back([&fn, t, b, e] { fn(t, b, e); });
+  }
+  for (auto &x : th) x.join();
+}
+
+}  // namespace
+
+extern "C" int binder_min_max(const double *a, long long n, double *mn, double *mx) {
+  return binder::guarded([&] {
+    BINDER_REQUIRE(a && mn && mx && n > 0, "null or empty input");
+    std::vector<double> lo(32, INFINITY), hi(32, -INFINITY);
+    std::vector<char> nan(32, 0);
+    host_parallel(n, [&](unsigned t, long long b, long long e) {
+      double l = INFINITY, h = -INFINITY;
+      char bad = 0;
+      for (long long i = b; i < e; ++i) {
+        const double v = a[i];
+        if (v != v) bad = 1;
+        l = v < l ? v : l;
+        h = v > h ? v : h;
+      }
+      lo[t] = l;
+      hi[t] = h;
+      nan[t] = bad;
+    });
+    double l = INFINITY, h = -INFINITY;
+    bool bad = false;
+    for (int t = 0; t < 32; ++t) {
+      l = lo[t] < l ? lo[t] : l;
+      h = hi[t] > h ? hi[t] : h;
+      bad = bad || nan[t];
+    }
+    *mn = bad ? NAN : l;
+    *mx = bad ? NAN : h;
+  });
+}
+
+extern "C" int binder_histogram_uniform(const double *a, long long n, int nbins,
+                                        const double *edges, long long *counts) {
+  return binder::guarded([&] {
+    BINDER_REQUIRE(a && edges && counts && nbins >= 1, "null argument");
+    const double first = edges[0], last = edges[nbins];
+    const double denom = last - first;
+    std::vector<std::vector<long long>> part(32, std::vector<long long>(nbins, 0));
+    host_parallel(n, [&](unsigned t, long long b, long long e) {
+      long long *c = part[t].data();
+      for (long long i = b; i < e; ++i) {
+        const double x = a[i];
+        if (!(x >= first && x <= last)) continue;  // outside the range, or NaN
+        const double f = ((x - first) / denom) * (double)nbins;
+        long long k = (long long)f;
+        if (k == nbins) k -= 1;
+        if (x < edges[k]) k -= 1;
+        if (x >= edges[k + 1] && k != nbins - 1) k += 1;
+        c[k] += 1;
+      }
+    });
+    for (int k = 0; k < nbins; ++k) {
+      long long s = 0;
+      for (int t = 0; t < 32; ++t) s += part[t][k];
+      counts[k] = s;
+    }
+  });
+}
+
+extern "C" int binder_bin_floor(const double *a, long long n, double lo, double hi,
+                                double width, int nbins, int *out) {
+  return binder::guarded([&] {
+    BINDER_REQUIRE(a && out, "null argument");
+    host_parallel(n, [&](unsigned, long long b, long long e) {
+      for (long long i = b; i < e; ++i) {
+        const double x = a[i];
+        int v;
+        if (x < lo) {
+          v = 0;
+        } else if (x > hi) {
+          v = nbins - 1;
+        } else {
+          const double q = std::floor((x - lo) / width);
+          // numpy's cast of a NaN or out-of-range value on x86-64
+          v = std::fabs(q) < 2147483648.0 ? (int)q : INT32_MIN;
+        }
+        out[i] = v;
+      }
+    });
+  });
+}

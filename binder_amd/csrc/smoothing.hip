@@ -114,11 +114,15 @@ __device__ __forceinline__ void normals4(u32x4 r, float n[4]) {
 
 // KS = number of 32-wide k-steps (compile time, so every fragment loop is
 // straight-line code and the B loads of a whole output tile issue together).
+// Block (bx, by) of a grid of gx x comps blocks of the non-persistent
+// contraction: rows bx * 64 .. + 63 of component by.  Its outputs and, for
+// the filter epilogue, its cost partial (slot by * gx + bx) depend only on
+// (a, bx, by, gx).
 template <int KS, bool VEC, int PRO, int EPI>
-__global__ __launch_bounds__(256) void k_contract(PassArgs a) {
+__device__ __forceinline__ void contract_block(const PassArgs &a, int bx, int by, int gx) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int comp = blockIdx.y;
-  const long long row0 = (long long)blockIdx.x * kRowsPerBlock + wave * 16;
+  const int comp = by;
+  const long long row0 = (long long)bx * kRowsPerBlock + wave * 16;
   const float *in = a.in + comp * a.cin;
   const float *xp = PRO == PRO_X ? a.xpro + comp * a.cin : nullptr;
   float *out = a.out + comp * a.cout;
@@ -287,11 +291,42 @@ __global__ __launch_bounds__(256) void k_contract(PassArgs a) {
       __syncthreads();
     }
     if (t == 0) {
-      const long long b = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+      const long long b = (long long)by * gx + bx;
       a.cost_part[b] = red[0][0];
       if (a.raw) a.raw_part[b] = red[1][0];
     }
   }
+}
+
+template <int KS, bool VEC, int PRO, int EPI>
+__global__ __launch_bounds__(256) void k_contract(PassArgs a) {
+  contract_block<KS, VEC, PRO, EPI>(a, blockIdx.x, blockIdx.y, gridDim.x);
+}
+
+// Pair of block `b` in a batched launch: prefix[p] <= b < prefix[p + 1]
+// (prefix[0] = 0, prefix[npairs] = total blocks).
+__device__ __forceinline__ int batch_pair(const int *prefix, int npairs, int b) {
+  int lo = 0, hi = npairs;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// The non-persistent contraction of several problems in one launch: pair p
+// owns blocks prefix[p] .. prefix[p + 1] - 1, which it runs as its own grid
+// of ceil(R / 64) x comps blocks.
+template <int KS, bool VEC, int PRO, int EPI>
+__global__ __launch_bounds__(256) void k_contract_batch(const PassArgs *args, const int *prefix,
+                                                        int npairs) {
+  const int b = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int p = batch_pair(prefix, npairs, b);
+  const PassArgs a = args[p];
+  const int lb = b - prefix[p];
+  const int gx = (int)((a.R + kRowsPerBlock - 1) / kRowsPerBlock);
+  contract_block<KS, VEC, PRO, EPI>(a, lb % gx, lb / gx, gx);
 }
 
 // Persistent variant for contraction lengths up to 192: one 1024-thread block
@@ -603,8 +638,11 @@ constexpr int kFusedThreads = 512;  // 2 wavefronts per SIMD: 256 VGPRs each
 
 // EXACT: H and R are multiples of 16, so every row and column a lane owns
 // is in range.
+// Block `bid` of `nblocks` of the fused x pass: tiles bid * 8 + wave,
+// strided by nblocks * 8, and cost partial slot bid.
 template <int NTH, bool EXACT>
-__global__ __launch_bounds__(kFusedThreads) void k_dct_x_fused(FusedArgs a, int comps) {
+__device__ __forceinline__ void fused_block(const FusedArgs &a, int comps, int bid,
+                                            int nblocks) {
   constexpr int KSH = (NTH + 1) / 2;
   constexpr int NF = NTH * KSH * 64;
   extern __shared__ __align__(16) uint4 msh[];
@@ -623,8 +661,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_dct_x_fused(FusedArgs a, int 
   const long long tiles_per_comp = (a.R + 15) >> 4;
   const long long total = tiles_per_comp * comps;
   double cost = 0.0, raw = 0.0;
-  for (long long tile = (long long)blockIdx.x * waves_per_block + wave; tile < total;
-       tile += (long long)gridDim.x * waves_per_block) {
+  for (long long tile = (long long)bid * waves_per_block + wave; tile < total;
+       tile += (long long)nblocks * waves_per_block) {
     const int comp = (int)(tile / tiles_per_comp);
     const uint32_t c = (uint32_t)((tile % tiles_per_comp) * 16) + (uint32_t)(lane & 15);
     const bool colok = EXACT || c < (uint32_t)a.R;
@@ -717,17 +755,33 @@ __global__ __launch_bounds__(kFusedThreads) void k_dct_x_fused(FusedArgs a, int 
     __syncthreads();
   }
   if (t == 0) {
-    a.cost_part[blockIdx.x] = red[0];
-    if (a.raw) a.raw_part[blockIdx.x] = red[kFusedThreads];
+    a.cost_part[bid] = red[0];
+    if (a.raw) a.raw_part[bid] = red[kFusedThreads];
   }
+}
+
+template <int NTH, bool EXACT>
+__global__ __launch_bounds__(kFusedThreads) void k_dct_x_fused(FusedArgs a, int comps) {
+  fused_block<NTH, EXACT>(a, comps, blockIdx.x, gridDim.x);
+}
+
+// The fused x pass of several problems in one launch: pair p's blocks
+// prefix[p] .. prefix[p + 1] - 1 are its own launch's blocks 0 .. n - 1.
+template <int NTH, bool EXACT>
+__global__ __launch_bounds__(kFusedThreads) void k_dct_x_fused_batch(const FusedArgs *args,
+                                                                    const int *prefix,
+                                                                    int npairs, int comps) {
+  const int b = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int p = batch_pair(prefix, npairs, b);
+  const FusedArgs a = args[p];
+  fused_block<NTH, EXACT>(a, comps, b - prefix[p], prefix[p + 1] - prefix[p]);
 }
 
 int g_num_cus = 0;
 
 // Deterministic sum of `count` partials (fixed assignment and tree order).
-__global__ __launch_bounds__(256) void k_sum_partials(const double *part,
-                                                      long long count,
-                                                      double *out) {
+__device__ __forceinline__ void sum_partials_block(const double *part, long long count,
+                                                   double *out) {
   __shared__ double red[256];
   double s = 0.0;
   for (long long i = threadIdx.x; i < count; i += 256) s += part[i];
@@ -738,6 +792,22 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *part,
     __syncthreads();
   }
   if (threadIdx.x == 0) *out = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const double *part, long long count,
+                                                      double *out) {
+  sum_partials_block(part, count, out);
+}
+
+struct SumArgs {
+  const double *part;
+  long long count;
+  double *out;
+};
+
+__global__ __launch_bounds__(256) void k_sum_partials_batch(const SumArgs *args) {
+  const SumArgs a = args[blockIdx.x];
+  sum_partials_block(a.part, a.count, a.out);
 }
 
 // nu ~ Gamma(alpha0 + N nv / 2, scale 1 / (beta0 + nv raw / 2)), then
@@ -925,13 +995,25 @@ const PassMatrix &MatrixCache::get(int kind, int n_in, int n_out,
   return ref;
 }
 
-long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in,
-                        float *out, int comps, long long cin, long long cout,
-                        long long R, const float *xpro, const float *xepi, int epi,
-                        const FilterArgs *f, KernelTimer *timer, const char *name,
-                        long long rin, const GridCoords *xg) {
-  BINDER_REQUIRE(m.n_in <= 32 * kMaxKSteps, "contraction length above 256");
+namespace {
+
+// A contraction pass ready to launch: its arguments and kernel variant.
+struct PassPlan {
   PassArgs a{};
+  int pro = PRO_NONE, epi = EPI_STORE, comps = 0;
+  bool vec = false, persist = false;
+  dim3 grid;
+  int blocks = 0;
+  size_t lds = 0;
+};
+
+PassPlan plan_contract(const PassMatrix &m, const float *in, float *out, int comps,
+                       long long cin, long long cout, long long R, const float *xpro,
+                       const float *xepi, int epi, const FilterArgs *f, long long rin,
+                       const GridCoords *xg) {
+  BINDER_REQUIRE(m.n_in <= 32 * kMaxKSteps, "contraction length above 256");
+  PassPlan P;
+  PassArgs &a = P.a;
   a.in = in;
   a.out = out;
   a.xpro = xpro;
@@ -960,10 +1042,10 @@ long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in
     a.cost_part = f->cost_part;
     a.raw_part = f->raw_part;
   }
-  const bool vec = (m.n_in % 8 == 0) && (R % 4 == 0) && (cin % 4 == 0) &&
-                   (cout % 4 == 0) && (rin % 4 == 0);
-  dim3 grid((unsigned)((R + kRowsPerBlock - 1) / kRowsPerBlock), comps);
-  TimedLaunch tl(timer, name, stream);
+  P.vec = (m.n_in % 8 == 0) && (R % 4 == 0) && (cin % 4 == 0) && (cout % 4 == 0) &&
+          (rin % 4 == 0);
+  P.grid = dim3((unsigned)((R + kRowsPerBlock - 1) / kRowsPerBlock), comps);
+  P.comps = comps;
   int pro = xpro != nullptr ? PRO_X : PRO_NONE;
   if (xg) {
     a.xg_axis = xg->axis;
@@ -976,29 +1058,42 @@ long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in
   }
   BINDER_REQUIRE(pro == PRO_NONE || epi == EPI_STORE || epi == EPI_FILTER,
                  "a pass either subtracts x in the prologue or adds it in the epilogue");
+  P.pro = pro;
+  P.epi = epi;
   const size_t mat_lds = (size_t)((m.n_out + 15) / 16) * a.ks * 64 * 16 * 2;
   // the filter epilogue reuses the matrix area for a 2 x 1024 double reduction
-  const size_t lds = mat_lds > 16384 ? mat_lds : 16384;
+  P.lds = mat_lds > 16384 ? mat_lds : 16384;
   const long long tiles = ((R + 15) / 16) * comps;
-  const bool persist = a.ks <= 6 && mat_lds <= kPersistMaxLds && tiles >= 2048;
+  P.persist = a.ks <= 6 && mat_lds <= kPersistMaxLds && tiles >= 2048;
+  P.blocks = P.persist ? persistent_blocks() : (int)(P.grid.x * P.grid.y);
+  return P;
+}
+
+long long launch_contract(const PassPlan &P, hipStream_t stream, KernelTimer *timer,
+                          const char *name) {
+  const PassArgs &a = P.a;
   const int blocks = persistent_blocks();
-#define BINDER_GO(V, P, E)                                       \
-  do {                                                           \
-    if (persist) launch_lds<V, P, E>(blocks, lds, comps, stream, a); \
-    else launch<V, P, E>(grid, stream, a);                       \
+  const size_t lds = P.lds;
+  const int comps = P.comps;
+  const dim3 grid = P.grid;
+  TimedLaunch tl(timer, name, stream);
+#define BINDER_GO(V, PR, E)                                            \
+  do {                                                                 \
+    if (P.persist) launch_lds<V, PR, E>(blocks, lds, comps, stream, a); \
+    else launch<V, PR, E>(grid, stream, a);                            \
   } while (0)
 #define BINDER_DISPATCH(V)                                                     \
-  if (pro == PRO_X) {                                                          \
-    if (epi == EPI_FILTER) BINDER_GO(V, PRO_X, EPI_FILTER);                    \
+  if (P.pro == PRO_X) {                                                        \
+    if (P.epi == EPI_FILTER) BINDER_GO(V, PRO_X, EPI_FILTER);                  \
     else BINDER_GO(V, PRO_X, EPI_STORE);                                       \
-  } else if (pro == PRO_GRID) {                                                \
-    if (epi == EPI_FILTER) BINDER_GO(V, PRO_GRID, EPI_FILTER);                 \
+  } else if (P.pro == PRO_GRID) {                                              \
+    if (P.epi == EPI_FILTER) BINDER_GO(V, PRO_GRID, EPI_FILTER);               \
     else BINDER_GO(V, PRO_GRID, EPI_STORE);                                    \
-  } else if (epi == EPI_FILTER) BINDER_GO(V, PRO_NONE, EPI_FILTER);            \
-  else if (epi == EPI_ADDX) BINDER_GO(V, PRO_NONE, EPI_ADDX);                  \
-  else if (epi == EPI_ADDGRID) BINDER_GO(V, PRO_NONE, EPI_ADDGRID);            \
+  } else if (P.epi == EPI_FILTER) BINDER_GO(V, PRO_NONE, EPI_FILTER);          \
+  else if (P.epi == EPI_ADDX) BINDER_GO(V, PRO_NONE, EPI_ADDX);                \
+  else if (P.epi == EPI_ADDGRID) BINDER_GO(V, PRO_NONE, EPI_ADDGRID);          \
   else BINDER_GO(V, PRO_NONE, EPI_STORE);
-  if (vec) {
+  if (P.vec) {
     BINDER_DISPATCH(true)
   } else {
     BINDER_DISPATCH(false)
@@ -1006,7 +1101,19 @@ long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in
 #undef BINDER_DISPATCH
 #undef BINDER_GO
   HIP_LAUNCH_OK();
-  return persist ? blocks : (long long)grid.x * grid.y;
+  return P.blocks;
+}
+
+}  // namespace
+
+long long contract_pass(hipStream_t stream, const PassMatrix &m, const float *in,
+                        float *out, int comps, long long cin, long long cout,
+                        long long R, const float *xpro, const float *xepi, int epi,
+                        const FilterArgs *f, KernelTimer *timer, const char *name,
+                        long long rin, const GridCoords *xg) {
+  return launch_contract(plan_contract(m, in, out, comps, cin, cout, R, xpro, xepi, epi, f, rin,
+                                       xg),
+                         stream, timer, name);
 }
 
 bool fused_x_supported(int n) { return n >= 2 && n % 2 == 0 && n <= 192; }
@@ -1023,12 +1130,23 @@ void launch_fused(int blocks, size_t lds, int comps, hipStream_t s, const FusedA
   hipLaunchKernelGGL(k, dim3(blocks), dim3(kFusedThreads), lds, s, a, comps);
 }
 
-long long fused_x_pass(hipStream_t stream, const PassMatrix &m, const float *in, float *out,
-                       int comps, long long cstride, long long R, const FilterArgs &f,
-                       KernelTimer *timer, const char *name) {
+namespace {
+
+// The fused x pass ready to launch.
+struct FusedPlan {
+  FusedArgs a{};
+  int nt = 0, comps = 0;
+  bool exact = false;
+  int blocks = 0;
+  size_t lds = 0;
+};
+
+FusedPlan plan_fused(const PassMatrix &m, const float *in, float *out, int comps,
+                     long long cstride, long long R, const FilterArgs &f) {
   const int n = m.n_in;
   BINDER_REQUIRE(fused_x_supported(n), "fused x pass needs an even length <= 192");
-  FusedArgs a{};
+  FusedPlan P;
+  FusedArgs &a = P.a;
   a.in = in;
   a.out = out;
   a.R = R;
@@ -1052,31 +1170,56 @@ long long fused_x_pass(hipStream_t stream, const PassMatrix &m, const float *in,
   a.raw_part = f.raw_part;
   const int H = n / 2, nt = (H + 15) / 16, ks = (nt + 1) / 2;
   const size_t mat = (size_t)8 * nt * ks * 64 * 16;
-  const size_t lds = mat > 16384 ? mat : 16384;  // the cost reduction reuses it
+  P.lds = mat > 16384 ? mat : 16384;  // the cost reduction reuses it
   const long long tiles = ((R + 15) / 16) * comps;
   long long blocks = (tiles + kFusedThreads / 64 - 1) / (kFusedThreads / 64);
   if (blocks > persistent_blocks()) blocks = persistent_blocks();
+  P.blocks = (int)blocks;
+  P.nt = nt < 6 ? nt : 6;
+  P.exact = H % 16 == 0 && R % 16 == 0;
+  P.comps = comps;
+  return P;
+}
+
+long long launch_fused_plan(const FusedPlan &P, hipStream_t stream, KernelTimer *timer,
+                            const char *name) {
   TimedLaunch tl(timer, name, stream);
-  const bool exact = H % 16 == 0 && R % 16 == 0;
-#define BINDER_FUSED(NT)                                                      \
-  case NT:                                                                    \
-    if (exact) launch_fused<NT, true>((int)blocks, lds, comps, stream, a);     \
-    else launch_fused<NT, false>((int)blocks, lds, comps, stream, a);          \
+#define BINDER_FUSED(NT)                                                                 \
+  case NT:                                                                               \
+    if (P.exact) launch_fused<NT, true>(P.blocks, P.lds, P.comps, stream, P.a);           \
+    else launch_fused<NT, false>(P.blocks, P.lds, P.comps, stream, P.a);                  \
     break;
-  switch (nt) {
+  switch (P.nt) {
     BINDER_FUSED(1)
     BINDER_FUSED(2)
     BINDER_FUSED(3)
     BINDER_FUSED(4)
     BINDER_FUSED(5)
     default:
-      if (exact) launch_fused<6, true>((int)blocks, lds, comps, stream, a);
-      else launch_fused<6, false>((int)blocks, lds, comps, stream, a);
+      if (P.exact) launch_fused<6, true>(P.blocks, P.lds, P.comps, stream, P.a);
+      else launch_fused<6, false>(P.blocks, P.lds, P.comps, stream, P.a);
   }
 #undef BINDER_FUSED
   HIP_LAUNCH_OK();
-  return blocks;
+  return P.blocks;
 }
+
+}  // namespace
+
+long long fused_x_pass(hipStream_t stream, const PassMatrix &m, const float *in, float *out,
+                       int comps, long long cstride, long long R, const FilterArgs &f,
+                       KernelTimer *timer, const char *name) {
+  return launch_fused_plan(plan_fused(m, in, out, comps, cstride, R, f), stream, timer, name);
+}
+
+// The five passes of the EM smoothing (3D, fused x): z forward (subtracts
+// x), y forward, fused x (filter, cost partials), the cost sum, z inverse,
+// y inverse (adds x) -- exactly the sequence run() launches.
+struct EmSmoothPlan {
+  PassPlan z_fwd, y_fwd, z_inv, y_inv;
+  FusedPlan x;
+  SumArgs sum{};
+};
 
 int persistent_blocks() {
   if (g_num_cus == 0) {
@@ -1182,6 +1325,7 @@ double Smoother::flops() const {
 
 void Smoother::run(const float *dream, const float *x, float *out,
                    const SmoothOptions &opt, KernelTimer *timer) {
+  hipStream_t st = opt.exec_stream ? opt.exec_stream : stream_;
   FilterArgs f{};
   f.D = D_.get();
   f.gamma = gamma_.get();
@@ -1202,54 +1346,72 @@ void Smoother::run(const float *dream, const float *x, float *out,
   long long nparts = 0;  // partial-sum slots written by the filter pass
   auto sum_partials = [&]() {
     if (opt.want_prior) {
-      TimedLaunch tl(timer, "sum_partials", stream_);
-      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, stream_,
+      TimedLaunch tl(timer, "sum_partials", st);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st,
                          part_cost_.get(), nparts,
                          opt.prior_out ? opt.prior_out : scal_.get());
       HIP_LAUNCH_OK();
     }
     if (opt.sample && opt.sample_gamma) {
-      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, stream_,
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st,
                          part_raw_.get(), nparts, scal_.get() + 1);
       HIP_LAUNCH_OK();
     }
   };
   using MC = MatrixCache;
+  if (fused_x_supported(nx_) && nz_ > 1 && !opt.sample) {
+    // the EM smoothing: the same plan a SmoothBatch launches
+    EmSmoothPlan P;
+    plan_em(dream, x, out, opt.want_prior ? (opt.prior_out ? opt.prior_out : scal_.get())
+                                          : nullptr, P, st);
+    launch_contract(P.z_fwd, st, timer, "dct_z_fwd");
+    launch_contract(P.y_fwd, st, timer, "dct_y_fwd");
+    launch_fused_plan(P.x, st, timer, "dct_x_fused");
+    if (P.sum.out) {
+      TimedLaunch tl(timer, "sum_partials", st);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, P.sum.part, P.sum.count,
+                         P.sum.out);
+      HIP_LAUNCH_OK();
+    }
+    launch_contract(P.z_inv, st, timer, "dct_z_inv");
+    launch_contract(P.y_inv, st, timer, "dct_y_inv");
+    return;
+  }
   if (fused_x_supported(nx_)) {
     // Five passes (3D) or three (2D).  The y and z passes contract their
     // contiguous axis and move it behind x ([X][Y][Z] -> [X][KZ][Y] ->
     // [X][KY][KZ] and back), so x stays outermost for the fused x pass,
     // which transforms, filters and transforms back in one read and write.
-    const PassMatrix &mx = mats_.get(MC::kDctFusedX, nx_, nx_, stream_);
-    const PassMatrix &fy = mats_.get(MC::kDctForward, ny_, ny_, stream_);
-    const PassMatrix &iy = mats_.get(MC::kDctInverse, ny_, ny_, stream_);
+    const PassMatrix &mx = mats_.get(MC::kDctFusedX, nx_, nx_, st);
+    const PassMatrix &fy = mats_.get(MC::kDctForward, ny_, ny_, st);
+    const PassMatrix &iy = mats_.get(MC::kDctInverse, ny_, ny_, st);
     const bool gen = index_grid_ && x != nullptr;
     if (nz_ > 1) {
-      const PassMatrix &fz = mats_.get(MC::kDctForward, nz_, nz_, stream_);
-      const PassMatrix &iz = mats_.get(MC::kDctInverse, nz_, nz_, stream_);
+      const PassMatrix &fz = mats_.get(MC::kDctForward, nz_, nz_, st);
+      const PassMatrix &iz = mats_.get(MC::kDctInverse, nz_, nz_, st);
       // z forward: row = x * Ny + y, k = z; y inverse: row = x * Nz + z, k' = y
       const GridCoords gz{2, {ny_, 1, 1}, {0, ny_, 0}};
       const GridCoords gy{1, {nz_, 1, 1}, {0, 0, nz_}};
-      contract_pass(stream_, fz, dream, bufs[0], N_, nv_, nv_, nv_ / nz_, x, nullptr,
+      contract_pass(st, fz, dream, bufs[0], N_, nv_, nv_, nv_ / nz_, x, nullptr,
                     EPI_STORE, nullptr, timer, "dct_z_fwd", ny_, gen ? &gz : nullptr);
-      contract_pass(stream_, fy, bufs[0], bufs[1], N_, nv_, nv_, nv_ / ny_, nullptr, nullptr,
+      contract_pass(st, fy, bufs[0], bufs[1], N_, nv_, nv_, nv_ / ny_, nullptr, nullptr,
                     EPI_STORE, nullptr, timer, "dct_y_fwd", nz_);
-      nparts = fused_x_pass(stream_, mx, bufs[1], bufs[0], N_, nv_, (long long)ny_ * nz_, f,
+      nparts = fused_x_pass(st, mx, bufs[1], bufs[0], N_, nv_, (long long)ny_ * nz_, f,
                             timer, "dct_x_fused");
       sum_partials();
-      contract_pass(stream_, iz, bufs[0], bufs[1], N_, nv_, nv_, nv_ / nz_, nullptr, nullptr,
+      contract_pass(st, iz, bufs[0], bufs[1], N_, nv_, nv_, nv_ / nz_, nullptr, nullptr,
                     EPI_STORE, nullptr, timer, "dct_z_inv", ny_);
-      contract_pass(stream_, iy, bufs[1], out, N_, nv_, nv_, nv_ / ny_, nullptr, x, EPI_ADDX,
+      contract_pass(st, iy, bufs[1], out, N_, nv_, nv_, nv_ / ny_, nullptr, x, EPI_ADDX,
                     nullptr, timer, "dct_y_inv", nz_, gen ? &gy : nullptr);
     } else {
       // rows are x, k (k') is y
       const GridCoords g2{1, {1, 1, 1}, {0, 0, 0}};
-      contract_pass(stream_, fy, dream, bufs[0], N_, nv_, nv_, nx_, x, nullptr, EPI_STORE,
+      contract_pass(st, fy, dream, bufs[0], N_, nv_, nv_, nx_, x, nullptr, EPI_STORE,
                     nullptr, timer, "dct_y_fwd", 1, gen ? &g2 : nullptr);
-      nparts = fused_x_pass(stream_, mx, bufs[0], bufs[1], N_, nv_, ny_, f, timer,
+      nparts = fused_x_pass(st, mx, bufs[0], bufs[1], N_, nv_, ny_, f, timer,
                             "dct_x_fused");
       sum_partials();
-      contract_pass(stream_, iy, bufs[1], out, N_, nv_, nv_, nx_, nullptr, x, EPI_ADDX,
+      contract_pass(st, iy, bufs[1], out, N_, nv_, nv_, nx_, nullptr, x, EPI_ADDX,
                     nullptr, timer, "dct_y_inv", 1, gen ? &g2 : nullptr);
     }
   } else {
@@ -1273,10 +1435,10 @@ void Smoother::run(const float *dream, const float *x, float *out,
     const float *src = dream;
     for (int i = 0; i < na; ++i) {
       const int n = dims[i];
-      const PassMatrix &m = mats_.get(MC::kDctForward, n, n, stream_);
+      const PassMatrix &m = mats_.get(MC::kDctForward, n, n, st);
       const bool last = (i == na - 1);
       const long long written =
-          contract_pass(stream_, m, src, bufs[bi], N_, nv_, nv_, nv_ / n,
+          contract_pass(st, m, src, bufs[bi], N_, nv_, nv_, nv_ / n,
                         i == 0 ? x : nullptr, nullptr, last ? EPI_FILTER : EPI_STORE,
                         last ? &f : nullptr, timer, fwd_names[i]);
       if (last) nparts = written;
@@ -1286,10 +1448,10 @@ void Smoother::run(const float *dream, const float *x, float *out,
     sum_partials();
     for (int i = 0; i < na; ++i) {
       const int n = dims[i];
-      const PassMatrix &m = mats_.get(MC::kDctInverse, n, n, stream_);
+      const PassMatrix &m = mats_.get(MC::kDctInverse, n, n, st);
       const bool last = (i == na - 1);
       float *dst = last ? out : bufs[bi];
-      contract_pass(stream_, m, src, dst, N_, nv_, nv_, nv_ / n, nullptr,
+      contract_pass(st, m, src, dst, N_, nv_, nv_, nv_ / n, nullptr,
                     last ? x : nullptr, last ? EPI_ADDX : EPI_STORE, nullptr, timer,
                     inv_names[i]);
       src = dst;
@@ -1297,10 +1459,265 @@ void Smoother::run(const float *dream, const float *x, float *out,
     }
   }
   if (opt.sample && opt.sample_gamma) {
-    hipLaunchKernelGGL(k_draw_nu, dim3(1), dim3(64), 0, stream_,
+    hipLaunchKernelGGL(k_draw_nu, dim3(1), dim3(64), 0, st,
                        scal_.get() + 1, gamma_.get(), N_, (double)nv_,
                        rho_[0] * rho_[1] * rho_[2], eta_[0], eta_[1], eta_[2],
                        kNuAlpha0, kNuBeta0, opt.key0, opt.key1, opt.keyx, opt.sweep);
+    HIP_LAUNCH_OK();
+  }
+}
+
+void Smoother::plan_em(const float *dream, const float *x, float *out, double *prior_out,
+                       EmSmoothPlan &P, hipStream_t st) {
+  BINDER_REQUIRE(fused_x_supported(nx_) && nz_ > 1, "EM plan needs the 3D fused-x schedule");
+  using MC = MatrixCache;
+  FilterArgs f{};
+  f.D = D_.get();
+  f.gamma = gamma_.get();
+  f.sigma2 = sigma_ * sigma_;
+  const double rp = rho_[0] * rho_[1] * rho_[2];
+  for (int i = 0; i < 3; ++i) f.etarho[i] = eta_[i] * eta_[i] * rp;
+  f.cost_part = part_cost_.get();
+  f.raw_part = part_raw_.get();
+  const PassMatrix &mx = mats_.get(MC::kDctFusedX, nx_, nx_, st);
+  const PassMatrix &fy = mats_.get(MC::kDctForward, ny_, ny_, st);
+  const PassMatrix &iy = mats_.get(MC::kDctInverse, ny_, ny_, st);
+  const PassMatrix &fz = mats_.get(MC::kDctForward, nz_, nz_, st);
+  const PassMatrix &iz = mats_.get(MC::kDctInverse, nz_, nz_, st);
+  const bool gen = index_grid_ && x != nullptr;
+  // z forward: row = x * Ny + y, k = z; y inverse: row = x * Nz + z, k' = y
+  const GridCoords gz{2, {ny_, 1, 1}, {0, ny_, 0}};
+  const GridCoords gy{1, {nz_, 1, 1}, {0, 0, nz_}};
+  float *b0 = bufA_.get(), *b1 = bufB_.get();
+  P.z_fwd = plan_contract(fz, dream, b0, N_, nv_, nv_, nv_ / nz_, x, nullptr, EPI_STORE, nullptr,
+                          ny_, gen ? &gz : nullptr);
+  P.y_fwd = plan_contract(fy, b0, b1, N_, nv_, nv_, nv_ / ny_, nullptr, nullptr, EPI_STORE,
+                          nullptr, nz_, nullptr);
+  P.x = plan_fused(mx, b1, b0, N_, nv_, (long long)ny_ * nz_, f);
+  P.sum = SumArgs{part_cost_.get(), (long long)P.x.blocks, prior_out};
+  P.z_inv = plan_contract(iz, b0, b1, N_, nv_, nv_, nv_ / nz_, nullptr, nullptr, EPI_STORE,
+                          nullptr, ny_, nullptr);
+  P.y_inv = plan_contract(iy, b1, out, N_, nv_, nv_, nv_ / ny_, nullptr, x, EPI_ADDX, nullptr,
+                          nz_, gen ? &gy : nullptr);
+}
+
+// Whether a contraction of length n over R rows (3 components) runs the
+// non-persistent kernel (the rule of plan_contract).
+static bool small_pass(int n, long long R, int comps) {
+  const int ks = (n + 31) / 32;
+  const size_t mat_lds = (size_t)((n + 15) / 16) * ks * 64 * 16 * 2;
+  const long long tiles = ((R + 15) / 16) * comps;
+  return !(ks <= 6 && mat_lds <= kPersistMaxLds && tiles >= 2048);
+}
+
+bool Smoother::batchable_em() const {
+  return fused_x_supported(nx_) && nz_ > 1 && nz_ <= 192 && ny_ <= 192 &&
+         small_pass(nz_, nv_ / nz_, N_) && small_pass(ny_, nv_ / ny_, N_);
+}
+
+// ---------------------------------------------------------------------------
+// SmoothBatch: the EM smoothing of several smoothers, pass by pass, one
+// launch per pass and kernel variant.  Each smoother's pass runs as the
+// same grid of blocks it runs alone (a block table maps the batched grid to
+// (smoother, block)), so its outputs and cost partials are those of its own
+// run().
+// ---------------------------------------------------------------------------
+struct SmoothBatch::Launch {
+  int kind = 0;  // 0 contraction, 1 fused x, 2 cost sums
+  int ks = 0, pro = 0, epi = 0, nt = 0, comps = 0;
+  bool vec = false, exact = false;
+  int n = 0, blocks = 0;
+  size_t lds = 0, args_off = 0, prefix_off = 0;
+};
+
+SmoothBatch::SmoothBatch() = default;
+SmoothBatch::~SmoothBatch() {
+  if (pin_) (void)hipHostFree(pin_);
+}
+
+void SmoothBatch::build(const std::vector<Job> &jobs, hipStream_t stream) {
+  launches_.clear();
+  std::vector<EmSmoothPlan> plans(jobs.size());
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    BINDER_REQUIRE(jobs[j].sm->batchable_em(), "smoother cannot join a batched EM smoothing");
+    jobs[j].sm->plan_em(jobs[j].dream, jobs[j].x, jobs[j].out, jobs[j].prior_out, plans[j],
+                        stream);
+  }
+  std::vector<unsigned char> hb;
+  auto append = [&](const void *p, size_t n) {
+    const size_t off = (hb.size() + 255) & ~(size_t)255;
+    hb.resize(off + n);
+    std::memcpy(hb.data() + off, p, n);
+    return off;
+  };
+  auto contract_group = [&](PassPlan EmSmoothPlan::*member) {
+    std::vector<int> done(jobs.size(), 0);
+    for (size_t j = 0; j < jobs.size(); ++j) {
+      if (done[j]) continue;
+      const PassPlan &r = plans[j].*member;
+      std::vector<PassArgs> args;
+      std::vector<int> prefix{0};
+      for (size_t i = j; i < jobs.size(); ++i) {
+        const PassPlan &q = plans[i].*member;
+        if (done[i] || q.a.ks != r.a.ks || q.vec != r.vec || q.pro != r.pro || q.epi != r.epi)
+          continue;
+        BINDER_REQUIRE(!q.persist, "batched contraction of a persistent pass");
+        done[i] = 1;
+        args.push_back(q.a);
+        prefix.push_back(prefix.back() + q.blocks);
+      }
+      Launch L;
+      L.kind = 0;
+      L.ks = r.a.ks;
+      L.vec = r.vec;
+      L.pro = r.pro;
+      L.epi = r.epi;
+      L.n = (int)args.size();
+      L.blocks = prefix.back();
+      L.args_off = append(args.data(), sizeof(PassArgs) * args.size());
+      L.prefix_off = append(prefix.data(), sizeof(int) * prefix.size());
+      launches_.push_back(L);
+    }
+  };
+  contract_group(&EmSmoothPlan::z_fwd);
+  contract_group(&EmSmoothPlan::y_fwd);
+  {
+    std::vector<int> done(jobs.size(), 0);
+    for (size_t j = 0; j < jobs.size(); ++j) {
+      if (done[j]) continue;
+      const FusedPlan &r = plans[j].x;
+      std::vector<FusedArgs> args;
+      std::vector<int> prefix{0};
+      size_t lds = 0;
+      for (size_t i = j; i < jobs.size(); ++i) {
+        const FusedPlan &q = plans[i].x;
+        if (done[i] || q.nt != r.nt || q.exact != r.exact || q.comps != r.comps) continue;
+        done[i] = 1;
+        args.push_back(q.a);
+        prefix.push_back(prefix.back() + q.blocks);
+        lds = std::max(lds, q.lds);
+      }
+      Launch L;
+      L.kind = 1;
+      L.nt = r.nt;
+      L.exact = r.exact;
+      L.comps = r.comps;
+      L.lds = lds;
+      L.n = (int)args.size();
+      L.blocks = prefix.back();
+      L.args_off = append(args.data(), sizeof(FusedArgs) * args.size());
+      L.prefix_off = append(prefix.data(), sizeof(int) * prefix.size());
+      launches_.push_back(L);
+    }
+  }
+  {
+    std::vector<SumArgs> args;
+    for (auto &p : plans)
+      if (p.sum.out) args.push_back(p.sum);
+    if (!args.empty()) {
+      Launch L;
+      L.kind = 2;
+      L.n = (int)args.size();
+      L.blocks = L.n;
+      L.args_off = append(args.data(), sizeof(SumArgs) * args.size());
+      launches_.push_back(L);
+    }
+  }
+  contract_group(&EmSmoothPlan::z_inv);
+  contract_group(&EmSmoothPlan::y_inv);
+  if (pin_cap_ < hb.size()) {
+    if (pin_) HIP_OK(hipHostFree(pin_));
+    pin_ = nullptr;
+    HIP_OK(hipHostMalloc((void **)&pin_, hb.size(), 0));
+    pin_cap_ = hb.size();
+  }
+  std::memcpy(pin_, hb.data(), hb.size());
+  if (dev_.n < hb.size()) dev_.alloc(hb.size());
+  HIP_OK(hipMemcpyAsync(dev_.get(), pin_, hb.size(), hipMemcpyHostToDevice, stream));
+}
+
+namespace {
+
+template <int KS, bool VEC, int PRO, int EPI>
+void launch_cb(int blocks, hipStream_t s, const PassArgs *a, const int *prefix, int n) {
+  hipLaunchKernelGGL((k_contract_batch<KS, VEC, PRO, EPI>), dim3(blocks), dim3(256), 0, s, a,
+                     prefix, n);
+}
+
+template <bool VEC, int PRO, int EPI>
+void launch_cb_ks(int ks, int blocks, hipStream_t s, const PassArgs *a, const int *prefix,
+                  int n) {
+  switch (ks) {
+    case 1: launch_cb<1, VEC, PRO, EPI>(blocks, s, a, prefix, n); break;
+    case 2: launch_cb<2, VEC, PRO, EPI>(blocks, s, a, prefix, n); break;
+    case 3: launch_cb<3, VEC, PRO, EPI>(blocks, s, a, prefix, n); break;
+    case 4: launch_cb<4, VEC, PRO, EPI>(blocks, s, a, prefix, n); break;
+    case 5: launch_cb<5, VEC, PRO, EPI>(blocks, s, a, prefix, n); break;
+    default: launch_cb<6, VEC, PRO, EPI>(blocks, s, a, prefix, n); break;
+  }
+}
+
+template <int NT, bool EXACT>
+void launch_fb(const SmoothBatchLaunchView &v, hipStream_t s) {
+  auto k = k_dct_x_fused_batch<NT, EXACT>;
+  static std::atomic<bool> attr_set{false};
+  if (!attr_set) {
+    HIP_OK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kPersistMaxLds));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k, dim3(v.blocks), dim3(kFusedThreads), v.lds, s,
+                     reinterpret_cast<const FusedArgs *>(v.args),
+                     reinterpret_cast<const int *>(v.prefix), v.n, v.comps);
+}
+
+}  // namespace
+
+void SmoothBatch::launch(hipStream_t s) const {
+  const unsigned char *dev = dev_.get();
+  for (const Launch &L : launches_) {
+    if (L.blocks <= 0) continue;
+    if (L.kind == 0) {
+      const auto *a = reinterpret_cast<const PassArgs *>(dev + L.args_off);
+      const auto *pf = reinterpret_cast<const int *>(dev + L.prefix_off);
+#define BINDER_CB(V, PR, E) launch_cb_ks<V, PR, E>(L.ks, L.blocks, s, a, pf, L.n)
+#define BINDER_CBV(V)                                                           \
+  if (L.pro == PRO_X) BINDER_CB(V, PRO_X, EPI_STORE);                           \
+  else if (L.pro == PRO_GRID) BINDER_CB(V, PRO_GRID, EPI_STORE);                \
+  else if (L.epi == EPI_ADDX) BINDER_CB(V, PRO_NONE, EPI_ADDX);                 \
+  else if (L.epi == EPI_ADDGRID) BINDER_CB(V, PRO_NONE, EPI_ADDGRID);           \
+  else BINDER_CB(V, PRO_NONE, EPI_STORE);
+      BINDER_REQUIRE(L.epi != EPI_FILTER, "batched contraction without filter epilogue");
+      if (L.vec) {
+        BINDER_CBV(true)
+      } else {
+        BINDER_CBV(false)
+      }
+#undef BINDER_CBV
+#undef BINDER_CB
+    } else if (L.kind == 1) {
+      const SmoothBatchLaunchView v{dev + L.args_off, dev + L.prefix_off, L.n, L.blocks, L.lds,
+                                    L.comps};
+#define BINDER_FB(NT)                                       \
+  case NT:                                                  \
+    if (L.exact) launch_fb<NT, true>(v, s);                 \
+    else launch_fb<NT, false>(v, s);                        \
+    break;
+      switch (L.nt) {
+        BINDER_FB(1)
+        BINDER_FB(2)
+        BINDER_FB(3)
+        BINDER_FB(4)
+        BINDER_FB(5)
+        default:
+          if (L.exact) launch_fb<6, true>(v, s);
+          else launch_fb<6, false>(v, s);
+      }
+#undef BINDER_FB
+    } else {
+      hipLaunchKernelGGL(k_sum_partials_batch, dim3(L.n), dim3(256), 0, s,
+                         reinterpret_cast<const SumArgs *>(dev + L.args_off));
+    }
     HIP_LAUNCH_OK();
   }
 }

@@ -92,7 +92,10 @@ struct SmoothOptions {
   uint32_t stream = 0;         // Philox stream word
   double *prior_out = nullptr; // device double: receives sum_n 0.5 g_n D p^2
   bool want_prior = true;      // false: skip the reduction of the prior cost
+  hipStream_t exec_stream = nullptr;  // stream to run on (null: the smoother's own)
 };
+
+struct EmSmoothPlan;
 
 class Smoother {
  public:
@@ -124,6 +127,15 @@ class Smoother {
   // Flops of one run (forward + inverse contractions over all components).
   double flops() const;
 
+  // The launches of run() for an EM smoothing (no sampling) of a 3D field,
+  // without launching them; prior_out (device, may be null) receives the
+  // prior cost.
+  void plan_em(const float *dream, const float *x, float *out, double *prior_out,
+               EmSmoothPlan &plan, hipStream_t stream);
+  // Whether this smoother's EM smoothing can join a SmoothBatch: 3D, fused x
+  // pass, and y / z contractions small enough for the non-persistent kernel.
+  bool batchable_em() const;
+
  private:
   int nx_, ny_, nz_, N_;
   long long nv_;
@@ -134,6 +146,42 @@ class Smoother {
   DBuf<float> D_, bufA_, bufB_;
   DBuf<double> gamma_, part_cost_, part_raw_, scal_;
   MatrixCache mats_;
+};
+
+// Device argument arrays of one batched launch (internal to SmoothBatch).
+struct SmoothBatchLaunchView {
+  const void *args, *prefix;
+  int n, blocks;
+  size_t lds;
+  int comps;
+};
+
+// The EM smoothing of several smoothers in lockstep: pass by pass, one
+// launch per pass and kernel variant over all of them.  build() plans the
+// launches for these jobs (their device arguments go up on `stream`);
+// launch() enqueues them.  Each smoother computes exactly what its own
+// run() computes.  Every smoother must satisfy batchable_em().
+class SmoothBatch {
+ public:
+  struct Job {
+    Smoother *sm;
+    const float *dream, *x;
+    float *out;
+    double *prior_out;
+  };
+  SmoothBatch();
+  ~SmoothBatch();
+  SmoothBatch(const SmoothBatch &) = delete;
+  SmoothBatch &operator=(const SmoothBatch &) = delete;
+  void build(const std::vector<Job> &jobs, hipStream_t stream);
+  void launch(hipStream_t stream) const;
+
+ private:
+  struct Launch;
+  std::vector<Launch> launches_;
+  DBuf<unsigned char> dev_;
+  unsigned char *pin_ = nullptr;
+  size_t pin_cap_ = 0;
 };
 
 }  // namespace binder

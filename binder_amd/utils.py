@@ -22,7 +22,11 @@ def bin_signal_robust(signal, number_of_bins, lower_percentile=0.01, higher_perc
         b[signal <= edges[0]] = 0
         b[signal >= edges[-1]] = number_of_bins - 1
         return b, edges
-    hist, centers = np.histogram(signal, 1000)
+    core = _host_core()
+    if core is not None:
+        hist, centers = core.histogram_uniform(signal, 1000)
+    else:
+        hist, centers = np.histogram(signal, 1000)
     cdf = np.cumsum(hist / np.sum(hist))
 
     def _edge(p):
@@ -37,10 +41,22 @@ def bin_signal_robust(signal, number_of_bins, lower_percentile=0.01, higher_perc
     hi = _edge(higher_percentile)
     width = (hi - lo) / (number_of_bins - 1)
     edges = np.linspace(lo, hi, number_of_bins + 1)
+    if core is not None:
+        return core.bin_floor(signal, lo, hi, width, number_of_bins), edges
     b = np.floor((signal - lo) / width).astype(np.int32)
     b[signal < lo] = 0
     b[signal > hi] = number_of_bins - 1
     return b, edges
+
+
+def _host_core():
+    """The library's multithreaded host binning (numpy's arithmetic), or None
+    where the library is not built (numpy then does the same computation)."""
+    try:
+        from . import _core
+        return _core
+    except ImportError:
+        return None
 
 
 def initialize_fourier_domain_smoothing(size, DCT=True, finite_difference_order=2,

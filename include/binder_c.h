@@ -171,6 +171,38 @@ int binder_registration_sampler_chains(binder_registration *reg, int nchains,
                                        int sample_transformation_parameters,
                                        int use_graph);
 
+/* Several registrations iterated in lockstep (batched EM).  No reference
+ * counterpart: the reference registers one pair per Registration object
+ * (BINDER/FastRegistration.py:443-450, BINDER/src/gpu/Registration.cu:529-530).
+ * A batch borrows npairs handles that share K and L (K <= 32) and
+ * dimensionality, each with setMILikelihood and setNonLinearTransformation
+ * done; the handles must outlive the batch.  binder_batch_em runs
+ * binder_registration_em(regs[p], max_iterations[p], convergence_th,
+ * update, 0) for every p at once: each iteration runs all active
+ * registrations' E-steps in one launch (a work list of (pair, logical block)
+ * items), their histogram reductions and theta steps in one launch each,
+ * their DCT smoothings on a pool of streams, and copies all objective terms
+ * to the host once.  Each registration's kernels compute what its own EM
+ * call computes, so its field, theta, objective trace and iteration count
+ * (iterations_out[p]) are those of the single call. */
+typedef struct binder_batch binder_batch;
+/* npairs slots; regs may be NULL (all slots empty) or hold NULL entries. */
+int binder_batch_create(binder_batch **out, int npairs, binder_registration *const *regs);
+void binder_batch_destroy(binder_batch *b);
+int binder_batch_em(binder_batch *b, const int *max_iterations, double convergence_th,
+                    int update_likelihood_parameters, int *iterations_out);
+/* Slot-wise use, so that pairs at different pyramid levels share launches:
+ * binder_batch_set puts reg (NULL: nothing) into `slot` with an EM call of
+ * max_iterations ahead of it (running slots share the update flag);
+ * binder_batch_run iterates every running slot until at least min_finished
+ * of them finish their EM call (or all have) and writes, per slot, the
+ * iteration count of a call that finished during this run, else -1.  A slot
+ * left running resumes exactly where it stands at the next run; a finished
+ * slot's handle holds the state its own EM call would leave. */
+int binder_batch_set(binder_batch *b, int slot, binder_registration *reg, int max_iterations,
+                     int update_likelihood_parameters);
+int binder_batch_run(binder_batch *b, double convergence_th, int min_finished, int *finished);
+
 /* get_final_locations_as_numpy_array() -- BINDER/src/gpu/bindings.cpp:142-157. */
 int binder_registration_get_final_locations(binder_registration *reg,
                                             float *out);
@@ -350,6 +382,18 @@ int binder_topology_correction_2d(const double *h_init, double e1, double e2,
                                   int max_outer_it, int max_it, int threads,
                                   const double *voxel_sizes, int Nx_v, int Ny_v,
                                   double *out, double *stats);
+
+/* Host helpers of bin_signal_robust (BINDER/utils.py:20-82), run over all
+ * host cores with numpy's arithmetic: binder_min_max gives the range (NaN if
+ * any value is NaN); binder_histogram_uniform counts a[0..n) into the nbins
+ * bins of edges[0..nbins] (numpy.histogram's uniform-bin rule, values
+ * outside [edges[0], edges[nbins]] dropped); binder_bin_floor writes
+ * floor((a - lo) / width), 0 below lo and nbins - 1 above hi. */
+int binder_min_max(const double *a, long long n, double *mn, double *mx);
+int binder_histogram_uniform(const double *a, long long n, int nbins, const double *edges,
+                             long long *counts);
+int binder_bin_floor(const double *a, long long n, double lo, double hi, double width,
+                     int nbins, int *out);
 
 /* Pyramid resampling of a field: skimage.transform.resize(field, shape,
  * order=1) as FastRegistration.move_resolution calls it

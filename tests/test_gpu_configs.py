@@ -326,9 +326,10 @@ def test_c3_em_vs_oracle(core, orc):
 # ---------------------------------------------------------------- c4 ------
 def test_c4_concurrent_pairs_and_chains_equal_alone(core):
     """Four pairs of uneven shapes (axes from {96, ..., 192}, warp RMS
-    U(2, 6)) registered with all four in flight give, pair by pair, the
-    iteration counts and the fields of the same pairs registered one at a
-    time, bit for bit; so do 16 chains of 1e6 site updates per pair run
+    U(2, 6)) registered with all four in flight (threads), and level by
+    level in lockstep with batched launches (Batch.EM), give, pair by pair,
+    the iteration counts and the fields of the same pairs registered one at
+    a time, bit for bit; so do 16 chains of 1e6 site updates per pair run
     for all pairs at once against each pair's chains alone."""
     from concurrent.futures import ThreadPoolExecutor
     from binder_amd import batch
@@ -337,9 +338,12 @@ def test_c4_concurrent_pairs_and_chains_equal_alone(core):
     pairs = [batch.make_pair(s) for s in specs]
     conc, _ = batch.register_pairs(pairs, workers=4)
     alone, _ = batch.register_pairs(pairs, workers=1)
-    for s, a, b in zip(specs, conc, alone):
+    lock, _ = batch.register_pairs_batched(pairs, workers=4)
+    for s, a, b, c in zip(specs, conc, alone, lock):
         assert [lv[1] for lv in a.level_log] == [lv[1] for lv in b.level_log], s
         assert np.array_equal(a.deformedVoxelPositions, b.deformedVoxelPositions), s
+        assert [lv[1] for lv in c.level_log] == [lv[1] for lv in b.level_log], s
+        assert np.array_equal(c.deformedVoxelPositions, b.deformedVoxelPositions), s
 
     def chains(fr):
         _, _, h = batch.run_chains(fr, chains=16, site_updates=1e6, keep=True)

@@ -141,3 +141,45 @@ def test_spline_binding(orc):
             assert np.isclose(prod.sum(), 1.0)
             for d in edge:
                 assert not np.any(np.isnan(np.array(sp.log_spline(d))[:order + 1]))
+
+
+def test_host_histogram_and_binning_match_numpy(core):
+    """The library's multithreaded host histogram and floor binning (used by
+    bin_signal_robust, BINDER/utils.py:20-82) give numpy's counts, edges and
+    bins exactly: smooth and noisy images, values on bin edges, a constant
+    image, a tiny range and negative values."""
+    from binder_amd import utils
+    rng = np.random.default_rng(5)
+    cases = [rng.random((40, 33, 27)),
+             np.round(rng.normal(0.3, 0.2, (50, 41, 9)), 3),
+             np.full((17, 5, 3), 0.25),
+             1e-9 * rng.random((64, 64, 2)) + 7.0,
+             rng.normal(-3.0, 2.0, (200_000,))]
+    edges_case = np.linspace(0.0, 1.0, 1001)
+    cases.append(np.concatenate([edges_case, edges_case[::7], rng.random(70_000)]))
+    for x in cases:
+        c, e = core.histogram_uniform(x, 1000)
+        cn, en = np.histogram(x, 1000)
+        assert np.array_equal(e, en) and np.array_equal(c, cn)
+        for L in (32, 7):
+            got, ge = utils.bin_signal_robust(x, L)
+            # the same computation with numpy's histogram and binning
+            hist, centers = np.histogram(np.asarray(x, np.float64), 1000)
+            cdf = np.cumsum(hist / np.sum(hist))
+
+            def edge(p):
+                idx = np.where(cdf <= p)[0]
+                add = 0.01
+                while len(idx) == 0:
+                    idx = np.where(cdf <= p + add)[0]
+                    add += 0.01
+                return centers[idx[-1]]
+            lo, hi = edge(0.01), edge(0.99)
+            width = (hi - lo) / (L - 1)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                b = np.floor((x - lo) / width).astype(np.int32)
+            b[x < lo] = 0
+            b[x > hi] = L - 1
+            assert np.array_equal(got, b), (x.shape, L)
+    with pytest.raises(ValueError):
+        core.histogram_uniform(np.array([1.0, np.nan]), 10)
