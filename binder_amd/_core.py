@@ -715,6 +715,38 @@ def bin_floor(a, lo, hi, width, number_of_bins):
     return out
 
 
+_at_set = _sig("binder_autotune_set", _c_int, _c_int)
+_at_clear = _sig("binder_autotune_clear", _c_int)
+_at_report = _sig("binder_autotune_report", _c_int, ctypes.c_char_p, _c_int, _P(_c_int))
+
+
+def autotune(mode=1):
+    """Launch-shape autotuner: 1 (or True) tunes -- the first run of a
+    tunable kernel per problem size times its candidate shapes and keeps the
+    fastest; 0 (False) turns it off; 2 takes each tunable's last non-default
+    shape untimed (to check that other shapes give the same results).
+    Results are bit for bit those of the default shapes."""
+    _check(_at_set(int(mode)))
+
+
+def autotune_clear():
+    _check(_at_clear())
+
+
+def autotune_report():
+    """[(op, key, choice, default, ms_choice, ms_default)] of the tuned
+    problems."""
+    need = _c_int(0)
+    _check(_at_report(None, 0, ctypes.byref(need)))
+    buf = ctypes.create_string_buffer(max(need.value, 1))
+    _check(_at_report(buf, len(buf), ctypes.byref(need)))
+    rows = []
+    for line in buf.value.decode().splitlines():
+        op, key, ch, df, mc, md = line.split()
+        rows.append((op, int(key), int(ch), int(df), float(mc), float(md)))
+    return rows
+
+
 def exported_symbols():
     """Names of the C entry points this module binds (for ABI checks)."""
     return [
@@ -732,7 +764,8 @@ def exported_symbols():
         "binder_registration_get_likelihood_parameters", "binder_registration_get_shape",
         "binder_registration_get_gamma", "binder_batch_create", "binder_batch_destroy",
         "binder_batch_em", "binder_batch_set", "binder_batch_run", "binder_min_max",
-        "binder_histogram_uniform", "binder_bin_floor",
+        "binder_histogram_uniform", "binder_bin_floor", "binder_autotune_set",
+        "binder_autotune_clear", "binder_autotune_report",
         "binder_registration_get_objective_trace", "binder_registration_e_step",
         "binder_registration_set_profiling", "binder_registration_get_kernel_times",
         "binder_nlt_create", "binder_nlt_destroy", "binder_nlt_smooth_deformation",

@@ -29,6 +29,7 @@ SOURCES = {
     # FP64 resampling: same operation order as the host reference restatement
     "resample.hip": ["-ffp-contract=off"],
     "warmstart.hip": [],
+    "autotune.hip": [],
 }
 
 

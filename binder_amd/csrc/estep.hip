@@ -580,7 +580,11 @@ __device__ __forceinline__ void estep_em32_block(const EStepArgs &a, int lb, int
   __syncthreads();
 
   constexpr int NZ = IS2D ? 1 : 4;
-  double l2sum = 0.0;  // sum of log2 Z_i
+  // sum of log2 Z_i in fixed point (round(log2 Z_i * 2^22), int64): exact
+  // and order-free, so the mll does not depend on the block decomposition;
+  // non-finite terms are flagged instead (bit 0: Z_i == 0, bit 1: NaN)
+  long long l2fix = 0;
+  int l2odd = 0;
   const long long per =
       ((a.nv + nb - 1) / nb + blockDim.x - 1) / blockDim.x * blockDim.x;
   const long long vbeg = (long long)lb * per;
@@ -650,7 +654,16 @@ __device__ __forceinline__ void estep_em32_block(const EStepArgs &a, int lb, int
       }
     }
 
-    l2sum += (double)__log2f(Z);
+    {
+      // lz * 2^22 is exact in FP32 and below 2^31 in magnitude for every
+      // finite log2 of a positive float: one rounding to an int32 per voxel.
+      // Branch-free (a branch here makes the compiler keep the whole x-slice
+      // state live across it: 247 VGPRs instead of 104)
+      const float lz = __log2f(Z);
+      const bool fin = fabsf(lz) <= kMllFixMax;
+      l2fix += (long long)__float2int_rn(fin ? lz * kMllFixScaleF : 0.f);
+      l2odd |= fin ? 0 : (lz != lz ? 2 : 1);
+    }
     const float invZ = 1.0f / Z;
     st_off32(a.dream, iu, (float)ix + sx * invZ);
     st_off32(a.dream, nv32 + iu, (float)iy + sy * invZ);
@@ -675,15 +688,24 @@ __device__ __forceinline__ void estep_em32_block(const EStepArgs &a, int lb, int
   unsigned long long *out = a.hpart + (long long)lb * KL;
   for (int i = threadIdx.x; i < KL; i += blockDim.x) out[i] = hist[i];
   // the private columns (all zero again) are free now: reuse them for the
-  // deterministic reduction of the log sums, so two blocks fit in one CU
-  double *red = reinterpret_cast<double *>(priv);
-  red[threadIdx.x] = l2sum;
+  // reduction of the fixed-point log sums and flags, so two blocks fit in
+  // one CU
+  long long *red = reinterpret_cast<long long *>(priv);
+  int *odd = reinterpret_cast<int *>(red + kEm32Threads);
+  red[threadIdx.x] = l2fix;
+  odd[threadIdx.x] = l2odd;
   __syncthreads();
   for (int s2 = kEm32Threads / 2; s2 > 0; s2 >>= 1) {
-    if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+    if ((int)threadIdx.x < s2) {
+      red[threadIdx.x] += red[threadIdx.x + s2];
+      odd[threadIdx.x] |= odd[threadIdx.x + s2];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) a.mllpart[lb] = -kLn2 * red[0];
+  if (threadIdx.x == 0) {
+    const long long v = odd[0] & 2 ? kMllFixNaN : (odd[0] & 1 ? kMllFixZero : red[0]);
+    reinterpret_cast<long long *>(a.mllpart)[lb] = v;
+  }
 }
 
 template <bool IS2D>
@@ -777,7 +799,37 @@ __device__ __forceinline__ void theta_block(const ThetaArgs &a) {
   __shared__ double sh[4096];   // H, then theta
   __shared__ double rowc[256];  // per-row prior cost / row sums
   double s = 0.0;
-  for (int i = tid; i < a.nmll; i += 1024) s += a.mllpart[i];
+  if (a.mll_fixed) {
+    // exact int64 sum of the fixed-point partials (integer additions in any
+    // order give the same sum), then one rounding
+    __shared__ long long lred[1024];
+    __shared__ int lodd[1024];
+    const long long *fx = reinterpret_cast<const long long *>(a.mllpart);
+    long long acc = 0;
+    int odd = 0;
+    for (int i = tid; i < a.nmll; i += 1024) {
+      const long long v = fx[i];
+      if (v == kMllFixNaN) odd |= 2;
+      else if (v == kMllFixZero) odd |= 1;
+      else acc += v;
+    }
+    lred[tid] = acc;
+    lodd[tid] = odd;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+      if (tid < w) {
+        lred[tid] += lred[tid + w];
+        lodd[tid] |= lodd[tid + w];
+      }
+      __syncthreads();
+    }
+    if (tid == 0)
+      s = lodd[0] & 2 ? (double)NAN
+                      : (lodd[0] & 1 ? (double)INFINITY
+                                     : -kLn2 * ((double)lred[0] / kMllFixScale));
+  } else {
+    for (int i = tid; i < a.nmll; i += 1024) s += a.mllpart[i];
+  }
   red[tid] = s;
   for (int kl = tid; kl < KL; kl += 1024) {
     const double h = a.Hd[kl];

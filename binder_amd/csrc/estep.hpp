@@ -11,6 +11,15 @@ namespace binder {
 constexpr double kHistScale = 2147483648.0;
 constexpr int kHistShift = 31;  // kHistScale == 2^kHistShift
 constexpr double kLn2 = 0.693147180559945309417232121458;
+// The EM E-step's per-block sums of log2 Z_i: int64 fixed point with scale
+// 2^22 (|log2 Z_i| <= 150 for every positive float, so each term fits an
+// int32 and 2^30 voxels cannot overflow the sum), plus two sentinels for a
+// block with a zero Z_i (mll = +inf) or a NaN.
+constexpr double kMllFixScale = 4194304.0;
+constexpr float kMllFixScaleF = 4194304.0f;
+constexpr float kMllFixMax = 150.0f;
+constexpr long long kMllFixNaN = (long long)(-9223372036854775807LL - 1);
+constexpr long long kMllFixZero = (long long)(-9223372036854775807LL);
 
 // Row stride (in floats) of the likelihood table theta[u][v] held in LDS by
 // the E-step kernels.  Lanes of one wavefront read theta at (fixed bin u,
@@ -86,6 +95,9 @@ struct ThetaArgs {
   double *stats;        // stats[0] = mll, stats[1] = prior_theta
   uint32_t k0, k1, sweep;
   const uint32_t *keyx;  // optional key mask, as in EStepArgs
+  // mllpart holds the EM E-step's (K <= 32) int64 fixed-point partials
+  // instead of doubles
+  int mll_fixed;
 };
 
 void launch_theta(const ThetaArgs &a, hipStream_t s);

@@ -19,6 +19,7 @@
 #include <thread>
 #include <utility>
 
+#include "autotune.hpp"
 #include "estep.hpp"
 #include "smoothing.hpp"
 #include "topology.hpp"
@@ -263,7 +264,10 @@ struct binder_registration {
   DBuf<float> em_d, em_dream, em_thT;
   DBuf<double> em_theta, em_stats;
   Event em_done[2];
-  int estep_blocks = 0;
+  // logical blocks of the EM E-step and grid of the sampling E-step (launch
+  // shapes the autotuner may change; neither changes any result), and the
+  // histogram partial rows the last E-step wrote
+  int estep_blocks = 0, sample_blocks = 0, hist_rows = 0;
   // transformation
   std::unique_ptr<Smoother> sm;
   // sampler
@@ -310,13 +314,69 @@ struct binder_registration {
                   const uint32_t *keyx = nullptr) {
     EStepArgs a = estep_args(sample, k0, k1, sweep);
     a.keyx = keyx;
+    hist_rows = sample ? sample_blocks : estep_blocks;
     TimedLaunch t(tm(), sample ? "estep_sample" : "estep", s());
-    launch_estep(a, is2d, sample, estep_blocks, s());
+    launch_estep(a, is2d, sample, hist_rows, s());
   }
 
   void hist_reduce_on(hipStream_t st) {
     TimedLaunch t(tm(), "hist_reduce", st);
-    launch_hist_reduce(hpart.get(), estep_blocks, K * L, hpart2.get(), Hd.get(), st);
+    launch_hist_reduce(hpart.get(), hist_rows, K * L, hpart2.get(), Hd.get(), st);
+  }
+
+  // Histogram partial rows (and mll partial slots) for E-steps of up to
+  // `rows` blocks.
+  void ensure_rows(int rows) {
+    const size_t KL = (size_t)K * L;
+    if (hpart.n < (size_t)rows * KL) {
+      HIP_OK(hipStreamSynchronize(s()));
+      hpart.alloc((size_t)rows * KL);
+      hpart2.alloc((size_t)((rows + 15) / 16) * KL);
+    }
+    if (mllpart.n < (size_t)rows) {
+      HIP_OK(hipStreamSynchronize(s()));
+      mllpart.alloc(rows);
+    }
+  }
+
+  // Autotuned E-step launch shapes (large grids only).  The EM E-step's
+  // partials are exact integers (fixed-point histogram and log sums) and the
+  // sampling E-step's draws depend on the voxel index only, so every
+  // candidate gives the same results.
+  void tune_estep_em() {
+    if (!has_mi || !estep_em32_supported(K) || nv < (1ll << 18)) return;
+    // half, the same and twice the default number of blocks
+    const int d = estep_blocks;
+    int c[3], nc = 0;
+    if (d / 2 >= 1) c[nc++] = d / 2;
+    c[nc++] = d;
+    if ((long long)d * 2 * 512 <= 2 * nv && d * 2 <= 8192) c[nc++] = d * 2;
+    const long long key = (nv << 22) ^ ((long long)K << 12) ^ ((long long)L << 2) ^ (is2d ? 1 : 0);
+    const int nb = Autotuner::get().choose("estep_em", key, c, nc, estep_blocks, s(), [&](int b) {
+      ensure_rows(b);
+      EStepArgs a = estep_args(false, 0, 0, 0);
+      launch_estep(a, is2d, false, b, s());
+    });
+    ensure_rows(nb);
+    estep_blocks = nb;
+  }
+
+  void tune_estep_sample() {
+    if (!has_mi || nv < (1ll << 18)) return;
+    const int d = sample_blocks;
+    int c[3], nc = 0;
+    if (d / 2 >= 1) c[nc++] = d / 2;
+    c[nc++] = d;
+    if ((long long)d * 2 * 256 <= 2 * nv && d * 2 <= 8192) c[nc++] = d * 2;
+    const long long key = (nv << 22) ^ ((long long)K << 12) ^ ((long long)L << 2) ^ (is2d ? 1 : 0);
+    const int nb = Autotuner::get().choose("estep_sample", key, c, nc, sample_blocks, s(),
+                                           [&](int b) {
+      ensure_rows(b);
+      EStepArgs a = estep_args(true, 0, 0, 0);
+      launch_estep(a, is2d, true, b, s());
+    });
+    ensure_rows(nb);
+    sample_blocks = nb;
   }
 
   // E-step + histogram reduction (+ theta step with the given mode).
@@ -346,7 +406,7 @@ struct binder_registration {
     t.thT = thT_out ? thT_out : thT.get();
     t.rowbuf = rowbuf.get();
     t.mllpart = mllpart.get();
-    t.nmll = estep_blocks;
+    t.nmll = hist_rows;
     t.K = K;
     t.L = L;
     t.alpha = alpha;
@@ -362,6 +422,9 @@ struct binder_registration {
     t.k1 = k1;
     t.keyx = keyx;
     t.sweep = sweep;
+    // the EM E-step for K <= 32 writes fixed-point mll partials; the
+    // sampling E-step writes FP64 zeros (its mll is not used)
+    t.mll_fixed = mode != kThetaDirichlet && estep_em32_supported(K);
     return t;
   }
 
@@ -586,12 +649,21 @@ std::unique_ptr<binder_registration> new_registration(const float *voxel_pos, in
   r->dream.zero(r->s());
   r->stats.alloc(16);
   r->stats.zero(r->s());
-  // 8 blocks of 4 wavefronts per CU on 256 CUs; each block keeps one
-  // partial histogram row
+  // The sampling E-step: blocks of 256 threads, up to 2048 (8 per CU on 256
+  // CUs).  The EM E-step (512-thread blocks with 64 KB of private LDS
+  // columns each): about 2048 voxels (4 per thread) per block, at most 2048
+  // and at least min(64, nv / 256) blocks, so that mid-size pyramid levels do
+  // not spend their time setting up and flushing blocks of a few hundred
+  // voxels.  Each block keeps one partial histogram row.
   long long blocks = (r->nv + 255) / 256;
   if (blocks > 2048) blocks = 2048;
-  r->estep_blocks = (int)blocks;
-  r->mllpart.alloc(blocks);
+  long long em = (r->nv + 2047) / 2048;
+  em = std::max(em, std::min(blocks, 64ll));
+  if (em > 2048) em = 2048;
+  r->estep_blocks = (int)em;
+  r->sample_blocks = (int)blocks;
+  r->hist_rows = (int)em;
+  r->mllpart.alloc(std::max(blocks, em));
   HIP_OK(hipStreamSynchronize(r->s()));
   return r;
 }
@@ -731,8 +803,11 @@ int binder_registration_set_mi_likelihood(binder_registration *r, double alpha,
     r->Hd.alloc((size_t)K * L);
     r->Hd.zero(r->s());
     r->rowbuf.alloc((size_t)K * L + K + 64);
-    r->hpart.alloc((size_t)r->estep_blocks * K * L);
-    r->hpart2.alloc((size_t)((r->estep_blocks + 15) / 16) * K * L);
+    {
+      const int rows = std::max(r->estep_blocks, r->sample_blocks);
+      r->hpart.alloc((size_t)rows * K * L);
+      r->hpart2.alloc((size_t)((rows + 15) / 16) * K * L);
+    }
     HIP_OK(hipStreamSynchronize(r->s()));
     r->has_mi = true;
   });
@@ -793,6 +868,8 @@ int binder_registration_em(binder_registration *r, int max_it, double th,
                             r->s()));
       r->dslot = 0;
     }
+    r->tune_estep_em();
+    r->hist_rows = r->estep_blocks;
     r->em_d.alloc(nf);
     r->em_dream.alloc(nf);
     r->em_stats.alloc(32);
@@ -894,6 +971,8 @@ void batch_fill(binder_batch *b, int p, binder_registration *r, int cap, int upd
                           r->s()));
     r->dslot = 0;
   }
+  r->tune_estep_em();
+  r->hist_rows = r->estep_blocks;
   r->em_d.alloc(nf);
   r->em_dream.alloc(nf);
   if (update) {
@@ -1208,6 +1287,7 @@ int binder_registration_sampler_run(binder_registration *r, int seed, int N_b,
     r->m2.alloc((size_t)npairs * r->nv);
     r->m2.zero(r->s());
     r->samples = N_s;
+    if (sp) r->tune_estep_sample();
     uint32_t k0, k1;
     sampler_key(seed, k0, k1);
     if (st) r->prepare_ring(N_b + N_s);
@@ -1232,6 +1312,7 @@ int binder_registration_sampler_chains(binder_registration *r, int nchains, cons
     const size_t nf = (size_t)N * r->nv, KL = (size_t)r->K * r->L;
     r->mean.alloc(nf);
     r->m2.alloc((size_t)npairs * r->nv);
+    if (sp) r->tune_estep_sample();
     if (st) r->prepare_ring(N_b + N_s);  // the current field is now slot 0
     // slot 0 keeps the start field unless the ring wraps onto it (more than
     // kWelfordBatch - 1 sweeps): only then does a chain restore it

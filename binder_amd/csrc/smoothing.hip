@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "autotune.hpp"
 #include "smoothing.hpp"
 
 namespace binder {
@@ -1069,14 +1070,32 @@ PassPlan plan_contract(const PassMatrix &m, const float *in, float *out, int com
   return P;
 }
 
+void launch_contract_blocks(const PassPlan &P, hipStream_t stream, int blocks);
+
 long long launch_contract(const PassPlan &P, hipStream_t stream, KernelTimer *timer,
                           const char *name) {
+  int blocks = persistent_blocks();
+  if (P.persist && P.epi != EPI_FILTER && 2 * P.lds <= 160 * 1024) {
+    // persistent blocks per CU: every tile's outputs are computed by one
+    // wavefront whatever block runs it, so one or two blocks per CU give the
+    // same results; the autotuner picks the faster
+    const int cands[2] = {blocks, 2 * blocks};
+    const long long key = ((long long)P.a.n_in << 52) ^ ((long long)P.a.n_out << 42) ^
+                          ((long long)P.a.R << 12) ^ ((long long)P.comps << 8) ^
+                          ((long long)P.pro << 5) ^ ((long long)P.epi << 2) ^ (P.vec ? 1 : 0);
+    blocks = Autotuner::get().choose("contract", key, cands, 2, blocks, stream,
+                                     [&](int b) { launch_contract_blocks(P, stream, b); });
+  }
+  TimedLaunch tl(timer, name, stream);
+  launch_contract_blocks(P, stream, blocks);
+  return P.blocks;
+}
+
+void launch_contract_blocks(const PassPlan &P, hipStream_t stream, int blocks) {
   const PassArgs &a = P.a;
-  const int blocks = persistent_blocks();
   const size_t lds = P.lds;
   const int comps = P.comps;
   const dim3 grid = P.grid;
-  TimedLaunch tl(timer, name, stream);
 #define BINDER_GO(V, PR, E)                                            \
   do {                                                                 \
     if (P.persist) launch_lds<V, PR, E>(blocks, lds, comps, stream, a); \
@@ -1101,7 +1120,6 @@ long long launch_contract(const PassPlan &P, hipStream_t stream, KernelTimer *ti
 #undef BINDER_DISPATCH
 #undef BINDER_GO
   HIP_LAUNCH_OK();
-  return P.blocks;
 }
 
 }  // namespace

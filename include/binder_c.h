@@ -383,6 +383,24 @@ int binder_topology_correction_2d(const double *h_init, double e1, double e2,
                                   const double *voxel_sizes, int Nx_v, int Ny_v,
                                   double *out, double *stats);
 
+/* Launch-shape autotuner (no reference counterpart: the reference fixes its
+ * launch shapes, BINDER/src/gpu/Registration.cu:756-761,
+ * BINDER/src/gpu/NonLinearTransformation.cu:341-345).  While enabled (also
+ * by BINDER_AUTOTUNE=1 in the environment), the first run of a tunable
+ * kernel on a problem size times its candidate launch shapes on the
+ * caller's stream and keeps the fastest for the process: the EM and sampling
+ * E-steps' block counts (grids of at least 2^18 voxels) and the persistent
+ * blocks per CU of the y/z DCT passes.  Every candidate computes the same
+ * results bit for bit (exact integer partial sums, tiles independent of the
+ * block that runs them).  binder_autotune_report writes one line per tuned
+ * problem, "op key choice default ms_choice ms_default" (needed: bytes
+ * including the terminating zero).  binder_autotune_set(mode): 0 off, 1 tune
+ * by timing, 2 take each tunable's last non-default shape untimed (for
+ * checking that non-default shapes give the default's results). */
+int binder_autotune_set(int mode);
+int binder_autotune_clear(void);
+int binder_autotune_report(char *buf, int cap, int *needed);
+
 /* Host helpers of bin_signal_robust (BINDER/utils.py:20-82), run over all
  * host cores with numpy's arithmetic: binder_min_max gives the range (NaN if
  * any value is NaN); binder_histogram_uniform counts a[0..n) into the nbins
