@@ -549,7 +549,7 @@ __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
 }
 
 // EM E-step for K <= 32 (same outputs as k_estep<IS2D, false>).  LDS:
-// block histogram (u64 fixed point) | theta columns | private per-lane bin
+// block histogram [L][K] (u64 fixed point) | theta columns | private per-lane bin
 // columns priv[wave][bin][lane] (u32 fixed-point spline mass).
 constexpr int kEm32Threads = 512;
 constexpr int kEm32MaxK = 32;
@@ -569,12 +569,14 @@ __device__ __forceinline__ void estep_em32_block(const EStepArgs &a, int lb, int
   uint32_t *priv = reinterpret_cast<uint32_t *>(th + ((a.L * Kp + 3) & ~3));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint32_t *mine = priv + wave * (kEm32MaxK * 64) + lane;  // bin v at mine[64 * v]
-  // theta is kept scaled by 2^-31 and the z weights by 2^31, so one product
-  // gives both the fixed-point spline weight and (exactly, the scalings
-  // being powers of two) the node's posterior weight
+  // the z weights are kept scaled by 2^31, so each node's weight
+  // w_j = B_j theta_j comes out scaled by 2^31 (exactly: a power of two), as
+  // the fixed-point value the private column adds, and Z, s_x, s_y, s_z with
+  // it.  (Integer private columns: LDS integer adds are several times
+  // faster than FP32 ones.)
   for (int i = threadIdx.x; i < KL; i += blockDim.x) {
     hist[i] = 0ull;
-    th[(i / a.K) * Kp + (i % a.K)] = a.thT[i] * (1.0f / kSplineScale);
+    th[(i / a.K) * Kp + (i % a.K)] = a.thT[i];
   }
   for (int v = 0; v < kEm32MaxK; ++v) mine[64 * v] = 0u;
   __syncthreads();
@@ -633,12 +635,16 @@ __device__ __forceinline__ void estep_em32_block(const EStepArgs &a, int lb, int
 #pragma unroll
         for (int zc = 0; zc < NZ; ++zc) {
           const int v = (qq >> (8 * zc)) & 0xFF;
-          const float sws = wxy * wzs[zc];  // spline weight x 2^31
-          const float wj = sws * thu[v];    // = spline weight x theta
-          w[yb * NZ + zc] = (uint32_t)sws;  // truncated: one conversion
+          const float wj = (wxy * wzs[zc]) * thu[v];  // 2^31 B_j theta_j
+          w[yb * NZ + zc] = (uint32_t)wj;             // truncated
           vv[yb * NZ + zc] = v;
           zy += wj;
-          if (!IS2D) sz += wj * (float)(zc - 1);
+          // z offset zc - 1 in {-1, 0, 1, 2}: no product with zero
+          if (!IS2D) {
+            if (zc == 0) sz -= wj;
+            else if (zc == 2) sz += wj;
+            else if (zc == 3) sz = fmaf(2.f, wj, sz);
+          }
         }
         zx += zy;
         sy += zy * (float)(yb - 1);
@@ -647,13 +653,19 @@ __device__ __forceinline__ void estep_em32_block(const EStepArgs &a, int lb, int
       sx += zx * (float)(xa - 1);
 #pragma unroll
       for (int j = 0; j < 4 * NZ; ++j) {
-        // private column: only this lane touches it; an integer LDS add with
-        // no return value, landing in program order
+        // private column: only this lane touches it, so the LDS adds (no
+        // return value) land in program order and the column sums do not
+        // depend on timing or on the block decomposition
         atomicAdd(mine + 64 * vv[j], w[j]);
         mask |= 1u << vv[j];
       }
     }
 
+    // the sums carry the 2^31 scale: Z itself for the log, while the ratios
+    // s / Z and the fixed-point masses below take the scaled values as they
+    // are (identical quotients)
+    const float Zs = Z;
+    Z = Zs * (1.0f / kSplineScale);
     {
       // lz * 2^22 is exact in FP32 and below 2^31 in magnitude for every
       // finite log2 of a positive float: one rounding to an int32 per voxel.
@@ -664,11 +676,15 @@ __device__ __forceinline__ void estep_em32_block(const EStepArgs &a, int lb, int
       l2fix += (long long)__float2int_rn(fin ? lz * kMllFixScaleF : 0.f);
       l2odd |= fin ? 0 : (lz != lz ? 2 : 1);
     }
-    const float invZ = 1.0f / Z;
+    const float invZ = 1.0f / Zs;
     st_off32(a.dream, iu, (float)ix + sx * invZ);
     st_off32(a.dream, nv32 + iu, (float)iy + sy * invZ);
     if (!IS2D) st_off32(a.dream, 2 * nv32 + iu, (float)iz + sz * invZ);
-    const bool add = Z > 0.f;
+    // fixed-point mass S_v / Z of bin v at the histogram's 2^31 scale (S_v,
+    // the private column, is sum_{j: v_j = v} w_j at that scale already);
+    // a voxel with Z = 0 adds zeros
+    const float invZs = Zs > 0.f ? invZ * kSplineScale : 0.f;
+    unsigned long long *hrow = hist + u * a.K;  // block histogram is [L][K]
     uint32_t m = __builtin_amdgcn_alignbit(mask, mask, rot);  // rotate right
     while (m) {
       const int b = __builtin_ctz(m);
@@ -676,17 +692,14 @@ __device__ __forceinline__ void estep_em32_block(const EStepArgs &a, int lb, int
       const int v = (b + rot) & 31;
       const uint32_t sv = mine[64 * v];
       mine[64 * v] = 0u;
-      if (add) {
-        const float mass = thu[v] * (float)sv * invZ;  // theta S_v / Z
-        const unsigned long long f = to_fixed(mass);
-        if (f) atomicAdd(hist + v * a.L + u, f);
-      }
+      atomicAdd(hrow + v, (unsigned long long)__float2uint_rn((float)sv * invZs));
     }
   }
 
   __syncthreads();
+  // partial row in the [K][L] order of the reduction
   unsigned long long *out = a.hpart + (long long)lb * KL;
-  for (int i = threadIdx.x; i < KL; i += blockDim.x) out[i] = hist[i];
+  for (int i = threadIdx.x; i < KL; i += blockDim.x) out[i] = hist[(i % a.L) * a.K + i / a.L];
   // the private columns (all zero again) are free now: reuse them for the
   // reduction of the fixed-point log sums and flags, so two blocks fit in
   // one CU
