@@ -385,7 +385,7 @@ def warmstart_report(args):
                                           ref_objective=cold["final_objective"])
             wo = warm["to_tolerance"].get("objective", {})
             row = {"seed": int(seed), "cold_s": cold["gpu_s"],
-                   "cold_converged_s": cold["converged_s"],
+                   "cold_converged_s": cold["converged_timed_s"],
                    "warm_own_s": warm["to_tolerance"]["own_field"].get("gpu_s"),
                    "warm_to_cold_objective_s": wo.get("gpu_s"),
                    "cold_iterations": cold["iterations"],
