@@ -41,16 +41,26 @@ def hipcc():
 
 
 def build(verbose=False):
+    from concurrent.futures import ThreadPoolExecutor
+
     os.makedirs(OUT_DIR, exist_ok=True)
     cc = hipcc()
-    objs = []
+    objs, cmds = [], []
     for src, extra in SOURCES.items():
         obj = os.path.join(OUT_DIR, src.replace(".hip", ".o"))
-        cmd = [cc, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append([cc, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # the translation units compile independently: one hipcc process each
+    jobs = max(1, min(len(cmds), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(jobs) as pool:
+        for cmd, r in zip(cmds, pool.map(
+                lambda c: subprocess.run(c, capture_output=True, text=True), cmds)):
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            if r.stderr:
+                print(r.stderr, end="", file=sys.stderr, flush=True)
+            if r.returncode != 0:
+                raise subprocess.CalledProcessError(r.returncode, cmd, r.stdout, r.stderr)
     tmp = LIB + ".tmp"
     cmd = [cc, *COMMON, "-shared", *objs, "-o", tmp]
     if verbose:

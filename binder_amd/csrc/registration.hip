@@ -193,6 +193,25 @@ void download_planes(const float *src, float *host, DBuf<float> &tmp, long long 
   }, (long long)4 << 20);
 }
 
+// Page-locked staging for uploads of bins (the fixed and moving bins of a
+// registration, one byte each), shared by every handle of the process and
+// held until the process ends; callers hold g_upload_mu until their copies
+// have completed.
+std::mutex g_upload_mu;
+uint8_t *g_upload = nullptr;
+size_t g_upload_bytes = 0;
+
+uint8_t *pinned_upload(size_t bytes) {
+  if (g_upload_bytes < bytes) {
+    if (g_upload) HIP_OK(hipHostFree(g_upload));
+    g_upload = nullptr;
+    g_upload_bytes = 0;
+    HIP_OK(hipHostMalloc((void **)&g_upload, bytes, hipHostMallocDefault));
+    g_upload_bytes = bytes;
+  }
+  return g_upload;
+}
+
 struct Stream {
   hipStream_t s = nullptr;
   Stream() { HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
@@ -252,7 +271,7 @@ struct binder_registration {
   bool has_mi = false;
   int K = 0, L = 0;
   double alpha = 0.0;
-  DBuf<uint8_t> bv, bins2d;
+  DBuf<uint8_t> bv, bins2d, nb8;  // nb8: moving bins as uploaded, before packing
   DBuf<uint32_t> quads;
   DBuf<double> theta, Hd, rowbuf, mllpart, stats;
   DBuf<float> thT;
@@ -741,20 +760,35 @@ int binder_registration_set_mi_likelihood(binder_registration *r, double alpha,
     BINDER_REQUIRE(theta && binned_nodes && binned_voxels, "null input array");
     BINDER_REQUIRE(K >= 1 && K <= 256 && L >= 1 && L <= 256 && K * L <= 4096,
                    "K and L must be in [1,256] with K*L <= 4096");
-    std::vector<uint8_t> v8(r->nv);
-    std::atomic<bool> bad{false};
-    parallel_ranges(r->nv, [&](long long b0, long long b1) {
+    // the fixed and (3D) moving bins go up as bytes, converted by host
+    // threads straight into page-locked memory and copied by one DMA each
+    const long long nn8 = r->is2d ? 0 : r->nn;
+    std::lock_guard<std::mutex> lock(g_upload_mu);
+    uint8_t *pin = pinned_upload((size_t)(r->nv + nn8));
+    std::atomic<bool> bad_v{false}, bad_n{false};
+    parallel_ranges(r->nv + nn8, [&](long long b0, long long b1) {
+      bool bv_ = false, bn_ = false;
       for (long long i = b0; i < b1; ++i) {
-        const int b = binned_voxels[i];
-        if (b < 0 || b >= L) bad = true;
-        v8[i] = (uint8_t)b;
+        if (i < r->nv) {
+          const int b = binned_voxels[i];
+          bv_ |= b < 0 || b >= L;
+          pin[i] = (uint8_t)b;
+        } else {
+          const int b = binned_nodes[i - r->nv];
+          bn_ |= b < 0 || b >= K;
+          pin[i] = (uint8_t)b;
+        }
       }
+      if (bv_) bad_v = true;
+      if (bn_) bad_n = true;
     });
-    BINDER_REQUIRE(!bad, "binned_voxels value outside [0, L)");
+    BINDER_REQUIRE(!bad_v, "binned_voxels value outside [0, L)");
+    BINDER_REQUIRE(!bad_n, "binned_nodes value outside [0, K)");
     r->K = K;
     r->L = L;
     r->alpha = alpha;
-    r->bv.upload(v8.data(), v8.size(), r->s());
+    r->bv.alloc((size_t)r->nv);
+    HIP_OK(hipMemcpyAsync(r->bv.get(), pin, (size_t)r->nv, hipMemcpyHostToDevice, r->s()));
     const int Xp = r->nxn + 2, Yp = r->nyn + 2;
     // the E-step indexes the padded moving grid with 32-bit offsets and
     // 24-bit multiplies
@@ -777,23 +811,14 @@ int binder_registration_set_mi_likelihood(binder_registration *r, double alpha,
       // z - zq + 4), zero outside the grid: the bins go up as bytes and
       // k_pack_quads lays the words out on the device
       const int Zq = r->nzn + 5;
-      std::vector<uint8_t> n8(r->nn);
-      parallel_ranges(r->nn, [&](long long b0, long long b1) {
-        for (long long i = b0; i < b1; ++i) {
-          const int b = binned_nodes[i];
-          if (b < 0 || b >= K) bad = true;
-          n8[i] = (uint8_t)b;
-        }
-      });
-      BINDER_REQUIRE(!bad, "binned_nodes value outside [0, K)");
-      DBuf<uint8_t> nb;
-      nb.upload(n8.data(), n8.size(), r->s());
+      r->nb8.alloc((size_t)r->nn);
+      HIP_OK(hipMemcpyAsync(r->nb8.get(), pin + r->nv, (size_t)r->nn, hipMemcpyHostToDevice,
+                            r->s()));
       const long long nq = (long long)Xp * Yp * Zq;
       r->quads.alloc((size_t)nq);
-      hipLaunchKernelGGL(k_pack_quads, dim3(grid_blocks(nq)), dim3(256), 0, r->s(), nb.get(),
-                         r->nxn, r->nyn, r->nzn, r->quads.get());
+      hipLaunchKernelGGL(k_pack_quads, dim3(grid_blocks(nq)), dim3(256), 0, r->s(),
+                         r->nb8.get(), r->nxn, r->nyn, r->nzn, r->quads.get());
       HIP_LAUNCH_OK();
-      HIP_OK(hipStreamSynchronize(r->s()));
     }
     std::vector<float> tt((size_t)K * L);
     for (int k = 0; k < K; ++k)

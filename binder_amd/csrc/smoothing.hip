@@ -35,6 +35,7 @@ typedef __attribute__((ext_vector_type(4))) unsigned int u32x4_t;
 
 constexpr int kRowsPerBlock = 64;  // 4 wavefronts x 16 rows
 constexpr int kMaxKSteps = 8;      // contraction length <= 256
+constexpr int kSymMaxTiles = 6;    // even/odd split: half length 32..96
 
 enum : int { EPI_STORE = 0, EPI_FILTER = 1, EPI_ADDX = 2, EPI_ADDGRID = 3 };
 // prologue: 0 none, 1 subtract xpro, 2 subtract the generated index grid
@@ -507,6 +508,244 @@ __global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
   }
 }
 
+// Persistent contraction of a square orthonormal DCT-II (INV = false) or
+// DCT-III (INV = true) of even length N = 2H on its even / odd halves.  The
+// matrix satisfies C[k][N-1-x] = (-1)^k C[k][x], so
+//   forward: out[2m]   = sum_{x<H} C[2m][x]   (in[x] + in[N-1-x]),
+//            out[2m+1] = sum_{x<H} C[2m+1][x] (in[x] - in[N-1-x]);
+//   inverse: E[x] = sum_m C[2m][x] in[2m], O[x] = sum_m C[2m+1][x] in[2m+1]
+//            (x < H), out[x] = E + O, out[N-1-x] = E - O.
+// Two H x H matrices replace one N x N: half the MFMA work and half the LDS
+// reads.  Tiles, rows and the output layout (rin) are those of
+// k_contract_lds; a lane's two output columns are 2m, 2m + 1 (forward) or
+// x, N - 1 - x (inverse).  NT = ceil(H / 16) output tiles per parity stay in
+// registers while the k-steps stream through: the next k-step's input is
+// loaded before the current one's MFMAs, so the HBM loads overlap the
+// matrix work.  The forward takes the prologues and a plain store, the
+// inverse a plain store or the x / index-grid epilogue.
+
+// Raw A data of k-step s (j < 8): forward the sums (p) and differences (q)
+// of x = x0 + j and N - 1 - x (prologue applied), inverse the coefficients
+// 2m (p) and 2m + 1 (q) of m = x0 + j.  H % 8 == 0 and N % 16 == 0, so a
+// group of 8 lies wholly inside [0, H) or wholly past it (zero), and rows
+// are 16-byte aligned.
+template <bool INV, int PRO>
+__device__ __forceinline__ void sym_load(const PassArgs &a, const float *src, const float *xr,
+                                         bool rowok, int x0, int N, int H, int comp, float rc,
+                                         float p[8], float q[8]) {
+  // loads into vectors that stay zero when the group lies past H (no early
+  // return: it would make the compiler keep p and q in scratch memory)
+  const bool ok = rowok && x0 < H;
+  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (!INV) {
+    // v[j] = in[x0 + j], w[j] = in[N - 1 - x0 - j]
+    float4 v0 = z4, v1 = z4, w0 = z4, w1 = z4;
+    if (ok) {
+      const float4 *pv = reinterpret_cast<const float4 *>(src + x0);
+      const float4 *pw = reinterpret_cast<const float4 *>(src + (N - 8 - x0));
+      v0 = pv[0];
+      v1 = pv[1];
+      w0 = pw[0];
+      w1 = pw[1];
+      if (PRO == PRO_X) {
+        const float4 *qv = reinterpret_cast<const float4 *>(xr + x0);
+        const float4 *qw = reinterpret_cast<const float4 *>(xr + (N - 8 - x0));
+        const float4 a0 = qv[0], a1 = qv[1], b0 = qw[0], b1 = qw[1];
+        v0.x -= a0.x; v0.y -= a0.y; v0.z -= a0.z; v0.w -= a0.w;
+        v1.x -= a1.x; v1.y -= a1.y; v1.z -= a1.z; v1.w -= a1.w;
+        w0.x -= b0.x; w0.y -= b0.y; w0.z -= b0.z; w0.w -= b0.w;
+        w1.x -= b1.x; w1.y -= b1.y; w1.z -= b1.z; w1.w -= b1.w;
+      }
+    }
+    float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+    float w[8] = {w1.w, w1.z, w1.y, w1.x, w0.w, w0.z, w0.y, w0.x};
+    if (PRO == PRO_GRID && ok) {
+      const bool along = comp == a.xg_axis;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        v[j] -= along ? (float)(x0 + j) : rc;
+        w[j] -= along ? (float)(N - 1 - x0 - j) : rc;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      p[j] = v[j] + w[j];
+      q[j] = v[j] - w[j];
+    }
+  } else {
+    float4 c0 = z4, c1 = z4, c2 = z4, c3 = z4;
+    if (ok) {
+      const float4 *pc = reinterpret_cast<const float4 *>(src + 2 * x0);
+      c0 = pc[0];
+      c1 = pc[1];
+      c2 = pc[2];
+      c3 = pc[3];
+    }
+    p[0] = c0.x; q[0] = c0.y; p[1] = c0.z; q[1] = c0.w;
+    p[2] = c1.x; q[2] = c1.y; p[3] = c1.z; q[3] = c1.w;
+    p[4] = c2.x; q[4] = c2.y; p[5] = c2.z; q[5] = c2.w;
+    p[6] = c3.x; q[6] = c3.y; p[7] = c3.z; q[7] = c3.w;
+  }
+}
+
+template <int NT, bool INV, int PRO, int EPI>
+__global__ __launch_bounds__(kPersistThreads) void k_contract_sym(PassArgs a, int comps) {
+  constexpr int KS = (NT + 1) / 2;
+  extern __shared__ __align__(16) uint4 bsh[];  // [E | O] hi, then [E | O] lo
+  const int N = a.n_in, H = N >> 1;
+  constexpr int nfrag = NT * KS * 64;  // uint4 per matrix
+  {
+    const uint4 *gh = reinterpret_cast<const uint4 *>(a.bhi);
+    const uint4 *gl = reinterpret_cast<const uint4 *>(a.blo);
+    for (int i = threadIdx.x; i < 2 * nfrag; i += blockDim.x) {
+      bsh[i] = gh[i];
+      bsh[2 * nfrag + i] = gl[i];
+    }
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6;
+  const int waves_per_block = blockDim.x >> 6;
+  const long long tiles_per_comp = (a.R + 15) >> 4;
+  const long long total = tiles_per_comp * comps;
+  const long long stride = (long long)gridDim.x * waves_per_block;
+  long long tile = (long long)blockIdx.x * waves_per_block + wave;
+  // input of a tile: its component, first row and this lane's row
+  struct TileIn {
+    int comp;
+    long long row0;
+    bool rowok;
+    const float *src, *xr;
+    float rc;
+  };
+  auto tile_in = [&](long long t, int lane) {
+    TileIn ti;
+    ti.comp = (int)(t / tiles_per_comp);
+    ti.row0 = (t % tiles_per_comp) * 16;
+    const long long arow = ti.row0 + (lane & 15);
+    ti.rowok = arow < a.R;
+    ti.src = a.in + ti.comp * a.cin + arow * N;
+    ti.xr = PRO == PRO_X ? a.xpro + ti.comp * a.cin + arow * N : nullptr;
+    ti.rc = 0.f;
+    if (PRO == PRO_GRID && ti.comp != a.xg_axis) ti.rc = grid_coord(a, ti.comp, arow);
+    return ti;
+  };
+  // raw data of the next k-step to split: loaded one k-step ahead, across
+  // tile boundaries (the last k-step of a tile loads the first of the next),
+  // so the HBM loads stream while the MFMAs run
+  float p[8], q[8];
+  TileIn cur{};
+  if (tile < total) {
+    int lane = threadIdx.x & 63;
+    cur = tile_in(tile, lane);
+    sym_load<INV, PRO>(a, cur.src, cur.xr, cur.rowok, 8 * (lane >> 4), N, H, cur.comp, cur.rc,
+                       p, q);
+  }
+  for (; tile < total; tile += stride) {
+    // the lane index is re-read per tile: values derived from it alone (the
+    // index-grid coordinates of every k-step, the output columns) are then
+    // recomputed per tile instead of held in registers across the loop
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
+    const int kq = 8 * (lane >> 4);
+    const long long next = tile + stride;
+    TileIn nxt{};
+    f32x4_t ae[NT], ao[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      ae[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      ao[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      bf16x8_t phi, plo, qhi, qlo;
+      split8(p, phi, plo);
+      split8(q, qhi, qlo);
+      if (s + 1 < KS) {
+        sym_load<INV, PRO>(a, cur.src, cur.xr, cur.rowok, (s + 1) * 32 + kq, N, H, cur.comp,
+                           cur.rc, p, q);
+      } else if (next < total) {
+        nxt = tile_in(next, lane);
+        sym_load<INV, PRO>(a, nxt.src, nxt.xr, nxt.rowok, kq, N, H, nxt.comp, nxt.rc, p, q);
+      }
+      const uint4 *bt = bsh + s * 64 + lane;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const uint4 *b = bt + t * KS * 64;
+        const bf16x8_t eh = __builtin_bit_cast(bf16x8_t, b[0]);
+        const bf16x8_t oh = __builtin_bit_cast(bf16x8_t, b[nfrag]);
+        const bf16x8_t el = __builtin_bit_cast(bf16x8_t, b[2 * nfrag]);
+        const bf16x8_t ol = __builtin_bit_cast(bf16x8_t, b[3 * nfrag]);
+        ae[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(plo, eh, ae[t], 0, 0, 0);
+        ao[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(qlo, oh, ao[t], 0, 0, 0);
+        ae[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(phi, el, ae[t], 0, 0, 0);
+        ao[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(qhi, ol, ao[t], 0, 0, 0);
+        ae[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(phi, eh, ae[t], 0, 0, 0);
+        ao[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(qhi, oh, ao[t], 0, 0, 0);
+        // one tile's B operands live at a time (the accumulators of all
+        // tiles already hold 8 NT registers)
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    // lane holds rows orow..orow+3 of its two output columns per tile (all
+    // four exist when orow < R: R % 4 == 0)
+    const int comp = cur.comp;
+    const long long orow = cur.row0 + 4 * (lane >> 4);
+    const bool rows4 = orow < a.R;
+    float *out = a.out + comp * a.cout;
+    const float *xs = EPI == EPI_ADDX && a.xepi ? a.xepi + comp * a.cout : nullptr;
+    float grc[4] = {0.f, 0.f, 0.f, 0.f};
+    if (EPI == EPI_ADDGRID && comp != a.xg_axis) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) grc[i] = grid_coord(a, comp, orow + i);
+    }
+    // output element (kc, orow) at obase + kc * ostride (out_offset); the
+    // stride is re-read per tile so the compiler does not keep the 2 NT
+    // column offsets live across the tile loop
+    long long obase;
+    uint32_t ostride;
+    if (a.rin == 0) {
+      obase = orow;
+      ostride = (uint32_t)a.R;
+    } else {
+      const uint32_t b = (uint32_t)orow / (uint32_t)a.rin;
+      obase = (long long)b * a.n_out * a.rin + (orow - (long long)b * a.rin);
+      ostride = (uint32_t)a.rin;
+    }
+    asm volatile("" : "+v"(ostride));
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int c = t * 16 + (lane & 15);  // m (forward) or x (inverse)
+      if (c >= H) continue;
+      const int kc0 = INV ? c : 2 * c, kc1 = INV ? N - 1 - c : 2 * c + 1;
+      float v0[4], v1[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        v0[i] = INV ? ae[t][i] + ao[t][i] : ae[t][i];
+        v1[i] = INV ? ae[t][i] - ao[t][i] : ao[t][i];
+        if (EPI == EPI_ADDGRID) {
+          const bool along = comp == a.xg_axis;
+          v0[i] += along ? (float)kc0 : grc[i];
+          v1[i] += along ? (float)kc1 : grc[i];
+        }
+      }
+      // rows orow..orow+3 are consecutive in the output (rin % 4 == 0)
+      const long long ob0 = obase + (long long)((uint32_t)kc0 * ostride);
+      const long long ob1 = obase + (long long)((uint32_t)kc1 * ostride);
+      if (rows4) {
+        if (EPI == EPI_ADDX && xs) {
+          const float4 t0 = *reinterpret_cast<const float4 *>(xs + ob0);
+          const float4 t1 = *reinterpret_cast<const float4 *>(xs + ob1);
+          v0[0] += t0.x; v0[1] += t0.y; v0[2] += t0.z; v0[3] += t0.w;
+          v1[0] += t1.x; v1[1] += t1.y; v1[2] += t1.z; v1[3] += t1.w;
+        }
+        *reinterpret_cast<float4 *>(out + ob0) = make_float4(v0[0], v0[1], v0[2], v0[3]);
+        *reinterpret_cast<float4 *>(out + ob1) = make_float4(v1[0], v1[1], v1[2], v1[3]);
+      }
+    }
+    cur = nxt;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // x axis in one pass: forward DCT-II, prior filter (+ noise, cost), inverse
 // DCT-III.  The data is [X][R] per component (x outermost, the R = Ny*Nz
@@ -863,6 +1102,31 @@ void launch_lds(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs
   }
 }
 
+template <int NT, bool INV, int PRO, int EPI>
+void launch_sym_nt(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs &a) {
+  auto k = k_contract_sym<NT, INV, PRO, EPI>;
+  static std::atomic<bool> attr_set{false};
+  if (!attr_set) {
+    HIP_OK(hipFuncSetAttribute((const void *)k,
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kPersistMaxLds));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(kPersistThreads), lds, s, a, comps);
+}
+
+// NT = ceil(H / 16) output tiles per parity, 2..kSymMaxTiles
+template <bool INV, int PRO, int EPI>
+void launch_sym(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs &a) {
+  switch ((a.n_in / 2 + 15) / 16) {
+    case 2: launch_sym_nt<2, INV, PRO, EPI>(blocks, lds, comps, s, a); break;
+    case 3: launch_sym_nt<3, INV, PRO, EPI>(blocks, lds, comps, s, a); break;
+    case 4: launch_sym_nt<4, INV, PRO, EPI>(blocks, lds, comps, s, a); break;
+    case 5: launch_sym_nt<5, INV, PRO, EPI>(blocks, lds, comps, s, a); break;
+    default: launch_sym_nt<6, INV, PRO, EPI>(blocks, lds, comps, s, a); break;
+  }
+}
+
 template <bool VEC, int PRO, int EPI>
 void launch(dim3 grid, hipStream_t s, const PassArgs &a) {
   switch (a.ks) {
@@ -990,6 +1254,28 @@ const PassMatrix &MatrixCache::get(int kind, int n_in, int n_out,
   pm->n_out = n_out;
   pm->hi.upload(hi.data(), hi.size(), stream);
   pm->lo.upload(lo.data(), lo.size(), stream);
+  if (n_in == n_out && n_in % 16 == 0 && n_in / 2 > 16 && n_in / 2 <= 16 * kSymMaxTiles) {
+    // even / odd halves: C_n[k][n-1-x] = (-1)^k C_n[k][x]
+    const int H = n_in / 2;
+    const bool inv = kind == kDctInverse;
+    std::vector<double> E((size_t)H * H), O((size_t)H * H);
+    for (int r = 0; r < H; ++r)
+      for (int c = 0; c < H; ++c) {
+        // forward: row r = x, column c = m; inverse: row r = m, column c = x
+        const int x = inv ? c : r, m = inv ? r : c;
+        E[(size_t)r * H + c] = dct_entry(n_in, 2 * m, x);
+        O[(size_t)r * H + c] = dct_entry(n_in, 2 * m + 1, x);
+      }
+    std::vector<uint32_t> ehi, elo, ohi, olo;
+    build_swizzled(E, H, H, ehi, elo);
+    build_swizzled(O, H, H, ohi, olo);
+    ehi.insert(ehi.end(), ohi.begin(), ohi.end());
+    elo.insert(elo.end(), olo.begin(), olo.end());
+    pm->shi.upload(ehi.data(), ehi.size(), stream);
+    pm->slo.upload(elo.data(), elo.size(), stream);
+    pm->sym_inverse = inv ? 1 : 0;
+    pm->sym_ks = (H + 31) / 32;
+  }
   HIP_OK(hipStreamSynchronize(stream));
   auto &ref = *pm;
   cache_[key] = std::move(pm);
@@ -1003,6 +1289,9 @@ struct PassPlan {
   PassArgs a{};
   int pro = PRO_NONE, epi = EPI_STORE, comps = 0;
   bool vec = false, persist = false;
+  // persistent even / odd contraction (k_contract_sym) of a forward or
+  // inverse square DCT
+  bool sym = false, sym_inv = false;
   dim3 grid;
   int blocks = 0;
   size_t lds = 0;
@@ -1066,6 +1355,20 @@ PassPlan plan_contract(const PassMatrix &m, const float *in, float *out, int com
   P.lds = mat_lds > 16384 ? mat_lds : 16384;
   const long long tiles = ((R + 15) / 16) * comps;
   P.persist = a.ks <= 6 && mat_lds <= kPersistMaxLds && tiles >= 2048;
+  if (P.persist && P.vec && m.sym_ks > 0) {
+    // the even / odd halves: forward passes with a prologue or a plain
+    // store, inverse passes with a plain store or the x epilogue
+    const bool ok = m.sym_inverse ? (pro == PRO_NONE && epi != EPI_FILTER) : epi == EPI_STORE;
+    const size_t sym_lds = (size_t)4 * ((m.n_in / 2 + 15) / 16) * m.sym_ks * 64 * 16;
+    if (ok && sym_lds <= kPersistMaxLds) {
+      P.sym = true;
+      P.sym_inv = m.sym_inverse != 0;
+      a.bhi = m.shi.get();
+      a.blo = m.slo.get();
+      a.ks = m.sym_ks;
+      P.lds = sym_lds;
+    }
+  }
   P.blocks = P.persist ? persistent_blocks() : (int)(P.grid.x * P.grid.y);
   return P;
 }
@@ -1079,11 +1382,15 @@ long long launch_contract(const PassPlan &P, hipStream_t stream, KernelTimer *ti
     // persistent blocks per CU: every tile's outputs are computed by one
     // wavefront whatever block runs it, so one or two blocks per CU give the
     // same results; the autotuner picks the faster
-    const int cands[2] = {blocks, 2 * blocks};
+    // the even / odd kernel streams tiles through a one-k-step prefetch, so
+    // half a block per CU (more tiles per wavefront) is a candidate too
+    const int cands[3] = {blocks, 2 * blocks, blocks / 2};
+    const int ncand = P.sym && blocks >= 2 ? 3 : 2;
     const long long key = ((long long)P.a.n_in << 52) ^ ((long long)P.a.n_out << 42) ^
                           ((long long)P.a.R << 12) ^ ((long long)P.comps << 8) ^
-                          ((long long)P.pro << 5) ^ ((long long)P.epi << 2) ^ (P.vec ? 1 : 0);
-    blocks = Autotuner::get().choose("contract", key, cands, 2, blocks, stream,
+                          ((long long)P.pro << 5) ^ ((long long)P.epi << 2) ^ (P.vec ? 1 : 0) ^
+                          (P.sym ? 2 : 0);
+    blocks = Autotuner::get().choose("contract", key, cands, ncand, blocks, stream,
                                      [&](int b) { launch_contract_blocks(P, stream, b); });
   }
   TimedLaunch tl(timer, name, stream);
@@ -1096,6 +1403,21 @@ void launch_contract_blocks(const PassPlan &P, hipStream_t stream, int blocks) {
   const size_t lds = P.lds;
   const int comps = P.comps;
   const dim3 grid = P.grid;
+  if (P.sym) {
+    if (P.sym_inv) {
+      if (P.epi == EPI_ADDX) launch_sym<true, PRO_NONE, EPI_ADDX>(blocks, lds, comps, stream, a);
+      else if (P.epi == EPI_ADDGRID)
+        launch_sym<true, PRO_NONE, EPI_ADDGRID>(blocks, lds, comps, stream, a);
+      else launch_sym<true, PRO_NONE, EPI_STORE>(blocks, lds, comps, stream, a);
+    } else {
+      if (P.pro == PRO_X) launch_sym<false, PRO_X, EPI_STORE>(blocks, lds, comps, stream, a);
+      else if (P.pro == PRO_GRID)
+        launch_sym<false, PRO_GRID, EPI_STORE>(blocks, lds, comps, stream, a);
+      else launch_sym<false, PRO_NONE, EPI_STORE>(blocks, lds, comps, stream, a);
+    }
+    HIP_LAUNCH_OK();
+    return;
+  }
 #define BINDER_GO(V, PR, E)                                            \
   do {                                                                 \
     if (P.persist) launch_lds<V, PR, E>(blocks, lds, comps, stream, a); \

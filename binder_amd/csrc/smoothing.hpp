@@ -18,6 +18,14 @@ namespace binder {
 struct PassMatrix {
   int n_in = 0, n_out = 0;
   DBuf<uint32_t> hi, lo;
+  // Even/odd split of a square DCT-II / DCT-III of size n % 16 == 0 (H = n / 2,
+  // 16 < H <= 96): two H x H matrices [even | odd] in the same B-operand order,
+  // used by the persistent contraction when the pass allows it (sym_ks > 0).
+  // Forward: E[x][m] = C_n[2m][x], O[x][m] = C_n[2m+1][x] on the sums and
+  // differences of mirrored inputs; inverse: E[m][x] = C_n[2m][x],
+  // O[m][x] = C_n[2m+1][x] on the even and odd coefficients.
+  int sym_inverse = 0, sym_ks = 0;
+  DBuf<uint32_t> shi, slo;
 };
 
 class MatrixCache {

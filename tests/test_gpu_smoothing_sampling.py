@@ -18,7 +18,7 @@ def _nlt(core, sizes, gamma, sigma, order=2):
 
 
 @pytest.mark.parametrize("sizes", [[5, 10], [15, 27], [11, 20, 7], [10, 13, 30],
-                                   [64, 48, 40], [128, 96, 96]])
+                                   [64, 48, 40], [128, 96, 96], [160, 192, 160]])
 def test_smooth_deformation_closed_form(core, orc, sizes):
     """Ortho DCT-II -> p = c/(1+gamma D) -> DCT-III; cost 0.5 gamma sum D p^2."""
     print(f"smooth_deformation sizes={sizes}", flush=True)
