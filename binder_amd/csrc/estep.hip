@@ -738,7 +738,7 @@ __global__ __launch_bounds__(kEm32Threads) void k_estep_em32_batch(const EStepAr
   estep_em32_block<IS2D>(a, (int)(w & 0xFFFFu), a.nblocks);
 }
 
-constexpr int kRowsPerReduce = 16;
+constexpr int kRowsPerReduce = kHistReduceRows;
 
 __device__ __forceinline__ void hist_reduce1_block(const unsigned long long *__restrict__ hpart,
                                                    int rows, int KL,
@@ -845,7 +845,18 @@ __device__ __forceinline__ void theta_block(const ThetaArgs &a) {
   }
   red[tid] = s;
   for (int kl = tid; kl < KL; kl += 1024) {
-    const double h = a.Hd[kl];
+    double h;
+    if (a.hsum) {
+      // exact u64 sum of the partial rows (any order), then the scaling of
+      // k_hist_reduce2
+      unsigned long long acc = 0;
+#pragma unroll 8
+      for (int r = 0; r < a.hrows; ++r) acc += a.hsum[(long long)r * KL + kl];
+      h = (double)acc * (1.0 / kHistScale);
+      a.Hd[kl] = h;
+    } else {
+      h = a.Hd[kl];
+    }
     if (a.mode == kThetaDirichlet)
       sh[kl] = gamma_mt(h + a.alpha, u32x4{(uint32_t)kl, a.sweep, 0u, kStreamTheta},
                         key0, key1);
@@ -1045,6 +1056,14 @@ void launch_hist_reduce_batch(const HistReduceArgs *d_args, int npairs, int max_
 void launch_theta_batch(const ThetaArgs *d_args, int npairs, hipStream_t s) {
   if (npairs <= 0) return;
   hipLaunchKernelGGL(k_theta_batch, dim3(npairs), dim3(1024), 0, s, d_args);
+  HIP_LAUNCH_OK();
+}
+
+void launch_hist_reduce1(const unsigned long long *hpart, int rows, int KL,
+                         unsigned long long *hpart2, hipStream_t s) {
+  const int rows2 = (rows + kRowsPerReduce - 1) / kRowsPerReduce;
+  hipLaunchKernelGGL(k_hist_reduce1, dim3((KL + 255) / 256, rows2), dim3(256), 0, s, hpart,
+                     rows, KL, hpart2);
   HIP_LAUNCH_OK();
 }
 

@@ -83,7 +83,11 @@ void launch_hist_reduce_batch(const HistReduceArgs *d_args, int npairs, int max_
 enum ThetaMode { kThetaKeep = 0, kThetaMap = 1, kThetaDirichlet = 2 };
 
 struct ThetaArgs {
-  const double *Hd;     // [K*L]
+  // [K*L] histogram; with hsum, written from the sum of the hrows partial
+  // rows hsum[r][K*L] (fixed point, as k_hist_reduce2 forms it), else read
+  double *Hd;
+  const unsigned long long *hsum;
+  int hrows;
   double *theta;        // [K*L] in/out
   float *thT;           // [L*K] out
   double *rowbuf;       // [K*L] scratch
@@ -101,6 +105,14 @@ struct ThetaArgs {
 };
 
 void launch_theta(const ThetaArgs &a, hipStream_t s);
+// The first stage of launch_hist_reduce alone: rows partial rows summed 16
+// at a time into hpart2 (ceil(rows / 16) rows).
+void launch_hist_reduce1(const unsigned long long *hpart, int rows, int KL,
+                         unsigned long long *hpart2, hipStream_t s);
+// Partial rows a theta launch sums itself instead of a separate reduction,
+// and the rows one reduction stage sums into one.
+constexpr int kThetaDirectRows = 8;
+constexpr int kHistReduceRows = 16;
 // launch_theta for npairs registrations, one block each.
 void launch_theta_batch(const ThetaArgs *d_args, int npairs, hipStream_t s);
 

@@ -343,6 +343,32 @@ struct binder_registration {
     launch_hist_reduce(hpart.get(), hist_rows, K * L, hpart2.get(), Hd.get(), st);
   }
 
+  // Histogram reduction and theta step.  Up to kThetaDirectRows partial
+  // rows are summed by the theta launch itself, up to kHistReduceRows times
+  // that many after one reduction stage, more after both; Hd is the same in
+  // every case (exact integer sums) and holds the histogram afterwards.
+  void hist_theta(hipStream_t st, int mode, uint32_t k0, uint32_t k1, uint32_t sweep,
+                  const uint32_t *keyx = nullptr, double *theta_out = nullptr,
+                  float *thT_out = nullptr, double *stats_out = nullptr) {
+    ThetaArgs t = theta_args(mode, k0, k1, sweep, keyx, theta_out, thT_out, stats_out);
+    const int rows2 = (hist_rows + kHistReduceRows - 1) / kHistReduceRows;
+    {
+      TimedLaunch tl(tm(), "hist_reduce", st);
+      if (hist_rows <= kThetaDirectRows) {
+        t.hsum = hpart.get();
+        t.hrows = hist_rows;
+      } else if (rows2 <= kThetaDirectRows) {
+        launch_hist_reduce1(hpart.get(), hist_rows, K * L, hpart2.get(), st);
+        t.hsum = hpart2.get();
+        t.hrows = rows2;
+      } else {
+        launch_hist_reduce(hpart.get(), hist_rows, K * L, hpart2.get(), Hd.get(), st);
+      }
+    }
+    TimedLaunch tl(tm(), mode == kThetaDirichlet ? "dirichlet" : "theta", st);
+    launch_theta(t, st);
+  }
+
   // Histogram partial rows (and mll partial slots) for E-steps of up to
   // `rows` blocks.
   void ensure_rows(int rows) {
@@ -402,8 +428,8 @@ struct binder_registration {
   void e_and_theta(bool sample, int theta_mode, uint32_t k0, uint32_t k1,
                    uint32_t sweep) {
     estep_only(sample, k0, k1, sweep);
-    hist_reduce_on(s());
-    if (theta_mode >= 0) run_theta(theta_mode, k0, k1, sweep);
+    if (theta_mode >= 0) hist_theta(s(), theta_mode, k0, k1, sweep);
+    else hist_reduce_on(s());
   }
 
   void run_theta(int mode, uint32_t k0, uint32_t k1, uint32_t sweep,
@@ -487,15 +513,13 @@ struct binder_registration {
         estep_only(true, k0, k1, (uint32_t)sweep, keyx);
         HIP_OK(hipEventRecord(estep_done.e, s()));
         HIP_OK(hipStreamWaitEvent(side_stream.s, estep_done.e, 0));
-        hist_reduce_on(side_stream.s);
-        run_theta(kThetaDirichlet, k0, k1, (uint32_t)sweep, side_stream.s, keyx);
+        hist_theta(side_stream.s, kThetaDirichlet, k0, k1, (uint32_t)sweep, keyx);
         HIP_OK(hipEventRecord(theta_done.e, side_stream.s));
       } else {
-        if (sp) {
-          estep_only(true, k0, k1, (uint32_t)sweep, keyx);
-          hist_reduce_on(s());
-        }
-        if (sl) run_theta(kThetaDirichlet, k0, k1, (uint32_t)sweep, nullptr, keyx);
+        if (sp) estep_only(true, k0, k1, (uint32_t)sweep, keyx);
+        if (sp && sl) hist_theta(s(), kThetaDirichlet, k0, k1, (uint32_t)sweep, keyx);
+        else if (sp) hist_reduce_on(s());
+        else if (sl) run_theta(kThetaDirichlet, k0, k1, (uint32_t)sweep, nullptr, keyx);
       }
       if (st) {
         SmoothOptions o;
@@ -919,14 +943,15 @@ int binder_registration_em(binder_registration *r, int max_it, double th,
         TimedLaunch t(r->tm(), "estep", r->s());
         launch_estep(a, r->is2d, false, r->estep_blocks, r->s());
       }
-      r->hist_reduce_on(r->s());
-      r->run_theta(update ? kThetaMap : kThetaKeep, 0, 0, 0, nullptr, nullptr,
-                   TH[update ? q : p], TT[update ? q : p], ST[p]);
+      r->hist_theta(r->s(), update ? kThetaMap : kThetaKeep, 0, 0, 0, nullptr,
+                    TH[update ? q : p], TT[update ? q : p], ST[p]);
       SmoothOptions opt;
       opt.prior_out = ST[p] + 2;
+      // the prior cost's sum also writes the three objective terms to the
+      // page-locked host buffer (no copy launch)
+      opt.host_stats = host + 8 * p;
+      opt.stats_in = ST[p];
       r->sm->run(DR[p], r->x.get(), F[q], opt, r->tm());
-      HIP_OK(hipMemcpyAsync(host + 8 * p, ST[p], 3 * sizeof(double), hipMemcpyDeviceToHost,
-                            r->s()));
       HIP_OK(hipEventRecord(r->em_done[p].e, r->s()));
     };
     double mlp_old = 1e100;

@@ -221,6 +221,24 @@ __global__ __launch_bounds__(256) void k_index_grid(double *__restrict__ out, in
   }
 }
 
+// The resized index grid from its resized axes: component c of output
+// voxel (x, y, z) is the resized ramp of axis c at that voxel's index along
+// c (axes = [x ramp (nxo) | y ramp (nyo) | z ramp (nzo)]).
+__global__ __launch_bounds__(256) void k_grid_from_axes(const double *__restrict__ axes,
+                                                        int nxo, int nyo, int nzo, int comps,
+                                                        double *__restrict__ out) {
+  const long long n = (long long)nxo * nyo * nzo * comps;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long v = e / comps;
+    const int c = (int)(e - v * comps);
+    const int z = (int)(v % nzo);
+    const int y = (int)((v / nzo) % nyo);
+    const int x = (int)(v / ((long long)nzo * nyo));
+    out[e] = c == 0 ? axes[x] : (c == 1 ? axes[nxo + y] : axes[nxo + nyo + z]);
+  }
+}
+
 // Gaussian passes on `a` (layout L, scratch b), then the zoom into `out`.
 // `range` is a device pointer to {min, max}.  Synchronises the stream.
 template <typename OUT>
@@ -333,17 +351,35 @@ extern "C" int binder_resize_index_grid(int nx, int ny, int nz, int comps, int n
     } sg{s};
     DBuf<double> a, b, o, range;
     a.alloc((size_t)nin);
-    hipLaunchKernelGGL(k_index_grid, dim3(grid_for(nin)), dim3(256), 0, s, a.get(), nx, ny, nz,
-                       comps);
-    HIP_LAUNCH_OK();
+    (void)nin;
     // min and max over every value of the grid: 0 and the largest index
     int hi = std::max(nx, ny);
     if (comps == 3) hi = std::max(hi, nz);
     const double lohi[2] = {0.0, (double)(hi - 1)};
     range.upload(lohi, 2, s);
+    // component c of the grid varies along axis c only, and the Gaussian
+    // and the linear zoom along any other axis leave a constant constant:
+    // each axis's index ramp is resized once, in one dimension, with the
+    // same kernels, and broadcast (the full-size volume never exists)
+    const int nin_ax[3] = {nx, ny, nz}, nout_ax[3] = {nxo, nyo, nzo};
+    DBuf<double> axes;
+    axes.alloc((size_t)nxo + nyo + nzo);
+    axes.zero(s);
+    int off = 0;
+    for (int c = 0; c < comps; ++c) {
+      a.alloc((size_t)nin_ax[c]);
+      b.release();
+      hipLaunchKernelGGL(k_index_grid, dim3(grid_for(nin_ax[c])), dim3(256), 0, s, a.get(),
+                         nin_ax[c], 1, 1, 1);
+      HIP_LAUNCH_OK();
+      resize_device<double>(a, b, Layout{1, 1}, nin_ax[c], 1, 1, 1, axes.get() + off,
+                            Layout{1, 1}, nout_ax[c], 1, 1, range.get(), scale, s);
+      off += nout_ax[c];
+    }
     o.alloc((size_t)nout);
-    resize_device<double>(a, b, Layout{1, comps}, nx, ny, nz, comps, o.get(),
-                          Layout{1, comps}, nxo, nyo, nzo, range.get(), scale, s);
+    hipLaunchKernelGGL(k_grid_from_axes, dim3(grid_for(nout)), dim3(256), 0, s, axes.get(), nxo,
+                       nyo, nzo, comps, o.get());
+    HIP_LAUNCH_OK();
     o.download(out, (size_t)nout, s);
     HIP_OK(hipStreamSynchronize(s));
   });

@@ -1039,6 +1039,20 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *part, long l
   sum_partials_block(part, count, out);
 }
 
+// k_sum_partials that also writes (stats[0], stats[1], sum) to page-locked
+// host memory, in place of a copy after it.
+__global__ __launch_bounds__(256) void k_sum_partials_host(const double *part, long long count,
+                                                           double *out, const double *stats,
+                                                           double *host) {
+  sum_partials_block(part, count, out);
+  if (threadIdx.x == 0) {
+    host[0] = stats[0];
+    host[1] = stats[1];
+    host[2] = *out;
+    __threadfence_system();
+  }
+}
+
 struct SumArgs {
   const double *part;
   long long count;
@@ -1684,14 +1698,20 @@ void Smoother::run(const float *dream, const float *x, float *out,
 
   float *bufs[2] = {bufA_.get(), bufB_.get()};
   long long nparts = 0;  // partial-sum slots written by the filter pass
+  BINDER_REQUIRE(!opt.host_stats || (opt.want_prior && opt.stats_in),
+                 "host statistics need the prior cost and the device statistics");
+  auto sum_kernel = [&](const double *part, long long count, double *out) {
+    TimedLaunch tl(timer, "sum_partials", st);
+    if (opt.host_stats)
+      hipLaunchKernelGGL(k_sum_partials_host, dim3(1), dim3(256), 0, st, part, count, out,
+                         opt.stats_in, opt.host_stats);
+    else
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, part, count, out);
+    HIP_LAUNCH_OK();
+  };
   auto sum_partials = [&]() {
-    if (opt.want_prior) {
-      TimedLaunch tl(timer, "sum_partials", st);
-      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st,
-                         part_cost_.get(), nparts,
-                         opt.prior_out ? opt.prior_out : scal_.get());
-      HIP_LAUNCH_OK();
-    }
+    if (opt.want_prior)
+      sum_kernel(part_cost_.get(), nparts, opt.prior_out ? opt.prior_out : scal_.get());
     if (opt.sample && opt.sample_gamma) {
       hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st,
                          part_raw_.get(), nparts, scal_.get() + 1);
@@ -1707,12 +1727,7 @@ void Smoother::run(const float *dream, const float *x, float *out,
     launch_contract(P.z_fwd, st, timer, "dct_z_fwd");
     launch_contract(P.y_fwd, st, timer, "dct_y_fwd");
     launch_fused_plan(P.x, st, timer, "dct_x_fused");
-    if (P.sum.out) {
-      TimedLaunch tl(timer, "sum_partials", st);
-      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, P.sum.part, P.sum.count,
-                         P.sum.out);
-      HIP_LAUNCH_OK();
-    }
+    if (P.sum.out) sum_kernel(P.sum.part, P.sum.count, P.sum.out);
     launch_contract(P.z_inv, st, timer, "dct_z_inv");
     launch_contract(P.y_inv, st, timer, "dct_y_inv");
     return;

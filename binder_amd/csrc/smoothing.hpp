@@ -100,6 +100,11 @@ struct SmoothOptions {
   uint32_t stream = 0;         // Philox stream word
   double *prior_out = nullptr; // device double: receives sum_n 0.5 g_n D p^2
   bool want_prior = true;      // false: skip the reduction of the prior cost
+  // with want_prior: the kernel that sums the prior cost also writes
+  // host_stats[0..2] = stats_in[0], stats_in[1], prior cost (host_stats in
+  // page-locked memory; stats_in on the device, complete before run())
+  double *host_stats = nullptr;
+  const double *stats_in = nullptr;
   hipStream_t exec_stream = nullptr;  // stream to run on (null: the smoother's own)
 };
 

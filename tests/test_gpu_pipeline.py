@@ -94,8 +94,12 @@ def test_resize_field_matches_host(core, shape_in, shape_out, comps):
     ((17, 9, 13), (9, 18, 5), 3),         # mixed shrink / grow, odd sizes
     ((96, 80, 1), (12, 10, 1), 2)])       # 2D
 def test_resize_index_grid(core, shape_in, shape_out, comps):
-    """The device-made index grid resampled equals the uploaded index grid
-    resampled (the same kernels: bit for bit) and the host restatement of
+    """The index grid resampled axis by axis (each component's 1-D ramp
+    resized with the same kernels, then broadcast) equals the uploaded
+    index grid resampled as a volume up to FP64 rounding (the Gaussian and
+    the zoom along the other axes scale a constant by a sum of weights that
+    is 1 to about 1e-16), equals it after the float32 rounding the
+    registration stores, and matches the host restatement of
     skimage.transform.resize(order=1)."""
     from binder_amd import resample
     print(f"resize index grid {shape_in} -> {shape_out} x{comps}", flush=True)
@@ -103,7 +107,9 @@ def test_resize_index_grid(core, shape_in, shape_out, comps):
     scale = shape_out[0] / shape_in[0]
     got = core.resize_index_grid(shape_in, comps, shape_out, scale)
     assert got.shape == shape_out + (comps,)
-    assert np.array_equal(got, core.resize_field(grid, shape_out, scale))
+    vol = core.resize_field(grid, shape_out, scale)
+    assert np.max(np.abs(got - vol)) <= 1e-13 * np.abs(vol).max()
+    assert np.array_equal(got.astype(np.float32), vol.astype(np.float32))
     exp = resample.resize(grid, shape_out, order=1) * scale
     assert np.max(np.abs(got - exp)) <= 1e-10 * np.abs(exp).max()
 
