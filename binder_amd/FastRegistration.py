@@ -4,9 +4,9 @@ gfx950 backend ``binder_amd._core``.
 
 Same constructor keywords, ``run(resolutions, max_number_of_iterations)``,
 ``move_resolution(resolution)`` and ``sample_param(...)`` as the reference,
-for the MI metric with the non-linear (DCT-prior) transformation, which is
-the path this backend implements.  The reference's SSD / feature likelihoods,
-affine / rigid transformations and smoothed pyramids are not part of it:
+for the MI and SSD metrics with the non-linear (DCT-prior), affine or rigid
+transformation.  The reference's feature likelihoods (MILikelihoodFilter,
+SSDLikelihoodFilter) and spline-smoothed pyramids are not part of it:
 asking for them raises NotImplementedError instead of silently running
 something else.  nibabel and scikit-image are replaced by
 ``binder_amd.nifti`` and ``binder_amd.resample``.
@@ -45,28 +45,28 @@ class FastRegistration:
                  initialize_with_center_of_mass=True, bin_robust=True,
                  include_image_as_feature=True, increase_gamma_at_DS_resolution=False):
         unsupported = []
-        if metric != 'MI':
+        if metric not in ('MI', 'SSD'):
             unsupported.append(f"metric={metric!r}")
         if feature_moving or feature_fixed or precomputed_features_moving is not None \
                 or precomputed_features_fixed is not None:
             unsupported.append("feature likelihoods")
-        if affine or rigid:
-            unsupported.append("affine/rigid transformations")
         if smooth:
             unsupported.append("smooth=True (spline-smoothed pyramid)")
         if visualizer:
             unsupported.append("visualizer=True")
         if unsupported:
             raise NotImplementedError(
-                "the gfx950 backend implements the MI metric with the non-linear DCT-prior "
-                "transformation; not available: " + ", ".join(unsupported))
+                "the gfx950 backend implements the MI and SSD metrics with the non-linear, "
+                "affine and rigid transformations; not available: " + ", ".join(unsupported))
         if spline_order != 3:
             raise ValueError("the gfx950 backend supports spline_order=3 only")
 
         self.registratorObj = None
         self.metric = metric
-        self.affine = False
-        self.rigid = False
+        # affine wins over rigid, as in the reference's set-up
+        # (FastRegistration.py:452-472)
+        self.affine = bool(affine)
+        self.rigid = bool(rigid) and not self.affine
         self.smooth = False
         self.visualizer = False
         self.feature_moving = False
@@ -153,6 +153,27 @@ class FastRegistration:
         # theta initialised to 1/L for all k, l (FastRegistration.py:224-225)
         self.theta = np.ones([self.num_node_intensity_levels, self.num_voxel_intensity_levels]) \
             / self.num_voxel_intensity_levels
+        if metric == 'SSD':
+            # sigma^2 starts at the mean squared bin difference over the
+            # common extent of the two images (FastRegistration.py:241-259)
+            tmp = np.minimum(self.full_binned_voxels.shape, self.full_binned_nodes.shape)
+            sl = tuple(slice(0, int(n)) for n in tmp)
+            diff = self.full_binned_nodes[sl].astype(np.float64) - \
+                self.full_binned_voxels[sl].astype(np.float64)
+            self.sigma_sq = float(np.sum(diff ** 2) / np.prod(tmp))
+            self.alpha_0, self.beta_0 = alpha_0, beta_0
+            if scale_shift:
+                self.scale_shift, self.scale, self.shift = 1, scale, shift
+            else:
+                self.scale_shift, self.scale, self.shift = 0, 1, 0
+        # starting linear transformation (FastRegistration.py:105-111): the
+        # identity in the homogeneous (affine) or plain (rigid) form of the
+        # image's dimensionality
+        nd = self.number_of_dimensions
+        if self.affine:
+            self.A, self.t = np.eye(nd + 1), np.zeros(nd)
+        if self.rigid:
+            self.R, self.t = np.eye(nd), np.zeros(nd)
 
         # full-resolution positions: the grid indices, made on first read
         # (the GPU pyramid generates them on the device and never reads them)
@@ -348,15 +369,11 @@ class FastRegistration:
                     prev, cur / res, vp, sh[0], sh[1], sh[2],
                     self.nodes_shape[0], self.nodes_shape[1], self.nodes_shape[2],
                     self.spline_order, self.non_zero_voxels)
-                self._set_nonlinear(self.registratorObj)
+                self._set_transformation(self.registratorObj)
             else:
                 self.deformedVoxelPositions = np.ascontiguousarray(self.deformedVoxelPositions)
                 self.registratorObj = self._new_registration(self.deformedVoxelPositions)
-            self.registratorObj.setMILikelihood(
-                self.alpha, np.ascontiguousarray(self.theta).ravel(),
-                self.num_node_intensity_levels, self.num_voxel_intensity_levels,
-                self.threads, np.ascontiguousarray(self.binned_nodes).ravel(),
-                np.ascontiguousarray(self.binned_voxels).ravel())
+            self._set_likelihood(self.registratorObj)
             cap = max_number_of_iterations * self.current_resolution
             if level_iterations is not None:
                 cap = int(level_iterations[li])
@@ -378,9 +395,7 @@ class FastRegistration:
                 if self.top_cor_obj and not chain:
                     self.deformedVoxelPositions = self.correct_topology(
                         self.deformedVoxelPositions)
-            self.theta = np.ascontiguousarray(np.array(
-                self.registratorObj.get_likelihood_parameters()).reshape(
-                    self.num_node_intensity_levels, self.num_voxel_intensity_levels))
+            self._keep_likelihood_parameters(self.registratorObj)
             iters = len(self.registratorObj.get_objective_trace())
             prev = self.registratorObj
             self.level_log.append((resolution, iters, time.perf_counter() - t0))
@@ -397,7 +412,7 @@ class FastRegistration:
             r = cls.from_dct(coef, self.orig_voxels_shape, res, vp, sh[0], sh[1], sh[2],
                              self.nodes_shape[0], self.nodes_shape[1], self.nodes_shape[2],
                              self.spline_order, self.non_zero_voxels)
-            self._set_nonlinear(r)
+            self._set_transformation(r)
             return r
         from .warmstart import coefficients_field
         start = coefficients_field(coef, self.orig_voxels_shape, res)
@@ -439,6 +454,9 @@ class FastRegistration:
         plain pyramid.  Returns True when the warm start was used."""
         if self.number_of_dimensions != 3:
             raise ValueError("the learned warm start is defined for 3D registrations")
+        if self.metric != 'MI' or self.affine or self.rigid:
+            raise ValueError("the learned warm start is defined for the MI metric with the "
+                             "non-linear transformation")
         coef = net.predict_pair(self.full_binned_voxels, self.full_binned_nodes,
                                 self.num_voxel_intensity_levels, self.num_node_intensity_levels)
         self.warm_coef = coef
@@ -491,8 +509,47 @@ class FastRegistration:
                                       self.nodes_shape[0], self.nodes_shape[1],
                                       self.nodes_shape[2], self.spline_order,
                                       self.non_zero_voxels)
-        self._set_nonlinear(r)
+        self._set_transformation(r)
         return r
+
+    def _set_transformation(self, r):
+        """The transformation model of the reference's set-up
+        (FastRegistration.py:452-472): affine, rigid or the DCT prior."""
+        if self.affine:
+            r.setAffineTransformation(np.ascontiguousarray(self.A).ravel(),
+                                      np.ascontiguousarray(self.t).ravel())
+        elif self.rigid:
+            r.setRigidTransformation(np.ascontiguousarray(self.R).ravel(),
+                                     np.ascontiguousarray(self.t).ravel())
+        else:
+            self._set_nonlinear(r)
+
+    def _set_likelihood(self, r):
+        """The likelihood model of the reference's set-up
+        (FastRegistration.py:474-488): MI with theta or SSD."""
+        bn = np.ascontiguousarray(self.binned_nodes).ravel()
+        bv = np.ascontiguousarray(self.binned_voxels).ravel()
+        if self.metric == 'SSD':
+            r.setSSDLikelihood(self.num_voxel_intensity_levels, self.threads, self.sigma_sq,
+                               self.scale_shift, self.scale, self.shift, bn, bv,
+                               self.alpha_0, self.beta_0)
+        else:
+            r.setMILikelihood(self.alpha, np.ascontiguousarray(self.theta).ravel(),
+                              self.num_node_intensity_levels, self.num_voxel_intensity_levels,
+                              self.threads, bn, bv)
+
+    def _keep_likelihood_parameters(self, r):
+        """What a level hands to the next (FastRegistration.py:516-524)."""
+        if self.metric == 'SSD':
+            # the reference unpacks get_likelihood_parameters() = [scale,
+            # shift] as (shift, scale): the next level starts from the fitted
+            # scale as its shift and the fitted shift as its scale.  Kept
+            # as is, so that a multi-level SSD run matches the reference's.
+            self.shift, self.scale = r.get_likelihood_parameters()
+        else:
+            self.theta = np.ascontiguousarray(np.array(
+                r.get_likelihood_parameters()).reshape(
+                    self.num_node_intensity_levels, self.num_voxel_intensity_levels))
 
     def _set_nonlinear(self, r):
         if hasattr(r, "setNonLinearTransformationDCTPrior") and \
@@ -569,11 +626,7 @@ class FastRegistration:
         self.registratorObj = self._new_registration(self.deformedVoxelPositions)
         if sample_posteriors == 0:
             self.registratorObj.set_dream_locations(np.asarray(self.dreamLocations).ravel())
-        self.registratorObj.setMILikelihood(self.alpha, np.ascontiguousarray(self.theta).ravel(),
-                                            self.num_node_intensity_levels,
-                                            self.num_voxel_intensity_levels, self.threads,
-                                            np.ascontiguousarray(self.binned_nodes).ravel(),
-                                            np.ascontiguousarray(self.binned_voxels).ravel())
+        self._set_likelihood(self.registratorObj)
         return self.registratorObj.sampler(self.seed, burnin, samples, sample_gamma,
                                            sample_gamma_every, sample_posteriors,
                                            sample_likelihood_parameters,

@@ -46,6 +46,13 @@ _reg_create = _sig("binder_registration_create", _c_int, _P(_vp), _vp, _c_int, _
 _reg_destroy = _sig("binder_registration_destroy", None, _vp)
 _reg_set_mi = _sig("binder_registration_set_mi_likelihood", _c_int, _vp, _c_double, _vp,
                    _c_int, _c_int, _c_int, _vp, _vp)
+_reg_set_ssd = _sig("binder_registration_set_ssd_likelihood", _c_int, _vp, _c_int, _c_int,
+                    _c_double, _c_int, _c_double, _c_double, _vp, _vp, _c_double, _c_double)
+_reg_get_ssd = _sig("binder_registration_get_ssd_parameters", _c_int, _vp, _vp)
+_reg_set_affine = _sig("binder_registration_set_affine_transformation", _c_int, _vp, _vp, _vp)
+_reg_set_rigid = _sig("binder_registration_set_rigid_transformation", _c_int, _vp, _vp, _vp)
+_reg_get_linear = _sig("binder_registration_get_linear_parameters", _c_int, _vp, _vp,
+                       _P(_c_int))
 _reg_set_nlt = _sig("binder_registration_set_nonlinear_transformation", _c_int, _vp,
                     *([_c_double] * 9), _vp)
 _reg_em = _sig("binder_registration_em", _c_int, _vp, _c_int, _c_double, _c_int, _c_int)
@@ -176,6 +183,7 @@ class Registration:
         self._shape = (int(Nx_v), int(Ny_v), int(Nz_v), N)
         self._nn = int(Nx_n) * int(Ny_n) * int(Nz_n)
         self._KL = None
+        self._ssd = False
 
     @classmethod
     def resampled_from(cls, src, scale, voxel_pos, Nx_v, Ny_v, Nz_v, Nx_n, Ny_n, Nz_n,
@@ -202,6 +210,7 @@ class Registration:
         self._shape = (int(Nx_v), int(Ny_v), int(Nz_v), N)
         self._nn = int(Nx_n) * int(Ny_n) * int(Nz_n)
         self._KL = None
+        self._ssd = False
         return self
 
     @classmethod
@@ -229,6 +238,7 @@ class Registration:
         self._shape = (int(Nx_v), int(Ny_v), int(Nz_v), 3)
         self._nn = int(Nx_n) * int(Ny_n) * int(Nz_n)
         self._KL = None
+        self._ssd = False
         return self
 
     def __del__(self):
@@ -247,6 +257,71 @@ class Registration:
         _check(_reg_set_mi(self._h, float(alpha), _ptr(th), int(K), int(L), int(threads),
                            _ptr(bn), _ptr(bv)))
         self._KL = (int(K), int(L))
+        self._ssd = False
+
+    def setSSDLikelihood(self, bins, threads, sigma_sq, scale_shift, scale, shift,
+                         binned_nodes, binned_voxels, alpha_0, beta_0):
+        """Sum-of-squared-differences likelihood (BINDER/src/cpu/bindings.cpp:120-152,
+        SSDLikelihood.cpp): both images binned to [0, bins), bins <= 64;
+        with scale_shift the EM fits shift and scale, and sigma_sq always."""
+        bn = _arr(binned_nodes, np.int32)
+        bv = _arr(binned_voxels, np.int32)
+        if bn.size != self._nn or bv.size != int(np.prod(self._shape[:3])):
+            raise ValueError("binned_nodes / binned_voxels have the wrong size")
+        _check(_reg_set_ssd(self._h, int(bins), int(threads), float(sigma_sq),
+                            int(bool(scale_shift)), float(scale), float(shift), _ptr(bn),
+                            _ptr(bv), float(alpha_0), float(beta_0)))
+        self._KL = (int(bins), int(bins))
+        self._ssd = True
+
+    def get_ssd_parameters(self):
+        """[scale, shift, sigma_sq] of the SSD likelihood.  No reference
+        counterpart (the reference prints them)."""
+        out = np.empty(3, np.float64)
+        _check(_reg_get_ssd(self._h, _ptr(out)))
+        return out.tolist()
+
+    def _linear_matrix(self, M, homogeneous):
+        N = self._shape[3]
+        m = np.asarray(M, np.float64).ravel()
+        if homogeneous and m.size == N * N:
+            # the N x N linear part: the homogeneous row and column are the
+            # identity's (FastRegistration passes np.eye(3) as the start)
+            h = np.eye(N + 1)
+            h[:N, :N] = m.reshape(N, N)
+            m = h.ravel()
+        want = (N + 1) ** 2 if homogeneous else N * N
+        if m.size != want:
+            raise ValueError(f"the matrix must have {want} values")
+        return np.ascontiguousarray(m)
+
+    def setAffineTransformation(self, A, t):
+        """Affine transformation model (BINDER/src/cpu/bindings.cpp:223-235,
+        AffineTransformation.cpp): A (N+1) x (N+1) homogeneous or its N x N
+        linear part, t of N values; the EM's least-squares fit replaces both."""
+        a = self._linear_matrix(A, True)
+        tv = _arr(t, np.float64)
+        if tv.size != self._shape[3]:
+            raise ValueError("t must have N values")
+        _check(_reg_set_affine(self._h, _ptr(a), _ptr(tv)))
+
+    def setRigidTransformation(self, R, t):
+        """Rigid transformation model (BINDER/src/cpu/bindings.cpp:236-247,
+        RigidTransformation.cpp): R N x N, t of N values."""
+        r = self._linear_matrix(R, False)
+        tv = _arr(t, np.float64)
+        if tv.size != self._shape[3]:
+            raise ValueError("t must have N values")
+        _check(_reg_set_rigid(self._h, _ptr(r), _ptr(tv)))
+
+    def get_linear_parameters(self):
+        """The affine ((N+1) x (N+1)) or rigid (N x N) matrix as last fitted,
+        row-major, followed by t.  No reference counterpart (the reference
+        prints them)."""
+        out = np.empty(19, np.float64)
+        n = _c_int(0)
+        _check(_reg_get_linear(self._h, _ptr(out), ctypes.byref(n)))
+        return out[:n.value].tolist()
 
     def setNonLinearTransformation(self, gamma_x, gamma_y, gamma_z, voxels_size_x,
                                    voxels_size_y, voxels_size_z, nodes_size_x, nodes_size_y,
@@ -345,7 +420,8 @@ class Registration:
     def get_likelihood_parameters(self):
         if self._KL is None:
             raise RuntimeError("setMILikelihood was not called")
-        out = np.empty(self._KL[0] * self._KL[1], np.float64)
+        # SSD: [scale, shift] (SSDLikelihood.cpp:328-334); MI: theta (K x L)
+        out = np.empty(2 if self._ssd else self._KL[0] * self._KL[1], np.float64)
         _check(_reg_get_theta(self._h, _ptr(out)))
         return out.tolist()
 
@@ -757,6 +833,10 @@ def exported_symbols():
         "binder_dct_prior_eigenvalues",
         "binder_registration_create",
         "binder_registration_destroy", "binder_registration_set_mi_likelihood",
+        "binder_registration_set_ssd_likelihood", "binder_registration_get_ssd_parameters",
+        "binder_registration_set_affine_transformation",
+        "binder_registration_set_rigid_transformation",
+        "binder_registration_get_linear_parameters",
         "binder_registration_set_nonlinear_transformation", "binder_registration_em",
         "binder_registration_sampler", "binder_registration_sampler_run",
         "binder_registration_sampler_result", "binder_registration_get_final_locations",

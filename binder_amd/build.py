@@ -30,6 +30,7 @@ SOURCES = {
     "resample.hip": ["-ffp-contract=off"],
     "warmstart.hip": [],
     "autotune.hip": [],
+    "models.hip": [],
 }
 
 

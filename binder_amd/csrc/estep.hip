@@ -169,10 +169,67 @@ __device__ __forceinline__ void slice_columns(const uint32_t q[16], int xa,
     s[yb] = xa == 0 ? q[yb] : (xa == 1 ? q[4 + yb] : (xa == 2 ? q[8 + yb] : q[12 + yb]));
 }
 
+// SSD likelihood of one voxel (fixed bin u) as node weights relative to its
+// closest node: with d_v = a - s v (a = u - shift, s = scale; a = u, s = 1
+// without shift and scale) and r the positive-weight node bin of smallest
+// |d|, exp(ll(v) - ll(r)) = exp(h s (v - r)(2a - s (v + r))), h = 1/(2 sigma^2):
+// the difference of squares d_r^2 - d_v^2 in factored form, so its rounding
+// is relative to the exponent itself, not to ll(v) (which can be ~1e9 when
+// sigma^2 is small).  The exponent is at most 0 up to rounding and is
+// clamped there, so a zero-weight node closer than r adds exactly 0.
+struct SsdParams {
+  float s, c1;              // scale, h * scale
+  double shift, scale, h, logc;
+};
+__device__ __forceinline__ SsdParams ssd_params(const EStepArgs &a) {
+  SsdParams p;
+  const double s2 = a.ssd_par[0];
+  p.shift = a.ssd_ss ? a.ssd_par[1] : 0.0;
+  p.scale = a.ssd_ss ? a.ssd_par[2] : 1.0;
+  p.h = 0.5 / s2;
+  p.logc = -log(sqrt(2.0 * M_PI * s2));  // log_constant (SSDLikelihood.cpp:55)
+  p.s = (float)p.scale;
+  p.c1 = (float)(p.h * p.scale);
+  return p;
+}
+struct SsdVoxel {
+  float av2, s, c1, rf;
+  __device__ __forceinline__ float w(int v) const {
+    const float vf = (float)v;
+    return __expf(fminf(c1 * (vf - rf) * fmaf(-s, vf + rf, av2), 0.f));
+  }
+};
+// r over the nodes with positive spline weight (columns c = 4 xa + yb of
+// packed bins q, z weights wz); also returns ll(r) / ln 2 in FP64.
+template <int NZ>
+__device__ __forceinline__ SsdVoxel ssd_voxel(const SsdParams &p, int u, const float wx[4],
+                                              const float wy[4], const float wz[4],
+                                              const uint32_t q[16], double &llr2) {
+  const double ad = (double)u - p.shift;
+  const float av = (float)ad;
+  float best = INFINITY;
+  int r = 0;
+#pragma unroll
+  for (int c = 0; c < 16; ++c) {
+    const bool col = wx[c >> 2] * wy[c & 3] > 0.f;
+#pragma unroll
+    for (int zc = 0; zc < NZ; ++zc) {
+      const int v = (q[c] >> (8 * zc)) & 0xFF;
+      const float dist = fabsf(fmaf(-p.s, (float)v, av));
+      const bool take = col && wz[zc] > 0.f && dist < best;
+      best = take ? dist : best;
+      r = take ? v : r;
+    }
+  }
+  const double dr = ad - p.scale * (double)r;
+  llr2 = (p.logc - p.h * dr * dr) * 1.4426950408889634074;  // / ln 2
+  return SsdVoxel{2.f * av, p.s, p.c1, (float)r};
+}
+
 // The 64 (3D) or 16 (2D) neighbour weights are evaluated twice -- once for
 // the normaliser and the mean, once for the histogram / the draw -- one
 // x-slice (16 or 4 nodes) at a time, so only a slice's packed bins are live.
-template <bool IS2D, bool SAMPLE>
+template <bool IS2D, bool SAMPLE, bool SSD>
 __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
   // Philox key, with the per-chain device mask when given (captured sweeps)
   const uint32_t key0 = a.keyx ? a.k0 ^ a.keyx[0] : a.k0;
@@ -188,8 +245,10 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
   const int Kp = theta_row_stride(a.K, a.L);
   for (int i = threadIdx.x; i < KL; i += blockDim.x) {
     hist[i] = 0ull;
-    th[(i / a.K) * Kp + (i % a.K)] = a.thT[i];
+    if (!SSD) th[(i / a.K) * Kp + (i % a.K)] = a.thT[i];
   }
+  SsdParams sp{};
+  if constexpr (SSD) sp = ssd_params(a);
   __syncthreads();
 
   constexpr int NZ = IS2D ? 1 : 4;
@@ -220,6 +279,14 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
 
     uint32_t qall[16];
     load_columns<IS2D>(a, ix, iy, iz, qall);
+    // likelihood of moving bin v at this voxel: the table, or the SSD weight
+    SsdVoxel sv{};
+    double llr2 = 0.0;
+    if constexpr (SSD) sv = ssd_voxel<NZ>(sp, u, wx, wy, wz, qall, llr2);
+    auto lik = [&](int v) -> float {
+      if constexpr (SSD) return sv.w(v);
+      else return thu[v];
+    };
 
     // pass 1: normaliser and first moments; the sampler also keeps the 16
     // (x,y) column sums for its two-level draw, the EM path walks the
@@ -237,7 +304,7 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
           const uint32_t qq = qall[xa * 4 + yb];
           float zy = 0.f;
 #pragma unroll
-          for (int zc = 0; zc < NZ; ++zc) zy += wxy * wz[zc] * thu[(qq >> (8 * zc)) & 0xFF];
+          for (int zc = 0; zc < NZ; ++zc) zy += wxy * wz[zc] * lik((qq >> (8 * zc)) & 0xFF);
           col[xa * 4 + yb] = zy;
           zx += zy;
         }
@@ -256,7 +323,7 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
           float zy = 0.f;
 #pragma unroll
           for (int zc = 0; zc < NZ; ++zc) {
-            const float wj = wxy * wz[zc] * thu[(q[yb] >> (8 * zc)) & 0xFF];
+            const float wj = wxy * wz[zc] * lik((q[yb] >> (8 * zc)) & 0xFF);
             zy += wj;
             if (!IS2D) sz += wj * (float)(zc - 1);
           }
@@ -272,7 +339,7 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
     unsigned long long hval = 0ull;
     if (!SAMPLE) {
       if (valid) {
-        mll += (double)__log2f(Z);
+        mll += (double)__log2f(Z) + llr2;
         const float invZ = 1.0f / Z;
         a.dream[i] = (float)ix + sx * invZ;
         a.dream[a.nv + i] = (float)iy + sy * invZ;
@@ -294,7 +361,7 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
 #pragma unroll
             for (int zc = 0; zc < NZ; ++zc) {
               const int v = (q[yb] >> (8 * zc)) & 0xFF;
-              const float wj = wxy * wz[zc] * thu[v];
+              const float wj = wxy * wz[zc] * lik(v);
               if (v == cur) {
                 acc += wj;
               } else {
@@ -357,7 +424,7 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
 #pragma unroll
         for (int zc = 0; zc < NZ; ++zc) {
           const int v = (qsel >> (8 * zc)) & 0xFF;
-          const float wj = wxy * wz[zc] * thu[v];
+          const float wj = wxy * wz[zc] * lik(v);
           cz += wj;
           if (wj > 0.f) {
             lastz = zc;
@@ -424,7 +491,7 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
 // counting columns below it, then the node inside the column with partial
 // sums formed by the same operations, so the scan always ends inside the
 // selected column.  LDS: theta rows (padded stride) | 32-bit counts [K*L].
-template <bool IS2D>
+template <bool IS2D, bool SSD>
 __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
   // Philox key, with the per-chain device mask when given (captured sweeps)
   const uint32_t key0 = a.keyx ? a.k0 ^ a.keyx[0] : a.k0;
@@ -436,8 +503,10 @@ __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
   unsigned int *cnt = reinterpret_cast<unsigned int *>(th + ((a.L * Kp + 3) & ~3));
   for (int i = threadIdx.x; i < KL; i += blockDim.x) {
     cnt[i] = 0u;
-    th[(i / a.K) * Kp + (i % a.K)] = a.thT[i];
+    if (!SSD) th[(i / a.K) * Kp + (i % a.K)] = a.thT[i];
   }
+  SsdParams sp{};
+  if constexpr (SSD) sp = ssd_params(a);
   __syncthreads();
 
   constexpr int NZ = IS2D ? 1 : 4;
@@ -483,14 +552,23 @@ __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
       if (!IS2D) spline3(tz, wz);
       uint32_t q[16];
       load_columns32<IS2D>(a, ix, iy, iz, q);
+      SsdVoxel sv{};
+      if constexpr (SSD) {
+        double llr2;
+        sv = ssd_voxel<NZ>(sp, u, wx, wy, wz, q, llr2);
+      }
+      auto lik = [&](int v) -> float {
+        if constexpr (SSD) return sv.w(v);
+        else return thu[v];
+      };
 
       float cum[16];
       float acc = 0.f;
 #pragma unroll
       for (int c = 0; c < 16; ++c) {
-        float zy = wz[0] * thu[q[c] & 0xFF];
+        float zy = wz[0] * lik(q[c] & 0xFF);
 #pragma unroll
-        for (int zc = 1; zc < NZ; ++zc) zy = fmaf(wz[zc], thu[(q[c] >> (8 * zc)) & 0xFF], zy);
+        for (int zc = 1; zc < NZ; ++zc) zy = fmaf(wz[zc], lik((q[c] >> (8 * zc)) & 0xFF), zy);
         acc = fmaf(wx[c >> 2] * wy[c & 3], zy, acc);
         cum[c] = acc;
       }
@@ -518,11 +596,11 @@ __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
       // node inside the column, partial sums with the column's own operations
       int zsel = 0;
       if (!IS2D) {
-        float zy = wz[0] * thu[qs & 0xFF];
+        float zy = wz[0] * lik(qs & 0xFF);
 #pragma unroll
         for (int zc = 1; zc < NZ; ++zc) {
           zsel += fmaf(wxy, zy, before) < target ? 1 : 0;
-          zy = fmaf(wz[zc], thu[(qs >> (8 * zc)) & 0xFF], zy);
+          zy = fmaf(wz[zc], lik((qs >> (8 * zc)) & 0xFF), zy);
         }
       }
       if (Z > 0.f) {
@@ -988,12 +1066,17 @@ void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
   if (sample) {
     const size_t ldss = (size_t)((Kp * a.L + 3) & ~3) * sizeof(float) +
                         (size_t)a.K * a.L * sizeof(unsigned int);
-    if (is2d) hipLaunchKernelGGL(k_esample<true>, g, b, ldss, s, a);
-    else hipLaunchKernelGGL(k_esample<false>, g, b, ldss, s, a);
+    if (a.ssd_par) {
+      if (is2d) hipLaunchKernelGGL((k_esample<true, true>), g, b, ldss, s, a);
+      else hipLaunchKernelGGL((k_esample<false, true>), g, b, ldss, s, a);
+    } else {
+      if (is2d) hipLaunchKernelGGL((k_esample<true, false>), g, b, ldss, s, a);
+      else hipLaunchKernelGGL((k_esample<false, false>), g, b, ldss, s, a);
+    }
     HIP_LAUNCH_OK();
     return;
   }
-  if (a.K <= kEm32MaxK) {
+  if (a.K <= kEm32MaxK && !a.ssd_par) {
     const size_t lds32 = estep_em32_lds(a.K, a.L);
     static std::atomic<bool> attr_set[2] = {false, false};
     auto k = is2d ? k_estep_em32<true> : k_estep_em32<false>;
@@ -1007,12 +1090,12 @@ void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
     HIP_LAUNCH_OK();
     return;
   }
-  if (is2d) {
-    if (sample) hipLaunchKernelGGL((k_estep<true, true>), g, b, lds, s, a);
-    else hipLaunchKernelGGL((k_estep<true, false>), g, b, lds, s, a);
+  if (a.ssd_par) {
+    if (is2d) hipLaunchKernelGGL((k_estep<true, false, true>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((k_estep<false, false, true>), g, b, lds, s, a);
   } else {
-    if (sample) hipLaunchKernelGGL((k_estep<false, true>), g, b, lds, s, a);
-    else hipLaunchKernelGGL((k_estep<false, false>), g, b, lds, s, a);
+    if (is2d) hipLaunchKernelGGL((k_estep<true, false, false>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((k_estep<false, false, false>), g, b, lds, s, a);
   }
   HIP_LAUNCH_OK();
 }

@@ -52,6 +52,16 @@ struct EStepArgs {
   // logical blocks of the EM E-step of this registration (batched launches;
   // a single launch uses its grid size)
   int nblocks;
+  // SSD likelihood (BINDER/src/cpu/SSDLikelihood.cpp:58-85) in place of the
+  // table thT: ssd_par = (sigma^2, shift, scale) on the device, ssd_ss the
+  // scale_shift flag.  Each voxel's node weights are B_j exp(ll_j - ll_r),
+  // ll_r the log-likelihood of its closest (positive-weight) node, with
+  // ll_j - ll_r formed as a difference of squares, so posteriors are exact
+  // however peaked the likelihood (the reference's log-space E-step,
+  // Registration.cpp:332-440); mll adds ll_r back in FP64.  The EM E-step
+  // then runs k_estep (FP64 mll partials) for any K.
+  const double *ssd_par;
+  int ssd_ss;
 };
 
 void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,

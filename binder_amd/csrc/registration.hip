@@ -21,6 +21,7 @@
 
 #include "autotune.hpp"
 #include "estep.hpp"
+#include "models.hpp"
 #include "smoothing.hpp"
 #include "topology.hpp"
 #include "resample.hpp"
@@ -287,8 +288,21 @@ struct binder_registration {
   // shapes the autotuner may change; neither changes any result), and the
   // histogram partial rows the last E-step wrote
   int estep_blocks = 0, sample_blocks = 0, hist_rows = 0;
-  // transformation
+  // likelihood model: 0 MI (theta table), 1 SSD (theta[0..3] hold
+  // sigma^2, shift, scale and the draw flag, which the E-step kernels read
+  // directly; thT is unused)
+  int lik = 0;
+  int ssd_ss = 0;
+  double ssd_a0 = 0.0, ssd_b0 = 0.0;
+  double ssd_given[2] = {1.0, 0.0};  // SSD scale, shift as set (fixed when scale_shift == 0)
+  // transformation: the DCT smoother, or an affine / rigid model
   std::unique_ptr<Smoother> sm;
+  std::unique_ptr<LinearModel> lin;
+  // the transformation step: out = constrained / sampled field of `dream`
+  void transform(const float *dream_p, float *out, const SmoothOptions &o) {
+    if (lin) lin->run(dream_p, x.get(), out, o);
+    else sm->run(dream_p, x.get(), out, o, tm());
+  }
   // sampler
   DBuf<double> mean, m2;
   int samples = -1;
@@ -300,7 +314,7 @@ struct binder_registration {
   void require_ready() {
     if (!has_mi)
       throw Error(BINDER_ESTATE, "setMILikelihood must be called first");
-    if (!sm)
+    if (!sm && !lin)
       throw Error(BINDER_ESTATE, "setNonLinearTransformation must be called first");
   }
 
@@ -325,6 +339,8 @@ struct binder_registration {
     a.k0 = k0;
     a.k1 = k1;
     a.sweep = sweep;
+    a.ssd_par = lik == 1 ? theta.get() : nullptr;
+    a.ssd_ss = ssd_ss;
     (void)sample;
     return a;
   }
@@ -349,7 +365,12 @@ struct binder_registration {
   // every case (exact integer sums) and holds the histogram afterwards.
   void hist_theta(hipStream_t st, int mode, uint32_t k0, uint32_t k1, uint32_t sweep,
                   const uint32_t *keyx = nullptr, double *theta_out = nullptr,
-                  float *thT_out = nullptr, double *stats_out = nullptr) {
+                  float *thT_out = nullptr, double *stats_out = nullptr,
+                  const double *theta_in = nullptr) {
+    if (lik == 1) {
+      ssd_theta(st, mode, k0, k1, sweep, keyx, theta_out, thT_out, stats_out, theta_in);
+      return;
+    }
     ThetaArgs t = theta_args(mode, k0, k1, sweep, keyx, theta_out, thT_out, stats_out);
     const int rows2 = (hist_rows + kHistReduceRows - 1) / kHistReduceRows;
     {
@@ -389,7 +410,7 @@ struct binder_registration {
   // sampling E-step's draws depend on the voxel index only, so every
   // candidate gives the same results.
   void tune_estep_em() {
-    if (!has_mi || !estep_em32_supported(K) || nv < (1ll << 18)) return;
+    if (!has_mi || lik != 0 || !estep_em32_supported(K) || nv < (1ll << 18)) return;
     // half, the same and twice the default number of blocks
     const int d = estep_blocks;
     int c[3], nc = 0;
@@ -432,11 +453,60 @@ struct binder_registration {
     else hist_reduce_on(s());
   }
 
+  // The SSD parameter step in place of the theta step: the same histogram
+  // reduction, then mll, the parameter update / draw of `mode` (theta modes:
+  // keep, MAP update, Dirichlet draw) and the prior cost.
+  // With `fresh` false the histogram is Hd as it stands (the last step's,
+  // which every path leaves there) instead of the last E-step's partials.
+  void ssd_theta(hipStream_t st, int mode, uint32_t k0, uint32_t k1, uint32_t sweep,
+                 const uint32_t *keyx, double *theta_out, float *thT_out,
+                 double *stats_out, const double *theta_in, bool fresh = true) {
+    SsdThetaArgs a{};
+    a.Hd = Hd.get();
+    const int rows2 = (hist_rows + kHistReduceRows - 1) / kHistReduceRows;
+    if (fresh) {
+      TimedLaunch tl(tm(), "hist_reduce", st);
+      if (hist_rows <= kThetaDirectRows) {
+        a.hsum = hpart.get();
+        a.hrows = hist_rows;
+      } else if (rows2 <= kThetaDirectRows) {
+        launch_hist_reduce1(hpart.get(), hist_rows, K * L, hpart2.get(), st);
+        a.hsum = hpart2.get();
+        a.hrows = rows2;
+      } else {
+        launch_hist_reduce(hpart.get(), hist_rows, K * L, hpart2.get(), Hd.get(), st);
+      }
+    }
+    a.mllpart = mllpart.get();
+    a.nmll = hist_rows;
+    a.mll_fixed = 0;  // the normalised E-step's FP64 partials
+    a.par_in = theta_in ? theta_in : theta.get();
+    a.par_out = theta_out ? theta_out : theta.get();
+    (void)thT_out;
+    a.bins = K;
+    a.shift_scale = ssd_ss;
+    a.mode = mode == kThetaMap ? kSsdUpdate : (mode == kThetaDirichlet ? kSsdDraw : kSsdKeep);
+    a.a0 = ssd_a0;
+    a.b0 = ssd_b0;
+    a.stats = stats_out ? stats_out : stats.get();
+    a.k0 = k0;
+    a.k1 = k1;
+    a.sweep = sweep;
+    a.keyx = keyx;
+    TimedLaunch tl(tm(), mode == kThetaDirichlet ? "dirichlet" : "theta", st);
+    launch_ssd_theta(a, st);
+  }
+
   void run_theta(int mode, uint32_t k0, uint32_t k1, uint32_t sweep,
                  hipStream_t st = nullptr, const uint32_t *keyx = nullptr,
                  double *theta_out = nullptr, float *thT_out = nullptr,
                  double *stats_out = nullptr) {
     if (!st) st = s();
+    if (lik == 1) {  // the histogram as it stands
+      ssd_theta(st, mode, k0, k1, sweep, keyx, theta_out, thT_out, stats_out, nullptr,
+                false);
+      return;
+    }
     const ThetaArgs t = theta_args(mode, k0, k1, sweep, keyx, theta_out, thT_out, stats_out);
     TimedLaunch tl(tm(), mode == kThetaDirichlet ? "dirichlet" : "theta", st);
     launch_theta(t, st);
@@ -532,7 +602,7 @@ struct binder_registration {
         o.stream = kStreamDeform;
         o.want_prior = false;  // the sampler reports no objective
         const int next = (dslot + 1) % ring;
-        sm->run(dream.get(), x.get(), slot(next), o, tm());
+        transform(dream.get(), slot(next), o);
         dslot = next;
       }
       if (sweep >= N_b) {
@@ -774,6 +844,95 @@ void binder_registration_destroy(binder_registration *reg) {
   }
 }
 
+}  // extern "C"
+
+namespace {
+
+// The likelihood set-up shared by the MI and SSD models: the fixed bins
+// (bytes, [0, L)) and the moving bins (bytes packed four z per word, [0, K))
+// on the device, the histogram buffers for K x L bins, and the fixed-bin
+// counts on the host when `ucount` is given.  Leaves the uploads queued on
+// the handle's stream (the caller synchronizes).
+void set_bins(binder_registration *r, int K, int L, const int *binned_nodes,
+              const int *binned_voxels) {
+  BINDER_REQUIRE(binned_nodes && binned_voxels, "null input array");
+  BINDER_REQUIRE(K >= 1 && K <= 256 && L >= 1 && L <= 256 && K * L <= 4096,
+                 "K and L must be in [1,256] with K*L <= 4096");
+  // the fixed and (3D) moving bins go up as bytes, converted by host
+  // threads straight into page-locked memory and copied by one DMA each
+  const long long nn8 = r->is2d ? 0 : r->nn;
+  std::lock_guard<std::mutex> lock(g_upload_mu);
+  uint8_t *pin = pinned_upload((size_t)(r->nv + nn8));
+  std::atomic<bool> bad_v{false}, bad_n{false};
+  parallel_ranges(r->nv + nn8, [&](long long b0, long long b1) {
+    bool bv_ = false, bn_ = false;
+    for (long long i = b0; i < b1; ++i) {
+      if (i < r->nv) {
+        const int b = binned_voxels[i];
+        bv_ |= b < 0 || b >= L;
+        pin[i] = (uint8_t)b;
+      } else {
+        const int b = binned_nodes[i - r->nv];
+        bn_ |= b < 0 || b >= K;
+        pin[i] = (uint8_t)b;
+      }
+    }
+    if (bv_) bad_v = true;
+    if (bn_) bad_n = true;
+  });
+  BINDER_REQUIRE(!bad_v, "binned_voxels value outside [0, L)");
+  BINDER_REQUIRE(!bad_n, "binned_nodes value outside [0, K)");
+  r->K = K;
+  r->L = L;
+  r->bv.alloc((size_t)r->nv);
+  HIP_OK(hipMemcpyAsync(r->bv.get(), pin, (size_t)r->nv, hipMemcpyHostToDevice, r->s()));
+  const int Xp = r->nxn + 2, Yp = r->nyn + 2;
+  // the E-step indexes the padded moving grid with 32-bit offsets and
+  // 24-bit multiplies
+  const long long slab = r->is2d ? (long long)Yp : (long long)Yp * (r->nzn + 5);
+  BINDER_REQUIRE(slab < (1ll << 24) && (long long)Xp * slab < (1ll << 30),
+                 "moving grid too large for the E-step's 32-bit indexing");
+  auto node_bin = [K](int b) {
+    BINDER_REQUIRE(b >= 0 && b < K, "binned_nodes value outside [0, K)");
+    return (uint8_t)b;
+  };
+  if (r->is2d) {
+    std::vector<uint8_t> p((size_t)Xp * Yp, 0);
+    for (int x = 0; x < r->nxn; ++x)
+      for (int y = 0; y < r->nyn; ++y)
+        p[(size_t)(x + 1) * Yp + (y + 1)] = node_bin(binned_nodes[(long long)x * r->nyn + y]);
+    r->bins2d.upload(p.data(), p.size(), r->s());
+    HIP_OK(hipStreamSynchronize(r->s()));
+  } else {
+    // word zq of a column packs the bins of z = zq - 4 .. zq - 1 (byte c =
+    // z - zq + 4), zero outside the grid: the bins go up as bytes and
+    // k_pack_quads lays the words out on the device
+    const int Zq = r->nzn + 5;
+    r->nb8.alloc((size_t)r->nn);
+    HIP_OK(hipMemcpyAsync(r->nb8.get(), pin + r->nv, (size_t)r->nn, hipMemcpyHostToDevice,
+                          r->s()));
+    const long long nq = (long long)Xp * Yp * Zq;
+    r->quads.alloc((size_t)nq);
+    hipLaunchKernelGGL(k_pack_quads, dim3(grid_blocks(nq)), dim3(256), 0, r->s(),
+                       r->nb8.get(), r->nxn, r->nyn, r->nzn, r->quads.get());
+    HIP_LAUNCH_OK();
+  }
+  r->Hd.alloc((size_t)K * L);
+  r->Hd.zero(r->s());
+  r->rowbuf.alloc((size_t)K * L + K + 64);
+  {
+    const int rows = std::max(r->estep_blocks, r->sample_blocks);
+    r->hpart.alloc((size_t)rows * K * L);
+    r->hpart2.alloc((size_t)((rows + 15) / 16) * K * L);
+  }
+  // the page-locked bins must have gone up before the lock is released
+  HIP_OK(hipStreamSynchronize(r->s()));
+}
+
+}  // namespace
+
+extern "C" {
+
 int binder_registration_set_mi_likelihood(binder_registration *r, double alpha,
                                           const double *theta, int K, int L,
                                           int threads, const int *binned_nodes,
@@ -781,84 +940,92 @@ int binder_registration_set_mi_likelihood(binder_registration *r, double alpha,
   (void)threads;
   return guarded([&] {
     BINDER_REQUIRE(r, "null handle");
-    BINDER_REQUIRE(theta && binned_nodes && binned_voxels, "null input array");
-    BINDER_REQUIRE(K >= 1 && K <= 256 && L >= 1 && L <= 256 && K * L <= 4096,
-                   "K and L must be in [1,256] with K*L <= 4096");
-    // the fixed and (3D) moving bins go up as bytes, converted by host
-    // threads straight into page-locked memory and copied by one DMA each
-    const long long nn8 = r->is2d ? 0 : r->nn;
-    std::lock_guard<std::mutex> lock(g_upload_mu);
-    uint8_t *pin = pinned_upload((size_t)(r->nv + nn8));
-    std::atomic<bool> bad_v{false}, bad_n{false};
-    parallel_ranges(r->nv + nn8, [&](long long b0, long long b1) {
-      bool bv_ = false, bn_ = false;
-      for (long long i = b0; i < b1; ++i) {
-        if (i < r->nv) {
-          const int b = binned_voxels[i];
-          bv_ |= b < 0 || b >= L;
-          pin[i] = (uint8_t)b;
-        } else {
-          const int b = binned_nodes[i - r->nv];
-          bn_ |= b < 0 || b >= K;
-          pin[i] = (uint8_t)b;
-        }
-      }
-      if (bv_) bad_v = true;
-      if (bn_) bad_n = true;
-    });
-    BINDER_REQUIRE(!bad_v, "binned_voxels value outside [0, L)");
-    BINDER_REQUIRE(!bad_n, "binned_nodes value outside [0, K)");
-    r->K = K;
-    r->L = L;
+    BINDER_REQUIRE(theta, "null input array");
+    set_bins(r, K, L, binned_nodes, binned_voxels);
     r->alpha = alpha;
-    r->bv.alloc((size_t)r->nv);
-    HIP_OK(hipMemcpyAsync(r->bv.get(), pin, (size_t)r->nv, hipMemcpyHostToDevice, r->s()));
-    const int Xp = r->nxn + 2, Yp = r->nyn + 2;
-    // the E-step indexes the padded moving grid with 32-bit offsets and
-    // 24-bit multiplies
-    const long long slab = r->is2d ? (long long)Yp : (long long)Yp * (r->nzn + 5);
-    BINDER_REQUIRE(slab < (1ll << 24) && (long long)Xp * slab < (1ll << 30),
-                   "moving grid too large for the E-step's 32-bit indexing");
-    auto node_bin = [K](int b) {
-      BINDER_REQUIRE(b >= 0 && b < K, "binned_nodes value outside [0, K)");
-      return (uint8_t)b;
-    };
-    if (r->is2d) {
-      std::vector<uint8_t> p((size_t)Xp * Yp, 0);
-      for (int x = 0; x < r->nxn; ++x)
-        for (int y = 0; y < r->nyn; ++y)
-          p[(size_t)(x + 1) * Yp + (y + 1)] = node_bin(binned_nodes[(long long)x * r->nyn + y]);
-      r->bins2d.upload(p.data(), p.size(), r->s());
-      HIP_OK(hipStreamSynchronize(r->s()));
-    } else {
-      // word zq of a column packs the bins of z = zq - 4 .. zq - 1 (byte c =
-      // z - zq + 4), zero outside the grid: the bins go up as bytes and
-      // k_pack_quads lays the words out on the device
-      const int Zq = r->nzn + 5;
-      r->nb8.alloc((size_t)r->nn);
-      HIP_OK(hipMemcpyAsync(r->nb8.get(), pin + r->nv, (size_t)r->nn, hipMemcpyHostToDevice,
-                            r->s()));
-      const long long nq = (long long)Xp * Yp * Zq;
-      r->quads.alloc((size_t)nq);
-      hipLaunchKernelGGL(k_pack_quads, dim3(grid_blocks(nq)), dim3(256), 0, r->s(),
-                         r->nb8.get(), r->nxn, r->nyn, r->nzn, r->quads.get());
-      HIP_LAUNCH_OK();
-    }
+    r->lik = 0;
     std::vector<float> tt((size_t)K * L);
     for (int k = 0; k < K; ++k)
       for (int l = 0; l < L; ++l) tt[(size_t)l * K + k] = (float)theta[(size_t)k * L + l];
     r->theta.upload(theta, (size_t)K * L, r->s());
     r->thT.upload(tt.data(), tt.size(), r->s());
-    r->Hd.alloc((size_t)K * L);
-    r->Hd.zero(r->s());
-    r->rowbuf.alloc((size_t)K * L + K + 64);
-    {
-      const int rows = std::max(r->estep_blocks, r->sample_blocks);
-      r->hpart.alloc((size_t)rows * K * L);
-      r->hpart2.alloc((size_t)((rows + 15) / 16) * K * L);
-    }
     HIP_OK(hipStreamSynchronize(r->s()));
     r->has_mi = true;
+  });
+}
+
+int binder_registration_set_ssd_likelihood(binder_registration *r, int bins, int threads,
+                                           double sigma_sq, int scale_shift, double scale,
+                                           double shift, const int *binned_nodes,
+                                           const int *binned_voxels, double alpha_0,
+                                           double beta_0) {
+  (void)threads;
+  return guarded([&] {
+    BINDER_REQUIRE(r, "null handle");
+    BINDER_REQUIRE(bins >= 2 && bins <= kSsdMaxBins, "SSD bins must be in [2, 64]");
+    BINDER_REQUIRE(sigma_sq > 0.0, "sigma_sq must be positive");
+    BINDER_REQUIRE(scale_shift == 0 || scale_shift == 1, "scale_shift must be 0 or 1");
+    set_bins(r, bins, bins, binned_nodes, binned_voxels);
+    r->lik = 1;
+    r->ssd_ss = scale_shift;
+    r->ssd_a0 = alpha_0;
+    r->ssd_b0 = beta_0;
+    // theta[0..3]: sigma^2, shift, scale, draw flag, in a bins^2 buffer (>= 4
+    // for bins >= 2) so the EM and sampler buffers of theta / thT keep their
+    // sizes
+    std::vector<double> par((size_t)bins * bins, 0.0);
+    par[0] = sigma_sq;
+    par[1] = scale_shift ? shift : 0.0;
+    par[2] = scale_shift ? scale : 1.0;
+    r->theta.upload(par.data(), par.size(), r->s());
+    r->thT.alloc((size_t)bins * bins);
+    // the scale and shift as given are what get_likelihood_parameters
+    // returns until an update (SSDLikelihood.cpp:326-333)
+    r->ssd_given[0] = scale;
+    r->ssd_given[1] = shift;
+    HIP_OK(hipStreamSynchronize(r->s()));
+    r->has_mi = true;
+  });
+}
+
+int binder_registration_get_ssd_parameters(binder_registration *r, double *out3) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && out3, "null argument");
+    if (r->lik != 1) throw Error(BINDER_ESTATE, "setSSDLikelihood was not called");
+    double par[kSsdParams];
+    r->theta.download(par, kSsdParams, r->s());
+    HIP_OK(hipStreamSynchronize(r->s()));
+    out3[0] = r->ssd_ss ? par[2] : r->ssd_given[0];
+    out3[1] = r->ssd_ss ? par[1] : r->ssd_given[1];
+    out3[2] = par[0];
+  });
+}
+
+int binder_registration_set_affine_transformation(binder_registration *r, const double *A,
+                                                  const double *t) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && A && t, "null argument");
+    r->lin = std::make_unique<LinearModel>(kLinAffine, r->N, r->nv, r->x.get(), A, t, r->s());
+    r->sm.reset();
+  });
+}
+
+int binder_registration_set_rigid_transformation(binder_registration *r, const double *R,
+                                                 const double *t) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && R && t, "null argument");
+    r->lin = std::make_unique<LinearModel>(kLinRigid, r->N, r->nv, r->x.get(), R, t, r->s());
+    r->sm.reset();
+  });
+}
+
+int binder_registration_get_linear_parameters(binder_registration *r, double *out,
+                                              int *count) {
+  return guarded([&] {
+    BINDER_REQUIRE(r && out, "null argument");
+    if (!r->lin) throw Error(BINDER_ESTATE, "no affine or rigid transformation was set");
+    const int n = r->lin->get(out);
+    if (count) *count = n;
   });
 }
 
@@ -875,6 +1042,7 @@ int binder_registration_set_nonlinear_transformation(
     r->sm = std::make_unique<Smoother>(r->nxv, r->nyv, r->nzv, g, rho, eta, D,
                                        std::sqrt(9.0 / (8.0 * M_PI)), r->s());
     r->sm->set_index_grid(r->x_index_grid);
+    r->lin.reset();  // the DCT prior replaces an affine / rigid model
   });
 }
 
@@ -901,6 +1069,7 @@ int binder_registration_set_nonlinear_transformation_dct_prior(
     r->sm = std::make_unique<Smoother>(r->nxv, r->nyv, r->nzv, g, rho, eta, t.data(), order,
                                        std::sqrt(9.0 / (8.0 * M_PI)), r->s());
     r->sm->set_index_grid(r->x_index_grid);
+    r->lin.reset();  // the DCT prior replaces an affine / rigid model
   });
 }
 
@@ -938,20 +1107,21 @@ int binder_registration_em(binder_registration *r, int max_it, double th,
       EStepArgs a = r->estep_args(false, 0, 0, 0);
       a.d = F[p];
       a.thT = TT[p];
+      if (a.ssd_par) a.ssd_par = TH[p];
       a.dream = DR[p];
       {
         TimedLaunch t(r->tm(), "estep", r->s());
         launch_estep(a, r->is2d, false, r->estep_blocks, r->s());
       }
       r->hist_theta(r->s(), update ? kThetaMap : kThetaKeep, 0, 0, 0, nullptr,
-                    TH[update ? q : p], TT[update ? q : p], ST[p]);
+                    TH[update ? q : p], TT[update ? q : p], ST[p], TH[p]);
       SmoothOptions opt;
       opt.prior_out = ST[p] + 2;
       // the prior cost's sum also writes the three objective terms to the
       // page-locked host buffer (no copy launch)
       opt.host_stats = host + 8 * p;
       opt.stats_in = ST[p];
-      r->sm->run(DR[p], r->x.get(), F[q], opt, r->tm());
+      r->transform(DR[p], F[q], opt);
       HIP_OK(hipEventRecord(r->em_done[p].e, r->s()));
     };
     double mlp_old = 1e100;
@@ -1003,6 +1173,8 @@ void batch_fill(binder_batch *b, int p, binder_registration *r, int cap, int upd
   if (!r) return;
   r->require_ready();
   BINDER_REQUIRE(estep_em32_supported(r->K), "batched EM needs K <= 32 moving bins");
+  BINDER_REQUIRE(r->lik == 0 && !r->lin && r->sm,
+                 "batched EM runs the MI likelihood with the DCT prior");
   if (b->K < 0) {
     b->K = r->K;
     b->L = r->L;
@@ -1661,6 +1833,15 @@ int binder_registration_get_likelihood_parameters(binder_registration *r,
   return guarded([&] {
     BINDER_REQUIRE(r && theta_out, "null argument");
     if (!r->has_mi) throw Error(BINDER_ESTATE, "setMILikelihood was not called");
+    if (r->lik == 1) {
+      // SSDLikelihood::get_parameters (:326-333): [scale, shift]
+      double p3[3];
+      const int rc = binder_registration_get_ssd_parameters(r, p3);
+      if (rc != BINDER_OK) throw Error(rc, binder_last_error());
+      theta_out[0] = p3[0];
+      theta_out[1] = p3[1];
+      return;
+    }
     r->theta.download(theta_out, (size_t)r->K * r->L, r->s());
     HIP_OK(hipStreamSynchronize(r->s()));
   });

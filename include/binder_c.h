@@ -96,6 +96,41 @@ int binder_registration_set_mi_likelihood(binder_registration *reg,
                                           const int *binned_nodes,
                                           const int *binned_voxels);
 
+/* setSSDLikelihood(bins, threads, sigma_sq, scale_shift, scale, shift,
+ *                  binned_nodes, binned_voxels, alpha_0, beta_0)
+ * -- BINDER/src/cpu/bindings.cpp:120-152, BINDER/src/cpu/Registration.cpp:472-479,
+ * BINDER/src/cpu/SSDLikelihood.cpp.  Both images binned to [0, bins), bins in
+ * [2, 64]; scale_shift 0 compares the bins directly, 1 fits shift and scale.
+ * The E-step, EM loop and sampler run as with the MI likelihood;
+ * get_likelihood_parameters then returns [scale, shift]. */
+int binder_registration_set_ssd_likelihood(binder_registration *reg, int bins,
+                                           int threads, double sigma_sq, int scale_shift,
+                                           double scale, double shift,
+                                           const int *binned_nodes,
+                                           const int *binned_voxels, double alpha_0,
+                                           double beta_0);
+/* No counterpart in the reference (its SSD parameters are printed only):
+ * (scale, shift, sigma_sq) of the SSD likelihood as they stand. */
+int binder_registration_get_ssd_parameters(binder_registration *reg, double *out3);
+
+/* setAffineTransformation(A, t) -- BINDER/src/cpu/bindings.cpp:223-235,
+ * BINDER/src/cpu/Registration.cpp:507-510, AffineTransformation.cpp:25-130.
+ * A is (N+1) x (N+1) row-major (homogeneous), t has N entries; both are the
+ * starting values, which the least-squares fit of each EM iteration
+ * replaces.  Replaces the DCT prior as the transformation model. */
+int binder_registration_set_affine_transformation(binder_registration *reg,
+                                                  const double *A, const double *t);
+/* setRigidTransformation(R, t) -- BINDER/src/cpu/bindings.cpp:236-247,
+ * BINDER/src/cpu/Registration.cpp:512-516, RigidTransformation.cpp:57-154.
+ * R is N x N row-major, t has N entries. */
+int binder_registration_set_rigid_transformation(binder_registration *reg,
+                                                 const double *R, const double *t);
+/* No counterpart in the reference (it prints A / R and t): the fitted
+ * matrix (affine (N+1)^2, rigid N^2, row-major) followed by t; *count gets
+ * the number of values written (at most 19). */
+int binder_registration_get_linear_parameters(binder_registration *reg, double *out,
+                                              int *count);
+
 /* setNonLinearTransformation(gamma_x..z, voxels_size_x..z, nodes_size_x..z, D)
  * -- BINDER/src/gpu/bindings.cpp:72-95, BINDER/src/cpu/Registration.cpp:492-502.
  * D is the [Nx_v,Ny_v,Nz_v] DCT-domain eigenvalue array. */

@@ -16,6 +16,17 @@ void *orc_reg_create(const double *final_locations, int threads,
 void orc_reg_destroy(void *h);
 void orc_reg_set_mi(void *h, double alpha, const double *theta, int K, int L,
                     const int *binned_nodes, const int *binned_voxels);
+void orc_reg_set_ssd(void *h, int bins, double sigma_sq, int scale_shift, double scale,
+                     double shift, const int *binned_nodes, const int *binned_voxels,
+                     double alpha_0, double beta_0);
+void orc_reg_get_ssd(void *h, double *out3);
+void orc_reg_set_affine(void *h, const double *A, const double *t);
+void orc_reg_set_rigid(void *h, const double *R, const double *t);
+int orc_reg_get_linear(void *h, double *out);
+double orc_reg_constrain(void *h);
+void orc_reg_ssd_update(void *h, int draw);
+double orc_reg_ssd_cost(void *h);
+void orc_reg_ssd_seed(void *h, int seed);
 void orc_reg_set_nlt(void *h, double gx, double gy, double gz, double rx,
                      double ry, double rz, double ex, double ey, double ez,
                      const double *D);

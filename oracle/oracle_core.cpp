@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <random>
 #include <vector>
@@ -57,6 +58,131 @@ static inline void spline_weights(int order, double d, double w[4]) {
     w[1] = d;
     w[2] = 0.0;
     w[3] = 0.0;
+  }
+}
+
+// log of the spline weights with the reference's sentinels for zero weights
+// (BINDER/src/cpu/Spline.cpp:111-146)
+static inline void log_spline_weights(int order, double d, double o[4]) {
+  o[3] = -1e100;
+  if (order == 3) {
+    o[3] = d < 1e-300 ? -1e100 : 3 * std::log(d) - std::log(6.0);
+    o[0] = 3 * std::log(1.0 - d) - std::log(6.0);
+    o[1] = std::log(2.0 / 3.0 - d * d + std::pow(d, 3) / 2.0);
+    o[2] = std::log(2.0 / 3.0 - std::pow(1.0 - d, 2) + std::pow(1.0 - d, 3) / 2.0);
+  } else if (order == 2) {
+    if (d > 0.5) {
+      o[0] = o[1] = o[2] = -1e100;
+    } else {
+      o[0] = std::log(0.125 * std::pow(-2.0 * d + 1.0, 2));
+      o[1] = std::log(0.75 - d * d);
+      o[2] = std::log(0.125 * std::pow(-2.0 * d - 1.0, 2));
+    }
+  } else {
+    o[0] = d >= 1 ? -1e100 : std::log(1 - d);
+    o[1] = d < 1e-300 ? -1e100 : std::log(d);
+    o[2] = -1e100;
+  }
+}
+
+// Inverse of an n x n matrix (n <= 4) by Gauss-Jordan elimination with
+// partial pivoting (row-major); the reference uses Eigen's inverse().
+static void invert(int n, const double *M, double *Minv) {
+  double a[4][8];
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < 2 * n; ++j) a[i][j] = j < n ? M[i * n + j] : (j - n == i ? 1.0 : 0.0);
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(a[r][c]) > std::fabs(a[p][c])) p = r;
+    if (p != c)
+      for (int j = 0; j < 2 * n; ++j) std::swap(a[p][j], a[c][j]);
+    const double inv = 1.0 / a[c][c];
+    for (int j = 0; j < 2 * n; ++j) a[c][j] *= inv;
+    for (int r = 0; r < n; ++r)
+      if (r != c) {
+        const double f = a[r][c];
+        for (int j = 0; j < 2 * n; ++j) a[r][j] -= f * a[c][j];
+      }
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) Minv[i * n + j] = a[i][j + n];
+}
+
+static double det_n(int n, const double *M) {
+  if (n == 2) return M[0] * M[3] - M[1] * M[2];
+  return M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) +
+         M[2] * (M[3] * M[7] - M[4] * M[6]);
+}
+
+// SVD H = U diag(s) V^T of an n x n matrix (n = 2 or 3, row-major), singular
+// values in decreasing order: V from the cyclic Jacobi eigen-decomposition of
+// H^T H, U = H V / s, and a zero singular value's left vector completed to
+// an orthonormal basis (Eigen's JacobiSVD in the reference).
+static void svd_small(int n, const double *H, double *U, double *s, double *V) {
+  double B[9] = {0}, W[9] = {0};
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      double acc = 0.0;
+      for (int k = 0; k < n; ++k) acc += H[k * n + i] * H[k * n + j];
+      B[i * n + j] = acc;
+      W[i * n + j] = i == j ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) off += B[p * n + q] * B[p * n + q];
+    if (off < 1e-300) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = B[p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        const double theta = (B[q * n + q] - B[p * n + p]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) /
+                         (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < n; ++k) {  // B <- J^T B J
+          const double bkp = B[k * n + p], bkq = B[k * n + q];
+          B[k * n + p] = c * bkp - sn * bkq;
+          B[k * n + q] = sn * bkp + c * bkq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double bpk = B[p * n + k], bqk = B[q * n + k];
+          B[p * n + k] = c * bpk - sn * bqk;
+          B[q * n + k] = sn * bpk + c * bqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double wkp = W[k * n + p], wkq = W[k * n + q];
+          W[k * n + p] = c * wkp - sn * wkq;
+          W[k * n + q] = sn * wkp + c * wkq;
+        }
+      }
+  }
+  int idx[3] = {0, 1, 2};
+  std::sort(idx, idx + n, [&](int a, int b) { return B[a * n + a] > B[b * n + b]; });
+  for (int j = 0; j < n; ++j) {
+    s[j] = std::sqrt(std::max(B[idx[j] * n + idx[j]], 0.0));
+    for (int k = 0; k < n; ++k) V[k * n + j] = W[k * n + idx[j]];
+  }
+  for (int j = 0; j < n; ++j) {
+    double col[3] = {0, 0, 0}, nrm = 0.0;
+    for (int i = 0; i < n; ++i) {
+      for (int k = 0; k < n; ++k) col[i] += H[i * n + k] * V[k * n + j];
+      nrm += col[i] * col[i];
+    }
+    nrm = std::sqrt(nrm);
+    if (nrm > 1e-12 * (s[0] > 0 ? s[0] : 1.0)) {
+      for (int i = 0; i < n; ++i) U[i * n + j] = col[i] / nrm;
+    } else if (n == 3 && j == 2) {  // u3 = u1 x u2
+      U[0 * 3 + 2] = U[1 * 3 + 0] * U[2 * 3 + 1] - U[2 * 3 + 0] * U[1 * 3 + 1];
+      U[1 * 3 + 2] = U[2 * 3 + 0] * U[0 * 3 + 1] - U[0 * 3 + 0] * U[2 * 3 + 1];
+      U[2 * 3 + 2] = U[0 * 3 + 0] * U[1 * 3 + 1] - U[1 * 3 + 0] * U[0 * 3 + 1];
+    } else if (n == 2 && j == 1) {
+      U[0 * 2 + 1] = -U[1 * 2 + 0];
+      U[1 * 2 + 1] = U[0 * 2 + 0];
+    } else {
+      for (int i = 0; i < n; ++i) U[i * n + j] = i == j ? 1.0 : 0.0;
+    }
   }
 }
 
@@ -225,6 +351,15 @@ struct Registration {
   std::mt19937 lk_gen;
   std::unique_ptr<Smoother> sm;
   std::vector<double> trace;  // rows of (mll, prior_theta, prior_d, mlp)
+  // SSD likelihood (SSDLikelihood.h:13-88): lik 1 replaces the MI table
+  int lik = 0;
+  int ssd_shift_scale = 0;
+  double sigma2 = 1.0, shift = 0.0, scale = 1.0, a0 = 0.0, b0 = 0.0, log_constant = 0.0;
+  int ssd_sample = 0;  // set by sample_parameters and never cleared (SSDLikelihood.cpp:291)
+  // affine / rigid transformation (AffineTransformation.cpp,
+  // RigidTransformation.cpp): tmodel 1 or 2 replaces the DCT smoother
+  int tmodel = 0;
+  std::vector<double> A_aff, R_rig, t_lin, x_means;
 
   Registration(const double *fl, int threads_, const double *vp, int nxv_,
                int nyv_, int nzv_, int nxn_, int nyn_, int nzn_, int order_,
@@ -326,11 +461,248 @@ struct Registration {
     return -std::log(Z);
   }
 
+  // compute_log_likelihood (SSDLikelihood.cpp:58-85) for fixed bin u and
+  // the moving bin k of the node (0 outside the grid: the value the
+  // reference uses for an outside node, voxel - 0 or voxel - shift)
+  double ssd_loglik(int u, int k) const {
+    const double d = ssd_shift_scale == 0 ? (double)u - (double)k
+                                           : (double)u - shift - scale * (double)k;
+    return (d * d) / (-2.0 * sigma2) + log_constant;
+  }
+
+  // estimate_posterior_and_dream_location_log (Registration.cpp:332-440),
+  // the E-step of the SSD likelihood (log_space, Registration.cpp:478)
+  double voxel_log(long long vi, bool sample, std::mt19937 *rng, double *Hacc) {
+    const int nb = order + 1;
+    const int ZZ = is2D ? 1 : nb;
+    static const int off3[4] = {-1, 0, 1, 2}, off2[4] = {-1, 0, 1, 0},
+                     off1[4] = {0, 1, 0, 0};
+    const int *off = order == 3 ? off3 : (order == 2 ? off2 : off1);
+    double lx[4], ly[4], lz[4] = {0.0, 0.0, 0.0, 0.0};
+    int ix, iy, iz = 0;
+    {
+      const double o = offset();
+      double t = final_loc[vi * N + 0];
+      ix = (int)std::floor(t + o);
+      log_spline_weights(order, t - std::floor(t + o), lx);
+      t = final_loc[vi * N + 1];
+      iy = (int)std::floor(t + o);
+      log_spline_weights(order, t - std::floor(t + o), ly);
+      if (!is2D) {
+        t = final_loc[vi * N + 2];
+        iz = (int)std::floor(t + o);
+        log_spline_weights(order, t - std::floor(t + o), lz);
+      }
+    }
+    const int u = bv[vi];
+    double lp[64];
+    int bins[64];
+    double mx = -std::numeric_limits<double>::infinity();
+    for (int a = 0; a < nb; ++a)
+      for (int b = 0; b < nb; ++b)
+        for (int c = 0; c < ZZ; ++c) {
+          const int j = (a * nb + b) * ZZ + c;
+          const int tz = is2D ? 0 : iz + off[c];
+          const int k = node_bin(ix + off[a], iy + off[b], tz);
+          bins[j] = k;
+          lp[j] = lx[a] + ly[b] + lz[c] + ssd_loglik(u, k);
+          mx = std::max(mx, lp[j]);
+        }
+    double nrm = 0.0;
+    for (int j = 0; j < nb * nb * ZZ; ++j) nrm += std::exp(lp[j] - mx);
+    const double lnz = std::log(nrm) + mx;
+    if (sample) {
+      std::uniform_real_distribution<double> dis(0.0, 1.0);
+      const double r = dis(*rng);
+      double cum = 0.0;
+      for (int a = 0; a < nb; ++a)
+        for (int b = 0; b < nb; ++b)
+          for (int c = 0; c < ZZ; ++c) {
+            const int j = (a * nb + b) * ZZ + c;
+            cum += std::exp(lp[j] - lnz);
+            if (r <= cum) {
+              Hacc[(long long)bins[j] * L + u] += 1.0;
+              dream[vi * N + 0] = (double)(ix + off[a]);
+              dream[vi * N + 1] = (double)(iy + off[b]);
+              if (!is2D) dream[vi * N + 2] = (double)(iz + off[c]);
+              return -lnz;
+            }
+          }
+    } else {
+      for (int a = 0; a < nb; ++a)
+        for (int b = 0; b < nb; ++b)
+          for (int c = 0; c < ZZ; ++c) {
+            const int j = (a * nb + b) * ZZ + c;
+            const double p = std::exp(lp[j] - lnz);
+            Hacc[(long long)bins[j] * L + u] += p;
+            dream[vi * N + 0] += p * (double)(ix + off[a]);
+            dream[vi * N + 1] += p * (double)(iy + off[b]);
+            if (!is2D) dream[vi * N + 2] += p * (double)(iz + off[c]);
+          }
+    }
+    return -lnz;
+  }
+
+  // The SSD statistics as sums over the soft joint histogram H[k][u] (the
+  // posterior mass of moving bin k at voxels of fixed bin u): the per-node
+  // sums of SSDLikelihood::accumulate_info (:118-150) grouped by bin.
+  //   g = sum post, gk = sum post k, gkk = sum post k^2, gu = sum post u,
+  //   guk = sum post u k, and sq(m0, m1) = sum post (u - m0 - m1 k)^2.
+  void ssd_moments(double &g, double &gk, double &gkk, double &gu, double &guk) const {
+    g = gk = gkk = gu = guk = 0.0;
+    for (int k = 0; k < K; ++k)
+      for (int u = 0; u < L; ++u) {
+        const double h = H[(size_t)k * L + u];
+        g += h;
+        gk += h * k;
+        gkk += h * k * (double)k;
+        gu += h * u;
+        guk += h * u * (double)k;
+      }
+  }
+  double ssd_sq(double m0, double m1) const {
+    double acc = 0.0;
+    for (int k = 0; k < K; ++k)
+      for (int u = 0; u < L; ++u) {
+        const double d = (double)u - m0 - m1 * (double)k;
+        acc += H[(size_t)k * L + u] * d * d;
+      }
+    return acc;
+  }
+
+  // update_parameters / update_shift_and_scale / update_sigma /
+  // sample_parameters (SSDLikelihood.cpp:152-304)
+  void ssd_update(bool draw) {
+    if (draw) ssd_sample = 1;
+    // with shift and scale, update_shift_and_scale / update_sigma draw once
+    // the flag is set (:205, :266); without them update_parameters is the
+    // MAP step whatever the flag (:154-171) and only sample_parameters draws
+    const bool smp = ssd_shift_scale == 1 ? ssd_sample != 0 : draw;
+    double g, gk, gkk, gu, guk;
+    ssd_moments(g, gk, gkk, gu, guk);
+    if (ssd_shift_scale == 1) {
+      // A = [[g, gk], [gk, gkk]], b = [gu, guk] (outside nodes count as k = 0)
+      const double M[4] = {g, gk, gk, gkk};
+      double Minv[4];
+      invert(2, M, Minv);
+      double x0 = Minv[0] * gu + Minv[1] * guk, x1 = Minv[2] * gu + Minv[3] * guk;
+      if (smp) {
+        // (shift, scale) = x + chol(A^-1) z, z ~ N(0, I)
+        std::normal_distribution<double> normal(0.0, 1.0);
+        const double z0 = normal(lk_gen), z1 = normal(lk_gen);
+        const double l00 = std::sqrt(Minv[0]);
+        const double l10 = Minv[2] / l00;
+        const double l11 = std::sqrt(Minv[3] - l10 * l10);
+        x0 += l00 * z0;
+        x1 += l10 * z0 + l11 * z1;
+      }
+      shift = x0;
+      scale = x1;
+    }
+    const double num = (ssd_shift_scale == 1 ? ssd_sq(shift, scale) : ssd_sq(0.0, 1.0)) +
+                       a0 * b0;
+    const double den = g + a0;
+    if (!smp) {
+      sigma2 = num / den;
+    } else {
+      // the reference's draw: 1 / Gamma(shape den / 2, scale 2 den / num)
+      std::gamma_distribution<double> gd(den * 0.5, 2.0 * den / num);
+      sigma2 = 1.0 / gd(lk_gen);
+    }
+  }
+
+  // compute_cost (SSDLikelihood.cpp:308-320): -log InvGamma(sigma^2; a0, a0 b0)
+  double ssd_cost() const {
+    const double a = a0, b = a0 * b0, x = sigma2;
+    return -(a * std::log(b) - std::lgamma(a) + (-a - 1.0) * std::log(x) - b / x);
+  }
+
+  // AffineTransformation::constrain_transformation (:25-130) and
+  // RigidTransformation::constrain_transformation (:57-154): the fitted
+  // transform of the voxel positions, written to final_loc; prior cost 0.
+  double lin_constrain() {
+    const int n1 = N + 1;
+    if (tmodel == 1) {
+      double M[16] = {0}, XY[12] = {0};
+      for (long long v = 0; v < nv; ++v) {
+        double xr[4] = {1.0, 0, 0, 0};
+        for (int n = 0; n < N; ++n) xr[n + 1] = xpos[v * N + n];
+        for (int i = 0; i < n1; ++i) {
+          for (int j = 0; j < n1; ++j) M[i * n1 + j] += xr[i] * xr[j];
+          for (int n = 0; n < N; ++n) XY[i * N + n] += xr[i] * dream[v * N + n];
+        }
+      }
+      double Minv[16];
+      invert(n1, M, Minv);
+      double T[12] = {0};  // (X^T X)^-1 X^T Y, rows 0..N
+      for (int i = 0; i < n1; ++i)
+        for (int n = 0; n < N; ++n)
+          for (int j = 0; j < n1; ++j) T[i * N + n] += Minv[i * n1 + j] * XY[j * N + n];
+      A_aff.assign((size_t)n1 * n1, 0.0);
+      for (int n = 0; n < N; ++n) {
+        t_lin[n] = T[0 * N + n];
+        for (int n2 = 0; n2 < N; ++n2) A_aff[n * n1 + n2] = T[(n2 + 1) * N + n];
+      }
+      A_aff[(size_t)n1 * n1 - 1] = 1.0;
+      for (long long v = 0; v < nv; ++v)
+        for (int n = 0; n < N; ++n) {
+          double acc = t_lin[n];
+          for (int n2 = 0; n2 < N; ++n2) acc += A_aff[n * n1 + n2] * xpos[v * N + n2];
+          final_loc[v * N + n] = acc;
+        }
+    } else {
+      double ym[3] = {0, 0, 0};
+      for (long long v = 0; v < nv; ++v)
+        for (int n = 0; n < N; ++n) ym[n] += dream[v * N + n];
+      for (int n = 0; n < N; ++n) ym[n] /= (double)nv;
+      double Hm[9] = {0};  // H(n, n2) = sum x~[n2] y~[n]
+      for (long long v = 0; v < nv; ++v)
+        for (int n = 0; n < N; ++n) {
+          const double yt = dream[v * N + n] - ym[n];
+          for (int n2 = 0; n2 < N; ++n2) Hm[n * N + n2] += (xpos[v * N + n2] - x_means[n2]) * yt;
+        }
+      double U[9], sv[3], V[9], VUt[9];
+      svd_small(N, Hm, U, sv, V);
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+          double acc = 0.0;
+          for (int k = 0; k < N; ++k) acc += V[i * N + k] * U[j * N + k];
+          VUt[i * N + j] = acc;
+        }
+      const double dsign = det_n(N, VUt);
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+          double acc = 0.0;
+          for (int k = 0; k < N; ++k) acc += V[i * N + k] * (k == N - 1 ? dsign : 1.0) * U[j * N + k];
+          R_rig[i * N + j] = acc;
+        }
+      for (int n = 0; n < N; ++n) {
+        double acc = ym[n];
+        for (int n2 = 0; n2 < N; ++n2) acc -= R_rig[n * N + n2] * x_means[n2];
+        t_lin[n] = acc;
+      }
+      for (long long v = 0; v < nv; ++v)
+        for (int n = 0; n < N; ++n) {
+          double acc = t_lin[n];
+          for (int n2 = 0; n2 < N; ++n2) acc += R_rig[n * N + n2] * xpos[v * N + n2];
+          final_loc[v * N + n] = acc;
+        }
+    }
+    return 0.0;
+  }
+
+  double constrain(bool sample, bool sample_gamma) {
+    if (tmodel != 0) return lin_constrain();  // sample_transformation = constrain
+    return sm->constrain(dream.data(), xpos.data(), final_loc.data(), sample, sample_gamma);
+  }
+
   // The E-step voxel loop (Registration.cpp:144-181 and :551-571).
   // Histogram and mll are accumulated per x-slab and summed in slab order so
   // the oracle's result does not depend on the thread count.
   double estep(bool sample, std::vector<std::mt19937> *rngs) {
     std::fill(dream.begin(), dream.end(), 0.0);  // initialize_dream_locations
+    // SSDLikelihood::initialize_parameters (:39-56)
+    if (lik == 1) log_constant = -std::log(std::sqrt(2.0 * M_PI * sigma2));
     std::vector<double> slabH((size_t)nxv * K * L, 0.0);
     std::vector<double> slab_mll(nxv, 0.0);
 #pragma omp parallel for schedule(static, 1) num_threads(threads)
@@ -339,8 +711,10 @@ struct Registration {
       double *Hs = &slabH[(size_t)x * K * L];
       double m = 0.0;
       for (int y = 0; y < nyv; ++y)
-        for (int z = 0; z < nzv; ++z)
-          m += voxel(((long long)x * nyv + y) * nzv + z, sample, rng, Hs);
+        for (int z = 0; z < nzv; ++z) {
+          const long long vi = ((long long)x * nyv + y) * nzv + z;
+          m += lik == 1 ? voxel_log(vi, sample, rng, Hs) : voxel(vi, sample, rng, Hs);
+        }
       slab_mll[x] = m;
     }
     H.assign((size_t)K * L, 0.0);
@@ -398,10 +772,12 @@ struct Registration {
     bool converged = false;
     while (it < max_it && !converged) {
       const double mll = estep(false, nullptr);
-      if (update) theta_update();
-      const double pt = theta_cost();
-      const double pd =
-          sm->constrain(dream.data(), xpos.data(), final_loc.data(), false, false);
+      if (update) {
+        if (lik == 1) ssd_update(false);
+        else theta_update();
+      }
+      const double pt = lik == 1 ? ssd_cost() : theta_cost();
+      const double pd = constrain(false, false);
       const double mlp = mll + pd + pt;
       trace.insert(trace.end(), {mll, pt, pd, mlp});
       if (debug)
@@ -422,14 +798,17 @@ struct Registration {
     std::mt19937 rng;
     rng.seed(seed);
     lk_gen = rng;   // lkObj->set_generator(rng) copies the engine
-    sm->gen = rng;  // transformationObj->set_generator(rng)
+    if (sm) sm->gen = rng;  // transformationObj->set_generator(rng)
     std::vector<double> avg((size_t)nv * N, 0.0), m2((size_t)nv * N * N, 0.0);
     for (int sweep = 0; sweep < Nb + Ns; ++sweep) {
       if (sp) estep(true, &rngs);
-      if (sl) theta_sample();
+      if (sl) {
+        if (lik == 1) ssd_update(true);
+        else theta_sample();
+      }
       if (st) {
         const bool sg = sample_gamma && (sweep % every == 0);
-        sm->constrain(dream.data(), xpos.data(), final_loc.data(), true, sg);
+        constrain(true, sg);
       }
       if (sweep >= Nb) {
         const double inv_n = 1.0 / ((double)(sweep - Nb) + 1.0);
@@ -489,6 +868,70 @@ void orc_reg_set_mi(void *h, double alpha, const double *theta, int K, int L,
   r->H.assign((size_t)K * L, 0.0);
   r->bn.assign(binned_nodes, binned_nodes + r->nn);
   r->bv.assign(binned_voxels, binned_voxels + r->nv);
+}
+
+// setSSDLikelihood (Registration.cpp:472-479; bindings.cpp:120-152): the
+// fixed and moving bins both in [0, bins)
+void orc_reg_set_ssd(void *h, int bins, double sigma_sq, int scale_shift, double scale,
+                     double shift, const int *binned_nodes, const int *binned_voxels,
+                     double alpha_0, double beta_0) {
+  auto *r = (Registration *)h;
+  r->lik = 1;
+  r->K = r->L = bins;
+  r->sigma2 = sigma_sq;
+  r->ssd_shift_scale = scale_shift;
+  r->scale = scale;
+  r->shift = shift;
+  r->a0 = alpha_0;
+  r->b0 = beta_0;
+  r->ssd_sample = 0;
+  r->H.assign((size_t)bins * bins, 0.0);
+  r->bn.assign(binned_nodes, binned_nodes + r->nn);
+  r->bv.assign(binned_voxels, binned_voxels + r->nv);
+}
+// SSDLikelihood::get_parameters (:326-333): [scale, shift]; sigma^2 third
+void orc_reg_get_ssd(void *h, double *out3) {
+  auto *r = (Registration *)h;
+  out3[0] = r->scale;
+  out3[1] = r->shift;
+  out3[2] = r->sigma2;
+}
+
+// setAffineTransformation / setRigidTransformation (Registration.cpp:507-518;
+// bindings.cpp:223-247): A is (N+1) x (N+1), R is N x N, t has N entries
+void orc_reg_set_affine(void *h, const double *A, const double *t) {
+  auto *r = (Registration *)h;
+  const int n1 = r->N + 1;
+  r->tmodel = 1;
+  r->A_aff.assign(A, A + (size_t)n1 * n1);
+  r->t_lin.assign(t, t + r->N);
+}
+void orc_reg_set_rigid(void *h, const double *R, const double *t) {
+  auto *r = (Registration *)h;
+  r->tmodel = 2;
+  r->R_rig.assign(R, R + (size_t)r->N * r->N);
+  r->t_lin.assign(t, t + r->N);
+  // x means of the voxel positions (RigidTransformation.cpp:22-37)
+  r->x_means.assign(r->N, 0.0);
+  for (long long v = 0; v < r->nv; ++v)
+    for (int n = 0; n < r->N; ++n) r->x_means[n] += r->xpos[v * r->N + n];
+  for (int n = 0; n < r->N; ++n) r->x_means[n] /= (double)r->nv;
+}
+// one transformation step from the current dream locations (test hook)
+double orc_reg_constrain(void *h) { return ((Registration *)h)->constrain(false, false); }
+// one SSD parameter update / draw from the last E-step's histogram and its
+// prior cost (test hooks)
+void orc_reg_ssd_update(void *h, int draw) { ((Registration *)h)->ssd_update(draw != 0); }
+double orc_reg_ssd_cost(void *h) { return ((Registration *)h)->ssd_cost(); }
+void orc_reg_ssd_seed(void *h, int seed) { ((Registration *)h)->lk_gen.seed(seed); }
+
+// the fitted transform: the matrix (affine (N+1)^2, rigid N^2) then t
+int orc_reg_get_linear(void *h, double *out) {
+  auto *r = (Registration *)h;
+  const std::vector<double> &M = r->tmodel == 1 ? r->A_aff : r->R_rig;
+  std::memcpy(out, M.data(), sizeof(double) * M.size());
+  std::memcpy(out + M.size(), r->t_lin.data(), sizeof(double) * r->t_lin.size());
+  return (int)(M.size() + r->t_lin.size());
 }
 
 void orc_reg_set_nlt(void *h, double gx, double gy, double gz, double rx,

@@ -47,6 +47,15 @@ _create = _sig("orc_reg_create", _vp, _vp, _i, _vp, _i, _i, _i, _i, _i, _i, _i, 
 _destroy = _sig("orc_reg_destroy", None, _vp)
 _set_mi = _sig("orc_reg_set_mi", None, _vp, _d, _vp, _i, _i, _vp, _vp)
 _set_nlt = _sig("orc_reg_set_nlt", None, _vp, *([_d] * 9), _vp)
+_set_ssd = _sig("orc_reg_set_ssd", None, _vp, _i, _d, _i, _d, _d, _vp, _vp, _d, _d)
+_get_ssd = _sig("orc_reg_get_ssd", None, _vp, _vp)
+_set_affine = _sig("orc_reg_set_affine", None, _vp, _vp, _vp)
+_set_rigid = _sig("orc_reg_set_rigid", None, _vp, _vp, _vp)
+_get_linear = _sig("orc_reg_get_linear", _i, _vp, _vp)
+_constrain = _sig("orc_reg_constrain", _d, _vp)
+_ssd_update = _sig("orc_reg_ssd_update", None, _vp, _i)
+_ssd_cost = _sig("orc_reg_ssd_cost", _d, _vp)
+_ssd_seed = _sig("orc_reg_ssd_seed", None, _vp, _i)
 _em = _sig("orc_reg_em", _i, _vp, _i, _d, _i, _i)
 _estep = _sig("orc_reg_estep", _d, _vp, _vp)
 _theta_update = _sig("orc_reg_theta_update", None, _vp)
@@ -110,6 +119,51 @@ class Registration:
         _set_mi(self._h, float(alpha), _p(self._th), int(K), int(L), _p(self._bn), _p(self._bv))
         self._KL = (int(K), int(L))
 
+    def setSSDLikelihood(self, bins, threads, sigma_sq, scale_shift, scale, shift,
+                         binned_nodes, binned_voxels, alpha_0, beta_0):
+        self._bn = _a(binned_nodes, np.int32)
+        self._bv = _a(binned_voxels, np.int32)
+        _set_ssd(self._h, int(bins), float(sigma_sq), int(scale_shift), float(scale),
+                 float(shift), _p(self._bn), _p(self._bv), float(alpha_0), float(beta_0))
+        self._KL = (int(bins), int(bins))
+        self._ssd = True
+
+    def get_ssd_parameters(self):
+        """(scale, shift, sigma^2) of the SSD likelihood."""
+        out = np.empty(3)
+        _get_ssd(self._h, _p(out))
+        return out.tolist()
+
+    def setAffineTransformation(self, A, t):
+        self._A = _a(A)
+        self._t = _a(t)
+        _set_affine(self._h, _p(self._A), _p(self._t))
+
+    def setRigidTransformation(self, R, t):
+        self._R = _a(R)
+        self._t = _a(t)
+        _set_rigid(self._h, _p(self._R), _p(self._t))
+
+    def constrain(self):
+        """One transformation step from the current dream locations (the
+        prior cost it returns)."""
+        return _constrain(self._h)
+
+    def ssd_update(self, draw=False, seed=None):
+        """One SSD parameter update (or draw) from the last e_step's histogram."""
+        if seed is not None:
+            _ssd_seed(self._h, int(seed))
+        _ssd_update(self._h, int(bool(draw)))
+
+    def ssd_cost(self):
+        return _ssd_cost(self._h)
+
+    def get_linear_parameters(self):
+        """The fitted affine ((N+1)^2, row-major) or rigid (N^2) matrix, then t."""
+        out = np.empty(32)
+        n = _get_linear(self._h, _p(out))
+        return out[:n].tolist()
+
     def setNonLinearTransformation(self, gamma_x, gamma_y, gamma_z, voxels_size_x,
                                    voxels_size_y, voxels_size_z, nodes_size_x, nodes_size_y,
                                    nodes_size_z, D):
@@ -147,6 +201,8 @@ class Registration:
         _set_dream(self._h, _p(d))
 
     def get_likelihood_parameters(self):
+        if getattr(self, "_ssd", False):
+            return self.get_ssd_parameters()[:2]  # [scale, shift]
         out = np.empty(self._KL[0] * self._KL[1])
         _get_theta(self._h, _p(out))
         return out.tolist()

@@ -1,0 +1,227 @@
+"""The SSD likelihood and the affine / rigid transformations on the HIP path
+against the FP64 oracle (pinned in test_oracle_models.py).
+
+  * the SSD E-step (per-voxel weights relative to the closest node, FP64
+    mll) for 32 and 64 bins, with and without shift and scale;
+  * 3 EM iterations with the SSD likelihood and the DCT prior;
+  * 3 EM iterations with the MI likelihood and the affine / rigid fit;
+  * the GPU parameter draws against N(MAP, A^-1) and the Gamma law of
+    1 / sigma^2;
+  * the FastRegistration pyramid with metric='SSD', affine=True and
+    rigid=True, HIP backend against the oracle backend.
+
+Reference: BINDER/src/cpu/SSDLikelihood.cpp, AffineTransformation.cpp,
+RigidTransformation.cpp, Registration.cpp:122-217 (EM), :515-642
+(sampler), BINDER/FastRegistration.py:452-525 (set-up per level).
+"""
+import numpy as np
+import pytest
+
+from binder_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(shape=(40, 36, 32), bins=32, seed=4):
+    pair = synth.make_pair(shape, 6, 2.0, seed, semi=(4.0, 12.0), bins=bins)
+    prob = synth.problem_from_pair(pair, gamma=1e-4, bins=bins)
+    return pair, prob
+
+
+def _sigma0(prob):
+    d = prob.binned_nodes.astype(np.float64) - prob.binned_voxels
+    return float(np.mean(d ** 2))
+
+
+def _ssd_reg(core, prob, ss, scale=1.0, shift=0.0, a0=1.0, b0=1.0, sigma2=None):
+    Nx, Ny, Nz = prob.shape
+    r = core.Registration(prob.final_locations.ravel(), 1, prob.voxel_pos.ravel(),
+                          prob.voxel_pos.ravel(), Nx, Ny, Nz, Nx, Ny, Nz, 3, prob.nnz)
+    r.setNonLinearTransformation(*prob.gammas, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, prob.D.ravel())
+    r.setSSDLikelihood(prob.K, 1, _sigma0(prob) if sigma2 is None else sigma2, ss, scale, shift,
+                       prob.binned_nodes.ravel(), prob.binned_voxels.ravel(), a0, b0)
+    return r
+
+
+@pytest.mark.parametrize("bins,ss", [(32, 0), (32, 1), (64, 1)])
+def test_ssd_estep_vs_oracle(core, orc, bins, ss):
+    print(f"SSD E-step bins={bins} shift_scale={ss}", flush=True)
+    _, prob = _pair(bins=bins)
+    rng = np.random.default_rng(2)
+    prob.final_locations = prob.voxel_pos + rng.normal(0, 0.8, prob.voxel_pos.shape)
+    g = _ssd_reg(core, prob, ss, 0.9, 1.5)
+    o = _ssd_reg(orc, prob, ss, 0.9, 1.5)
+    mg, Hg = g.e_step(return_histogram=True)
+    mo, Ho = o.e_step(return_histogram=True)
+    dg = np.asarray(g.get_dream_locations(), np.float64)
+    do = np.asarray(o.get_dream_locations())
+    print(f"  mll {mg:.6f} vs {mo:.6f}; H max {np.abs(Hg - Ho).max():.2e}; "
+          f"dream max {np.abs(dg - do).max():.2e}", flush=True)
+    assert abs(mg - mo) <= 2e-6 * abs(mo)
+    assert np.abs(Hg - Ho).max() <= 1e-5 * Ho.max()
+    assert np.isclose(Hg.sum(), np.prod(prob.shape), rtol=1e-6)
+    assert np.abs(dg - do).max() <= 2e-4
+
+
+@pytest.mark.parametrize("bins,ss", [(32, 0), (32, 1), (64, 1)])
+def test_ssd_em_vs_oracle(core, orc, bins, ss):
+    print(f"SSD EM x3 bins={bins} shift_scale={ss}", flush=True)
+    _, prob = _pair(bins=bins)
+    g = _ssd_reg(core, prob, ss, a0=2.0, b0=3.0)
+    o = _ssd_reg(orc, prob, ss, a0=2.0, b0=3.0)
+    g.EM(3, -1e30, 1, 0)
+    o.EM(3, -1e30, 1, 0)
+    tg, to = g.get_objective_trace(), o.get_objective_trace()
+    pg, po = np.array(g.get_ssd_parameters()), np.array(o.get_ssd_parameters())
+    fg = np.asarray(g.get_final_locations_as_numpy_array(), np.float64)
+    fo = o.get_final_locations_as_numpy_array()
+    rel = np.abs(tg - to) / np.abs(to)
+    print(f"  trace rel {rel.max(0)}; params {pg} vs {po}; field max "
+          f"{np.abs(fg - fo).max():.2e}", flush=True)
+    assert tg.shape == to.shape == (3, 4)
+    assert rel[:, 0].max() <= 1e-5 and rel[:, 3].max() <= 1e-5
+    assert rel[:, 1].max() <= 1e-5  # -log InvGamma(sigma^2)
+    assert np.allclose(pg, po, rtol=1e-5, atol=1e-5)
+    assert np.abs(fg - fo).max() <= 1e-3
+    # get_likelihood_parameters is [scale, shift]
+    assert np.allclose(g.get_likelihood_parameters(), pg[:2])
+
+
+@pytest.mark.parametrize("kind", ["affine", "rigid"])
+@pytest.mark.parametrize("shape", [(40, 36, 32), (64, 56, 1)])
+def test_linear_em_vs_oracle(core, orc, kind, shape):
+    print(f"{kind} EM x3 {shape}", flush=True)
+    _, prob = _pair(shape=shape)
+    nd = 2 if shape[2] == 1 else 3
+    rng = np.random.default_rng(6)
+    # start away from the identity so the fit has something to do
+    M = np.eye(nd) + rng.normal(0, 0.02, (nd, nd))
+    prob.final_locations = prob.voxel_pos @ M.T + rng.normal(0, 0.5, nd)
+    regs = []
+    for be in (core, orc):
+        r = synth.build_registration(be, prob)
+        if kind == "affine":
+            r.setAffineTransformation(np.eye(nd + 1).ravel(), np.zeros(nd))
+        else:
+            r.setRigidTransformation(np.eye(nd).ravel(), np.zeros(nd))
+        r.EM(3, -1e30, 1, 0)
+        regs.append(r)
+    g, o = regs
+    pg, po = np.array(g.get_linear_parameters()), np.array(o.get_linear_parameters())
+    tg, to = g.get_objective_trace(), o.get_objective_trace()
+    fg = np.asarray(g.get_final_locations_as_numpy_array(), np.float64)
+    fo = o.get_final_locations_as_numpy_array()
+    print(f"  params max diff {np.abs(pg - po).max():.2e}; mll rel "
+          f"{(np.abs(tg - to)[:, 0] / np.abs(to[:, 0])).max():.2e}; field max "
+          f"{np.abs(fg - fo).max():.2e}", flush=True)
+    assert tg.shape == (3, 4) and np.all(tg[:, 2] == 0.0)  # no prior cost
+    assert np.abs(pg - po).max() <= 1e-4
+    assert np.abs(fg - fo).max() <= 1e-3
+    assert np.all(np.abs(tg - to)[:, 0] <= 1e-5 * np.abs(to[:, 0]))
+    if kind == "rigid":
+        R = pg[:nd * nd].reshape(nd, nd)
+        assert np.allclose(R @ R.T, np.eye(nd), atol=1e-10)
+
+
+def _map_and_cov(H):
+    K, L = H.shape
+    k, u = np.meshgrid(np.arange(K, dtype=np.float64), np.arange(L, dtype=np.float64),
+                       indexing="ij")
+    g, gk, gkk = H.sum(), (H * k).sum(), (H * k * k).sum()
+    A = np.array([[g, gk], [gk, gkk]])
+    b = np.array([(H * u).sum(), (H * u * k).sum()])
+    return np.linalg.solve(A, b), np.linalg.inv(A)
+
+
+def test_ssd_draws_on_gpu(core):
+    """Parameter draws of the sampler (sample_parameters, SSDLikelihood.cpp:
+    273-304) from one histogram: (shift, scale) ~ N(MAP, A^-1); without
+    shift and scale 1 / sigma^2 ~ Gamma(den / 2, 2 den / num)."""
+    _, prob = _pair(shape=(24, 20, 16), bins=16)
+    g = _ssd_reg(core, prob, 1)
+    _, H = g.e_step(return_histogram=True)
+    mean, cov = _map_and_cov(H)
+    n = 1500
+    draws = np.empty((n, 2))
+    for i in range(n):
+        # no E-draw and no deformation draw: each call draws the parameters
+        # once from the histogram as it stands
+        g.sampler_run(100 + i, 0, 1, 0, 1, 0, 1, 0)
+        scale, shift, _ = g.get_ssd_parameters()
+        draws[i] = shift, scale
+    se = np.sqrt(np.diag(cov) / n)
+    print(f"shift/scale draws: mean {draws.mean(0)} vs {mean}, var "
+          f"{draws.var(0)} vs {np.diag(cov)}", flush=True)
+    assert np.all(np.abs(draws.mean(0) - mean) < 4.5 * se)
+    assert np.allclose(draws.var(0), np.diag(cov), rtol=0.12)
+
+    a0, b0 = 2.0, 3.0
+    g0 = _ssd_reg(core, prob, 0, a0=a0, b0=b0)
+    _, H0 = g0.e_step(return_histogram=True)
+    k, u = np.meshgrid(np.arange(16.0), np.arange(16.0), indexing="ij")
+    num = (H0 * (u - k) ** 2).sum() + a0 * b0
+    den = H0.sum() + a0
+    shape, scl = den / 2, 2 * den / num
+    prec = np.empty(n)
+    for i in range(n):
+        g0.sampler_run(5000 + i, 0, 1, 0, 1, 0, 1, 0)
+        prec[i] = 1.0 / g0.get_ssd_parameters()[2]
+    m, v = shape * scl, shape * scl ** 2
+    print(f"1/sigma^2 draws: mean {prec.mean():.6g} vs {m:.6g}, var {prec.var():.4g} vs "
+          f"{v:.4g}", flush=True)
+    assert abs(prec.mean() - m) < 4.5 * np.sqrt(v / n)
+    assert np.isclose(prec.var(), v, rtol=0.12)
+
+
+def test_ssd_sampler_runs(core, orc):
+    """The full sampler with the SSD likelihood (E-draws, parameter draws,
+    deformation draws) from the EM solution.  The reference's sigma^2 draw
+    (:283-301) has mean sum-of-squares / (sum h)^2, so sigma^2 falls to
+    ~1e-6 bins^2 within a few sweeps and the E-draws become near
+    deterministic: the per-voxel log-space weights keep that regime exact.
+    Different generators: the HIP chain's mean is as far from the oracle's
+    as from a second HIP chain's, and sigma^2 reaches the same scale."""
+    _, prob = _pair(shape=(24, 20, 16), bins=16)
+    out = []
+    for be, seed in ((core, 7), (core, 8), (orc, 7)):
+        r = _ssd_reg(be, prob, 1)
+        r.EM(5, -1e30, 1, 0)
+        mean, cov = r.sampler(seed, 5, 30, 0, 1, 1, 1, 1)
+        out.append((np.asarray(mean, np.float64), np.asarray(cov), r.get_ssd_parameters()))
+    (mg, cg, pg), (mg2, _, pg2), (mo, co, po) = out
+    rms = lambda a, b: float(np.sqrt(np.mean(np.sum((a - b) ** 2, -1))))
+    d_go, d_gg = rms(mg, mo), rms(mg, mg2)
+    print(f"sampler mean RMS: HIP-oracle {d_go:.3f} vox, HIP-HIP {d_gg:.3f} vox; params "
+          f"{pg} / {pg2} vs {po}", flush=True)
+    assert np.all(np.isfinite(mg)) and np.all(np.isfinite(cg))
+    assert d_go <= 1.5 * d_gg + 0.05
+    assert np.allclose(pg[:2], po[:2], atol=0.05)
+    assert 0.2 < pg[2] / po[2] < 5.0
+
+
+def _pyramid(backend, monkeypatch, pair, **kw):
+    import binder_amd.FastRegistration as fr
+    monkeypatch.setattr(fr, "Registration", backend)
+    reg = fr.FastRegistration(voxels=pair.fixed, nodes=pair.moving, threads=8,
+                              num_voxel_intesity_levels=32, num_node_intensity_levels=32,
+                              gamma=1e-5, topology_correction=False, **kw)
+    reg.run([2, 1], 1000, level_iterations=[3, 3])
+    return reg
+
+
+@pytest.mark.parametrize("kw", [dict(metric="SSD"), dict(metric="SSD", scale_shift=False),
+                                dict(affine=True), dict(rigid=True)],
+                         ids=["ssd", "ssd-noscale", "affine", "rigid"])
+def test_fastregistration_models_vs_oracle(core, orc, monkeypatch, kw):
+    print(f"FastRegistration {kw} pyramid [2, 1] x 3 iterations", flush=True)
+    pair, _ = _pair(shape=(40, 36, 32))
+    g = _pyramid(core, monkeypatch, pair, **kw)
+    o = _pyramid(orc, monkeypatch, pair, **kw)
+    fg = np.asarray(g.deformedVoxelPositions, np.float64)
+    fo = np.asarray(o.deformedVoxelPositions, np.float64)
+    print(f"  field max {np.abs(fg - fo).max():.2e}", flush=True)
+    assert np.abs(fg - fo).max() <= 2e-3
+    if kw.get("metric") == "SSD":
+        # the reference's hand-over unpacks [scale, shift] as (shift, scale)
+        assert np.allclose([g.shift, g.scale], [o.shift, o.scale], rtol=1e-5, atol=1e-6)
+        assert np.allclose(g.registratorObj.get_likelihood_parameters(), [g.shift, g.scale])

@@ -114,7 +114,7 @@ def test_toy_2d_registration_oracle(orc, monkeypatch):
 def test_fastregistration_rejects_out_of_scope_options(orc, monkeypatch):
     import binder_amd.FastRegistration as fr
     monkeypatch.setattr(fr, "Registration", orc)
-    for kw in ({"metric": "SSD"}, {"affine": True}, {"feature_moving": True},
+    for kw in ({"metric": "NCC"}, {"feature_moving": True}, {"metric": "SSD", "feature_fixed": True},
                {"smooth": True}):
         with pytest.raises(NotImplementedError):
             fr.FastRegistration(voxels=FIXED_2D, nodes=MOVING_2D, topology_correction=False, **kw)
@@ -209,3 +209,41 @@ def test_cli_oracle(orc, monkeypatch, tmp_path):
     import binder_amd.FastRegistration as fr
     monkeypatch.setattr(fr, "Registration", orc)
     run_cli(tmp_path, ["--no-topology-correction"])
+
+
+@pytest.mark.parametrize("kw", [{"affine": True}, {"rigid": True}], ids=["affine", "rigid"])
+def test_toy_2d_linear_registration_oracle(orc, monkeypatch, kw):
+    """The toy pair differs by a translation of two rows: the affine and the
+    rigid fit (MI likelihood) recover it and the warped moving image equals
+    the fixed one."""
+    import binder_amd.FastRegistration as fr
+    monkeypatch.setattr(fr, "Registration", orc)
+    reg = fr.FastRegistration(voxels=FIXED_2D, nodes=MOVING_2D, num_voxel_intesity_levels=4,
+                              num_node_intensity_levels=4, gamma=100, threads=1,
+                              seed=12345, alpha=1.1, bin_robust=False,
+                              topology_correction=False, **kw)
+    reg.run(resolutions=[1], max_number_of_iterations=100)
+    pos = np.moveaxis(reg.deformedVoxelPositions, 3, 0)[0:2, :, :, 0]
+    assert np.allclose(resample.warp(MOVING_2D, pos, order=0), FIXED_2D)
+    par = np.asarray(reg.registratorObj.get_linear_parameters())
+    assert abs(par[-2] - 2.0) < 0.2 and abs(par[-1]) < 0.25
+
+
+def test_toy_2d_ssd_hand_over(orc, monkeypatch):
+    """metric='SSD': sigma^2 starts at the mean squared bin difference over
+    the common extent (FastRegistration.py:241-246) and a level hands
+    get_likelihood_parameters() = [scale, shift] on as (shift, scale), as
+    the reference's unpacking at FastRegistration.py:521 does."""
+    import binder_amd.FastRegistration as fr
+    monkeypatch.setattr(fr, "Registration", orc)
+    reg = fr.FastRegistration(voxels=FIXED_2D, nodes=MOVING_2D, num_voxel_intesity_levels=4,
+                              num_node_intensity_levels=4, gamma=100, threads=1, metric="SSD",
+                              alpha_0=1, beta_0=1, bin_robust=False, topology_correction=False)
+    bf = utils.bin_signal(FIXED_2D[:, :, None] * 1.0, 4)
+    bm = utils.bin_signal(MOVING_2D[:, :, None] * 1.0, 4)
+    n = np.minimum(bf.shape, bm.shape)
+    want = np.sum((bm[:n[0], :n[1], :n[2]] - bf[:n[0], :n[1], :n[2]]) ** 2) / np.prod(n)
+    assert np.isclose(reg.sigma_sq, want)
+    reg.run(resolutions=[1], max_number_of_iterations=20)
+    scale, shift = reg.registratorObj.get_likelihood_parameters()
+    assert (reg.shift, reg.scale) == (scale, shift)

@@ -1,0 +1,294 @@
+"""The FP64 oracle's SSD likelihood and affine / rigid transformations pinned
+against brute force and closed forms.
+
+The reference's SSDLikelihood.cpp, AffineTransformation.cpp and
+RigidTransformation.cpp include Eigen, which this image does not have, so
+they cannot be compiled into oracle/_ref; the reference's tests hold no
+vectors for them.  The checks here are chosen so that a dropped term, a
+wrong sign or index or a transposed operand fails one of them:
+
+  * the log-space E-step (Registration.cpp:332-440 with
+    SSDLikelihood::compute_log_likelihood, :58-85) against a NumPy loop
+    with scipy's logsumexp, outside nodes taking moving bin 0;
+  * the MAP step (update_shift_and_scale / update_sigma, :176-271, and the
+    shift/scale-free update_parameters, :152-173) against a weighted
+    least-squares fit by numpy.linalg.lstsq on the soft joint histogram;
+  * compute_cost (:308-320) against scipy.stats.invgamma;
+  * the draws (sample_parameters, :273-304): mean and covariance of
+    (shift, scale) against N(MAP, A^-1), the moments of 1 / sigma^2 against
+    Gamma(shape den / 2, scale 2 den / num), and which update draws once the
+    draw flag is set;
+  * the affine fit (AffineTransformation.cpp:25-130) against lstsq on
+    [1, x] -> y, and the rigid fit (RigidTransformation.cpp:57-154) against
+    exact rotations: the reference builds H = sum y~ x~^T, so a field
+    y = Q x + c gives R = Q^T (the transpose of the Kabsch rotation), and
+    a reflection still gives det R = +1.
+"""
+import numpy as np
+import pytest
+from scipy.special import logsumexp
+from scipy.stats import invgamma
+
+from binder_amd.utils import initialize_fourier_domain_smoothing
+
+
+def _bspline3(t):
+    return np.array([(1 - t) ** 3 / 6, 2 / 3 - t ** 2 + t ** 3 / 2,
+                     2 / 3 - (1 - t) ** 2 + (1 - t) ** 3 / 2, t ** 3 / 6])
+
+
+def _ssd_ll(u, k, sigma2, shift, scale, ss):
+    d = u - shift - scale * k if ss else u - k
+    return -d * d / (2.0 * sigma2) - np.log(np.sqrt(2.0 * np.pi * sigma2))
+
+
+def _ssd_estep_bruteforce(d, bn, bv, bins, sigma2, shift, scale, ss):
+    shape = bv.shape
+    nd = d.shape[-1]
+    H = np.zeros((bins, bins))
+    dream = np.zeros_like(d)
+    mll = 0.0
+    for idx in np.ndindex(*shape):
+        p = d[idx]
+        base = np.floor(p).astype(int)
+        w = [_bspline3(p[a] - base[a]) for a in range(nd)]
+        u = bv[idx]
+        nodes, lp = [], []
+        for off in np.ndindex(*(4,) * nd):
+            node = base + np.array(off) - 1
+            inside = all(0 <= node[a] < bn.shape[a] for a in range(nd))
+            k = bn[tuple(node)] if inside else 0
+            lw = np.sum([np.log(w[a][off[a]]) for a in range(nd)])
+            nodes.append((node, k))
+            lp.append(lw + _ssd_ll(u, k, sigma2, shift, scale, ss))
+        lp = np.array(lp)
+        lz = logsumexp(lp)
+        mll -= lz
+        for (node, k), q in zip(nodes, np.exp(lp - lz)):
+            H[k, u] += q
+            dream[idx] += q * node
+    return mll, H, dream
+
+
+def _ssd_registration(orc, shape, bins, sigma2, ss, scale=1.0, shift=0.0, seed=3,
+                      a0=1.0, b0=1.0, spread=1.2):
+    rng = np.random.default_rng(seed)
+    nd = 2 if shape[2] == 1 else 3
+    pos = np.stack(np.indices(shape, dtype=np.float64), -1)[..., :nd]
+    d = pos + rng.normal(0, spread, pos.shape)
+    d[0] += 5.5  # some voxels partly outside the moving grid
+    bn = rng.integers(0, bins, shape).astype(np.int32)
+    # the fixed bins follow the moving ones through a scale and shift
+    bv = np.clip(np.rint(0.8 * bn + 2 + rng.normal(0, 1.5, shape)), 0, bins - 1).astype(np.int32)
+    r = orc.Registration(d.ravel(), 1, pos.ravel(), pos.ravel(), *shape, *shape, 3,
+                         int(np.prod(shape)))
+    D = initialize_fourier_domain_smoothing(list(shape[:nd]), True, 2, nd).reshape(shape)
+    r.setNonLinearTransformation(1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, D.ravel())
+    r.setSSDLikelihood(bins, 1, sigma2, ss, scale, shift, bn.ravel(), bv.ravel(), a0, b0)
+    return r, d, bn, bv
+
+
+@pytest.mark.parametrize("shape,ss", [((5, 4, 6), 0), ((5, 4, 6), 1), ((7, 6, 1), 1)])
+def test_oracle_ssd_estep_bruteforce(orc, shape, ss):
+    bins, sigma2, scale, shift = 8, 2.5, 0.7, 1.3
+    r, d, bn, bv = _ssd_registration(orc, shape, bins, sigma2, ss, scale, shift)
+    mll, H = r.e_step(return_histogram=True)
+    dream = np.asarray(r.get_dream_locations()).reshape(d.shape)
+    if d.shape[-1] == 2:  # drop the unit z axis of the 2D grid
+        d2, bn2, bv2 = d[:, :, 0, :], bn[..., 0], bv[..., 0]
+    else:
+        d2, bn2, bv2 = d, bn, bv
+    bmll, bH, bdream = _ssd_estep_bruteforce(d2, bn2, bv2, bins, sigma2, shift, scale, ss)
+    assert np.isclose(mll, bmll, rtol=1e-12)
+    assert np.allclose(H, bH, atol=1e-11)
+    assert np.allclose(dream, bdream.reshape(d.shape), atol=1e-11)
+    assert np.isclose(H.sum(), np.prod(shape))
+
+
+def _wls(H, ss):
+    """Weighted least squares of u on [1, k] with weights H[k, u]."""
+    K, L = H.shape
+    k, u = np.meshgrid(np.arange(K, dtype=np.float64), np.arange(L, dtype=np.float64),
+                       indexing="ij")
+    w = np.sqrt(H.ravel())
+    if ss:
+        X = np.stack([np.ones(K * L), k.ravel()], 1)
+        coef = np.linalg.lstsq(X * w[:, None], u.ravel() * w, rcond=None)[0]
+        res = u.ravel() - X @ coef
+        return coef, res
+    return np.array([0.0, 1.0]), (u - k).ravel()
+
+
+@pytest.mark.parametrize("ss", [0, 1])
+def test_oracle_ssd_map_step_weighted_lstsq(orc, ss):
+    a0, b0 = 3.0, 2.0
+    r, *_ = _ssd_registration(orc, (9, 8, 7), 16, 6.0, ss, 1.0, 0.0, a0=a0, b0=b0, seed=5)
+    _, H = r.e_step(return_histogram=True)
+    r.ssd_update(draw=False)
+    scale, shift, sigma2 = r.get_ssd_parameters()
+    coef, res = _wls(H, ss)
+    want_s2 = (np.sum(H.ravel() * res ** 2) + a0 * b0) / (H.sum() + a0)
+    if ss:
+        assert np.isclose(shift, coef[0], rtol=1e-10, atol=1e-12)
+        assert np.isclose(scale, coef[1], rtol=1e-10)
+    else:
+        assert (scale, shift) == (1.0, 0.0)
+    assert np.isclose(sigma2, want_s2, rtol=1e-10)
+    # get_likelihood_parameters is [scale, shift] (SSDLikelihood.cpp:328-334)
+    assert r.get_likelihood_parameters() == [scale, shift]
+
+
+@pytest.mark.parametrize("a0,b0,s2", [(1.0, 1.0, 3.0), (4.0, 0.5, 0.7), (1000.0, 1.0, 1.2)])
+def test_oracle_ssd_cost_inverse_gamma(orc, a0, b0, s2):
+    r, *_ = _ssd_registration(orc, (4, 4, 4), 4, s2, 0, a0=a0, b0=b0)
+    want = -invgamma(a0, scale=a0 * b0).logpdf(s2)
+    assert np.isclose(r.ssd_cost(), want, rtol=1e-12)
+
+
+def test_oracle_ssd_draws_shift_scale_gaussian(orc):
+    """With shift and scale the draw is (shift, scale) ~ N(MAP, A^-1),
+    A = [[sum h, sum h k], [sum h k, sum h k^2]]."""
+    r, *_ = _ssd_registration(orc, (6, 5, 4), 8, 4.0, 1, seed=7)
+    _, H = r.e_step(return_histogram=True)
+    coef, _ = _wls(H, 1)
+    k = np.arange(H.shape[0], dtype=np.float64)
+    g = H.sum(1)
+    A = np.array([[g.sum(), (g * k).sum()], [(g * k).sum(), (g * k * k).sum()]])
+    cov = np.linalg.inv(A)
+    n = 3000
+    draws = np.empty((n, 2))
+    for i in range(n):
+        # the flag only changes what later calls do; each draw starts from
+        # the same histogram
+        r.ssd_update(draw=True, seed=1000 + i)
+        scale, shift, _ = r.get_ssd_parameters()
+        draws[i] = shift, scale
+    se = np.sqrt(np.diag(cov) / n)
+    assert np.all(np.abs(draws.mean(0) - coef) < 4.5 * se)
+    emp = np.cov(draws.T)
+    assert np.allclose(np.diag(emp), np.diag(cov), rtol=0.1)
+    corr_e = emp[0, 1] / np.sqrt(emp[0, 0] * emp[1, 1])
+    corr_t = cov[0, 1] / np.sqrt(cov[0, 0] * cov[1, 1])
+    assert abs(corr_e - corr_t) < 0.06
+
+
+def test_oracle_ssd_draws_sigma_gamma(orc):
+    """Without shift and scale, 1 / sigma^2 ~ Gamma(shape den / 2, scale
+    2 den / num), num = sum h (u - k)^2 + a0 b0, den = sum h + a0 (the
+    reference's parametrisation, :283-301)."""
+    a0, b0 = 2.0, 3.0
+    r, *_ = _ssd_registration(orc, (5, 4, 3), 8, 5.0, 0, a0=a0, b0=b0, seed=9)
+    _, H = r.e_step(return_histogram=True)
+    _, res = _wls(H, 0)
+    num = np.sum(H.ravel() * res ** 2) + a0 * b0
+    den = H.sum() + a0
+    shape, scale = den / 2, 2 * den / num
+    n = 4000
+    prec = np.empty(n)
+    for i in range(n):
+        r.ssd_update(draw=True, seed=77 + i)
+        prec[i] = 1.0 / r.get_ssd_parameters()[2]
+    mean, var = shape * scale, shape * scale ** 2
+    assert abs(prec.mean() - mean) < 4.5 * np.sqrt(var / n)
+    assert np.isclose(prec.var(), var, rtol=0.1)
+
+
+def test_oracle_ssd_draw_flag(orc):
+    """Once a draw has run, the MAP step of the shift/scale model draws
+    (update_shift_and_scale / update_sigma test the flag), the MAP step
+    without shift and scale does not (update_parameters, :152-173)."""
+    for ss in (0, 1):
+        r, *_ = _ssd_registration(orc, (6, 5, 4), 8, 4.0, ss, seed=11)
+        r.e_step()
+        r.ssd_update(draw=False)
+        map_par = r.get_ssd_parameters()
+        r.ssd_update(draw=True, seed=5)
+        r.ssd_update(draw=False, seed=6)
+        after = r.get_ssd_parameters()
+        if ss:
+            assert after != map_par
+        else:
+            assert after == map_par
+
+
+# ---------------------------------------------------------------------------
+# affine / rigid
+# ---------------------------------------------------------------------------
+def _linear_registration(orc, shape, dream):
+    nd = dream.shape[-1]
+    pos = np.stack(np.indices(shape, dtype=np.float64), -1)[..., :nd]
+    r = orc.Registration(pos.ravel(), 1, pos.ravel(), pos.ravel(), *shape, *shape, 3,
+                         int(np.prod(shape)))
+    r.set_dream_locations(dream.ravel())
+    return r, pos.reshape(-1, nd)
+
+
+@pytest.mark.parametrize("shape", [(7, 6, 5), (9, 8, 1)])
+def test_oracle_affine_lstsq(orc, shape):
+    rng = np.random.default_rng(4)
+    nd = 2 if shape[2] == 1 else 3
+    pos = np.stack(np.indices(shape, dtype=np.float64), -1)[..., :nd].reshape(-1, nd)
+    M = np.eye(nd) + rng.normal(0, 0.1, (nd, nd))
+    c = rng.normal(0, 2.0, nd)
+    y = pos @ M.T + c + rng.normal(0, 0.3, pos.shape)
+    r, _ = _linear_registration(orc, shape, y)
+    r.setAffineTransformation(np.eye(nd + 1).ravel(), np.zeros(nd))
+    assert r.constrain() == 0.0
+    par = np.asarray(r.get_linear_parameters())
+    n1 = nd + 1
+    A, t = par[:n1 * n1].reshape(n1, n1), par[n1 * n1:]
+    X = np.hstack([np.ones((pos.shape[0], 1)), pos])
+    T = np.linalg.lstsq(X, y, rcond=None)[0]  # rows: t, then the columns of M^T
+    assert np.allclose(t, T[0], atol=1e-10)
+    assert np.allclose(A[:nd, :nd], T[1:].T, atol=1e-10)
+    assert np.array_equal(A[nd], np.eye(n1)[nd]) and np.all(A[:nd, nd] == 0)
+    final = r.get_final_locations_as_numpy_array().reshape(-1, nd)
+    assert np.allclose(final, X @ T, atol=1e-9)
+
+
+def _rotation(nd, rng):
+    q, _ = np.linalg.qr(rng.normal(size=(nd, nd)))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+@pytest.mark.parametrize("shape", [(7, 6, 5), (9, 8, 1)])
+def test_oracle_rigid_exact_rotation_transposed(orc, shape):
+    rng = np.random.default_rng(8)
+    nd = 2 if shape[2] == 1 else 3
+    pos = np.stack(np.indices(shape, dtype=np.float64), -1)[..., :nd].reshape(-1, nd)
+    Q = _rotation(nd, rng)
+    c = rng.normal(0, 3.0, nd)
+    y = pos @ Q.T + c
+    r, _ = _linear_registration(orc, shape, y)
+    r.setRigidTransformation(np.eye(nd).ravel(), np.zeros(nd))
+    r.constrain()
+    par = np.asarray(r.get_linear_parameters())
+    R, t = par[:nd * nd].reshape(nd, nd), par[nd * nd:]
+    assert np.allclose(R, Q.T, atol=1e-10)
+    xm, ym = pos.mean(0), y.mean(0)
+    assert np.allclose(t, ym - Q.T @ xm, atol=1e-9)
+    final = r.get_final_locations_as_numpy_array().reshape(-1, nd)
+    assert np.allclose(final, pos @ Q + t, atol=1e-9)
+    # the identity is its own transpose: y = x + c gives R = I, t = c
+    r2, _ = _linear_registration(orc, shape, pos + c)
+    r2.setRigidTransformation(np.eye(nd).ravel(), np.zeros(nd))
+    r2.constrain()
+    par2 = np.asarray(r2.get_linear_parameters())
+    assert np.allclose(par2[:nd * nd].reshape(nd, nd), np.eye(nd), atol=1e-10)
+    assert np.allclose(par2[nd * nd:], c, atol=1e-9)
+
+
+def test_oracle_rigid_reflection_gives_rotation(orc):
+    rng = np.random.default_rng(12)
+    shape = (6, 7, 5)
+    pos = np.stack(np.indices(shape, dtype=np.float64), -1).reshape(-1, 3)
+    P = _rotation(3, rng) @ np.diag([1.0, 1.0, -1.0])
+    y = pos @ P.T + rng.normal(0, 0.05, pos.shape)
+    r, _ = _linear_registration(orc, shape, y)
+    r.setRigidTransformation(np.eye(3).ravel(), np.zeros(3))
+    r.constrain()
+    R = np.asarray(r.get_linear_parameters())[:9].reshape(3, 3)
+    assert np.allclose(R @ R.T, np.eye(3), atol=1e-10)
+    assert np.isclose(np.linalg.det(R), 1.0, atol=1e-10)
