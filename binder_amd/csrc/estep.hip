@@ -187,7 +187,7 @@ __device__ __forceinline__ SsdParams ssd_params(const EStepArgs &a) {
   p.shift = a.ssd_ss ? a.ssd_par[1] : 0.0;
   p.scale = a.ssd_ss ? a.ssd_par[2] : 1.0;
   p.h = 0.5 / s2;
-  p.logc = -log(sqrt(2.0 * M_PI * s2));  // log_constant (SSDLikelihood.cpp:55)
+  p.logc = -log(sqrt(2.0 * M_PI * s2));  // log_constant (SSDLikelihood.cpp:57)
   p.s = (float)p.scale;
   p.c1 = (float)(p.h * p.scale);
   return p;

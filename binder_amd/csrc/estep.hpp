@@ -52,7 +52,7 @@ struct EStepArgs {
   // logical blocks of the EM E-step of this registration (batched launches;
   // a single launch uses its grid size)
   int nblocks;
-  // SSD likelihood (BINDER/src/cpu/SSDLikelihood.cpp:58-85) in place of the
+  // SSD likelihood (BINDER/src/cpu/SSDLikelihood.cpp:60-86) in place of the
   // table thT: ssd_par = (sigma^2, shift, scale) on the device, ssd_ss the
   // scale_shift flag.  Each voxel's node weights are B_j exp(ll_j - ll_r),
   // ll_r the log-likelihood of its closest (positive-weight) node, with

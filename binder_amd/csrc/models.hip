@@ -2,12 +2,12 @@
 //
 // SSD (BINDER/src/cpu/SSDLikelihood.cpp): the likelihood of a voxel with
 // fixed bin u at a node with moving bin k (bin 0 outside the grid, the value
-// the reference uses there, :58-85) depends on (u, k) only; the E-step
+// the reference uses there, :60-86) depends on (u, k) only; the E-step
 // kernels evaluate it per voxel relative to the voxel's closest node (exact
 // however small sigma^2 is), writing the same soft joint histogram H[k][u]
 // and FP64 mll partials as for the MI likelihood.  The reference's
-// accumulate_info sums (:118-150) are moments of H, so the parameter step
-// (update_shift_and_scale, update_sigma, sample_parameters, :176-304) runs
+// accumulate_info sums (:117-146) are moments of H, so the parameter step
+// (update_shift_and_scale, update_sigma, sample_parameters, :169-307) runs
 // on those moments in one block.
 //
 // Affine / rigid (AffineTransformation.cpp:25-130,
@@ -105,8 +105,8 @@ __global__ __launch_bounds__(kSsdThreads) void k_ssd_theta(SsdThetaArgs a) {
     if (a.mode != kSsdKeep) {
       if (a.mode == kSsdDraw) flag = 1.0;
       // with shift and scale the MAP step draws once the flag is set
-      // (SSDLikelihood.cpp:205, :266); without them only a draw step draws
-      // (:154-171 vs :283-301)
+      // (SSDLikelihood.cpp:211, :273); without them only a draw step draws
+      // (:148-167 vs :284-307)
       const bool draw = a.shift_scale ? flag != 0.0 : a.mode == kSsdDraw;
       const uint32_t key0 = a.keyx ? a.k0 ^ a.keyx[0] : a.k0;
       const uint32_t key1 = a.keyx ? a.k1 ^ a.keyx[1] : a.k1;
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kSsdThreads) void k_ssd_theta(SsdThetaArgs a) {
       if (!draw) {
         s2 = num / den;
       } else {
-        // the reference's draw (:240-246): 1 / Gamma(shape den / 2, scale 2 den / num)
+        // the reference's draw (:278, :300): 1 / Gamma(shape den / 2, scale 2 den / num)
         const double gd =
             gamma_mt(den * 0.5, u32x4{1u, a.sweep, 0u, kStreamSsd}, key0, key1) *
             (2.0 * den / num);
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kSsdThreads) void k_ssd_theta(SsdThetaArgs a) {
     a.par_out[1] = sh;
     a.par_out[2] = sc;
     a.par_out[3] = flag;
-    // -log InvGamma(sigma^2; a0, a0 b0) (:308-320)
+    // -log InvGamma(sigma^2; a0, a0 b0) (:311-323)
     const double pa = a.a0, pb = a.a0 * a.b0;
     a.stats[0] = mll;
     a.stats[1] = -(pa * log(pb) - lgamma(pa) + (-pa - 1.0) * log(s2) - pb / s2);

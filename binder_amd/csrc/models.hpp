@@ -1,7 +1,7 @@
 // The reference's other likelihood and transformation models on the GPU:
-// the SSD likelihood (BINDER/src/cpu/SSDLikelihood.cpp) as a likelihood table
-// for the E-step kernels with its parameter step from the soft joint
-// histogram, and the affine / rigid transformations
+// the SSD likelihood's parameter step (BINDER/src/cpu/SSDLikelihood.cpp) from
+// the soft joint histogram (the E-step kernels evaluate the likelihood
+// itself), and the affine / rigid transformations
 // (BINDER/src/cpu/AffineTransformation.cpp, RigidTransformation.cpp) as FP64
 // voxel reductions, a one-thread solve and an apply pass.
 #pragma once
@@ -12,7 +12,7 @@ namespace binder {
 
 // ---------------------------------------------------------------------------
 // SSD likelihood.  Parameters on the device: par[0] sigma^2, par[1] shift,
-// par[2] scale, par[3] the reference's `sample` flag (SSDLikelihood.h:47:
+// par[2] scale, par[3] the reference's `sample` flag (SSDLikelihood.h:46:
 // set by the first parameter draw, after which every update draws).
 // ---------------------------------------------------------------------------
 constexpr int kSsdParams = 4;

@@ -980,7 +980,7 @@ int binder_registration_set_ssd_likelihood(binder_registration *r, int bins, int
     r->theta.upload(par.data(), par.size(), r->s());
     r->thT.alloc((size_t)bins * bins);
     // the scale and shift as given are what get_likelihood_parameters
-    // returns until an update (SSDLikelihood.cpp:326-333)
+    // returns until an update (SSDLikelihood.cpp:328-334)
     r->ssd_given[0] = scale;
     r->ssd_given[1] = shift;
     HIP_OK(hipStreamSynchronize(r->s()));
@@ -1834,7 +1834,7 @@ int binder_registration_get_likelihood_parameters(binder_registration *r,
     BINDER_REQUIRE(r && theta_out, "null argument");
     if (!r->has_mi) throw Error(BINDER_ESTATE, "setMILikelihood was not called");
     if (r->lik == 1) {
-      // SSDLikelihood::get_parameters (:326-333): [scale, shift]
+      // SSDLikelihood::get_parameters (:328-334): [scale, shift]
       double p3[3];
       const int rc = binder_registration_get_ssd_parameters(r, p3);
       if (rc != BINDER_OK) throw Error(rc, binder_last_error());

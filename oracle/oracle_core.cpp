@@ -355,7 +355,7 @@ struct Registration {
   int lik = 0;
   int ssd_shift_scale = 0;
   double sigma2 = 1.0, shift = 0.0, scale = 1.0, a0 = 0.0, b0 = 0.0, log_constant = 0.0;
-  int ssd_sample = 0;  // set by sample_parameters and never cleared (SSDLikelihood.cpp:291)
+  int ssd_sample = 0;  // set by sample_parameters and never cleared (SSDLikelihood.cpp:285)
   // affine / rigid transformation (AffineTransformation.cpp,
   // RigidTransformation.cpp): tmodel 1 or 2 replaces the DCT smoother
   int tmodel = 0;
@@ -461,7 +461,7 @@ struct Registration {
     return -std::log(Z);
   }
 
-  // compute_log_likelihood (SSDLikelihood.cpp:58-85) for fixed bin u and
+  // compute_log_likelihood (SSDLikelihood.cpp:60-86) for fixed bin u and
   // the moving bin k of the node (0 outside the grid: the value the
   // reference uses for an outside node, voxel - 0 or voxel - shift)
   double ssd_loglik(int u, int k) const {
@@ -545,7 +545,7 @@ struct Registration {
 
   // The SSD statistics as sums over the soft joint histogram H[k][u] (the
   // posterior mass of moving bin k at voxels of fixed bin u): the per-node
-  // sums of SSDLikelihood::accumulate_info (:118-150) grouped by bin.
+  // sums of SSDLikelihood::accumulate_info (:117-146) grouped by bin.
   //   g = sum post, gk = sum post k, gkk = sum post k^2, gu = sum post u,
   //   guk = sum post u k, and sq(m0, m1) = sum post (u - m0 - m1 k)^2.
   void ssd_moments(double &g, double &gk, double &gkk, double &gu, double &guk) const {
@@ -571,12 +571,12 @@ struct Registration {
   }
 
   // update_parameters / update_shift_and_scale / update_sigma /
-  // sample_parameters (SSDLikelihood.cpp:152-304)
+  // sample_parameters (SSDLikelihood.cpp:148-307)
   void ssd_update(bool draw) {
     if (draw) ssd_sample = 1;
     // with shift and scale, update_shift_and_scale / update_sigma draw once
-    // the flag is set (:205, :266); without them update_parameters is the
-    // MAP step whatever the flag (:154-171) and only sample_parameters draws
+    // the flag is set (:211, :273); without them update_parameters is the
+    // MAP step whatever the flag (:148-167) and only sample_parameters draws
     const bool smp = ssd_shift_scale == 1 ? ssd_sample != 0 : draw;
     double g, gk, gkk, gu, guk;
     ssd_moments(g, gk, gkk, gu, guk);
@@ -611,7 +611,7 @@ struct Registration {
     }
   }
 
-  // compute_cost (SSDLikelihood.cpp:308-320): -log InvGamma(sigma^2; a0, a0 b0)
+  // compute_cost (SSDLikelihood.cpp:311-323): -log InvGamma(sigma^2; a0, a0 b0)
   double ssd_cost() const {
     const double a = a0, b = a0 * b0, x = sigma2;
     return -(a * std::log(b) - std::lgamma(a) + (-a - 1.0) * std::log(x) - b / x);
@@ -701,7 +701,7 @@ struct Registration {
   // the oracle's result does not depend on the thread count.
   double estep(bool sample, std::vector<std::mt19937> *rngs) {
     std::fill(dream.begin(), dream.end(), 0.0);  // initialize_dream_locations
-    // SSDLikelihood::initialize_parameters (:39-56)
+    // SSDLikelihood::initialize_parameters (:39-58)
     if (lik == 1) log_constant = -std::log(std::sqrt(2.0 * M_PI * sigma2));
     std::vector<double> slabH((size_t)nxv * K * L, 0.0);
     std::vector<double> slab_mll(nxv, 0.0);
@@ -889,7 +889,7 @@ void orc_reg_set_ssd(void *h, int bins, double sigma_sq, int scale_shift, double
   r->bn.assign(binned_nodes, binned_nodes + r->nn);
   r->bv.assign(binned_voxels, binned_voxels + r->nv);
 }
-// SSDLikelihood::get_parameters (:326-333): [scale, shift]; sigma^2 third
+// SSDLikelihood::get_parameters (:328-334): [scale, shift]; sigma^2 third
 void orc_reg_get_ssd(void *h, double *out3) {
   auto *r = (Registration *)h;
   out3[0] = r->scale;

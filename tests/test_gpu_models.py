@@ -135,7 +135,7 @@ def _map_and_cov(H):
 
 def test_ssd_draws_on_gpu(core):
     """Parameter draws of the sampler (sample_parameters, SSDLikelihood.cpp:
-    273-304) from one histogram: (shift, scale) ~ N(MAP, A^-1); without
+    284-307) from one histogram: (shift, scale) ~ N(MAP, A^-1); without
     shift and scale 1 / sigma^2 ~ Gamma(den / 2, 2 den / num)."""
     _, prob = _pair(shape=(24, 20, 16), bins=16)
     g = _ssd_reg(core, prob, 1)
@@ -176,7 +176,7 @@ def test_ssd_draws_on_gpu(core):
 def test_ssd_sampler_runs(core, orc):
     """The full sampler with the SSD likelihood (E-draws, parameter draws,
     deformation draws) from the EM solution.  The reference's sigma^2 draw
-    (:283-301) has mean sum-of-squares / (sum h)^2, so sigma^2 falls to
+    (:284-307) has mean sum-of-squares / (sum h)^2, so sigma^2 falls to
     ~1e-6 bins^2 within a few sweeps and the E-draws become near
     deterministic: the per-voxel log-space weights keep that regime exact.
     Different generators: the HIP chain's mean is as far from the oracle's

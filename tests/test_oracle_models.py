@@ -8,13 +8,13 @@ vectors for them.  The checks here are chosen so that a dropped term, a
 wrong sign or index or a transposed operand fails one of them:
 
   * the log-space E-step (Registration.cpp:332-440 with
-    SSDLikelihood::compute_log_likelihood, :58-85) against a NumPy loop
+    SSDLikelihood::compute_log_likelihood, :60-86) against a NumPy loop
     with scipy's logsumexp, outside nodes taking moving bin 0;
-  * the MAP step (update_shift_and_scale / update_sigma, :176-271, and the
-    shift/scale-free update_parameters, :152-173) against a weighted
+  * the MAP step (update_shift_and_scale / update_sigma, :169-282, and the
+    shift/scale-free update_parameters, :148-167) against a weighted
     least-squares fit by numpy.linalg.lstsq on the soft joint histogram;
-  * compute_cost (:308-320) against scipy.stats.invgamma;
-  * the draws (sample_parameters, :273-304): mean and covariance of
+  * compute_cost (:311-323) against scipy.stats.invgamma;
+  * the draws (sample_parameters, :284-307): mean and covariance of
     (shift, scale) against N(MAP, A^-1), the moments of 1 / sigma^2 against
     Gamma(shape den / 2, scale 2 den / num), and which update draws once the
     draw flag is set;
@@ -175,7 +175,7 @@ def test_oracle_ssd_draws_shift_scale_gaussian(orc):
 def test_oracle_ssd_draws_sigma_gamma(orc):
     """Without shift and scale, 1 / sigma^2 ~ Gamma(shape den / 2, scale
     2 den / num), num = sum h (u - k)^2 + a0 b0, den = sum h + a0 (the
-    reference's parametrisation, :283-301)."""
+    reference's parametrisation, :284-307)."""
     a0, b0 = 2.0, 3.0
     r, *_ = _ssd_registration(orc, (5, 4, 3), 8, 5.0, 0, a0=a0, b0=b0, seed=9)
     _, H = r.e_step(return_histogram=True)
@@ -196,7 +196,7 @@ def test_oracle_ssd_draws_sigma_gamma(orc):
 def test_oracle_ssd_draw_flag(orc):
     """Once a draw has run, the MAP step of the shift/scale model draws
     (update_shift_and_scale / update_sigma test the flag), the MAP step
-    without shift and scale does not (update_parameters, :152-173)."""
+    without shift and scale does not (update_parameters, :148-167)."""
     for ss in (0, 1):
         r, *_ = _ssd_registration(orc, (6, 5, 4), 8, 4.0, ss, seed=11)
         r.e_step()
