@@ -225,3 +225,91 @@ def test_fastregistration_models_vs_oracle(core, orc, monkeypatch, kw):
         # the reference's hand-over unpacks [scale, shift] as (shift, scale)
         assert np.allclose([g.shift, g.scale], [o.shift, o.scale], rtol=1e-5, atol=1e-6)
         assert np.allclose(g.registratorObj.get_likelihood_parameters(), [g.shift, g.scale])
+
+
+def _merge(parts):
+    """Chan et al. pooled mean / M2 of [(n, mean, M2)] in order."""
+    n, mean, m2 = parts[0]
+    mean, m2 = mean.copy(), m2.copy()
+    for nb, mb, m2b in parts[1:]:
+        tot = n + nb
+        d = mb - mean
+        mean = mean + d * (nb / tot)
+        m2 = m2 + m2b + np.einsum("...i,...j->...ij", d, d) * (n * nb / tot)
+        n = tot
+    return n, mean, m2
+
+
+@pytest.mark.parametrize("model", ["ssd", "affine"])
+def test_model_chains_match_per_seed_runs(core, model):
+    """Graph-captured chains with the SSD likelihood (parameter draws in the
+    captured sweeps) and with the affine fit (the least-squares fit as the
+    deformation step): each chain draws what sampler_run with its seed
+    draws, graph replays equal eager launches, the state is restored."""
+    print(f"{model} chains 24x20x16: 2 chains x (2 + 5) sweeps", flush=True)
+    _, prob = _pair(shape=(24, 20, 16), bins=16)
+
+    def handle():
+        if model == "ssd":
+            r = _ssd_reg(core, prob, 1)
+        else:
+            r = synth.build_registration(core, prob)
+            r.setAffineTransformation(np.eye(4).ravel(), np.zeros(3))
+        r.EM(3, -1e30, 1, 0)
+        return r
+
+    seeds, Nb, Ns = [21, 22], 2, 5
+    parts = []
+    for sd in seeds:
+        r = handle()
+        r.sampler_run(sd, Nb, Ns)
+        mean, cov = r.sampler_result()
+        parts.append((Ns, mean, cov * (Ns - 1)))
+    n, mref, m2ref = _merge(parts)
+    r = handle()
+    start = r.get_final_locations_as_numpy_array()
+    par = np.asarray(r.get_likelihood_parameters())
+    r.sampler_chains(seeds, Nb, Ns, use_graph=False)
+    me, ce = r.sampler_result()
+    np.testing.assert_allclose(me, mref, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(ce, m2ref / (n - 1), rtol=1e-9, atol=1e-9)
+    np.testing.assert_array_equal(r.get_final_locations_as_numpy_array(), start)
+    np.testing.assert_array_equal(np.asarray(r.get_likelihood_parameters()), par)
+    r.sampler_chains(seeds, Nb, Ns, use_graph=True)
+    mg, cg = r.sampler_result()
+    np.testing.assert_array_equal(mg, me)
+    np.testing.assert_array_equal(cg, ce)
+
+
+@pytest.mark.parametrize("ss", [0, 1])
+def test_ssd_2d_em_vs_oracle(core, orc, ss):
+    print(f"SSD EM x3 2D 64x56 shift_scale={ss}", flush=True)
+    _, prob = _pair(shape=(64, 56, 1), bins=32)
+    g = _ssd_reg(core, prob, ss)
+    o = _ssd_reg(orc, prob, ss)
+    g.EM(3, -1e30, 1, 0)
+    o.EM(3, -1e30, 1, 0)
+    tg, to = g.get_objective_trace(), o.get_objective_trace()
+    fg = np.asarray(g.get_final_locations_as_numpy_array(), np.float64)
+    fo = o.get_final_locations_as_numpy_array()
+    pg, po = np.array(g.get_ssd_parameters()), np.array(o.get_ssd_parameters())
+    rel = np.abs(tg - to) / np.abs(to)
+    print(f"  trace rel {rel.max(0)}; params {pg} vs {po}; field max "
+          f"{np.abs(fg - fo).max():.2e}", flush=True)
+    assert rel[:, 0].max() <= 1e-5 and rel[:, 3].max() <= 1e-5
+    assert np.allclose(pg, po, rtol=1e-5, atol=1e-5)
+    assert np.abs(fg - fo).max() <= 1e-3
+
+
+def test_batch_rejects_other_models(core):
+    """The lockstep batch runs the MI likelihood with the DCT prior only."""
+    _, prob = _pair(shape=(24, 20, 16), bins=16)
+    r = _ssd_reg(core, prob, 1)
+    with pytest.raises((RuntimeError, ValueError)):
+        b = core.Batch([r])
+        b.EM([2], -1e30, 1)
+    a = synth.build_registration(core, prob)
+    a.setRigidTransformation(np.eye(3).ravel(), np.zeros(3))
+    with pytest.raises((RuntimeError, ValueError)):
+        b = core.Batch([a])
+        b.EM([2], -1e30, 1)
