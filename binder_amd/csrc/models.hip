@@ -209,10 +209,14 @@ __device__ void svd_small_dev(int n, const double *H, double *U, double *s, doub
       W[i * n + j] = i == j ? 1.0 : 0.0;
     }
   for (int sweep = 0; sweep < 60; ++sweep) {
-    double off = 0.0;
-    for (int p = 0; p < n; ++p)
+    // stop once the off-diagonal part is below 1e-15 of the diagonal
+    // (relative: the moments scale with the voxel count)
+    double off = 0.0, dia = 0.0;
+    for (int p = 0; p < n; ++p) {
+      dia += B[p * n + p] * B[p * n + p];
       for (int q = p + 1; q < n; ++q) off += B[p * n + q] * B[p * n + q];
-    if (off < 1e-300) break;
+    }
+    if (off <= 1e-30 * dia || off < 1e-300) break;
     for (int p = 0; p < n; ++p)
       for (int q = p + 1; q < n; ++q) {
         const double apq = B[p * n + q];
@@ -271,21 +275,32 @@ __device__ void svd_small_dev(int n, const double *H, double *U, double *s, doub
   }
 }
 
-// Sums the partials in block order, fits the transform, writes the
+// Sums the block partials (each thread a fixed strided subset, then a
+// fixed tree: the same order every call), fits the transform, writes the
 // parameters, the zero prior cost and (optionally) the host statistics.
-__global__ __launch_bounds__(64) void k_lin_solve(const double *__restrict__ part, int nblk,
-                                                  int kind, int N, long long nv,
-                                                  const double *__restrict__ xstat,
-                                                  double *par, double *prior_out,
-                                                  const double *stats_in, double *host_stats) {
-  __shared__ double S[kLinMoments];
-  if ((int)threadIdx.x < kLinMoments) {
-    double acc = 0.0;
-    for (int b = 0; b < nblk; ++b) acc += part[b * kLinMoments + threadIdx.x];
-    S[threadIdx.x] = acc;
+__global__ __launch_bounds__(kLinThreads) void k_lin_solve(const double *__restrict__ part,
+                                                           int nblk, int kind, int N,
+                                                           long long nv,
+                                                           const double *__restrict__ xstat,
+                                                           double *par, double *prior_out,
+                                                           const double *stats_in,
+                                                           double *host_stats) {
+  __shared__ double red[kLinMoments][kLinThreads];
+  {
+    double acc[kLinMoments] = {0};
+    for (int b = threadIdx.x; b < nblk; b += kLinThreads)
+      for (int j = 0; j < kLinMoments; ++j) acc[j] += part[b * kLinMoments + j];
+    for (int j = 0; j < kLinMoments; ++j) red[j][threadIdx.x] = acc[j];
   }
   __syncthreads();
+  for (int w = kLinThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int j = 0; j < kLinMoments; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + w];
+    __syncthreads();
+  }
   if (threadIdx.x != 0) return;
+  double S[kLinMoments];
+  for (int j = 0; j < kLinMoments; ++j) S[j] = red[j][0];
   const int n1 = N + 1;
   if (kind == kLinAffine) {
     // T = (X^T X)^-1 X^T Y with X = [1, x]: rows 0 (sum y) and 1 + i (sum x_i y)
@@ -448,7 +463,7 @@ void LinearModel::run(const float *dream, const float *x, float *out, const Smoo
   hipLaunchKernelGGL(k_lin_moments, dim3(blocks_), dim3(kLinThreads), 0, st, dream, x, nv_, N_,
                      part_.get());
   HIP_LAUNCH_OK();
-  hipLaunchKernelGGL(k_lin_solve, dim3(1), dim3(64), 0, st, part_.get(), blocks_, kind_, N_,
+  hipLaunchKernelGGL(k_lin_solve, dim3(1), dim3(kLinThreads), 0, st, part_.get(), blocks_, kind_, N_,
                      nv_, xstat_.get(), par_.get(), opt.want_prior ? opt.prior_out : nullptr,
                      opt.stats_in, opt.host_stats);
   HIP_LAUNCH_OK();

@@ -129,10 +129,13 @@ static void svd_small(int n, const double *H, double *U, double *s, double *V) {
       W[i * n + j] = i == j ? 1.0 : 0.0;
     }
   for (int sweep = 0; sweep < 60; ++sweep) {
-    double off = 0.0;
-    for (int p = 0; p < n; ++p)
+    // stop once the off-diagonal part is below 1e-15 of the diagonal
+    double off = 0.0, dia = 0.0;
+    for (int p = 0; p < n; ++p) {
+      dia += B[p * n + p] * B[p * n + p];
       for (int q = p + 1; q < n; ++q) off += B[p * n + q] * B[p * n + q];
-    if (off < 1e-300) break;
+    }
+    if (off <= 1e-30 * dia || off < 1e-300) break;
     for (int p = 0; p < n; ++p)
       for (int q = p + 1; q < n; ++q) {
         const double apq = B[p * n + q];
