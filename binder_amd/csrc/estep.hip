@@ -1023,13 +1023,19 @@ __global__ __launch_bounds__(256) void k_welford(WelfordBatch b, double *mean, d
   constexpr int NP = NC * (NC + 1) / 2;
   const long long stride = (long long)gridDim.x * blockDim.x;
   if (b.n0_base) b.n0 += *b.n0_base;
+  // no sample folded in yet: the accumulators start from zero in registers
+  // instead of being read (the first update then gives mean = x, M2 = 0
+  // exactly, as from zeroed memory), so they need no clearing beforehand
+  const bool fresh = b.n0 == 0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
        i += stride) {
-    double mu[NC], M[NP];
+    double mu[NC] = {}, M[NP] = {};
+    if (!fresh) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) mu[c] = mean[c * nv + i];
+      for (int c = 0; c < NC; ++c) mu[c] = mean[c * nv + i];
 #pragma unroll
-    for (int p = 0; p < NP; ++p) M[p] = m2[p * nv + i];
+      for (int p = 0; p < NP; ++p) M[p] = m2[p * nv + i];
+    }
     if constexpr (CNT > 0) {
       float xs[CNT][NC];
       const uint32_t iu = (uint32_t)i, nv32 = (uint32_t)nv;

@@ -129,8 +129,8 @@ void launch_theta_batch(const ThetaArgs *d_args, int npairs, hipStream_t s);
 // FP64 Welford update of mean [N][nv] and upper-triangular M2 [N(N+1)/2][nv]
 // with `count` samples x[0..count) (each [N][nv]) taken in that order, n0
 // samples having been folded in before: per voxel the accumulators are
-// loaded once, updated sample by sample exactly as one update per sweep
-// would, and stored once.
+// loaded once (not at all when n0 == 0: they start from zero), updated
+// sample by sample exactly as one update per sweep would, and stored once.
 constexpr int kWelfordBatch = 16;
 struct WelfordBatch {
   const float *x[kWelfordBatch];

@@ -89,14 +89,6 @@ __global__ __launch_bounds__(256) void k_check_index_grid(const float *__restric
   if (!ok) *flag = 0;
 }
 
-// x[i] = 0 (a kernel rather than hipMemsetAsync, so that it can sit in a
-// captured graph)
-__global__ __launch_bounds__(256) void k_zero_f64(double *__restrict__ x, long long n) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    x[i] = 0.0;
-}
-
 // Padded, packed moving bins of the 3D E-step: word (x, y, zq) of the
 // [nx+2][ny+2][nz+5] array holds the bins of z = zq - 4 .. zq - 1 of node
 // (x - 1, y - 1) in bytes 0..3, zero outside the grid.
@@ -1505,9 +1497,13 @@ int binder_registration_sampler_run(binder_registration *r, int seed, int N_b,
     const int N = r->N;
     const int npairs = N * (N + 1) / 2;
     r->mean.alloc((size_t)N * r->nv);
-    r->mean.zero(r->s());
     r->m2.alloc((size_t)npairs * r->nv);
-    r->m2.zero(r->s());
+    if (N_s == 0) {
+      // no Welford launch: the result is zero (otherwise the first launch
+      // writes every voxel's accumulators without reading them)
+      r->mean.zero(r->s());
+      r->m2.zero(r->s());
+    }
     r->samples = N_s;
     if (sp) r->tune_estep_sample();
     uint32_t k0, k1;
@@ -1570,11 +1566,8 @@ int binder_registration_sampler_chains(binder_registration *r, int nchains, cons
     kslot.alloc(4);
     const uint32_t *keyx = kslot.get();
     const long long *n0b = reinterpret_cast<const long long *>(kslot.get() + 2);
-    hipLaunchKernelGGL(k_zero_f64, dim3(1024), dim3(256), 0, r->s(), r->mean.get(),
-                       (long long)nf);
-    hipLaunchKernelGGL(k_zero_f64, dim3(1024), dim3(256), 0, r->s(), r->m2.get(),
-                       (long long)npairs * r->nv);
-    HIP_LAUNCH_OK();
+    // no clearing of mean / M2: chain 0's first Welford launch (sample
+    // offset 0) writes them without reading (N_s >= 1)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     auto release = [&] {
