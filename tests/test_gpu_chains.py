@@ -36,9 +36,10 @@ def _merge(parts):
     return n, mean, m2
 
 
-@pytest.mark.parametrize("Nb,Ns", [(2, 5), (3, 14)])
+@pytest.mark.parametrize("Nb,Ns", [(2, 5), (3, 14), (3, 31)])
 def test_chains_match_per_seed_runs_and_graph_replays(core, Nb, Ns):
-    # (3, 14): 17 sweeps, the field ring wraps within a chain
+    # (3, 14) and (3, 31): 17 and 34 sweeps, the 16-slot field ring wraps
+    # within a chain, once and twice
     print(f"sampler chains 24x20x18: 3 chains x ({Nb} burn-in + {Ns}) sweeps", flush=True)
     prob = _problem()
     seeds = [11, 12, 13]
