@@ -171,14 +171,18 @@ __device__ __forceinline__ void slice_columns(const uint32_t q[16], int xa,
 
 // SSD likelihood of one voxel (fixed bin u) as node weights relative to its
 // closest node: with d_v = a - s v (a = u - shift, s = scale; a = u, s = 1
-// without shift and scale) and r the positive-weight node bin of smallest
-// |d|, exp(ll(v) - ll(r)) = exp(h s (v - r)(2a - s (v + r))), h = 1/(2 sigma^2):
-// the difference of squares d_r^2 - d_v^2 in factored form, so its rounding
-// is relative to the exponent itself, not to ll(v) (which can be ~1e9 when
-// sigma^2 is small).  The exponent is at most 0 up to rounding and is
-// clamped there, so a zero-weight node closer than r adds exactly 0.
+// without shift and scale), r the positive-weight node bin of smallest |d|
+// and delta = v - r, ll(v) - ll(r) = h (d_r^2 - d_v^2) =
+// delta (2 h s d_r - h s^2 delta), h = 1/(2 sigma^2).  The per-voxel
+// coefficients come from d_r in FP64, so the exponent's rounding is relative
+// to the exponent itself, not to ll(v) (which can be ~1e9 when sigma^2 is
+// small), and d_r carries no cancellation of a - s r.  The exponent is at
+// most 0 up to rounding and is clamped there, so a zero-weight node closer
+// than r adds exactly 0.  Both coefficients carry log2(e) for the base-2
+// exponential.
+constexpr double kLog2e = 1.4426950408889634074;
 struct SsdParams {
-  float s, c1;              // scale, h * scale
+  float s;                  // scale
   double shift, scale, h, logc;
 };
 __device__ __forceinline__ SsdParams ssd_params(const EStepArgs &a) {
@@ -189,14 +193,14 @@ __device__ __forceinline__ SsdParams ssd_params(const EStepArgs &a) {
   p.h = 0.5 / s2;
   p.logc = -log(sqrt(2.0 * M_PI * s2));  // log_constant (SSDLikelihood.cpp:57)
   p.s = (float)p.scale;
-  p.c1 = (float)(p.h * p.scale);
   return p;
 }
 struct SsdVoxel {
-  float av2, s, c1, rf;
+  float a2, b2;  // 2 h s d_r log2(e), h s^2 log2(e)
+  int r;
   __device__ __forceinline__ float w(int v) const {
-    const float vf = (float)v;
-    return __expf(fminf(c1 * (vf - rf) * fmaf(-s, vf + rf, av2), 0.f));
+    const float dl = (float)(v - r);
+    return __builtin_amdgcn_exp2f(fminf(dl * fmaf(-b2, dl, a2), 0.f));
   }
 };
 // r over the nodes with positive spline weight (columns c = 4 xa + yb of
@@ -222,8 +226,9 @@ __device__ __forceinline__ SsdVoxel ssd_voxel(const SsdParams &p, int u, const f
     }
   }
   const double dr = ad - p.scale * (double)r;
-  llr2 = (p.logc - p.h * dr * dr) * 1.4426950408889634074;  // / ln 2
-  return SsdVoxel{2.f * av, p.s, p.c1, (float)r};
+  llr2 = (p.logc - p.h * dr * dr) * kLog2e;  // / ln 2
+  return SsdVoxel{(float)(2.0 * p.h * p.scale * dr * kLog2e),
+                  (float)(p.h * p.scale * p.scale * kLog2e), r};
 }
 
 // The 64 (3D) or 16 (2D) neighbour weights are evaluated twice -- once for
