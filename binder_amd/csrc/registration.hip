@@ -842,9 +842,8 @@ namespace {
 
 // The likelihood set-up shared by the MI and SSD models: the fixed bins
 // (bytes, [0, L)) and the moving bins (bytes packed four z per word, [0, K))
-// on the device, the histogram buffers for K x L bins, and the fixed-bin
-// counts on the host when `ucount` is given.  Leaves the uploads queued on
-// the handle's stream (the caller synchronizes).
+// on the device and the histogram buffers for K x L bins.  Leaves the
+// uploads queued on the handle's stream (the caller synchronizes).
 void set_bins(binder_registration *r, int K, int L, const int *binned_nodes,
               const int *binned_voxels) {
   BINDER_REQUIRE(binned_nodes && binned_voxels, "null input array");
