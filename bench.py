@@ -867,6 +867,83 @@ def copy_calibration(mib=1024):
     log(f"copy calibration: 3 copies of {mib} MiB (read {mib} MiB + write {mib} MiB each)")
 
 
+def models_report(args):
+    """The reference's other models at 160x192x160, 32 bins (SSD likelihood,
+    affine and rigid transformations; BINDER/src/cpu/SSDLikelihood.cpp,
+    AffineTransformation.cpp, RigidTransformation.cpp) beside MI + DCT prior:
+    per EM iteration and per Gibbs sweep on the GPU, and 3 EM iterations from
+    the identity against the FP64 oracle (all host cores)."""
+    from binder_amd import _core as core, synth
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pycore as orc
+    core.autotune(0 if args.no_autotune else 1)
+    pair, prob = synth.config_problem("c2", seed=args.seed)
+    nv = int(np.prod(prob.shape))
+    sig0 = float(np.mean((prob.binned_nodes.astype(np.float64) - prob.binned_voxels) ** 2))
+    th = host_threads()
+
+    def make(be, kind, threads=1):
+        Nx, Ny, Nz = prob.shape
+        r = be.Registration(prob.final_locations.ravel(), threads, prob.voxel_pos.ravel(),
+                            prob.voxel_pos.ravel(), Nx, Ny, Nz, Nx, Ny, Nz, 3, prob.nnz)
+        r.setNonLinearTransformation(*prob.gammas, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, prob.D.ravel())
+        if kind == "affine":
+            r.setAffineTransformation(np.eye(4).ravel(), np.zeros(3))
+        if kind == "rigid":
+            r.setRigidTransformation(np.eye(3).ravel(), np.zeros(3))
+        if kind == "ssd":
+            r.setSSDLikelihood(prob.K, threads, sig0, 1, 1.0, 0.0, prob.binned_nodes.ravel(),
+                               prob.binned_voxels.ravel(), 1.0, 1.0)
+        else:
+            r.setMILikelihood(prob.alpha, prob.theta.ravel(), prob.K, prob.L, threads,
+                              prob.binned_nodes.ravel(), prob.binned_voxels.ravel())
+        return r
+
+    out = {"workload": "160x192x160, 32 bins, c2 synthetic pair (seed %d)" % args.seed,
+           "models": {}}
+    for kind in ("mi", "ssd", "affine", "rigid"):
+        ent = {}
+        g = make(core, kind)
+        g.EM(3, -1e30, 1, 0)
+        fg = np.asarray(g.get_final_locations_as_numpy_array(), np.float64)
+        tg = g.get_objective_trace()
+        t0 = time.perf_counter()
+        o = make(orc, kind, th)
+        o.EM(3, -1e30, 1, 0)
+        ent["oracle_3_iterations_s"] = round(time.perf_counter() - t0, 2)
+        fo = o.get_final_locations_as_numpy_array()
+        to = o.get_objective_trace()
+        ent["vs_oracle_3_iterations"] = {
+            "field_max_vox": float(np.abs(fg - fo).max()),
+            "mll_rel": float(np.max(np.abs(tg[:, 0] - to[:, 0]) / np.abs(to[:, 0]))),
+            "mlp_rel": float(np.max(np.abs(tg[:, 3] - to[:, 3]) / np.abs(to[:, 3])))}
+        if kind == "ssd":
+            ent["ssd_parameters_gpu_oracle"] = [g.get_ssd_parameters(), o.get_ssd_parameters()]
+        if kind in ("affine", "rigid"):
+            pg, po = np.array(g.get_linear_parameters()), np.array(o.get_linear_parameters())
+            ent["linear_parameters_max_diff"] = float(np.abs(pg - po).max())
+        del o
+        g.EM(3, -1e30, 1, 0)  # warm
+        t0 = time.perf_counter()
+        g.EM(20, -1e30, 1, 0)
+        ent["em_ms_per_iteration"] = round(1e3 * (time.perf_counter() - t0) / 20, 4)
+        g.set_profiling(True)
+        g.EM(5, -1e30, 1, 0)
+        ent["em_kernel_avg_us"] = {k: round(1e3 * v[0] / max(v[1], 1), 2)
+                                   for k, v in g.get_kernel_times().items()}
+        g.set_profiling(False)
+        g.sampler_run(args.seed, 2, 8)  # warm
+        t0 = time.perf_counter()
+        g.sampler_run(args.seed + 1, 0, 40)
+        sw = (time.perf_counter() - t0) / 40
+        ent["sweep_ms"] = round(1e3 * sw, 4)
+        ent["voxel_updates_per_s"] = round(nv / sw, 1)
+        out["models"][kind] = ent
+        log(f"{kind}: EM {ent['em_ms_per_iteration']} ms/iteration, sweep {ent['sweep_ms']} ms, "
+            f"vs oracle {ent['vs_oracle_3_iterations']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -905,6 +982,8 @@ def main():
                     help="only the warm-start network's inference / training-step report")
     ap.add_argument("--warmstart-first", type=int, default=0, metavar="I",
                     help="first held-out pair of --warmstart-eval (0-based)")
+    ap.add_argument("--models", action="store_true",
+                    help="only the SSD / affine / rigid model report (GPU times, oracle parity)")
     ap.add_argument("--batched", type=int, default=0, metavar="N",
                     help="only the many-pairs / many-chains report over N pairs")
     ap.add_argument("--batch-workers", type=int, default=8)
@@ -922,6 +1001,9 @@ def main():
         return
     if args.batched:
         print(json.dumps(batched_report(args)), flush=True)
+        return
+    if args.models:
+        print(json.dumps(models_report(args)), flush=True)
         return
     if args.cnn:
         print(json.dumps(cnn_report(args)), flush=True)
