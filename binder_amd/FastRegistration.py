@@ -58,8 +58,8 @@ class FastRegistration:
             raise NotImplementedError(
                 "the gfx950 backend implements the MI and SSD metrics with the non-linear, "
                 "affine and rigid transformations; not available: " + ", ".join(unsupported))
-        if spline_order != 3:
-            raise ValueError("the gfx950 backend supports spline_order=3 only")
+        if spline_order not in (1, 2, 3):
+            raise ValueError("spline_order must be 1, 2 or 3")
 
         self.registratorObj = None
         self.metric = metric

@@ -61,6 +61,35 @@ __device__ __forceinline__ int safe_floor(float p, float &frac) {
   return (int)f;
 }
 
+// Node index and B-spline weights of one axis at the node offsets -1..2 the
+// E-step kernels visit (Spline.cpp:44-108, Registration.cpp:36-52): cubic
+// on floor(p) (weights at -1..2); quadratic on floor(p + 1/2) (-1..1, the
+// slot at +2 weighs 0); linear on floor(p) (0 and 1, the slots at -1 and +2
+// weigh 0).  Zero-weight slots add nothing to any sum and are never drawn.
+__device__ __forceinline__ int spline_node(float p, int order, float w[4]) {
+  float d;
+  if (order == 2) {
+    const float f = fminf(fmaxf(floorf(p + 0.5f), -1.0e7f), 1.0e7f);
+    d = p - f;
+    const float a = fmaf(-2.f, d, 1.f), b = fmaf(-2.f, d, -1.f);
+    w[0] = 0.125f * (a * a);
+    w[1] = 0.75f - d * d;
+    w[2] = 0.125f * (b * b);
+    w[3] = 0.f;
+    return (int)f;
+  }
+  const int i = safe_floor(p, d);
+  if (order == 1) {
+    w[0] = 0.f;
+    w[1] = 1.f - d;
+    w[2] = d;
+    w[3] = 0.f;
+  } else {
+    spline3(d, w);
+  }
+  return i;
+}
+
 // Adds `val` at hist[idx] for every lane with idx >= 0.  Lanes that share an
 // index are first summed across the wavefront (one LDS atomic per distinct
 // index for up to two distinct indices); the rest add directly.  Must be
@@ -272,15 +301,11 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
     const long long ii = valid ? i : a.nv - 1;
     const int u = a.bv[ii];
     const float *thu = th + u * Kp;
-    float tx, ty, tz = 0.f;
-    const int ix = safe_floor(a.d[ii], tx);
-    const int iy = safe_floor(a.d[a.nv + ii], ty);
-    int iz = 0;
-    if (!IS2D) iz = safe_floor(a.d[2 * a.nv + ii], tz);
     float wx[4], wy[4], wz[4] = {1.f, 0.f, 0.f, 0.f};
-    spline3(tx, wx);
-    spline3(ty, wy);
-    if (!IS2D) spline3(tz, wz);
+    const int ix = spline_node(a.d[ii], a.order, wx);
+    const int iy = spline_node(a.d[a.nv + ii], a.order, wy);
+    int iz = 0;
+    if (!IS2D) iz = spline_node(a.d[2 * a.nv + ii], a.order, wz);
 
     uint32_t qall[16];
     load_columns<IS2D>(a, ix, iy, iz, qall);
@@ -546,15 +571,11 @@ __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
       const uint32_t iu = (uint32_t)i, nv32 = (uint32_t)a.nv;
       const int u = ld_off32(a.bv, iu);
       const float *thu = th + u * Kp;
-      float tx, ty, tz = 0.f;
-      const int ix = safe_floor(ld_off32(a.d, iu), tx);
-      const int iy = safe_floor(ld_off32(a.d, nv32 + iu), ty);
-      int iz = 0;
-      if (!IS2D) iz = safe_floor(ld_off32(a.d, 2 * nv32 + iu), tz);
       float wx[4], wy[4], wz[4] = {1.f, 0.f, 0.f, 0.f};
-      spline3(tx, wx);
-      spline3(ty, wy);
-      if (!IS2D) spline3(tz, wz);
+      const int ix = spline_node(ld_off32(a.d, iu), a.order, wx);
+      const int iy = spline_node(ld_off32(a.d, nv32 + iu), a.order, wy);
+      int iz = 0;
+      if (!IS2D) iz = spline_node(ld_off32(a.d, 2 * nv32 + iu), a.order, wz);
       uint32_t q[16];
       load_columns32<IS2D>(a, ix, iy, iz, q);
       SsdVoxel sv{};
@@ -686,15 +707,11 @@ __device__ __forceinline__ void estep_em32_block(const EStepArgs &a, int lb, int
     if (iu >= vend32) break;
     const int u = ld_off32(a.bv, iu);
     const float *thu = th + u * Kp;
-    float tx, ty, tz = 0.f;
-    const int ix = safe_floor(ld_off32(a.d, iu), tx);
-    const int iy = safe_floor(ld_off32(a.d, nv32 + iu), ty);
-    int iz = 0;
-    if (!IS2D) iz = safe_floor(ld_off32(a.d, 2 * nv32 + iu), tz);
     float wx[4], wy[4], wz[4] = {1.f, 0.f, 0.f, 0.f};
-    spline3(tx, wx);
-    spline3(ty, wy);
-    if (!IS2D) spline3(tz, wz);
+    const int ix = spline_node(ld_off32(a.d, iu), a.order, wx);
+    const int iy = spline_node(ld_off32(a.d, nv32 + iu), a.order, wy);
+    int iz = 0;
+    if (!IS2D) iz = spline_node(ld_off32(a.d, 2 * nv32 + iu), a.order, wz);
     float wzs[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * kSplineScale;

@@ -62,6 +62,8 @@ struct EStepArgs {
   // then runs k_estep (FP64 mll partials) for any K.
   const double *ssd_par;
   int ssd_ss;
+  // B-spline order of the node weights: 1, 2 or 3 (0 is taken as 3)
+  int order;
 };
 
 void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,

@@ -243,6 +243,12 @@ struct binder_registration {
   KernelTimer timer;
   PinnedStats host_stats;
   int nxv, nyv, nzv, nxn, nyn, nzn, N;
+  int order = 3;  // B-spline order of the E-step's node weights
+  // the smoothing's sigma for that order (Spline.cpp:20-26)
+  double sigma_spline() const {
+    return order == 3 ? std::sqrt(9.0 / (8.0 * M_PI))
+                      : (order == 2 ? std::sqrt(8.0 / (9.0 * M_PI)) : 0.4);
+  }
   long long nv, nn;
   double nnz;
   bool is2d;
@@ -333,6 +339,7 @@ struct binder_registration {
     a.sweep = sweep;
     a.ssd_par = lik == 1 ? theta.get() : nullptr;
     a.ssd_ss = ssd_ss;
+    a.order = order;
     (void)sample;
     return a;
   }
@@ -709,12 +716,11 @@ std::unique_ptr<binder_registration> new_registration(const float *voxel_pos, in
                                                       double non_zero_voxels) {
   BINDER_REQUIRE(Nx_v >= 2 && Ny_v >= 2 && Nz_v >= 1, "bad fixed-grid shape");
   BINDER_REQUIRE(Nx_n >= 1 && Ny_n >= 1 && Nz_n >= 1, "bad moving-grid shape");
-  BINDER_REQUIRE(spline_order == 3,
-                 "the HIP backend implements the cubic B-spline only "
-                 "(spline_order must be 3)");
+  BINDER_REQUIRE(spline_order >= 1 && spline_order <= 3, "spline_order must be 1, 2 or 3");
   BINDER_REQUIRE((Nz_v == 1) == (Nz_n == 1),
                  "fixed and moving images must both be 2D or both 3D");
   auto r = std::make_unique<binder_registration>();
+  r->order = spline_order;
   r->nxv = Nx_v;
   r->nyv = Ny_v;
   r->nzv = Nz_v;
@@ -1031,7 +1037,7 @@ int binder_registration_set_nonlinear_transformation(
     const double rho[3] = {voxels_size_x, voxels_size_y, voxels_size_z};
     const double eta[3] = {nodes_size_x, nodes_size_y, nodes_size_z};
     r->sm = std::make_unique<Smoother>(r->nxv, r->nyv, r->nzv, g, rho, eta, D,
-                                       std::sqrt(9.0 / (8.0 * M_PI)), r->s());
+                                       r->sigma_spline(), r->s());
     r->sm->set_index_grid(r->x_index_grid);
     r->lin.reset();  // the DCT prior replaces an affine / rigid model
   });
@@ -1058,7 +1064,7 @@ int binder_registration_set_nonlinear_transformation_dct_prior(
         if (a < nd) t[o] = (2.0 - 2.0 * std::cos(M_PI * (double)k / (double)dims[a])) / v2;
     }
     r->sm = std::make_unique<Smoother>(r->nxv, r->nyv, r->nzv, g, rho, eta, t.data(), order,
-                                       std::sqrt(9.0 / (8.0 * M_PI)), r->s());
+                                       r->sigma_spline(), r->s());
     r->sm->set_index_grid(r->x_index_grid);
     r->lin.reset();  // the DCT prior replaces an affine / rigid model
   });

@@ -159,12 +159,14 @@ def problem_from_pair(pair, gamma=1e-6, alpha=2.0, bins=32, fd_order=2):
                    alpha=alpha, nnz=int(np.count_nonzero(fixed_b)))
 
 
-def build_registration(core, prob, threads=1):
+def build_registration(core, prob, threads=1, spline_order=3):
     """Construct a `core.Registration` (binder_amd._core or the oracle's
-    mirror) for `prob` with the MI likelihood and the DCT prior set."""
+    mirror) for `prob` with the MI likelihood and the DCT prior set, its node
+    weights B-splines of `spline_order` (1, 2 or 3)."""
     Nx, Ny, Nz = prob.shape
     r = core.Registration(prob.final_locations.ravel(), threads, prob.voxel_pos.ravel(),
-                          prob.voxel_pos.ravel(), Nx, Ny, Nz, Nx, Ny, Nz, 3, prob.nnz)
+                          prob.voxel_pos.ravel(), Nx, Ny, Nz, Nx, Ny, Nz, spline_order,
+                          prob.nnz)
     r.setNonLinearTransformation(*prob.gammas, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, prob.D.ravel())
     r.setMILikelihood(prob.alpha, prob.theta.ravel(), prob.K, prob.L, threads,
                       prob.binned_nodes.ravel(), prob.binned_voxels.ravel())

@@ -60,7 +60,8 @@ typedef struct binder_registration binder_registration;
  * -- BINDER/src/gpu/bindings.cpp:18-47, BINDER/src/cpu/Registration.cpp:25-70.
  * final_locations and voxel_pos are [Nx_v,Ny_v,Nz_v,N]; node_pos is unused
  * after construction in the reference and may be NULL here.
- * spline_order must be 3 (the cubic B-spline; others -> BINDER_EINVAL). */
+ * spline_order is 1, 2 or 3 (linear, quadratic or cubic B-spline node
+ * weights, BINDER/src/cpu/Spline.cpp:8-110; others -> BINDER_EINVAL). */
 int binder_registration_create(binder_registration **out,
                                const float *final_locations, int threads,
                                const float *voxel_pos, const float *node_pos,

@@ -120,7 +120,7 @@ def test_fastregistration_rejects_out_of_scope_options(orc, monkeypatch):
             fr.FastRegistration(voxels=FIXED_2D, nodes=MOVING_2D, topology_correction=False, **kw)
     with pytest.raises(ValueError):
         fr.FastRegistration(voxels=FIXED_2D, nodes=MOVING_2D, topology_correction=False,
-                            spline_order=2)
+                            spline_order=4)
 
 
 def test_pyramid_bookkeeping(orc, monkeypatch):
