@@ -66,27 +66,34 @@ __device__ __forceinline__ int safe_floor(float p, float &frac) {
 // on floor(p) (weights at -1..2); quadratic on floor(p + 1/2) (-1..1, the
 // slot at +2 weighs 0); linear on floor(p) (0 and 1, the slots at -1 and +2
 // weigh 0).  Zero-weight slots add nothing to any sum and are never drawn.
+// CUBIC: order 3 only, compiled without the other orders' branches (the hot
+// kernels' default instantiation); otherwise `order` (1, 2, or 3 / 0 for
+// cubic) is read at run time.
+template <bool CUBIC>
 __device__ __forceinline__ int spline_node(float p, int order, float w[4]) {
   float d;
-  if (order == 2) {
-    const float f = fminf(fmaxf(floorf(p + 0.5f), -1.0e7f), 1.0e7f);
-    d = p - f;
-    const float a = fmaf(-2.f, d, 1.f), b = fmaf(-2.f, d, -1.f);
-    w[0] = 0.125f * (a * a);
-    w[1] = 0.75f - d * d;
-    w[2] = 0.125f * (b * b);
-    w[3] = 0.f;
-    return (int)f;
+  if constexpr (!CUBIC) {
+    if (order == 2) {
+      const float f = fminf(fmaxf(floorf(p + 0.5f), -1.0e7f), 1.0e7f);
+      d = p - f;
+      const float a = fmaf(-2.f, d, 1.f), b = fmaf(-2.f, d, -1.f);
+      w[0] = 0.125f * (a * a);
+      w[1] = 0.75f - d * d;
+      w[2] = 0.125f * (b * b);
+      w[3] = 0.f;
+      return (int)f;
+    }
+    if (order == 1) {
+      const int i = safe_floor(p, d);
+      w[0] = 0.f;
+      w[1] = 1.f - d;
+      w[2] = d;
+      w[3] = 0.f;
+      return i;
+    }
   }
   const int i = safe_floor(p, d);
-  if (order == 1) {
-    w[0] = 0.f;
-    w[1] = 1.f - d;
-    w[2] = d;
-    w[3] = 0.f;
-  } else {
-    spline3(d, w);
-  }
+  spline3(d, w);
   return i;
 }
 
@@ -302,10 +309,10 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
     const int u = a.bv[ii];
     const float *thu = th + u * Kp;
     float wx[4], wy[4], wz[4] = {1.f, 0.f, 0.f, 0.f};
-    const int ix = spline_node(a.d[ii], a.order, wx);
-    const int iy = spline_node(a.d[a.nv + ii], a.order, wy);
+    const int ix = spline_node<false>(a.d[ii], a.order, wx);
+    const int iy = spline_node<false>(a.d[a.nv + ii], a.order, wy);
     int iz = 0;
-    if (!IS2D) iz = spline_node(a.d[2 * a.nv + ii], a.order, wz);
+    if (!IS2D) iz = spline_node<false>(a.d[2 * a.nv + ii], a.order, wz);
 
     uint32_t qall[16];
     load_columns<IS2D>(a, ix, iy, iz, qall);
@@ -521,7 +528,7 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
 // counting columns below it, then the node inside the column with partial
 // sums formed by the same operations, so the scan always ends inside the
 // selected column.  LDS: theta rows (padded stride) | 32-bit counts [K*L].
-template <bool IS2D, bool SSD>
+template <bool IS2D, bool SSD, bool CUBIC>
 __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
   // Philox key, with the per-chain device mask when given (captured sweeps)
   const uint32_t key0 = a.keyx ? a.k0 ^ a.keyx[0] : a.k0;
@@ -572,10 +579,10 @@ __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
       const int u = ld_off32(a.bv, iu);
       const float *thu = th + u * Kp;
       float wx[4], wy[4], wz[4] = {1.f, 0.f, 0.f, 0.f};
-      const int ix = spline_node(ld_off32(a.d, iu), a.order, wx);
-      const int iy = spline_node(ld_off32(a.d, nv32 + iu), a.order, wy);
+      const int ix = spline_node<CUBIC>(ld_off32(a.d, iu), a.order, wx);
+      const int iy = spline_node<CUBIC>(ld_off32(a.d, nv32 + iu), a.order, wy);
       int iz = 0;
-      if (!IS2D) iz = spline_node(ld_off32(a.d, 2 * nv32 + iu), a.order, wz);
+      if (!IS2D) iz = spline_node<CUBIC>(ld_off32(a.d, 2 * nv32 + iu), a.order, wz);
       uint32_t q[16];
       load_columns32<IS2D>(a, ix, iy, iz, q);
       SsdVoxel sv{};
@@ -663,7 +670,7 @@ constexpr float kSplineScale = 2147483648.0f;  // 2^31
 // voxel range, the histogram partial row and the mll partial of that block.
 // The result depends only on (a, lb, nb), never on which physical block runs
 // it, so one pair's blocks give the same partials alone or in a batch.
-template <bool IS2D>
+template <bool IS2D, bool CUBIC>
 __device__ __forceinline__ void estep_em32_block(const EStepArgs &a, int lb, int nb) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int KL = a.K * a.L;
@@ -708,10 +715,10 @@ __device__ __forceinline__ void estep_em32_block(const EStepArgs &a, int lb, int
     const int u = ld_off32(a.bv, iu);
     const float *thu = th + u * Kp;
     float wx[4], wy[4], wz[4] = {1.f, 0.f, 0.f, 0.f};
-    const int ix = spline_node(ld_off32(a.d, iu), a.order, wx);
-    const int iy = spline_node(ld_off32(a.d, nv32 + iu), a.order, wy);
+    const int ix = spline_node<CUBIC>(ld_off32(a.d, iu), a.order, wx);
+    const int iy = spline_node<CUBIC>(ld_off32(a.d, nv32 + iu), a.order, wy);
     int iz = 0;
-    if (!IS2D) iz = spline_node(ld_off32(a.d, 2 * nv32 + iu), a.order, wz);
+    if (!IS2D) iz = spline_node<CUBIC>(ld_off32(a.d, 2 * nv32 + iu), a.order, wz);
     float wzs[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * kSplineScale;
@@ -821,21 +828,21 @@ __device__ __forceinline__ void estep_em32_block(const EStepArgs &a, int lb, int
   }
 }
 
-template <bool IS2D>
+template <bool IS2D, bool CUBIC>
 __global__ __launch_bounds__(kEm32Threads) void k_estep_em32(EStepArgs a) {
-  estep_em32_block<IS2D>(a, xcd_logical_block(blockIdx.x, gridDim.x), gridDim.x);
+  estep_em32_block<IS2D, CUBIC>(a, xcd_logical_block(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 // The same E-step for several registrations in one launch: work item w
 // (pair << 16 | logical block) runs estep_em32_block of that pair, with the
 // pair's own block count.  Items are ordered pair-major, and consecutive
 // items share an XCD, as one pair's logical blocks do alone.
-template <bool IS2D>
+template <bool IS2D, bool CUBIC>
 __global__ __launch_bounds__(kEm32Threads) void k_estep_em32_batch(const EStepArgs *args,
                                                                  const uint32_t *work) {
   const uint32_t w = work[xcd_logical_block(blockIdx.x, gridDim.x)];
   const EStepArgs a = args[w >> 16];
-  estep_em32_block<IS2D>(a, (int)(w & 0xFFFFu), a.nblocks);
+  estep_em32_block<IS2D, CUBIC>(a, (int)(w & 0xFFFFu), a.nblocks);
 }
 
 constexpr int kRowsPerReduce = kHistReduceRows;
@@ -1083,6 +1090,9 @@ __global__ __launch_bounds__(256) void k_welford(WelfordBatch b, double *mean, d
   }
 }
 
+// The spline order runs the kernels' order-3-only instantiation.
+inline bool cubic_order(int order) { return order == 3 || order == 0; }
+
 }  // namespace
 
 void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
@@ -1094,25 +1104,29 @@ void launch_estep(const EStepArgs &a, bool is2d, bool sample, int blocks,
   if (sample) {
     const size_t ldss = (size_t)((Kp * a.L + 3) & ~3) * sizeof(float) +
                         (size_t)a.K * a.L * sizeof(unsigned int);
-    if (a.ssd_par) {
-      if (is2d) hipLaunchKernelGGL((k_esample<true, true>), g, b, ldss, s, a);
-      else hipLaunchKernelGGL((k_esample<false, true>), g, b, ldss, s, a);
-    } else {
-      if (is2d) hipLaunchKernelGGL((k_esample<true, false>), g, b, ldss, s, a);
-      else hipLaunchKernelGGL((k_esample<false, false>), g, b, ldss, s, a);
-    }
+    // [ssd][is2d][cubic]
+    static void (*const ks[2][2][2])(EStepArgs) = {
+        {{k_esample<false, false, false>, k_esample<false, false, true>},
+         {k_esample<true, false, false>, k_esample<true, false, true>}},
+        {{k_esample<false, true, false>, k_esample<false, true, true>},
+         {k_esample<true, true, false>, k_esample<true, true, true>}}};
+    hipLaunchKernelGGL(ks[a.ssd_par != nullptr][is2d][cubic_order(a.order)], g, b, ldss, s, a);
     HIP_LAUNCH_OK();
     return;
   }
   if (a.K <= kEm32MaxK && !a.ssd_par) {
     const size_t lds32 = estep_em32_lds(a.K, a.L);
-    static std::atomic<bool> attr_set[2] = {false, false};
-    auto k = is2d ? k_estep_em32<true> : k_estep_em32<false>;
-    if (!attr_set[is2d]) {
+    static std::atomic<bool> attr_set[2][2] = {};
+    static void (*const ks[2][2])(EStepArgs) = {
+        {k_estep_em32<false, false>, k_estep_em32<false, true>},
+        {k_estep_em32<true, false>, k_estep_em32<true, true>}};
+    const bool cubic = cubic_order(a.order);
+    auto k = ks[is2d][cubic];
+    if (!attr_set[is2d][cubic]) {
       HIP_OK(hipFuncSetAttribute((const void *)k,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  160 * 1024));
-      attr_set[is2d] = true;
+      attr_set[is2d][cubic] = true;
     }
     hipLaunchKernelGGL(k, g, dim3(kEm32Threads), lds32, s, a);
     HIP_LAUNCH_OK();
@@ -1137,15 +1151,18 @@ size_t estep_em32_lds(int K, int L) {
 bool estep_em32_supported(int K) { return K <= kEm32MaxK; }
 
 void launch_estep_em_batch(const EStepArgs *d_args, const uint32_t *d_work, int nwork,
-                           bool is2d, int K, int L, hipStream_t s) {
+                           bool is2d, bool cubic, int K, int L, hipStream_t s) {
   if (nwork <= 0) return;
   BINDER_REQUIRE(K <= kEm32MaxK, "batched E-step needs K <= 32");
-  static std::atomic<bool> attr_set[2] = {false, false};
-  auto k = is2d ? k_estep_em32_batch<true> : k_estep_em32_batch<false>;
-  if (!attr_set[is2d]) {
+  static std::atomic<bool> attr_set[2][2] = {};
+  static void (*const ks[2][2])(const EStepArgs *, const uint32_t *) = {
+      {k_estep_em32_batch<false, false>, k_estep_em32_batch<false, true>},
+      {k_estep_em32_batch<true, false>, k_estep_em32_batch<true, true>}};
+  auto k = ks[is2d][cubic];
+  if (!attr_set[is2d][cubic]) {
     HIP_OK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                160 * 1024));
-    attr_set[is2d] = true;
+    attr_set[is2d][cubic] = true;
   }
   hipLaunchKernelGGL(k, dim3(nwork), dim3(kEm32Threads), estep_em32_lds(K, L), s, d_args,
                      d_work);

@@ -76,11 +76,12 @@ void launch_hist_reduce(const unsigned long long *hpart, int rows, int KL,
 // Several registrations' EM E-steps in one launch (K <= 32 only): d_work
 // holds nwork items (pair << 16 | logical block), pair-major; every pair's
 // EStepArgs (d_args[pair]) carries its own block count, so its partials are
-// those of its own single launch.
+// those of its own single launch.  cubic: every pair's spline order is 3
+// (the order-3-only kernel); otherwise each pair's order is read at run time.
 bool estep_em32_supported(int K);
 size_t estep_em32_lds(int K, int L);
 void launch_estep_em_batch(const EStepArgs *d_args, const uint32_t *d_work, int nwork,
-                           bool is2d, int K, int L, hipStream_t s);
+                           bool is2d, bool cubic, int K, int L, hipStream_t s);
 
 struct HistReduceArgs {
   const unsigned long long *hpart;  // [rows][KL]

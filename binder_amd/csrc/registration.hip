@@ -1312,9 +1312,11 @@ void batch_run(binder_batch *b, double th, int min_finished, int *finished) {
     const int n = (int)list.size();
     hipStream_t bs = b->stream.s;
     unsigned char *dev = A.dev.get();
+    bool cubic = true;
+    for (int p : list) cubic = cubic && b->slots[p].r->order == 3;
     launch_estep_em_batch(reinterpret_cast<const EStepArgs *>(dev),
                           reinterpret_cast<const uint32_t *>(dev + A.off_w), A.nwork, b->is2d,
-                          b->K, b->L, bs);
+                          cubic, b->K, b->L, bs);
     launch_hist_reduce_batch(reinterpret_cast<const HistReduceArgs *>(dev + A.off_h), n,
                              A.max_rows, (int)KL, bs);
     launch_theta_batch(reinterpret_cast<const ThetaArgs *>(dev + A.off_t), n, bs);
