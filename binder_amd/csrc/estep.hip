@@ -914,16 +914,18 @@ __device__ __forceinline__ void theta_block(const ThetaArgs &a) {
   const uint32_t key0 = a.keyx ? a.k0 ^ a.keyx[0] : a.k0;
   const uint32_t key1 = a.keyx ? a.k1 ^ a.keyx[1] : a.k1;
   const int KL = a.K * a.L;
-  const int tid = threadIdx.x;
-  __shared__ double red[1024];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  constexpr int kWaves = 1024 / 64;
   __shared__ double sh[4096];   // H, then theta
   __shared__ double rowc[256];  // per-row prior cost / row sums
-  double s = 0.0;
+  __shared__ double wsum[kWaves];
+  __shared__ long long wfix[kWaves];
+  __shared__ int wodd[kWaves];
+  // mll partials: per thread (index stride 1024), then per wavefront (a
+  // fixed butterfly), then the 16 wavefronts in order by thread 0 at the end
   if (a.mll_fixed) {
     // exact int64 sum of the fixed-point partials (integer additions in any
     // order give the same sum), then one rounding
-    __shared__ long long lred[1024];
-    __shared__ int lodd[1024];
     const long long *fx = reinterpret_cast<const long long *>(a.mllpart);
     long long acc = 0;
     int odd = 0;
@@ -933,32 +935,39 @@ __device__ __forceinline__ void theta_block(const ThetaArgs &a) {
       else if (v == kMllFixZero) odd |= 1;
       else acc += v;
     }
-    lred[tid] = acc;
-    lodd[tid] = odd;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-      if (tid < w) {
-        lred[tid] += lred[tid + w];
-        lodd[tid] |= lodd[tid + w];
-      }
-      __syncthreads();
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+      acc += __shfl_xor(acc, m, 64);
+      odd |= __shfl_xor(odd, m, 64);
     }
-    if (tid == 0)
-      s = lodd[0] & 2 ? (double)NAN
-                      : (lodd[0] & 1 ? (double)INFINITY
-                                     : -kLn2 * ((double)lred[0] / kMllFixScale));
+    if (lane == 0) {
+      wfix[wid] = acc;
+      wodd[wid] = odd;
+    }
   } else {
+    double s = 0.0;
     for (int i = tid; i < a.nmll; i += 1024) s += a.mllpart[i];
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m, 64);
+    if (lane == 0) wsum[wid] = s;
   }
-  red[tid] = s;
   for (int kl = tid; kl < KL; kl += 1024) {
     double h;
     if (a.hsum) {
       // exact u64 sum of the partial rows (any order), then the scaling of
-      // k_hist_reduce2
+      // k_hist_reduce2; 16 rows in flight per step (rows past the end load
+      // the last row and add zero)
       unsigned long long acc = 0;
-#pragma unroll 8
-      for (int r = 0; r < a.hrows; ++r) acc += a.hsum[(long long)r * KL + kl];
+      for (int r0 = 0; r0 < a.hrows; r0 += 16) {
+        unsigned long long v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int r = r0 + j < a.hrows ? r0 + j : a.hrows - 1;
+          v[j] = a.hsum[(long long)r * KL + kl];
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc += r0 + j < a.hrows ? v[j] : 0ull;
+      }
       h = (double)acc * (1.0 / kHistScale);
       a.Hd[kl] = h;
     } else {
@@ -973,10 +982,6 @@ __device__ __forceinline__ void theta_block(const ThetaArgs &a) {
       sh[kl] = a.theta[kl];
   }
   __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
-    if (tid < w) red[tid] += red[tid + w];
-    __syncthreads();
-  }
   if (a.mode != kThetaKeep) {
     // row sums in index order (MILikelihood.cpp:74-79)
     for (int k = tid; k < a.K; k += 1024) {
@@ -1008,9 +1013,23 @@ __device__ __forceinline__ void theta_block(const ThetaArgs &a) {
   }
   __syncthreads();
   if (tid == 0) {
+    double mll;
+    if (a.mll_fixed) {
+      long long acc = 0;
+      int odd = 0;
+      for (int w = 0; w < kWaves; ++w) {
+        acc += wfix[w];
+        odd |= wodd[w];
+      }
+      mll = odd & 2 ? (double)NAN
+                    : (odd & 1 ? (double)INFINITY : -kLn2 * ((double)acc / kMllFixScale));
+    } else {
+      mll = 0.0;
+      for (int w = 0; w < kWaves; ++w) mll += wsum[w];
+    }
     double c = 0.0;
     for (int k = 0; k < a.K; ++k) c += rowc[k];
-    a.stats[0] = red[0];
+    a.stats[0] = mll;
     a.stats[1] = c;
   }
 }

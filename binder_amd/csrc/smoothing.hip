@@ -70,6 +70,40 @@ struct PassArgs {
   double *cost_part, *raw_part;
 };
 
+// An optional sum of an earlier launch's partials that a contraction launch
+// runs before its own work, in its first block's first wavefront, in place
+// of a launch of its own: *out = sum of part[0 .. count), and with host,
+// (stats[0], stats[1], sum) to page-locked host memory.  (A kernel argument
+// of its own: the batched contractions' PassArgs stay as small as they are.)
+struct SumJob {
+  const double *part;
+  long long count;
+  double *out;
+  const double *stats;
+  double *host;
+};
+
+// Lane i adds partials i, i + 64, ... in index order, then a fixed
+// butterfly; lane 0 writes.  Deterministic (the same assignment and order
+// for every launch shape).
+__device__ __forceinline__ void maybe_sum_job(const SumJob &j) {
+  if (!j.out || blockIdx.x != 0 || blockIdx.y != 0 || threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  double s = 0.0;
+  for (long long i = lane; i < j.count; i += 64) s += j.part[i];
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m, 64);
+  if (lane == 0) {
+    *j.out = s;
+    if (j.host) {
+      j.host[0] = j.stats[0];
+      j.host[1] = j.stats[1];
+      j.host[2] = s;
+      __threadfence_system();
+    }
+  }
+}
+
 
 // Split 8 floats into bf16 hi and lo vectors: hi = bf16(v), lo = bf16(v - hi),
 // both rounded to nearest even (v_cvt_pk_bf16_f32).
@@ -301,7 +335,8 @@ __device__ __forceinline__ void contract_block(const PassArgs &a, int bx, int by
 }
 
 template <int KS, bool VEC, int PRO, int EPI>
-__global__ __launch_bounds__(256) void k_contract(PassArgs a) {
+__global__ __launch_bounds__(256) void k_contract(PassArgs a, SumJob j) {
+  maybe_sum_job(j);
   contract_block<KS, VEC, PRO, EPI>(a, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
@@ -339,8 +374,9 @@ constexpr int kPersistThreads = 1024;
 constexpr size_t kPersistMaxLds = 144 * 1024;
 
 template <int KS, bool VEC, int PRO, int EPI>
-__global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a,
-                                                                 int comps) {
+__global__ __launch_bounds__(kPersistThreads) void k_contract_lds(PassArgs a, int comps,
+                                                                 SumJob j) {
+  maybe_sum_job(j);
   extern __shared__ __align__(16) uint4 bsh[];  // [nt][KS][64] hi, then lo
   const int nt_count = (a.n_out + 15) >> 4;
   const int nfrag = nt_count * KS * 64;
@@ -595,7 +631,9 @@ __device__ __forceinline__ void sym_load(const PassArgs &a, const float *src, co
 constexpr int kSymThreads = 512;
 
 template <int NT, bool INV, int PRO, int EPI>
-__global__ __launch_bounds__(kSymThreads) void k_contract_sym(PassArgs a, int comps) {
+__global__ __launch_bounds__(kSymThreads) void k_contract_sym(PassArgs a, int comps,
+                                                             SumJob j) {
+  maybe_sum_job(j);
   constexpr int KS = (NT + 1) / 2;
   extern __shared__ __align__(16) uint4 bsh[];  // [E | O] hi, then [E | O] lo
   const int N = a.n_in, H = N >> 1;
@@ -1093,12 +1131,13 @@ __global__ void k_draw_nu(const double *raw, double *gamma, int N, double nv,
 }
 
 template <int KS, bool VEC, int PRO, int EPI>
-void launch_ks(dim3 grid, hipStream_t s, const PassArgs &a) {
-  hipLaunchKernelGGL((k_contract<KS, VEC, PRO, EPI>), grid, dim3(256), 0, s, a);
+void launch_ks(dim3 grid, hipStream_t s, const PassArgs &a, const SumJob &j) {
+  hipLaunchKernelGGL((k_contract<KS, VEC, PRO, EPI>), grid, dim3(256), 0, s, a, j);
 }
 
 template <int KS, bool VEC, int PRO, int EPI>
-void launch_persist(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs &a) {
+void launch_persist(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs &a,
+                    const SumJob &j) {
   auto k = k_contract_lds<KS, VEC, PRO, EPI>;
   static std::atomic<bool> attr_set{false};
   if (!attr_set) {
@@ -1107,23 +1146,25 @@ void launch_persist(int blocks, size_t lds, int comps, hipStream_t s, const Pass
                                (int)kPersistMaxLds));
     attr_set = true;
   }
-  hipLaunchKernelGGL(k, dim3(blocks), dim3(kPersistThreads), lds, s, a, comps);
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(kPersistThreads), lds, s, a, comps, j);
 }
 
 template <bool VEC, int PRO, int EPI>
-void launch_lds(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs &a) {
+void launch_lds(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs &a,
+                const SumJob &j) {
   switch (a.ks) {
-    case 1: launch_persist<1, VEC, PRO, EPI>(blocks, lds, comps, s, a); break;
-    case 2: launch_persist<2, VEC, PRO, EPI>(blocks, lds, comps, s, a); break;
-    case 3: launch_persist<3, VEC, PRO, EPI>(blocks, lds, comps, s, a); break;
-    case 4: launch_persist<4, VEC, PRO, EPI>(blocks, lds, comps, s, a); break;
-    case 5: launch_persist<5, VEC, PRO, EPI>(blocks, lds, comps, s, a); break;
-    default: launch_persist<6, VEC, PRO, EPI>(blocks, lds, comps, s, a); break;
+    case 1: launch_persist<1, VEC, PRO, EPI>(blocks, lds, comps, s, a, j); break;
+    case 2: launch_persist<2, VEC, PRO, EPI>(blocks, lds, comps, s, a, j); break;
+    case 3: launch_persist<3, VEC, PRO, EPI>(blocks, lds, comps, s, a, j); break;
+    case 4: launch_persist<4, VEC, PRO, EPI>(blocks, lds, comps, s, a, j); break;
+    case 5: launch_persist<5, VEC, PRO, EPI>(blocks, lds, comps, s, a, j); break;
+    default: launch_persist<6, VEC, PRO, EPI>(blocks, lds, comps, s, a, j); break;
   }
 }
 
 template <int NT, bool INV, int PRO, int EPI>
-void launch_sym_nt(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs &a) {
+void launch_sym_nt(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs &a,
+                   const SumJob &j) {
   auto k = k_contract_sym<NT, INV, PRO, EPI>;
   static std::atomic<bool> attr_set{false};
   if (!attr_set) {
@@ -1132,32 +1173,33 @@ void launch_sym_nt(int blocks, size_t lds, int comps, hipStream_t s, const PassA
                                (int)kPersistMaxLds));
     attr_set = true;
   }
-  hipLaunchKernelGGL(k, dim3(blocks), dim3(kSymThreads), lds, s, a, comps);
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(kSymThreads), lds, s, a, comps, j);
 }
 
 // NT = ceil(H / 16) output tiles per parity, 2..kSymMaxTiles
 template <bool INV, int PRO, int EPI>
-void launch_sym(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs &a) {
+void launch_sym(int blocks, size_t lds, int comps, hipStream_t s, const PassArgs &a,
+                const SumJob &j) {
   switch ((a.n_in / 2 + 15) / 16) {
-    case 2: launch_sym_nt<2, INV, PRO, EPI>(blocks, lds, comps, s, a); break;
-    case 3: launch_sym_nt<3, INV, PRO, EPI>(blocks, lds, comps, s, a); break;
-    case 4: launch_sym_nt<4, INV, PRO, EPI>(blocks, lds, comps, s, a); break;
-    case 5: launch_sym_nt<5, INV, PRO, EPI>(blocks, lds, comps, s, a); break;
-    default: launch_sym_nt<6, INV, PRO, EPI>(blocks, lds, comps, s, a); break;
+    case 2: launch_sym_nt<2, INV, PRO, EPI>(blocks, lds, comps, s, a, j); break;
+    case 3: launch_sym_nt<3, INV, PRO, EPI>(blocks, lds, comps, s, a, j); break;
+    case 4: launch_sym_nt<4, INV, PRO, EPI>(blocks, lds, comps, s, a, j); break;
+    case 5: launch_sym_nt<5, INV, PRO, EPI>(blocks, lds, comps, s, a, j); break;
+    default: launch_sym_nt<6, INV, PRO, EPI>(blocks, lds, comps, s, a, j); break;
   }
 }
 
 template <bool VEC, int PRO, int EPI>
-void launch(dim3 grid, hipStream_t s, const PassArgs &a) {
+void launch(dim3 grid, hipStream_t s, const PassArgs &a, const SumJob &j) {
   switch (a.ks) {
-    case 1: launch_ks<1, VEC, PRO, EPI>(grid, s, a); break;
-    case 2: launch_ks<2, VEC, PRO, EPI>(grid, s, a); break;
-    case 3: launch_ks<3, VEC, PRO, EPI>(grid, s, a); break;
-    case 4: launch_ks<4, VEC, PRO, EPI>(grid, s, a); break;
-    case 5: launch_ks<5, VEC, PRO, EPI>(grid, s, a); break;
-    case 6: launch_ks<6, VEC, PRO, EPI>(grid, s, a); break;
-    case 7: launch_ks<7, VEC, PRO, EPI>(grid, s, a); break;
-    default: launch_ks<8, VEC, PRO, EPI>(grid, s, a); break;
+    case 1: launch_ks<1, VEC, PRO, EPI>(grid, s, a, j); break;
+    case 2: launch_ks<2, VEC, PRO, EPI>(grid, s, a, j); break;
+    case 3: launch_ks<3, VEC, PRO, EPI>(grid, s, a, j); break;
+    case 4: launch_ks<4, VEC, PRO, EPI>(grid, s, a, j); break;
+    case 5: launch_ks<5, VEC, PRO, EPI>(grid, s, a, j); break;
+    case 6: launch_ks<6, VEC, PRO, EPI>(grid, s, a, j); break;
+    case 7: launch_ks<7, VEC, PRO, EPI>(grid, s, a, j); break;
+    default: launch_ks<8, VEC, PRO, EPI>(grid, s, a, j); break;
   }
 }
 
@@ -1315,6 +1357,8 @@ struct PassPlan {
   dim3 grid;
   int blocks = 0;
   size_t lds = 0;
+  // optional partial sum run by the launch (SumJob), none by default
+  SumJob sum{};
 };
 
 PassPlan plan_contract(const PassMatrix &m, const float *in, float *out, int comps,
@@ -1421,28 +1465,31 @@ long long launch_contract(const PassPlan &P, hipStream_t stream, KernelTimer *ti
 
 void launch_contract_blocks(const PassPlan &P, hipStream_t stream, int blocks) {
   const PassArgs &a = P.a;
+  const SumJob &j = P.sum;
   const size_t lds = P.lds;
   const int comps = P.comps;
   const dim3 grid = P.grid;
   if (P.sym) {
     if (P.sym_inv) {
-      if (P.epi == EPI_ADDX) launch_sym<true, PRO_NONE, EPI_ADDX>(blocks, lds, comps, stream, a);
+      if (P.epi == EPI_ADDX)
+        launch_sym<true, PRO_NONE, EPI_ADDX>(blocks, lds, comps, stream, a, j);
       else if (P.epi == EPI_ADDGRID)
-        launch_sym<true, PRO_NONE, EPI_ADDGRID>(blocks, lds, comps, stream, a);
-      else launch_sym<true, PRO_NONE, EPI_STORE>(blocks, lds, comps, stream, a);
+        launch_sym<true, PRO_NONE, EPI_ADDGRID>(blocks, lds, comps, stream, a, j);
+      else launch_sym<true, PRO_NONE, EPI_STORE>(blocks, lds, comps, stream, a, j);
     } else {
-      if (P.pro == PRO_X) launch_sym<false, PRO_X, EPI_STORE>(blocks, lds, comps, stream, a);
+      if (P.pro == PRO_X)
+        launch_sym<false, PRO_X, EPI_STORE>(blocks, lds, comps, stream, a, j);
       else if (P.pro == PRO_GRID)
-        launch_sym<false, PRO_GRID, EPI_STORE>(blocks, lds, comps, stream, a);
-      else launch_sym<false, PRO_NONE, EPI_STORE>(blocks, lds, comps, stream, a);
+        launch_sym<false, PRO_GRID, EPI_STORE>(blocks, lds, comps, stream, a, j);
+      else launch_sym<false, PRO_NONE, EPI_STORE>(blocks, lds, comps, stream, a, j);
     }
     HIP_LAUNCH_OK();
     return;
   }
 #define BINDER_GO(V, PR, E)                                            \
   do {                                                                 \
-    if (P.persist) launch_lds<V, PR, E>(blocks, lds, comps, stream, a); \
-    else launch<V, PR, E>(grid, stream, a);                            \
+    if (P.persist) launch_lds<V, PR, E>(blocks, lds, comps, stream, a, j); \
+    else launch<V, PR, E>(grid, stream, a, j);                         \
   } while (0)
 #define BINDER_DISPATCH(V)                                                     \
   if (P.pro == PRO_X) {                                                        \
@@ -1716,6 +1763,12 @@ void Smoother::run(const float *dream, const float *x, float *out,
       hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, part, count, out);
     HIP_LAUNCH_OK();
   };
+  // The EM's prior-cost sum rides on the pass after the filter (its first
+  // wavefront sums the filter pass's partials) instead of a launch of its
+  // own; the sampler keeps the separate launches.
+  auto fold_sum = [&](PassPlan &P, const double *part, long long count, double *out) {
+    P.sum = SumJob{part, count, out, opt.host_stats ? opt.stats_in : nullptr, opt.host_stats};
+  };
   auto sum_partials = [&]() {
     if (opt.want_prior)
       sum_kernel(part_cost_.get(), nparts, opt.prior_out ? opt.prior_out : scal_.get());
@@ -1734,7 +1787,7 @@ void Smoother::run(const float *dream, const float *x, float *out,
     launch_contract(P.z_fwd, st, timer, "dct_z_fwd");
     launch_contract(P.y_fwd, st, timer, "dct_y_fwd");
     launch_fused_plan(P.x, st, timer, "dct_x_fused");
-    if (P.sum.out) sum_kernel(P.sum.part, P.sum.count, P.sum.out);
+    if (P.sum.out) fold_sum(P.z_inv, P.sum.part, P.sum.count, P.sum.out);
     launch_contract(P.z_inv, st, timer, "dct_z_inv");
     launch_contract(P.y_inv, st, timer, "dct_y_inv");
     return;
@@ -1760,9 +1813,13 @@ void Smoother::run(const float *dream, const float *x, float *out,
                     EPI_STORE, nullptr, timer, "dct_y_fwd", nz_);
       nparts = fused_x_pass(st, mx, bufs[1], bufs[0], N_, nv_, (long long)ny_ * nz_, f,
                             timer, "dct_x_fused");
-      sum_partials();
-      contract_pass(st, iz, bufs[0], bufs[1], N_, nv_, nv_, nv_ / nz_, nullptr, nullptr,
-                    EPI_STORE, nullptr, timer, "dct_z_inv", ny_);
+      PassPlan zi = plan_contract(iz, bufs[0], bufs[1], N_, nv_, nv_, nv_ / nz_, nullptr,
+                                  nullptr, EPI_STORE, nullptr, ny_, nullptr);
+      if (!opt.sample && opt.want_prior)
+        fold_sum(zi, part_cost_.get(), nparts, opt.prior_out ? opt.prior_out : scal_.get());
+      else
+        sum_partials();
+      launch_contract(zi, st, timer, "dct_z_inv");
       contract_pass(st, iy, bufs[1], out, N_, nv_, nv_, nv_ / ny_, nullptr, x, EPI_ADDX,
                     nullptr, timer, "dct_y_inv", nz_, gen ? &gy : nullptr);
     } else {
@@ -1772,9 +1829,13 @@ void Smoother::run(const float *dream, const float *x, float *out,
                     nullptr, timer, "dct_y_fwd", 1, gen ? &g2 : nullptr);
       nparts = fused_x_pass(st, mx, bufs[0], bufs[1], N_, nv_, ny_, f, timer,
                             "dct_x_fused");
-      sum_partials();
-      contract_pass(st, iy, bufs[1], out, N_, nv_, nv_, nx_, nullptr, x, EPI_ADDX,
-                    nullptr, timer, "dct_y_inv", 1, gen ? &g2 : nullptr);
+      PassPlan yi = plan_contract(iy, bufs[1], out, N_, nv_, nv_, nx_, nullptr, x, EPI_ADDX,
+                                  nullptr, 1, gen ? &g2 : nullptr);
+      if (!opt.sample && opt.want_prior)
+        fold_sum(yi, part_cost_.get(), nparts, opt.prior_out ? opt.prior_out : scal_.get());
+      else
+        sum_partials();
+      launch_contract(yi, st, timer, "dct_y_inv");
     }
   } else {
     // Six rotating passes: axes contracted innermost first, each pass
