@@ -124,7 +124,7 @@ void launch_hist_reduce1(const unsigned long long *hpart, int rows, int KL,
                          unsigned long long *hpart2, hipStream_t s);
 // Partial rows a theta launch sums itself instead of a separate reduction,
 // and the rows one reduction stage sums into one.
-constexpr int kThetaDirectRows = 32;
+constexpr int kThetaDirectRows = 64;
 constexpr int kHistReduceRows = 16;
 // launch_theta for npairs registrations, one block each.
 void launch_theta_batch(const ThetaArgs *d_args, int npairs, hipStream_t s);
