@@ -855,10 +855,15 @@ __device__ __forceinline__ void hist_reduce1_block(const unsigned long long *__r
   if (kl >= KL) return;
   const int r0 = by * kRowsPerReduce;
   if (r0 >= rows) return;
-  unsigned long long s = 0;
+  // all rows of the group in flight at once (rows past the end load the
+  // last row and add zero)
+  unsigned long long v[kRowsPerReduce];
 #pragma unroll
   for (int r = 0; r < kRowsPerReduce; ++r)
-    if (r0 + r < rows) s += hpart[(long long)(r0 + r) * KL + kl];
+    v[r] = hpart[(long long)(r0 + r < rows ? r0 + r : rows - 1) * KL + kl];
+  unsigned long long s = 0;
+#pragma unroll
+  for (int r = 0; r < kRowsPerReduce; ++r) s += r0 + r < rows ? v[r] : 0ull;
   hpart2[(long long)by * KL + kl] = s;
 }
 

@@ -85,7 +85,7 @@ void launch_estep_em_batch(const EStepArgs *d_args, const uint32_t *d_work, int 
 
 struct HistReduceArgs {
   const unsigned long long *hpart;  // [rows][KL]
-  unsigned long long *hpart2;       // [ceil(rows / 16)][KL]
+  unsigned long long *hpart2;       // [ceil(rows / kHistReduceRows)][KL]
   double *Hd;                       // [KL]
   int rows, KL;
 };
@@ -118,13 +118,13 @@ struct ThetaArgs {
 };
 
 void launch_theta(const ThetaArgs &a, hipStream_t s);
-// The first stage of launch_hist_reduce alone: rows partial rows summed 16
-// at a time into hpart2 (ceil(rows / 16) rows).
+// The first stage of launch_hist_reduce alone: rows partial rows summed
+// kHistReduceRows at a time into hpart2 (ceil(rows / kHistReduceRows) rows).
 void launch_hist_reduce1(const unsigned long long *hpart, int rows, int KL,
                          unsigned long long *hpart2, hipStream_t s);
 // Partial rows a theta launch sums itself instead of a separate reduction,
 // and the rows one reduction stage sums into one.
-constexpr int kThetaDirectRows = 64;
+constexpr int kThetaDirectRows = 32;
 constexpr int kHistReduceRows = 16;
 // launch_theta for npairs registrations, one block each.
 void launch_theta_batch(const ThetaArgs *d_args, int npairs, hipStream_t s);

@@ -396,7 +396,7 @@ struct binder_registration {
     if (hpart.n < (size_t)rows * KL) {
       HIP_OK(hipStreamSynchronize(s()));
       hpart.alloc((size_t)rows * KL);
-      hpart2.alloc((size_t)((rows + 15) / 16) * KL);
+      hpart2.alloc((size_t)((rows + kHistReduceRows - 1) / kHistReduceRows) * KL);
     }
     if (mllpart.n < (size_t)rows) {
       HIP_OK(hipStreamSynchronize(s()));
@@ -920,7 +920,7 @@ void set_bins(binder_registration *r, int K, int L, const int *binned_nodes,
   {
     const int rows = std::max(r->estep_blocks, r->sample_blocks);
     r->hpart.alloc((size_t)rows * K * L);
-    r->hpart2.alloc((size_t)((rows + 15) / 16) * K * L);
+    r->hpart2.alloc((size_t)((rows + kHistReduceRows - 1) / kHistReduceRows) * K * L);
   }
   // the page-locked bins must have gone up before the lock is released
   HIP_OK(hipStreamSynchronize(r->s()));
