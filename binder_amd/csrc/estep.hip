@@ -561,6 +561,15 @@ __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
   const long long wid = (long long)lb * nwaves + wave;
   const long long rbeg = wid * nrows / wtotal, rend = (wid + 1) * nrows / wtotal;
   u32x4 rr{0u, 0u, 0u, 0u};
+  // 32-bit offsets: the host keeps N * nv below 2^30.  The next row's fixed
+  // bin and position are loaded while the current row computes (rows past
+  // the end reload the last voxel; their values are not used).
+  const uint32_t nv32 = (uint32_t)a.nv, vlast = nv32 - 1u;
+  uint32_t pi = (uint32_t)(rbeg * 64 + lane);
+  pi = pi < vlast ? pi : vlast;
+  int pu = ld_off32(a.bv, pi);
+  float pdx = ld_off32(a.d, pi), pdy = ld_off32(a.d, nv32 + pi), pdz = 0.f;
+  if (!IS2D) pdz = ld_off32(a.d, 2 * nv32 + pi);
   for (long long row = rbeg; row < rend; ++row) {
     {
       const int qv = (int)(row & 3);
@@ -574,17 +583,24 @@ __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
       if (i >= a.nv) break;  // only the last row is partial
       const uint32_t word = rr.x;
       rr = u32x4{rr.y, rr.z, rr.w, rr.x};
-      // 32-bit offsets: the host keeps N * nv below 2^30
-      const uint32_t iu = (uint32_t)i, nv32 = (uint32_t)a.nv;
-      const int u = ld_off32(a.bv, iu);
+      const uint32_t iu = (uint32_t)i;
+      const int u = pu;
       const float *thu = th + u * Kp;
       float wx[4], wy[4], wz[4] = {1.f, 0.f, 0.f, 0.f};
-      const int ix = spline_node<CUBIC>(ld_off32(a.d, iu), a.order, wx);
-      const int iy = spline_node<CUBIC>(ld_off32(a.d, nv32 + iu), a.order, wy);
+      const int ix = spline_node<CUBIC>(pdx, a.order, wx);
+      const int iy = spline_node<CUBIC>(pdy, a.order, wy);
       int iz = 0;
-      if (!IS2D) iz = spline_node<CUBIC>(ld_off32(a.d, 2 * nv32 + iu), a.order, wz);
+      if (!IS2D) iz = spline_node<CUBIC>(pdz, a.order, wz);
       uint32_t q[16];
       load_columns32<IS2D>(a, ix, iy, iz, q);
+      {
+        uint32_t ni = iu + 64u;
+        ni = ni < vlast ? ni : vlast;
+        pu = ld_off32(a.bv, ni);
+        pdx = ld_off32(a.d, ni);
+        pdy = ld_off32(a.d, nv32 + ni);
+        if (!IS2D) pdz = ld_off32(a.d, 2 * nv32 + ni);
+      }
       SsdVoxel sv{};
       if constexpr (SSD) {
         double llr2;
@@ -709,21 +725,42 @@ __device__ __forceinline__ void estep_em32_block(const EStepArgs &a, int lb, int
   // moving-grid offsets (checked when the bins are set): no 64-bit address
   // arithmetic in the voxel loop
   const uint32_t nv32 = (uint32_t)a.nv, vend32 = (uint32_t)vend;
+  // the next voxel's fixed bin and position are loaded while the current one
+  // computes (indices past the block's range reload its last voxel, unused)
+  const uint32_t vlast = vend32 > 0u ? vend32 - 1u : 0u;
+  uint32_t pi = (uint32_t)vbeg + threadIdx.x;
+  pi = pi < vlast ? pi : vlast;
+  int pu = 0;
+  float pdx = 0.f, pdy = 0.f, pdz = 0.f;
+  if ((uint32_t)vbeg < vend32) {
+    pu = ld_off32(a.bv, pi);
+    pdx = ld_off32(a.d, pi);
+    pdy = ld_off32(a.d, nv32 + pi);
+    if (!IS2D) pdz = ld_off32(a.d, 2 * nv32 + pi);
+  }
   for (uint32_t base = (uint32_t)vbeg; base < vend32; base += blockDim.x) {
     const uint32_t iu = base + threadIdx.x;
     if (iu >= vend32) break;
-    const int u = ld_off32(a.bv, iu);
+    const int u = pu;
     const float *thu = th + u * Kp;
     float wx[4], wy[4], wz[4] = {1.f, 0.f, 0.f, 0.f};
-    const int ix = spline_node<CUBIC>(ld_off32(a.d, iu), a.order, wx);
-    const int iy = spline_node<CUBIC>(ld_off32(a.d, nv32 + iu), a.order, wy);
+    const int ix = spline_node<CUBIC>(pdx, a.order, wx);
+    const int iy = spline_node<CUBIC>(pdy, a.order, wy);
     int iz = 0;
-    if (!IS2D) iz = spline_node<CUBIC>(ld_off32(a.d, 2 * nv32 + iu), a.order, wz);
+    if (!IS2D) iz = spline_node<CUBIC>(pdz, a.order, wz);
     float wzs[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * kSplineScale;
     uint32_t qall[16];
     load_columns32<IS2D>(a, ix, iy, iz, qall);
+    {
+      uint32_t ni = iu + blockDim.x;
+      ni = ni < vlast ? ni : vlast;
+      pu = ld_off32(a.bv, ni);
+      pdx = ld_off32(a.d, ni);
+      pdy = ld_off32(a.d, nv32 + ni);
+      if (!IS2D) pdz = ld_off32(a.d, 2 * nv32 + ni);
+    }
 
     float Z = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
     uint32_t mask = 0u;
