@@ -562,8 +562,8 @@ __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
   const long long rbeg = wid * nrows / wtotal, rend = (wid + 1) * nrows / wtotal;
   u32x4 rr{0u, 0u, 0u, 0u};
   // 32-bit offsets: the host keeps N * nv below 2^30.  The next row's fixed
-  // bin and position are loaded while the current row computes (rows past
-  // the end reload the last voxel; their values are not used).
+  // bin and position are loaded while the current row computes (indices past
+  // the volume reload its last voxel; their values are not used).
   const uint32_t nv32 = (uint32_t)a.nv, vlast = nv32 - 1u;
   uint32_t pi = (uint32_t)(rbeg * 64 + lane);
   pi = pi < vlast ? pi : vlast;
@@ -594,7 +594,9 @@ __global__ __launch_bounds__(256) void k_esample(EStepArgs a) {
       uint32_t q[16];
       load_columns32<IS2D>(a, ix, iy, iz, q);
       {
-        uint32_t ni = iu + 64u;
+        // the wavefront's last row reloads its own voxel (no fetch from the
+        // next wavefront's run)
+        uint32_t ni = row + 1 < rend ? iu + 64u : iu;
         ni = ni < vlast ? ni : vlast;
         pu = ld_off32(a.bv, ni);
         pdx = ld_off32(a.d, ni);
