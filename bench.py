@@ -109,7 +109,7 @@ def registration(args, pair, net=None, ref=None, ref_objective=None):
     chunk's field is topology-corrected as the pipeline would return it).
     Each chunk is judged by up to three criteria:
       * oracle field: within 0.1 voxel RMS of `ref` (the FP64 oracle's
-        converged field of the same pipeline, the north-star definition);
+        converged field of the same pipeline: the time-to-tolerance target);
       * objective: mlp at or below `ref_objective` (the cold start's final
         objective, used to judge the warm start);
       * own field: within 0.1 voxel RMS of A's own converged field.

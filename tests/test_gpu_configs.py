@@ -109,7 +109,7 @@ def test_c1_last_level_from_shared_start(core, orc, monkeypatch):
     host) with the oracle's theta.  The float32 positions carry about 1e-5 px
     of rounding at 190 px, which the EM dynamics amplify at a few pixels:
     after one iteration the field is within 1e-3 px max of the oracle's and
-    mll / mlp within 1e-5 relative; after 3 the RMS is
+    mll / mlp within 1e-5 relative (the EM tolerance of these tests); after 3 the RMS is
     within 1e-3 px; after 100 the RMS is within 0.05 px, with the printed
     trajectory showing that the worst 1 % of pixels carry nearly all of the
     squared difference."""
