@@ -1013,7 +1013,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the registration path is single-GPU in this build)", file=sys.stderr)
+              "(the registration and sampling path runs on one GPU; sharding it is a design, not built)", file=sys.stderr)
         sys.exit(2)
     print(json.dumps(run(args)), flush=True)
 

@@ -167,7 +167,7 @@ void download_planes(const float *src, float *host, DBuf<float> &tmp, long long 
     return;
   }
   // a full-size field: one DMA into page-locked memory, then a threaded copy
-  // into the caller's (pageable) buffer, instead of the driver's staged
+  // into the caller's (pageable) buffer, instead of the HIP runtime's staged
   // pageable transfer
   std::lock_guard<std::mutex> lock(g_pinned_mu);
   if (g_pinned_bytes < bytes) {
