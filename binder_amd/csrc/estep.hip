@@ -302,20 +302,40 @@ __global__ __launch_bounds__(256) void k_estep(EStepArgs a) {
       ((a.nv + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
   const long long vbeg = (long long)lb * per;
   const long long vend = vbeg + per < a.nv ? vbeg + per : a.nv;
+  // the next voxel's fixed bin and position are loaded while the current one
+  // computes (lanes past the range use the volume's last voxel, as below)
+  int pu = 0;
+  float pdx = 0.f, pdy = 0.f, pdz = 0.f;
+  if (vbeg < vend) {
+    const long long p0 = vbeg + threadIdx.x;
+    const long long pii = p0 < vend ? p0 : a.nv - 1;
+    pu = a.bv[pii];
+    pdx = a.d[pii];
+    pdy = a.d[a.nv + pii];
+    if (!IS2D) pdz = a.d[2 * a.nv + pii];
+  }
   for (long long base = vbeg; base < vend; base += blockDim.x) {
     const long long i = base + threadIdx.x;
     const bool valid = i < vend;
     const long long ii = valid ? i : a.nv - 1;
-    const int u = a.bv[ii];
+    const int u = pu;
     const float *thu = th + u * Kp;
     float wx[4], wy[4], wz[4] = {1.f, 0.f, 0.f, 0.f};
-    const int ix = spline_node<false>(a.d[ii], a.order, wx);
-    const int iy = spline_node<false>(a.d[a.nv + ii], a.order, wy);
+    const int ix = spline_node<false>(pdx, a.order, wx);
+    const int iy = spline_node<false>(pdy, a.order, wy);
     int iz = 0;
-    if (!IS2D) iz = spline_node<false>(a.d[2 * a.nv + ii], a.order, wz);
+    if (!IS2D) iz = spline_node<false>(pdz, a.order, wz);
 
     uint32_t qall[16];
     load_columns<IS2D>(a, ix, iy, iz, qall);
+    {
+      const long long nx = i + blockDim.x;
+      const long long nii = nx < vend ? nx : a.nv - 1;
+      pu = a.bv[nii];
+      pdx = a.d[nii];
+      pdy = a.d[a.nv + nii];
+      if (!IS2D) pdz = a.d[2 * a.nv + nii];
+    }
     // likelihood of moving bin v at this voxel: the table, or the SSD weight
     SsdVoxel sv{};
     double llr2 = 0.0;
